@@ -1,0 +1,144 @@
+"""Benchmark: PB solves/sec + p99 latency at 1e-6 relative residual,
+mixed precision vs FP64 on one MI355X.
+
+Workload: the 161^3 linearised Poisson-Boltzmann solve (h = 0.4 A, 5 levels,
+1000 synthetic atoms in a 25 A sphere, 0.15 M salt), V(2,2) cycles with
+Galerkin coarsening, stop at 1e-6 L1 relative residual.  One step = one
+complete solve from inputs resident in HBM: operator assembly (VbuildA,
+VbuildP, VbuildG) and the V-cycles to tolerance.
+
+Prints one JSON line (see the contract at the end of main()).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="c1")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    return ap.parse_args()
+
+
+def timed_steps(ctx, mode, steps, warmup):
+    for _ in range(warmup):
+        ctx.rebuild()
+        ctx.solve(mode)
+    lat = []
+    iters = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s = time.perf_counter()
+        ctx.rebuild()
+        ctx.solve(mode)
+        lat.append((time.perf_counter() - s) * 1e3)
+        iters.append(ctx.stats()["iters"])
+    wall = time.perf_counter() - t0
+    return wall, np.array(lat), iters
+
+
+def cpu_baseline(cfg, inp, threads):
+    """The CPU oracle (C restatement of the reference pmgc, OpenMP) timed on
+    this host on one complete solve of the same workload."""
+    if threads:
+        os.environ["OMP_NUM_THREADS"] = str(threads)
+    from tests import oracle as orc
+
+    n, nlev = cfg["n"], cfg["nlev"]
+    t0 = time.perf_counter()
+    orc.solve(inp, n, nlev)
+    dt = time.perf_counter() - t0
+    nthreads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": 1.0 / dt, "unit": "solves/s", "cores": nthreads, "kind": "port",
+            "sample": f"1 full {n}^3 solve (setup + V-cycles to 1e-6), {dt:.2f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(the solver runs on a single MI355X).", file=sys.stderr)
+        sys.exit(2)
+    import multigrid_amd as mg
+
+    cfg, inp = mg.config_inputs(args.config)
+    n, nlev = cfg["n"], cfg["nlev"]
+    iparm, rparm = mg.make_params(n, nlev)
+    ctx = mg.Context(iparm, rparm, inp)
+
+    # FP64 GPU path (the drop-in arithmetic)
+    wall64, lat64, it64 = timed_steps(ctx, mg.PMG_FP64, args.steps, args.warmup)
+    u64 = ctx.u()
+    # mixed precision: FP32 V-cycles inside FP64 defect correction
+    ctx.ktime_enable(16384)
+    wallmx, latmx, itmx = timed_steps(ctx, mg.PMG_MIXED, args.steps, args.warmup)
+    kms, ksweeps = ctx.ktime()
+    umx = ctx.u()
+    st = ctx.stats()
+    err = float(np.max(np.abs(umx - u64)) / np.max(np.abs(u64)))
+
+    # roofline of the dominant kernel: fine-level red-black half-sweep (FP32
+    # in the mixed path).  Algorithmic bytes per fine interior point and
+    # half-sweep: own-colour oC, cc, fc (3 x 1/2) + oE, oN, uC (3) +
+    # other-colour x read (1/2) + own x write (1/2) = 5.5 arrays.
+    pts = (n - 2) ** 3
+    bytes_half = 5.5 * 4 * pts
+    launches = 2 * ksweeps
+    avg_launch_s = (kms / 1e3) / max(launches, 1)
+    achieved = bytes_half / avg_launch_s / 1e9 if launches else 0.0
+
+    ctx.close()
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(cfg, inp, args.cpu_threads)
+
+    value = args.steps / wallmx
+    out = {
+        "metric": "PB solves/sec + p99 latency at 1e-6 rel residual: mixed-precision vs FP64 GPU",
+        "value": value,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wallmx / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-in-f64",
+        "data": "synthetic",
+        "config": {"workload": f"linearised PBE {n}^3 h={cfg['h']} nlev={nlev} "
+                               f"{cfg['natoms']} atoms, V(2,2) Galerkin, stop 1e-6",
+                   "n": n, "nlev": nlev, "batch": 1},
+        "p50_ms": float(np.percentile(latmx, 50)),
+        "p99_ms": float(np.percentile(latmx, 99)),
+        "fp64": {"value": args.steps / wall64, "p50_ms": float(np.percentile(lat64, 50)),
+                 "p99_ms": float(np.percentile(lat64, 99)), "vcycles": it64[-1]},
+        "mixed_vs_fp64_speedup": (args.steps / wallmx) / (args.steps / wall64),
+        "mixed_outer_iters": itmx[-1],
+        "mixed_potential_rel_err_vs_fp64": err,
+        "setup_ms": st["setup_ms"],
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "fine RB-GS half-sweep (FP32)",
+                     "avg_launch_us": avg_launch_s * 1e6},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,168 @@
+/*
+ * pmg_mi355x.h -- C ABI of the MI355X-native PMG multigrid solver.
+ *
+ * Drop-in boundary for the reference's PMG driver entry points.  All arrays
+ * use the reference's layouts: 1-based, Fortran order (i fastest),
+ * VAT3(a,i,j,k) = a[(k-1)*nx*ny + (j-1)*nx + (i-1)] (pmgc/vmatrix.h:79-80);
+ * iparm/rparm are the APBS control arrays (pmgc/mgsubd.cc:538-576);
+ * iwork/rwork are caller-allocated with the sizes pmg_mgsz returns.
+ *
+ * Return codes (the reference prints and exits instead):
+ *   PMG_OK 0, PMG_ITMAX 1 (itmax reached), PMG_EBADPARM 2 (unsupported or
+ *   invalid iparm/rparm), PMG_ENOTSPD 3 (coarse operator breakdown),
+ *   PMG_ENOMEM 4 (device allocation failed), PMG_EDEVICE 5 (no usable GPU
+ *   or a HIP runtime error), PMG_EBUSY 6 (serving admission refused).
+ *
+ * Host pointers are accepted; the library copies inputs to HBM, builds the
+ * hierarchy there, solves, and copies results back.  The `*_ctx` calls keep
+ * a solve's hierarchy resident between calls.
+ */
+#ifndef PMG_MI355X_H
+#define PMG_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PMG_OK 0
+#define PMG_ITMAX 1
+#define PMG_EBADPARM 2
+#define PMG_ENOTSPD 3
+#define PMG_ENOMEM 4
+#define PMG_EDEVICE 5
+#define PMG_EBUSY 6
+
+/* Replaces pmgc::Vmgsz (pmgc/mgdrvd.h:236-255, mgdrvd.cc:505-601).
+ * out[12] = {nxc,nyc,nzc,nf,nc,narr,narrc,n_rpc,n_iz,n_ipc,iretot,iintot}.
+ * Host-only arithmetic; needs no GPU. */
+int pmg_mgsz(int mgcoar, int mgdisc, int mgsolv, int nx, int ny, int nz,
+             int nlev, int *out);
+
+/* Replaces pmgc::Vmaxlev (pmgc/mgsubd.cc:456-484).  Host-only. */
+int pmg_maxlev(int nx, int ny, int nz);
+
+/* Replaces pmgc::Vmgdriv (pmgc/mgdrvd.h:79-98, mgdrvd.cc:60-65): the linear
+ * PMG solve, same pointer list minus the sycl::queue.  FP64 throughout.
+ * On return u holds the solution with Dirichlet boundary values restored
+ * (mgdrvd.cc:500-502); iwork holds iz/ipc and rwork holds rpc/cc/fc/pc/ac
+ * as the reference leaves them (mgdrvd.cc:145-166).  Unlike the reference,
+ * a1cf..ccf, fcf and tcf are not overwritten. */
+int pmg_mgdriv(int *iparm, double *rparm, int *iwork, double *rwork,
+               double *u, const double *xf, const double *yf, const double *zf,
+               const double *gxcf, const double *gycf, const double *gzcf,
+               double *a1cf, double *a2cf, double *a3cf, double *ccf,
+               double *fcf, double *tcf);
+
+/* Replaces pmgc::Vnewdriv (pmgc/newdrvd.h, newdrvd.cc:60-155): the
+ * nonlinear (sinh) Poisson-Boltzmann Newton solve with PMG inner solves.
+ * rwork needs iretot + 2*nf entries (newdrvd.cc:123). */
+int pmg_newdriv(int *iparm, double *rparm, int *iwork, double *rwork,
+                double *u, const double *xf, const double *yf,
+                const double *zf, const double *gxcf, const double *gycf,
+                const double *gzcf, double *a1cf, double *a2cf, double *a3cf,
+                double *ccf, double *fcf, double *tcf);
+
+/* Replaces pmgc::Vmypdefinitlpbe / Vmypdefinitnpbe
+ * (pmgc/mypdec.cc:72-106): the ion table used by the nonlinear term. */
+int pmg_mypdefinit(int nion, const double *charge, const double *sconc);
+
+/* Iterations, error flag and final relative residual of the last
+ * pmg_mgdriv / pmg_newdriv call on this thread.  The reference computes
+ * these and drops them (pmgc/mgdrvd.cc:194-195, 452-455). */
+int pmg_last_stats(int *iters, int *ierror, double *relres);
+
+/* --- resident-hierarchy API (no reference counterpart: the reference
+ * rebuilds everything per Vmgdriv call; these keep a solve's operators in
+ * HBM for repeated solves, mixed precision and serving) --- */
+
+typedef struct pmg_ctx pmg_ctx;
+
+/* Precision modes of pmg_ctx_solve. */
+#define PMG_FP64 0  /* plain FP64 V-cycles: the drop-in arithmetic      */
+#define PMG_MIXED 1 /* FP32 V-cycle inside FP64 defect correction       */
+
+/* Create a context: decodes iparm/rparm exactly as pmg_mgdriv, uploads the
+ * fine-level inputs and builds the hierarchy (VbuildA, VbuildP, VbuildG)
+ * in FP64 on the GPU. */
+int pmg_ctx_create(pmg_ctx **out, const int *iparm, const double *rparm,
+                   const double *xf, const double *yf, const double *zf,
+                   const double *gxcf, const double *gycf, const double *gzcf,
+                   const double *a1cf, const double *a2cf, const double *a3cf,
+                   const double *ccf, const double *fcf);
+/* Solve on the resident hierarchy (mode PMG_FP64 or PMG_MIXED).  The
+ * solution stays in HBM until pmg_ctx_get_u. */
+int pmg_ctx_solve(pmg_ctx *ctx, int mode);
+/* Copy the solution (with Dirichlet boundary values) to host u (nx*ny*nz). */
+int pmg_ctx_get_u(pmg_ctx *ctx, double *u);
+/* Stats of the last solve on ctx: iterations, ierror, relative residual,
+ * device milliseconds of setup and of the solve. */
+int pmg_ctx_stats(pmg_ctx *ctx, int *iters, int *ierror, double *relres,
+                  double *setup_ms, double *solve_ms);
+/* Rebuild the hierarchy on the GPU from the resident inputs. */
+int pmg_ctx_rebuild(pmg_ctx *ctx);
+/* Device bytes held by the context. */
+size_t pmg_ctx_bytes(const pmg_ctx *ctx);
+void pmg_ctx_destroy(pmg_ctx *ctx);
+
+/* Copy out the FP64 operators built on the GPU for level lev (1-based):
+ * ac (4 or 14 planes), cc, fc, and pc (27 planes of P from level lev+1 to
+ * lev, coarse-sized; ignored on the coarsest level).  Any pointer may be
+ * NULL.  Used to check VbuildA/VbuildP/VbuildG output. */
+int pmg_ctx_get_level(pmg_ctx *ctx, int lev, double *ac, double *cc,
+                      double *fc, double *pc);
+
+/* Kernel timing (no reference counterpart): when enabled, the fine-level
+ * smoothing launches of every later solve on ctx are bracketed by HIP
+ * events on the context's stream (at most max_events events per solve);
+ * pmg_ctx_ktime returns the accumulated device milliseconds and the number
+ * of full red-black sweeps they cover. */
+int pmg_ctx_ktime_enable(pmg_ctx *ctx, int max_events);
+int pmg_ctx_ktime(pmg_ctx *ctx, double *ms, long long *sweeps);
+
+/* --- single-kernel entry points on host arrays (parity checking of the
+ * individual HIP kernels against the reference routines) --- */
+
+/* Vmatvec / Vmresid (pmgc/matvecd.cc:244-261, 512-528), numdia 7 or 27. */
+int pmg_matvec(int nx, int ny, int nz, int numdia, const double *ac,
+               const double *cc, const double *x, double *y);
+int pmg_mresid(int nx, int ny, int nz, int numdia, const double *ac,
+               const double *cc, const double *fc, const double *x,
+               double *r);
+/* Vgsrb7x / Vgsrb27x (pmgc/gsd.cc:60-212): itmax sweeps, in place on x. */
+int pmg_gsrb(int nx, int ny, int nz, int numdia, const double *ac,
+             const double *cc, const double *fc, double *x, int itmax,
+             int iadjoint);
+/* Vrestrc / VinterpPMG (pmgc/matvecd.cc:801-817, 966-983). */
+int pmg_restrc(int nxf, int nyf, int nzf, int nxc, int nyc, int nzc,
+               const double *xin, double *xout, const double *pc);
+int pmg_interp(int nxc, int nyc, int nzc, int nxf, int nyf, int nzf,
+               const double *xin, double *xout, const double *pc);
+/* VbuildP / VbuildG (pmgc/buildPd.cc:1118-1147, buildGd.cc:3276-3368). */
+int pmg_buildP(int nxf, int nyf, int nzf, int nxc, int nyc, int nzc,
+               int mgprol, int numdia, const double *acf, double *pc);
+int pmg_buildG(int nxf, int nyf, int nzf, int nxc, int nyc, int nzc,
+               int numdia, const double *pc, const double *acf, double *acc);
+
+/* --- synthetic Poisson-Boltzmann inputs (no reference counterpart: the
+ * reference assembles them from PQR files in PBE_pmgc_class.h:191-344) ---
+ * Fills the fine-level Vmgdriv inputs for a seeded synthetic molecule.
+ * charge_mode 0: +-1 with net zero; 1: uniform over {-1,-0.5,0,0.5,1}.
+ * gxcf/gycf/gzcf need 4*ny*nz, 4*nx*nz, 4*nx*ny entries; others n^3,
+ * xf/yf/zf n.  Host-only; needs no GPU. */
+int pmg_pbe_synth(int n, double h, int natoms, double sphere_r,
+                  double ionic_strength, int charge_mode, uint64_t seed,
+                  double *xf, double *yf, double *zf, double *gxcf,
+                  double *gycf, double *gzcf, double *a1cf, double *a2cf,
+                  double *a3cf, double *ccf, double *fcf, double *tcf,
+                  double *atoms /* NULL or 5*natoms: x,y,z,radius,charge */);
+
+/* Version string of the library build. */
+const char *pmg_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

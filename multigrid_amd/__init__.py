@@ -1,0 +1,309 @@
+"""multigrid_amd -- MI355X-native PMG (APBS pmgc) multigrid solver.
+
+Python host mirror of the reference's PMG entry points
+(pmgc/mgdrvd.h:79-255, pmgc/newdrvd.h, pmgc/mgsubd.cc:538-576,
+pmgc/mypdec.cc:72-106) over the C ABI of libpmg_mi355x.so
+(include/pmg_mi355x.h).  Arrays are numpy, 1-based Fortran order
+flattened (i fastest), exactly as the reference's pointers.
+
+Names follow the reference: Vmgsz, Vpackmg, Vmgdriv, Vnewdriv,
+Vmypdefinitlpbe / Vmypdefinitnpbe.  Errors that the reference reports with
+printf/exit raise PMGError here; itmax exhaustion returns ierror = 1 as the
+reference's Vmvcs does (pmgc/mgcsd.cc:529).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libpmg_mi355x.so")
+
+PMG_OK, PMG_ITMAX, PMG_EBADPARM, PMG_ENOTSPD, PMG_ENOMEM, PMG_EDEVICE, PMG_EBUSY = range(7)
+PMG_FP64, PMG_MIXED = 0, 1
+_ERRS = {
+    PMG_EBADPARM: "invalid or unsupported iparm/rparm or array shape",
+    PMG_ENOTSPD: "coarse operator breakdown",
+    PMG_ENOMEM: "device memory allocation failed",
+    PMG_EDEVICE: "no usable GPU or HIP runtime error",
+    PMG_EBUSY: "admission refused",
+}
+
+
+class PMGError(RuntimeError):
+    def __init__(self, code, where=""):
+        self.code = code
+        super().__init__(f"{where}: {_ERRS.get(code, 'error')} (code {code})")
+
+
+_lib = None
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+
+
+def lib():
+    """Load the HIP library; fails loudly if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise ImportError(
+                f"{LIBPATH} is missing: build it with "
+                "`python -c 'import __graft_entry__ as g; g.build()'`")
+        L = ctypes.CDLL(LIBPATH)
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+def _declare(L):
+    dp, ip, i, d = _D, _I, ctypes.c_int, ctypes.c_double
+    vp = ctypes.c_void_p
+    sig = {
+        "pmg_mgsz": (i, [i, i, i, i, i, i, i, ip]),
+        "pmg_maxlev": (i, [i, i, i]),
+        "pmg_mgdriv": (i, [ip, dp, ip, dp, dp] + [dp] * 12),
+        "pmg_newdriv": (i, [ip, dp, ip, dp, dp] + [dp] * 12),
+        "pmg_mypdefinit": (i, [i, dp, dp]),
+        "pmg_last_stats": (i, [ip, ip, dp]),
+        "pmg_ctx_create": (i, [ctypes.POINTER(vp), ip, dp] + [dp] * 11),
+        "pmg_ctx_solve": (i, [vp, i]),
+        "pmg_ctx_get_u": (i, [vp, dp]),
+        "pmg_ctx_stats": (i, [vp, ip, ip, dp, dp, dp]),
+        "pmg_ctx_rebuild": (i, [vp]),
+        "pmg_ctx_bytes": (ctypes.c_size_t, [vp]),
+        "pmg_ctx_destroy": (None, [vp]),
+        "pmg_ctx_get_level": (i, [vp, i, dp, dp, dp, dp]),
+        "pmg_ctx_ktime_enable": (i, [vp, i]),
+        "pmg_ctx_ktime": (i, [vp, dp, ctypes.POINTER(ctypes.c_longlong)]),
+        "pmg_matvec": (i, [i, i, i, i, dp, dp, dp, dp]),
+        "pmg_mresid": (i, [i, i, i, i, dp, dp, dp, dp, dp]),
+        "pmg_gsrb": (i, [i, i, i, i, dp, dp, dp, dp, i, i]),
+        "pmg_restrc": (i, [i, i, i, i, i, i, dp, dp, dp]),
+        "pmg_interp": (i, [i, i, i, i, i, i, dp, dp, dp]),
+        "pmg_buildP": (i, [i, i, i, i, i, i, i, i, dp, dp]),
+        "pmg_buildG": (i, [i, i, i, i, i, i, i, dp, dp, dp]),
+        "pmg_pbe_synth": (i, [i, d, i, d, d, i, ctypes.c_uint64] + [dp] * 13),
+        "pmg_version": (ctypes.c_char_p, []),
+    }
+    for name, (res, args) in sig.items():
+        if hasattr(L, name):
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+
+
+def _dp(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_D)
+
+
+def _ip(a):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_I)
+
+
+def _check(rc, where):
+    if rc >= PMG_EBADPARM:
+        raise PMGError(rc, where)
+    return rc
+
+
+# ---------------------------------------------------------------------------
+# Sizing and control arrays
+# ---------------------------------------------------------------------------
+def Vmgsz(mgcoar, mgdisc, mgsolv, nx, ny, nz, nlev):
+    """Workspace sizes (pmgc/mgdrvd.cc:505-601) as a dict."""
+    out = np.zeros(12, np.int32)
+    _check(lib().pmg_mgsz(mgcoar, mgdisc, mgsolv, nx, ny, nz, nlev, _ip(out)), "Vmgsz")
+    keys = ["nxc", "nyc", "nzc", "nf", "nc", "narr", "narrc", "n_rpc", "n_iz",
+            "n_ipc", "iretot", "iintot"]
+    return dict(zip(keys, (int(v) for v in out)))
+
+
+def Vmaxlev(nx, ny, nz):
+    """Deepest admissible level count (pmgc/mgsubd.cc:456-484)."""
+    return int(lib().pmg_maxlev(nx, ny, nz))
+
+
+def Vpackmg(nrwk, niwk, nx, ny, nz, nlev, nu1=2, nu2=2, mgkey=0, itmax=200,
+            istop=1, ipcon=0, nonlin=0, mgsmoo=1, mgprol=1, mgcoar=2,
+            mgsolv=0, mgdisc=0, iinfo=0, errtol=1e-6, ipkey=0, omegal=0.8,
+            omegan=0.9, irite=8, iperf=0):
+    """Pack iparm (22 ints) and rparm (10 doubles) (pmgc/mgsubd.cc:538-576)."""
+    iparm = np.zeros(22, np.int32)
+    rparm = np.zeros(10, np.float64)
+    vals = [nrwk, niwk, nx, ny, nz, nlev, nu1, nu2, mgkey, itmax, istop, iinfo,
+            irite, ipkey, ipcon, nonlin, mgprol, mgcoar, mgdisc, mgsmoo, mgsolv,
+            iperf]
+    iparm[:] = vals
+    rparm[0], rparm[8], rparm[9] = errtol, omegal, omegan
+    return iparm, rparm
+
+
+def make_params(n, nlev, nonlin=0, **kw):
+    """iparm/rparm for an n^3 grid with workspace sizes filled in."""
+    sz = Vmgsz(kw.get("mgcoar", 2), kw.get("mgdisc", 0), kw.get("mgsolv", 0), n, n, n, nlev)
+    nrwk = sz["iretot"] + (2 * sz["nf"] if nonlin else 0)
+    return Vpackmg(nrwk, sz["iintot"], n, n, n, nlev, nonlin=nonlin, **kw)
+
+
+# ---------------------------------------------------------------------------
+# Drop-in drivers
+# ---------------------------------------------------------------------------
+def _driver(fn, name, iparm, rparm, iwork, rwork, u, xf, yf, zf, gxcf, gycf,
+            gzcf, a1cf, a2cf, a3cf, ccf, fcf, tcf):
+    rc = fn(_ip(iparm), _dp(rparm), _ip(iwork), _dp(rwork), _dp(u), _dp(xf),
+            _dp(yf), _dp(zf), _dp(gxcf), _dp(gycf), _dp(gzcf), _dp(a1cf),
+            _dp(a2cf), _dp(a3cf), _dp(ccf), _dp(fcf), _dp(tcf))
+    return _check(rc, name)
+
+
+def Vmgdriv(iparm, rparm, iwork, rwork, u, xf, yf, zf, gxcf, gycf, gzcf,
+            a1cf, a2cf, a3cf, ccf, fcf, tcf):
+    """Linear PMG solve (pmgc/mgdrvd.cc:60-171) on the GPU, FP64.
+    Returns ierror (0 converged, 1 itmax reached)."""
+    return _driver(lib().pmg_mgdriv, "Vmgdriv", iparm, rparm, iwork, rwork, u,
+                   xf, yf, zf, gxcf, gycf, gzcf, a1cf, a2cf, a3cf, ccf, fcf, tcf)
+
+
+def Vnewdriv(iparm, rparm, iwork, rwork, u, xf, yf, zf, gxcf, gycf, gzcf,
+             a1cf, a2cf, a3cf, ccf, fcf, tcf):
+    """Nonlinear PB Newton solve (pmgc/newdrvd.cc:60-155) on the GPU."""
+    return _driver(lib().pmg_newdriv, "Vnewdriv", iparm, rparm, iwork, rwork,
+                   u, xf, yf, zf, gxcf, gycf, gzcf, a1cf, a2cf, a3cf, ccf, fcf, tcf)
+
+
+def Vmypdefinitlpbe(nion, charge, sconc):
+    """Ion table (pmgc/mypdec.cc:72-88)."""
+    charge = np.ascontiguousarray(charge, np.float64)
+    sconc = np.ascontiguousarray(sconc, np.float64)
+    return _check(lib().pmg_mypdefinit(int(nion), _dp(charge), _dp(sconc)),
+                  "Vmypdefinitlpbe")
+
+
+Vmypdefinitnpbe = Vmypdefinitlpbe
+
+
+def last_stats():
+    it, ie, rr = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+    lib().pmg_last_stats(ctypes.byref(it), ctypes.byref(ie), ctypes.byref(rr))
+    return it.value, ie.value, rr.value
+
+
+# ---------------------------------------------------------------------------
+# Resident-hierarchy context
+# ---------------------------------------------------------------------------
+class Context:
+    """A solve whose inputs and hierarchy stay resident in HBM."""
+
+    def __init__(self, iparm, rparm, inp):
+        self._h = ctypes.c_void_p()
+        L = lib()
+        rc = L.pmg_ctx_create(ctypes.byref(self._h), _ip(iparm), _dp(rparm),
+                              _dp(inp["xf"]), _dp(inp["yf"]), _dp(inp["zf"]),
+                              _dp(inp["gxcf"]), _dp(inp["gycf"]), _dp(inp["gzcf"]),
+                              _dp(inp["a1cf"]), _dp(inp["a2cf"]), _dp(inp["a3cf"]),
+                              _dp(inp["ccf"]), _dp(inp["fcf"]))
+        _check(rc, "pmg_ctx_create")
+        self.n = int(iparm[2]), int(iparm[3]), int(iparm[4])
+        self.nlev = int(iparm[5])
+
+    def solve(self, mode=PMG_FP64):
+        return _check(lib().pmg_ctx_solve(self._h, mode), "pmg_ctx_solve")
+
+    def u(self):
+        out = np.empty(self.n[0] * self.n[1] * self.n[2], np.float64)
+        _check(lib().pmg_ctx_get_u(self._h, _dp(out)), "pmg_ctx_get_u")
+        return out
+
+    def stats(self):
+        it, ie = ctypes.c_int(), ctypes.c_int()
+        rr, su, so = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        lib().pmg_ctx_stats(self._h, ctypes.byref(it), ctypes.byref(ie),
+                            ctypes.byref(rr), ctypes.byref(su), ctypes.byref(so))
+        return dict(iters=it.value, ierror=ie.value, relres=rr.value,
+                    setup_ms=su.value, solve_ms=so.value)
+
+    def rebuild(self):
+        return _check(lib().pmg_ctx_rebuild(self._h), "pmg_ctx_rebuild")
+
+    def nbytes(self):
+        return int(lib().pmg_ctx_bytes(self._h))
+
+    def level(self, lev):
+        """FP64 operators of level lev (1-based): (ac, cc, fc, pc)."""
+        n = self.n
+        dims = [n]
+        for _ in range(1, self.nlev):
+            dims.append(tuple((v - 1) // 2 + 1 for v in dims[-1]))
+        d = dims[lev - 1]
+        nl = d[0] * d[1] * d[2]
+        ac = np.zeros((4 if lev == 1 else 14) * nl)
+        cc, fc = np.zeros(nl), np.zeros(nl)
+        pc = None
+        if lev < self.nlev:
+            c = dims[lev]
+            pc = np.zeros(27 * c[0] * c[1] * c[2])
+        _check(lib().pmg_ctx_get_level(self._h, lev, _dp(ac), _dp(cc), _dp(fc),
+                                       _dp(pc)), "pmg_ctx_get_level")
+        return ac, cc, fc, pc
+
+    def ktime_enable(self, max_events=8192):
+        return _check(lib().pmg_ctx_ktime_enable(self._h, max_events), "ktime")
+
+    def ktime(self):
+        ms, cnt = ctypes.c_double(), ctypes.c_longlong()
+        lib().pmg_ctx_ktime(self._h, ctypes.byref(ms), ctypes.byref(cnt))
+        return ms.value, cnt.value
+
+    def close(self):
+        if self._h:
+            lib().pmg_ctx_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# Synthetic PB inputs (host, no GPU needed)
+# ---------------------------------------------------------------------------
+def pbe_synth(n, h, natoms, sphere_r, ionic_strength=0.15, charge_mode=0, seed=20261202):
+    """Fine-level Vmgdriv inputs for a seeded synthetic molecule."""
+    N = n * n * n
+    out = {k: np.zeros(n) for k in ("xf", "yf", "zf")}
+    for k in ("gxcf", "gycf", "gzcf"):
+        out[k] = np.zeros(4 * n * n)
+    for k in ("a1cf", "a2cf", "a3cf", "ccf", "fcf", "tcf"):
+        out[k] = np.zeros(N)
+    atoms = np.zeros(5 * max(natoms, 1))
+    rc = lib().pmg_pbe_synth(n, h, natoms, sphere_r, ionic_strength, charge_mode,
+                             ctypes.c_uint64(seed), *(_dp(out[k]) for k in (
+                                 "xf", "yf", "zf", "gxcf", "gycf", "gzcf", "a1cf",
+                                 "a2cf", "a3cf", "ccf", "fcf", "tcf")), _dp(atoms))
+    _check(rc, "pbe_synth")
+    out["atoms"] = atoms[: 5 * natoms].reshape(natoms, 5)
+    return out
+
+
+# Benchmark configurations (sizes and molecule parameters of the five
+# workloads this solver is measured on).
+CONFIGS = {
+    "c0": dict(n=65, h=0.5, nlev=4, natoms=200, sphere_r=14.0, ionic=0.15, charge_mode=0, seed=20261202),
+    "c1": dict(n=161, h=0.4, nlev=5, natoms=1000, sphere_r=25.0, ionic=0.15, charge_mode=0, seed=20261203),
+    "c2": dict(n=289, h=0.35, nlev=6, natoms=5000, sphere_r=45.0, ionic=0.15, charge_mode=1, seed=20261204),
+    "c3": dict(n=129, h=0.5, nlev=5, natoms=800, sphere_r=20.0, ionic=0.30, charge_mode=0, seed=20261205),
+}
+
+
+def config_inputs(name, **over):
+    c = dict(CONFIGS[name])
+    c.update(over)
+    inp = pbe_synth(c["n"], c["h"], c["natoms"], c["sphere_r"], c["ionic"],
+                    c["charge_mode"], c["seed"])
+    return c, inp

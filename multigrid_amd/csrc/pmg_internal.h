@@ -1,0 +1,121 @@
+// pmg_internal.h -- kernel launchers shared by the PMG driver and the C ABI.
+//
+// Layout on the device is the reference's: 1-based Fortran order, i fastest,
+// operators as planes of n = nx*ny*nz (pmgc/vmatrix.h:79-80,
+// pmgc/matvecd.cc:185-189), P as 27 coarse-sized planes
+// (pmgc/matvecd.cc:809-816).  Every launcher checks the shapes it is given
+// against the grid it launches before launching.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace pmg {
+
+struct Dim {
+  int nx = 0, ny = 0, nz = 0;
+  size_t n() const { return (size_t)nx * ny * nz; }
+  size_t interior() const {
+    return (size_t)(nx - 2) * (ny - 2) * (nz - 2);
+  }
+};
+
+inline int corsr(int n) { return (n - 1) / 2 + 1; }
+
+// Largest number of per-block partial sums any reduction launcher writes for
+// a level of dims d (the caller sizes the partials buffer with it).
+size_t max_partials(const Dim &d);
+
+// ---- smoothers (pmgc/gsd.cc) ----
+template <class T>
+void gs7(const Dim &d, const T *ac, const T *cc, const T *fc, T *x,
+         int nsweeps, int iadjoint, hipStream_t s);
+template <class T>
+void gs27(const Dim &d, const T *ac, const T *cc, const T *fc, T *x,
+          int nsweeps, int iadjoint, hipStream_t s);
+
+// ---- residual r = f - A x (pmgc/matvecd.cc:381-528) ----
+// TO is the output type (FP64 operator may write an FP32 residual for the
+// mixed-precision inner solve).  If part != nullptr, per-block L1 partial
+// sums of the residual (in double) are written there; returns their count.
+template <class T, class TO>
+int resid(const Dim &d, int numdia, const T *ac, const T *cc, const T *fc,
+          const T *x, TO *r, double *part, hipStream_t s);
+// Nonlinear residual r = f - A x - c_vec(cc, x) (pmgc/matvecd.cc:530-693),
+// 7-point fine operator only, FP64.
+int nmresid7(const Dim &d, const double *ac, const double *cc,
+             const double *fc, const double *x, double *r, double *part,
+             hipStream_t s);
+
+// ---- y = A x, with partial sums of x.w0 and x.y (pmgc/mgcsd.cc:429-452) --
+template <class T>
+int matvec_dots(const Dim &d, int numdia, const T *ac, const T *cc,
+                const T *x, const T *w0, T *y, double *part, hipStream_t s);
+
+// ---- grid transfer (pmgc/matvecd.cc:695-983) ----
+template <class T>
+void restrict_(const Dim &f, const Dim &c, const T *xin, T *xout, const T *pc,
+               hipStream_t s);
+// xf(interior) += (*xdamp) * (P xc); xdamp == nullptr means plain
+// interpolation into xf with the fine boundary set to zero.
+template <class T>
+void interp(const Dim &c, const Dim &f, const T *xc, const T *pc, T *xf,
+            const double *xdamp, hipStream_t s);
+
+// ---- reductions ----
+// mode 0: out[0] = sum part[b*stride]; mode 1: out[0] = sum part[b*stride]
+// / sum part[b*stride+1]; mode 2: out[0], out[1] = both sums.
+void reduce(const double *part, int nblk, int stride, int mode, double *out,
+            hipStream_t s);
+// Deterministic interior L1 norm / dot into part (returns block count).
+template <class T>
+int nrm1_part(const Dim &d, const T *x, double *part, hipStream_t s);
+template <class T>
+int dot_part(const Dim &d, const T *x, const T *y, double *part,
+             hipStream_t s);
+
+// ---- coarsest-level CG (pmgc/cgd.cc:60-124), one workgroup, FP64 ----
+// Returns false if the level is too large for the single-workgroup kernel.
+bool cg_fits(const Dim &d);
+template <class T>
+void cg_single(const Dim &d, int numdia, const double *ac, const double *cc,
+               const T *f, T *x, int itmax, double errtol, hipStream_t s);
+
+// ---- BLAS-1 helpers ----
+template <class T>
+void fill(T *x, size_t n, T v, hipStream_t s);
+template <class T>
+void axpy_interior(const Dim &d, double alpha, const T *x, T *y,
+                   hipStream_t s);
+template <class TA, class TB>
+void convert(const TA *a, TB *b, size_t n, hipStream_t s);
+// u(interior) += e(interior) (u FP64, e any type)
+template <class T>
+void add_correction(const Dim &d, const T *e, double *u, hipStream_t s);
+void copy_interior(const Dim &d, const double *x, double *y, hipStream_t s);
+void fbound00(const Dim &d, double *x, hipStream_t s);
+void fbound_dirichlet(const Dim &d, double *x, const double *gx,
+                      const double *gy, const double *gz, hipStream_t s);
+
+// ---- ion terms (pmgc/mypdec.cc) ----
+void set_ions(int nion, const double *charge, const double *sconc);
+void dc_vec(const double *coef, const double *u, double *out, size_t n,
+            hipStream_t s);
+// x_out = x + damp * dx on the interior (boundary of x_out = boundary of x)
+void xpay_interior(const Dim &d, const double *x, double damp,
+                   const double *dx, double *out, hipStream_t s);
+
+// ---- setup (pmgc/buildAd.cc, buildPd.cc, buildGd.cc), FP64 ----
+void buildA(const Dim &d, double *ac, double *cc, double *fc,
+            const double *xf, const double *yf, const double *zf,
+            const double *gx, const double *gy, const double *gz,
+            const double *a1, const double *a2, const double *a3,
+            const double *ccf, const double *fcf, hipStream_t s);
+void buildP(const Dim &f, const Dim &c, int mgprol, int numdia,
+            const double *acf, double *pc, hipStream_t s);
+void buildG(const Dim &f, const Dim &c, int numdia, const double *pc,
+            const double *acf, double *acc, hipStream_t s);
+
+}  // namespace pmg

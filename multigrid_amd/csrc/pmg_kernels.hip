@@ -1,0 +1,1230 @@
+// pmg_kernels.hip -- HIP kernels of the PMG hot path for gfx950 (CDNA4).
+//
+// Every kernel works on the reference's 1-based Fortran-order arrays with i
+// fastest (pmgc/vmatrix.h:79-80).  Work is mapped with i on the 64-lane
+// wavefront dimension so every stencil load is a coalesced row segment;
+// blocks are 64 x 4 threads over (i, j) and march KCH planes in k.
+// Reductions are deterministic: wavefront shuffles, then LDS in a fixed
+// order, one partial per block, and a single-block final pass.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+
+#include "pmg_internal.h"
+
+namespace pmg {
+
+constexpr int BX = 64, BY = 4, KCH = 4;
+
+__host__ __device__ inline size_t ix(int nx, int ny, int i, int j, int k) {
+  return ((size_t)(k - 1) * (size_t)ny + (size_t)(j - 1)) * (size_t)nx +
+         (size_t)(i - 1);
+}
+
+inline int cdiv(int a, int b) { return (a + b - 1) / b; }
+
+static void check_launch(const char *what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    fprintf(stderr, "pmg: launch of %s failed: %s\n", what,
+            hipGetErrorString(e));
+}
+
+// ---------------------------------------------------------------------------
+// Block reduction (deterministic for a fixed block shape)
+// ---------------------------------------------------------------------------
+template <int NT>
+__device__ inline double block_sum(double v, double *red) {
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = tid & 63, wid = tid >> 6;
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (wid == 0) {
+    t = lane < NT / 64 ? red[lane] : 0.0;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
+  }
+  __syncthreads();
+  return t;  // valid in thread 0
+}
+
+// ---------------------------------------------------------------------------
+// 27-point neighbour sums in the reference's grouping (matvecd.cc:137-170)
+// ---------------------------------------------------------------------------
+template <class T, class XT>
+__device__ inline void sum27(const T *__restrict__ ac, size_t n, size_t sy,
+                             size_t sz, size_t p, const XT *x, double &sO,
+                             double &sU, double &sD) {
+  const T *oE = ac + n, *oN = ac + 2 * n, *uC = ac + 3 * n, *oNE = ac + 4 * n,
+          *oNW = ac + 5 * n, *uE = ac + 6 * n, *uW = ac + 7 * n,
+          *uN = ac + 8 * n, *uS = ac + 9 * n, *uNE = ac + 10 * n,
+          *uNW = ac + 11 * n, *uSE = ac + 12 * n, *uSW = ac + 13 * n;
+  using A = T;
+  A o = oN[p] * (A)x[p + sy] + oN[p - sy] * (A)x[p - sy] +
+        oE[p] * (A)x[p + 1] + oE[p - 1] * (A)x[p - 1] +
+        oNE[p] * (A)x[p + 1 + sy] + oNW[p] * (A)x[p - 1 + sy] +
+        oNW[p + 1 - sy] * (A)x[p + 1 - sy] +
+        oNE[p - 1 - sy] * (A)x[p - 1 - sy];
+  A u = uC[p] * (A)x[p + sz] + uN[p] * (A)x[p + sy + sz] +
+        uS[p] * (A)x[p - sy + sz] + uE[p] * (A)x[p + 1 + sz] +
+        uW[p] * (A)x[p - 1 + sz] + uNE[p] * (A)x[p + 1 + sy + sz] +
+        uNW[p] * (A)x[p - 1 + sy + sz] + uSE[p] * (A)x[p + 1 - sy + sz] +
+        uSW[p] * (A)x[p - 1 - sy + sz];
+  A dd = uC[p - sz] * (A)x[p - sz] + uS[p + sy - sz] * (A)x[p + sy - sz] +
+         uN[p - sy - sz] * (A)x[p - sy - sz] +
+         uW[p + 1 - sz] * (A)x[p + 1 - sz] +
+         uE[p - 1 - sz] * (A)x[p - 1 - sz] +
+         uSW[p + 1 + sy - sz] * (A)x[p + 1 + sy - sz] +
+         uSE[p - 1 + sy - sz] * (A)x[p - 1 + sy - sz] +
+         uNW[p + 1 - sy - sz] * (A)x[p + 1 - sy - sz] +
+         uNE[p - 1 - sy - sz] * (A)x[p - 1 - sy - sz];
+  sO = o;
+  sU = u;
+  sD = dd;
+}
+
+template <class T, class XT>
+__device__ inline T sum7(const T *__restrict__ ac, size_t n, size_t sy,
+                         size_t sz, size_t p, const XT *x) {
+  const T *oE = ac + n, *oN = ac + 2 * n, *uC = ac + 3 * n;
+  return oN[p] * (T)x[p + sy] + oN[p - sy] * (T)x[p - sy] +
+         oE[p] * (T)x[p + 1] + oE[p - 1] * (T)x[p - 1] +
+         uC[p - sz] * (T)x[p - sz] + uC[p] * (T)x[p + sz];
+}
+
+size_t max_partials(const Dim &d) {
+  return (size_t)cdiv(d.nx, BX) * cdiv(d.ny, BY) * cdiv(d.nz, KCH) + 64;
+}
+
+// ---------------------------------------------------------------------------
+// Smoothers: Vgsrb7x (gsd.cc:60-111) and Vgsrb27x (gsd.cc:113-212)
+// ---------------------------------------------------------------------------
+
+// One red or black half-sweep.  Each thread owns a pair of i-adjacent
+// points of which exactly one has the active colour.
+template <class T>
+__global__ __launch_bounds__(256) void k_gs7(int nx, int ny, int nz,
+                                             const T *__restrict__ ac,
+                                             const T *__restrict__ cc,
+                                             const T *__restrict__ fc, T *x,
+                                             int colour) {
+  const size_t n = (size_t)nx * ny * nz, sy = nx, sz = (size_t)nx * ny;
+  const int ip = blockIdx.x * BX + threadIdx.x;
+  const int j = 2 + blockIdx.y * BY + threadIdx.y;
+  const int k0 = 2 + blockIdx.z * KCH;
+  if (j > ny - 1) return;
+  for (int kk = 0; kk < KCH; kk++) {
+    const int k = k0 + kk;
+    if (k > nz - 1) break;
+    const int i = 2 + 2 * ip + ((colour + j + k) & 1);
+    if (i > nx - 1) continue;
+    const size_t p = ix(nx, ny, i, j, k);
+    const T s = sum7<T, T>(ac, n, sy, sz, p, x);
+    x[p] = (fc[p] + s) / (ac[p] + cc[p]);
+  }
+}
+
+// One colour of the 8-colour sweep: (i%2) + 2(j%2) + 4(k%2) == colour.
+template <class T>
+__global__ __launch_bounds__(256) void k_gs27(int nx, int ny, int nz,
+                                              const T *__restrict__ ac,
+                                              const T *__restrict__ cc,
+                                              const T *__restrict__ fc, T *x,
+                                              int colour) {
+  const size_t n = (size_t)nx * ny * nz, sy = nx, sz = (size_t)nx * ny;
+  const int i = 2 + (colour & 1) + 2 * (int)(blockIdx.x * BX + threadIdx.x);
+  const int j = 2 + ((colour >> 1) & 1) + 2 * (int)(blockIdx.y * BY + threadIdx.y);
+  const int k = 2 + ((colour >> 2) & 1) + 2 * (int)blockIdx.z;
+  if (i > nx - 1 || j > ny - 1 || k > nz - 1) return;
+  const size_t p = ix(nx, ny, i, j, k);
+  double sO, sU, sD;
+  sum27<T, T>(ac, n, sy, sz, p, x, sO, sU, sD);
+  x[p] = (fc[p] + (T)((T)((T)sO + (T)sU) + (T)sD)) / (ac[p] + cc[p]);
+}
+
+template <class T>
+void gs7(const Dim &d, const T *ac, const T *cc, const T *fc, T *x,
+         int nsweeps, int iadjoint, hipStream_t s) {
+  if (d.nx < 3 || d.ny < 3 || d.nz < 3) return;
+  const int npairs = cdiv(d.nx - 2, 2);
+  dim3 g(cdiv(npairs, BX), cdiv(d.ny - 2, BY), cdiv(d.nz - 2, KCH));
+  dim3 b(BX, BY);
+  for (int it = 0; it < nsweeps; it++)
+    for (int c = 0; c <= 1; c++) {
+      const int col = (1 - iadjoint) * c + iadjoint * (1 - c);
+      k_gs7<T><<<g, b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, fc, x, col);
+    }
+  check_launch("gs7");
+}
+
+template <class T>
+void gs27(const Dim &d, const T *ac, const T *cc, const T *fc, T *x,
+          int nsweeps, int iadjoint, hipStream_t s) {
+  if (d.nx < 3 || d.ny < 3 || d.nz < 3) return;
+  dim3 b(BX, BY);
+  for (int it = 0; it < nsweeps; it++)
+    for (int c = 0; c < 8; c++) {
+      const int col = c * (1 - iadjoint) + iadjoint * (7 - c);
+      const int px = col & 1, py = (col >> 1) & 1, pz = (col >> 2) & 1;
+      // number of points of this parity in 2..n-1
+      const int ni = (d.nx - 2 - px + 1) / 2, nj = (d.ny - 2 - py + 1) / 2,
+                nk = (d.nz - 2 - pz + 1) / 2;
+      if (ni <= 0 || nj <= 0 || nk <= 0) continue;
+      dim3 g(cdiv(ni, BX), cdiv(nj, BY), nk);
+      k_gs27<T><<<g, b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, fc, x, col);
+    }
+  check_launch("gs27");
+}
+
+// ---------------------------------------------------------------------------
+// Residual (Vmresid7_1s / Vmresid27_1s) with optional L1 partial sums
+// ---------------------------------------------------------------------------
+template <class T, class TO, int ND>
+__global__ __launch_bounds__(256) void k_resid(
+    int nx, int ny, int nz, const T *__restrict__ ac,
+    const T *__restrict__ cc, const T *__restrict__ fc,
+    const T *__restrict__ x, TO *__restrict__ r, double *__restrict__ part) {
+  __shared__ double red[4];
+  const size_t n = (size_t)nx * ny * nz, sy = nx, sz = (size_t)nx * ny;
+  const int i = 2 + blockIdx.x * BX + threadIdx.x;
+  const int j = 2 + blockIdx.y * BY + threadIdx.y;
+  const int k0 = 2 + blockIdx.z * KCH;
+  double acc = 0.0;
+  if (i <= nx - 1 && j <= ny - 1) {
+    for (int kk = 0; kk < KCH; kk++) {
+      const int k = k0 + kk;
+      if (k > nz - 1) break;
+      const size_t p = ix(nx, ny, i, j, k);
+      T v;
+      if (ND == 7) {
+        const T *oE = ac + n, *oN = ac + 2 * n, *uC = ac + 3 * n;
+        v = fc[p] + oN[p] * x[p + sy] + oN[p - sy] * x[p - sy] +
+            oE[p] * x[p + 1] + oE[p - 1] * x[p - 1] +
+            uC[p - sz] * x[p - sz] + uC[p] * x[p + sz] -
+            (ac[p] + cc[p]) * x[p];
+      } else {
+        double sO, sU, sD;
+        sum27<T, T>(ac, n, sy, sz, p, x, sO, sU, sD);
+        v = (T)((T)((T)(fc[p] + (T)sO) + (T)sU) + (T)sD) -
+            (ac[p] + cc[p]) * x[p];
+      }
+      if (r) r[p] = (TO)v;
+      acc += fabs((double)v);
+    }
+  }
+  if (part) {
+    double t = block_sum<256>(acc, red);
+    if (threadIdx.x == 0 && threadIdx.y == 0)
+      part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] =
+          t;
+  }
+}
+
+template <class T, class TO>
+int resid(const Dim &d, int numdia, const T *ac, const T *cc, const T *fc,
+          const T *x, TO *r, double *part, hipStream_t s) {
+  dim3 g(cdiv(d.nx - 2, BX), cdiv(d.ny - 2, BY), cdiv(d.nz - 2, KCH));
+  dim3 b(BX, BY);
+  if (numdia == 7)
+    k_resid<T, TO, 7><<<g, b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, fc, x, r,
+                                      part);
+  else
+    k_resid<T, TO, 27><<<g, b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, fc, x, r,
+                                       part);
+  check_launch("resid");
+  return (int)(g.x * g.y * g.z);
+}
+
+// ---------------------------------------------------------------------------
+// Ion terms (Vc_vecpmg / Vdc_vecpmg, mypdec.cc:136-212, 348-414)
+// ---------------------------------------------------------------------------
+constexpr int MAXIONS = 50;
+__constant__ double c_charge[MAXIONS];
+__constant__ double c_sconc[MAXIONS];
+__constant__ int c_nion;
+static int h_nion = 0;
+
+void set_ions(int nion, const double *charge, const double *sconc) {
+  if (nion > MAXIONS) nion = MAXIONS;
+  double q[MAXIONS] = {0}, c[MAXIONS] = {0};
+  for (int i = 0; i < nion; i++) q[i] = charge[i], c[i] = sconc[i];
+  hipMemcpyToSymbol(HIP_SYMBOL(c_charge), q, sizeof(q));
+  hipMemcpyToSymbol(HIP_SYMBOL(c_sconc), c, sizeof(c));
+  hipMemcpyToSymbol(HIP_SYMBOL(c_nion), &nion, sizeof(int));
+  h_nion = nion;
+}
+
+__device__ inline double ion_term(double coef, double u, int deriv) {
+  const double ZSMALL = 1.0e-20, ZLARGE = 1.0e20, SMIN = -85.0, SMAX = 85.0;
+  double out = 0.0;
+  for (int s = 0; s < c_nion; s++) {
+    const double q = c_charge[s];
+    const double zcf2 = deriv ? c_sconc[s] * q * q : -1.0 * c_sconc[s] * q;
+    const double zu2 = -1.0 * q;
+    const double az = fmin(ZSMALL, fabs(zcf2 * coef)) * ZLARGE;
+    const double an = fmax(fmin(zu2 * u, 0.0), SMIN);
+    const double ap = fmin(fmax(zu2 * u, 0.0), SMAX);
+    out = out + zcf2 * coef * exp(az * (an + ap));
+  }
+  return out;
+}
+
+__global__ void k_dc_vec(const double *coef, const double *u, double *out,
+                         size_t n) {
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n;
+       t += (size_t)gridDim.x * blockDim.x)
+    out[t] = ion_term(coef[t], u[t], 1);
+}
+
+void dc_vec(const double *coef, const double *u, double *out, size_t n,
+            hipStream_t s) {
+  int nb = (int)((n + 255) / 256);
+  if (nb > 65535) nb = 65535;
+  k_dc_vec<<<nb, 256, 0, s>>>(coef, u, out, n);
+  check_launch("dc_vec");
+}
+
+__global__ __launch_bounds__(256) void k_nmresid7(
+    int nx, int ny, int nz, const double *__restrict__ ac,
+    const double *__restrict__ cc, const double *__restrict__ fc,
+    const double *__restrict__ x, double *__restrict__ r,
+    double *__restrict__ part) {
+  __shared__ double red[4];
+  const size_t n = (size_t)nx * ny * nz, sy = nx, sz = (size_t)nx * ny;
+  const int i = 2 + blockIdx.x * BX + threadIdx.x;
+  const int j = 2 + blockIdx.y * BY + threadIdx.y;
+  const int k0 = 2 + blockIdx.z * KCH;
+  double acc = 0.0;
+  if (i <= nx - 1 && j <= ny - 1) {
+    const double *oE = ac + n, *oN = ac + 2 * n, *uC = ac + 3 * n;
+    for (int kk = 0; kk < KCH; kk++) {
+      const int k = k0 + kk;
+      if (k > nz - 1) break;
+      const size_t p = ix(nx, ny, i, j, k);
+      const double w1 = ion_term(cc[p], x[p], 0);
+      const double v = fc[p] + oN[p] * x[p + sy] + oN[p - sy] * x[p - sy] +
+                       oE[p] * x[p + 1] + oE[p - 1] * x[p - 1] +
+                       uC[p - sz] * x[p - sz] + uC[p] * x[p + sz] -
+                       ac[p] * x[p] - w1;
+      if (r) r[p] = v;
+      acc += fabs(v);
+    }
+  }
+  if (part) {
+    double t = block_sum<256>(acc, red);
+    if (threadIdx.x == 0 && threadIdx.y == 0)
+      part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] =
+          t;
+  }
+}
+
+int nmresid7(const Dim &d, const double *ac, const double *cc,
+             const double *fc, const double *x, double *r, double *part,
+             hipStream_t s) {
+  dim3 g(cdiv(d.nx - 2, BX), cdiv(d.ny - 2, BY), cdiv(d.nz - 2, KCH));
+  k_nmresid7<<<g, dim3(BX, BY), 0, s>>>(d.nx, d.ny, d.nz, ac, cc, fc, x, r,
+                                        part);
+  check_launch("nmresid7");
+  return (int)(g.x * g.y * g.z);
+}
+
+// ---------------------------------------------------------------------------
+// y = A x with partial sums of x.w0 and x.y (Vmatvec + 2 Vxdot, D2)
+// ---------------------------------------------------------------------------
+template <class T, int ND>
+__global__ __launch_bounds__(256) void k_matvec_dots(
+    int nx, int ny, int nz, const T *__restrict__ ac,
+    const T *__restrict__ cc, const T *__restrict__ x,
+    const T *__restrict__ w0, T *__restrict__ y, double *__restrict__ part) {
+  __shared__ double red[4];
+  const size_t n = (size_t)nx * ny * nz, sy = nx, sz = (size_t)nx * ny;
+  const int i = 2 + blockIdx.x * BX + threadIdx.x;
+  const int j = 2 + blockIdx.y * BY + threadIdx.y;
+  const int k0 = 2 + blockIdx.z * KCH;
+  double a0 = 0.0, a1 = 0.0;
+  if (i <= nx - 1 && j <= ny - 1) {
+    for (int kk = 0; kk < KCH; kk++) {
+      const int k = k0 + kk;
+      if (k > nz - 1) break;
+      const size_t p = ix(nx, ny, i, j, k);
+      T v;
+      if (ND == 7) {
+        const T *oE = ac + n, *oN = ac + 2 * n, *uC = ac + 3 * n;
+        v = -oN[p] * x[p + sy] - oN[p - sy] * x[p - sy] - oE[p] * x[p + 1] -
+            oE[p - 1] * x[p - 1] - uC[p - sz] * x[p - sz] -
+            uC[p] * x[p + sz] + (ac[p] + cc[p]) * x[p];
+      } else {
+        double sO, sU, sD;
+        sum27<T, T>(ac, n, sy, sz, p, x, sO, sU, sD);
+        v = (T)((T)(-(T)sO + -(T)sU) + -(T)sD) + (ac[p] + cc[p]) * x[p];
+      }
+      if (y) y[p] = v;
+      if (w0) a0 += (double)x[p] * (double)w0[p];
+      a1 += (double)x[p] * (double)v;
+    }
+  }
+  if (part) {
+    double t0 = block_sum<256>(a0, red);
+    double t1 = block_sum<256>(a1, red);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+      size_t b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+      part[2 * b] = t0;
+      part[2 * b + 1] = t1;
+    }
+  }
+}
+
+template <class T>
+int matvec_dots(const Dim &d, int numdia, const T *ac, const T *cc,
+                const T *x, const T *w0, T *y, double *part, hipStream_t s) {
+  dim3 g(cdiv(d.nx - 2, BX), cdiv(d.ny - 2, BY), cdiv(d.nz - 2, KCH));
+  dim3 b(BX, BY);
+  if (numdia == 7)
+    k_matvec_dots<T, 7><<<g, b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, x, w0, y,
+                                        part);
+  else
+    k_matvec_dots<T, 27><<<g, b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, x, w0, y,
+                                         part);
+  check_launch("matvec_dots");
+  return (int)(g.x * g.y * g.z);
+}
+
+// ---------------------------------------------------------------------------
+// Grid transfer: Vrestrc2 (matvecd.cc:755-795), VinterpPMG2 (:878-960)
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(256) void k_restrict(int nxf, int nyf, int nxc,
+                                                  int nyc, int nzc,
+                                                  const T *__restrict__ xin,
+                                                  T *__restrict__ xout,
+                                                  const T *__restrict__ pc) {
+  const int ii = 2 + blockIdx.x * BX + threadIdx.x;
+  const int jj = 2 + blockIdx.y * BY + threadIdx.y;
+  const int kk = 2 + blockIdx.z;
+  if (ii > nxc - 1 || jj > nyc - 1) return;
+  const size_t nc = (size_t)nxc * nyc * nzc, pC = ix(nxc, nyc, ii, jj, kk);
+  const size_t sy = nxf, sz = (size_t)nxf * nyf;
+  const size_t f = ix(nxf, nyf, 2 * ii - 1, 2 * jj - 1, 2 * kk - 1);
+  // in-plane offsets in P plane order C,N,S,E,W,NE,NW,SE,SW
+  const long long off[9] = {0,
+                            (long long)sy,
+                            -(long long)sy,
+                            1,
+                            -1,
+                            1 + (long long)sy,
+                            -1 + (long long)sy,
+                            1 - (long long)sy,
+                            -1 - (long long)sy};
+  const long long zo[3] = {0, (long long)sz, -(long long)sz};
+  T t[3];
+#pragma unroll
+  for (int b = 0; b < 3; b++) {
+    T acc = pc[(size_t)(9 * b) * nc + pC] * xin[f + zo[b]];
+#pragma unroll
+    for (int q = 1; q < 9; q++)
+      acc += pc[(size_t)(9 * b + q) * nc + pC] * xin[f + zo[b] + off[q]];
+    t[b] = acc;
+  }
+  xout[pC] = t[0] + t[1] + t[2];
+}
+
+template <class T>
+void restrict_(const Dim &f, const Dim &c, const T *xin, T *xout, const T *pc,
+               hipStream_t s) {
+  if (c.nx < 3 || c.ny < 3 || c.nz < 3) return;
+  if (f.nx != 2 * c.nx - 1 || f.ny != 2 * c.ny - 1 || f.nz != 2 * c.nz - 1) {
+    fprintf(stderr, "pmg: restrict shape mismatch\n");
+    return;
+  }
+  dim3 g(cdiv(c.nx - 2, BX), cdiv(c.ny - 2, BY), c.nz - 2);
+  k_restrict<T><<<g, dim3(BX, BY), 0, s>>>(f.nx, f.ny, c.nx, c.ny, c.nz, xin,
+                                           xout, pc);
+  check_launch("restrict");
+}
+
+enum : int {
+  oPC, oPN, oPS, oPE, oPW, oPNE, oPNW, oPSE, oPSW,
+  uPC, uPN, uPS, uPE, uPW, uPNE, uPNW, uPSE, uPSW,
+  dPC, dPN, dPS, dPE, dPW, dPNE, dPNW, dPSE, dPSW
+};
+
+// One thread per coarse cell (ii, jj, kk) in 1..nc-1: the 8 fine points of
+// the cell, as VinterpPMG2.  ADD: xf += xdamp * value on fine interior
+// points (D3); otherwise xf = value (boundary zeroed separately, D8).
+template <class T, bool ADD>
+__global__ __launch_bounds__(256) void k_interp(int nxc, int nyc, int nzc,
+                                                int nxf, int nyf, int nzf,
+                                                const T *__restrict__ xc,
+                                                const T *__restrict__ pc,
+                                                T *__restrict__ xf,
+                                                const double *xdamp) {
+  const int ii = 1 + blockIdx.x * BX + threadIdx.x;
+  const int jj = 1 + blockIdx.y * BY + threadIdx.y;
+  const int kk = 1 + blockIdx.z;
+  if (ii > nxc - 1 || jj > nyc - 1) return;
+  const size_t nc = (size_t)nxc * nyc * nzc;
+  const size_t c0 = ix(nxc, nyc, ii, jj, kk);
+  const size_t cx = 1, cy = nxc, cz = (size_t)nxc * nyc;
+#define PW(pl, o) pc[(size_t)(pl)*nc + c0 + (o)]
+#define XC(o) xc[c0 + (o)]
+  const T x000 = XC(0), x100 = XC(cx), x010 = XC(cy), x110 = XC(cx + cy),
+          x001 = XC(cz), x101 = XC(cx + cz), x011 = XC(cy + cz),
+          x111 = XC(cx + cy + cz);
+  T v[8];
+  v[0] = x000;
+  v[1] = PW(oPE, 0) * x000 + PW(oPW, cx) * x100;
+  v[2] = PW(oPN, 0) * x000 + PW(oPS, cy) * x010;
+  v[4] = PW(uPC, 0) * x000 + PW(dPC, cz) * x001;
+  v[3] = PW(oPNE, 0) * x000 + PW(oPNW, cx) * x100 + PW(oPSE, cy) * x010 +
+         PW(oPSW, cx + cy) * x110;
+  v[5] = PW(uPE, 0) * x000 + PW(uPW, cx) * x100 + PW(dPE, cz) * x001 +
+         PW(dPW, cx + cz) * x101;
+  v[6] = PW(uPN, 0) * x000 + PW(uPS, cy) * x010 + PW(dPN, cz) * x001 +
+         PW(dPS, cy + cz) * x011;
+  v[7] = +PW(uPNE, 0) * x000 + PW(uPNW, cx) * x100 + PW(uPSE, cy) * x010 +
+         PW(uPSW, cx + cy) * x110 + PW(dPNE, cz) * x001 +
+         PW(dPNW, cx + cz) * x101 + PW(dPSE, cy + cz) * x011 +
+         PW(dPSW, cx + cy + cz) * x111;
+#undef PW
+#undef XC
+  const int i = 2 * ii - 1, j = 2 * jj - 1, k = 2 * kk - 1;
+  T damp = (T)0;
+  if (ADD) damp = (T)(*xdamp);
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const int fi = i + (q & 1), fj = j + ((q >> 1) & 1), fk = k + (q >> 2);
+    const size_t p = ix(nxf, nyf, fi, fj, fk);
+    if (ADD) {
+      if (fi >= 2 && fi <= nxf - 1 && fj >= 2 && fj <= nyf - 1 && fk >= 2 &&
+          fk <= nzf - 1)
+        xf[p] += damp * v[q];
+    } else {
+      xf[p] = v[q];
+    }
+  }
+}
+
+template <class T>
+void interp(const Dim &c, const Dim &f, const T *xc, const T *pc, T *xf,
+            const double *xdamp, hipStream_t s) {
+  if (f.nx != 2 * c.nx - 1 || f.ny != 2 * c.ny - 1 || f.nz != 2 * c.nz - 1) {
+    fprintf(stderr, "pmg: interp shape mismatch\n");
+    return;
+  }
+  dim3 g(cdiv(c.nx - 1, BX), cdiv(c.ny - 1, BY), c.nz - 1);
+  if (xdamp) {
+    k_interp<T, true><<<g, dim3(BX, BY), 0, s>>>(c.nx, c.ny, c.nz, f.nx, f.ny,
+                                                 f.nz, xc, pc, xf, xdamp);
+  } else {
+    k_interp<T, false><<<g, dim3(BX, BY), 0, s>>>(c.nx, c.ny, c.nz, f.nx,
+                                                  f.ny, f.nz, xc, pc, xf,
+                                                  nullptr);
+    if constexpr (sizeof(T) == 8) fbound00(f, (double *)xf, s);
+  }
+  check_launch("interp");
+}
+
+// ---------------------------------------------------------------------------
+// Reductions
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_reduce(const double *part, int nblk,
+                                                 int stride, int mode,
+                                                 double *out) {
+  __shared__ double red[16];
+  double a = 0.0, b = 0.0;
+  for (int t = threadIdx.x; t < nblk; t += 1024) {
+    a += part[(size_t)t * stride];
+    if (mode) b += part[(size_t)t * stride + 1];
+  }
+  // block_sum indexes by threadIdx.x + threadIdx.y*blockDim.x
+  double sa = block_sum<1024>(a, red);
+  double sb = mode ? block_sum<1024>(b, red) : 0.0;
+  if (threadIdx.x == 0) {
+    if (mode == 1) {
+      out[0] = sb != 0.0 ? sa / sb : 0.0;  // zero correction: xdamp = 0
+    } else {
+      out[0] = sa;
+      if (mode == 2) out[1] = sb;
+    }
+  }
+}
+
+void reduce(const double *part, int nblk, int stride, int mode, double *out,
+            hipStream_t s) {
+  k_reduce<<<1, 1024, 0, s>>>(part, nblk, stride, mode, out);
+  check_launch("reduce");
+}
+
+template <class T, int MODE>
+__global__ __launch_bounds__(256) void k_norm_part(int nx, int ny, int nz,
+                                                   const T *__restrict__ x,
+                                                   const T *__restrict__ y,
+                                                   double *part) {
+  __shared__ double red[4];
+  const int i = 2 + blockIdx.x * BX + threadIdx.x;
+  const int j = 2 + blockIdx.y * BY + threadIdx.y;
+  const int k0 = 2 + blockIdx.z * KCH;
+  double acc = 0.0;
+  if (i <= nx - 1 && j <= ny - 1)
+    for (int kk = 0; kk < KCH; kk++) {
+      const int k = k0 + kk;
+      if (k > nz - 1) break;
+      const size_t p = ix(nx, ny, i, j, k);
+      if (MODE == 0)
+        acc += fabs((double)x[p]);
+      else
+        acc += (double)x[p] * (double)y[p];
+    }
+  double t = block_sum<256>(acc, red);
+  if (threadIdx.x == 0 && threadIdx.y == 0)
+    part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = t;
+}
+
+template <class T>
+int nrm1_part(const Dim &d, const T *x, double *part, hipStream_t s) {
+  dim3 g(cdiv(d.nx - 2, BX), cdiv(d.ny - 2, BY), cdiv(d.nz - 2, KCH));
+  k_norm_part<T, 0><<<g, dim3(BX, BY), 0, s>>>(d.nx, d.ny, d.nz, x, x, part);
+  check_launch("nrm1");
+  return (int)(g.x * g.y * g.z);
+}
+
+template <class T>
+int dot_part(const Dim &d, const T *x, const T *y, double *part,
+             hipStream_t s) {
+  dim3 g(cdiv(d.nx - 2, BX), cdiv(d.ny - 2, BY), cdiv(d.nz - 2, KCH));
+  k_norm_part<T, 1><<<g, dim3(BX, BY), 0, s>>>(d.nx, d.ny, d.nz, x, y, part);
+  check_launch("dot");
+  return (int)(g.x * g.y * g.z);
+}
+
+// ---------------------------------------------------------------------------
+// Coarsest level: Vcghs (cgd.cc:60-124) in one 1024-thread workgroup.
+// The search direction p lives in LDS over the whole level grid; x, r and
+// A p live in registers (NPT interior points per thread).  FP64 throughout.
+// ---------------------------------------------------------------------------
+constexpr int CG_NT = 1024;
+constexpr size_t CG_LDS_MAX = 60 * 1024;
+
+bool cg_fits(const Dim &d) {
+  return d.n() * sizeof(double) <= CG_LDS_MAX &&
+         d.interior() <= (size_t)CG_NT * 8;
+}
+
+template <class T, int NPT>
+__global__ __launch_bounds__(CG_NT) void k_cg_single(
+    int nx, int ny, int nz, int numdia, const double *__restrict__ ac,
+    const double *__restrict__ cc, const T *__restrict__ f,
+    T *__restrict__ xout, int itmax, double errtol) {
+  extern __shared__ double P[];
+  __shared__ double red[16];
+  const int tid = threadIdx.x;
+  const size_t n = (size_t)nx * ny * nz, sy = nx, sz = (size_t)nx * ny;
+  const int ni = nx - 2, nj = ny - 2;
+  const int nint = ni * nj * (nz - 2);
+  for (size_t q = tid; q < n; q += CG_NT) P[q] = 0.0;
+  size_t pidx[NPT];
+  bool val[NPT];
+  double xr[NPT], rr[NPT], apv[NPT];
+#pragma unroll
+  for (int m = 0; m < NPT; m++) {
+    const int t = tid + m * CG_NT;
+    val[m] = t < nint;
+    const int tt = val[m] ? t : 0;
+    const int i = 2 + tt % ni, j = 2 + (tt / ni) % nj, k = 2 + tt / (ni * nj);
+    pidx[m] = ix(nx, ny, i, j, k);
+    xr[m] = 0.0;
+    rr[m] = val[m] ? (double)f[pidx[m]] : 0.0;  // r = f - A*0
+    apv[m] = 0.0;
+  }
+  __syncthreads();
+  double s = 0.0;
+#pragma unroll
+  for (int m = 0; m < NPT; m++) s += rr[m] * rr[m];
+  double t0 = block_sum<CG_NT>(s, red);
+  if (tid == 0) red[8] = t0;
+  __syncthreads();
+  const double denom = sqrt(red[8]);
+  __syncthreads();
+  int iters = 0;
+  double rhok1 = 0.0;
+  if (denom != 0.0) {
+    for (;;) {
+      s = 0.0;
+#pragma unroll
+      for (int m = 0; m < NPT; m++) s += rr[m] * rr[m];
+      double t1 = block_sum<CG_NT>(s, red);
+      if (tid == 0) red[8] = t1;
+      __syncthreads();
+      const double rhok2 = red[8];
+      __syncthreads();
+      if (sqrt(rhok2) / denom <= errtol) break;
+      if (iters >= itmax) break;
+      if (iters == 0) {
+#pragma unroll
+        for (int m = 0; m < NPT; m++)
+          if (val[m]) P[pidx[m]] = rr[m];
+      } else {
+        const double beta = rhok2 / rhok1;
+        const double ib = 1.0 / beta;
+#pragma unroll
+        for (int m = 0; m < NPT; m++)
+          if (val[m]) P[pidx[m]] = (P[pidx[m]] + ib * rr[m]) * beta;
+      }
+      __syncthreads();
+      s = 0.0;
+#pragma unroll
+      for (int m = 0; m < NPT; m++) {
+        if (!val[m]) continue;
+        const size_t p = pidx[m];
+        double v;
+        if (numdia == 7) {
+          const double *oE = ac + n, *oN = ac + 2 * n, *uC = ac + 3 * n;
+          v = -oN[p] * P[p + sy] - oN[p - sy] * P[p - sy] - oE[p] * P[p + 1] -
+              oE[p - 1] * P[p - 1] - uC[p - sz] * P[p - sz] -
+              uC[p] * P[p + sz] + (ac[p] + cc[p]) * P[p];
+        } else {
+          double sO, sU, sD;
+          sum27<double, double>(ac, n, sy, sz, p, P, sO, sU, sD);
+          v = -sO + -sU + -sD + (ac[p] + cc[p]) * P[p];
+        }
+        apv[m] = v;
+        s += P[p] * v;
+      }
+      double t2 = block_sum<CG_NT>(s, red);
+      if (tid == 0) red[8] = t2;
+      __syncthreads();
+      const double pAp = red[8];
+      __syncthreads();
+      const double alpha = rhok2 / pAp;
+      rhok1 = rhok2;
+#pragma unroll
+      for (int m = 0; m < NPT; m++) {
+        if (!val[m]) continue;
+        xr[m] += alpha * P[pidx[m]];
+        rr[m] += -alpha * apv[m];
+      }
+      iters++;
+    }
+  }
+  // x = 0 on the boundary (Vazeros before the solve), solution inside
+  for (size_t q = tid; q < n; q += CG_NT) {
+    const int i = 1 + (int)(q % nx), j = 1 + (int)((q / nx) % ny),
+              k = 1 + (int)(q / ((size_t)nx * ny));
+    if (i == 1 || i == nx || j == 1 || j == ny || k == 1 || k == nz)
+      xout[q] = (T)0;
+  }
+#pragma unroll
+  for (int m = 0; m < NPT; m++)
+    if (val[m]) xout[pidx[m]] = (T)xr[m];
+}
+
+template <class T>
+void cg_single(const Dim &d, int numdia, const double *ac, const double *cc,
+               const T *f, T *x, int itmax, double errtol, hipStream_t s) {
+  if (!cg_fits(d)) {
+    fprintf(stderr, "pmg: coarsest level too large for cg_single\n");
+    return;
+  }
+  const size_t lds = d.n() * sizeof(double);
+  const size_t ni = d.interior();
+  if (ni <= CG_NT)
+    k_cg_single<T, 1><<<1, CG_NT, lds, s>>>(d.nx, d.ny, d.nz, numdia, ac, cc,
+                                            f, x, itmax, errtol);
+  else if (ni <= 2 * CG_NT)
+    k_cg_single<T, 2><<<1, CG_NT, lds, s>>>(d.nx, d.ny, d.nz, numdia, ac, cc,
+                                            f, x, itmax, errtol);
+  else if (ni <= 4 * CG_NT)
+    k_cg_single<T, 4><<<1, CG_NT, lds, s>>>(d.nx, d.ny, d.nz, numdia, ac, cc,
+                                            f, x, itmax, errtol);
+  else
+    k_cg_single<T, 8><<<1, CG_NT, lds, s>>>(d.nx, d.ny, d.nz, numdia, ac, cc,
+                                            f, x, itmax, errtol);
+  check_launch("cg_single");
+}
+
+// ---------------------------------------------------------------------------
+// BLAS-1 helpers
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void k_fill(T *x, size_t n, T v) {
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n;
+       t += (size_t)gridDim.x * blockDim.x)
+    x[t] = v;
+}
+template <class T>
+void fill(T *x, size_t n, T v, hipStream_t s) {
+  if (!n) return;
+  int nb = (int)((n + 255) / 256);
+  if (nb > 16384) nb = 16384;
+  k_fill<T><<<nb, 256, 0, s>>>(x, n, v);
+  check_launch("fill");
+}
+
+template <class TA, class TB>
+__global__ void k_convert(const TA *a, TB *b, size_t n) {
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n;
+       t += (size_t)gridDim.x * blockDim.x)
+    b[t] = (TB)a[t];
+}
+template <class TA, class TB>
+void convert(const TA *a, TB *b, size_t n, hipStream_t s) {
+  if (!n) return;
+  int nb = (int)((n + 255) / 256);
+  if (nb > 16384) nb = 16384;
+  k_convert<TA, TB><<<nb, 256, 0, s>>>(a, b, n);
+  check_launch("convert");
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_axpy(int nx, int ny, int nz,
+                                              double alpha,
+                                              const T *__restrict__ x,
+                                              T *__restrict__ y) {
+  const int i = 2 + blockIdx.x * BX + threadIdx.x;
+  const int j = 2 + blockIdx.y * BY + threadIdx.y;
+  const int k0 = 2 + blockIdx.z * KCH;
+  if (i > nx - 1 || j > ny - 1) return;
+  for (int kk = 0; kk < KCH; kk++) {
+    const int k = k0 + kk;
+    if (k > nz - 1) break;
+    const size_t p = ix(nx, ny, i, j, k);
+    y[p] += (T)alpha * x[p];
+  }
+}
+template <class T>
+void axpy_interior(const Dim &d, double alpha, const T *x, T *y,
+                   hipStream_t s) {
+  dim3 g(cdiv(d.nx - 2, BX), cdiv(d.ny - 2, BY), cdiv(d.nz - 2, KCH));
+  k_axpy<T><<<g, dim3(BX, BY), 0, s>>>(d.nx, d.ny, d.nz, alpha, x, y);
+  check_launch("axpy");
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_addcorr(int nx, int ny, int nz,
+                                                 const T *__restrict__ e,
+                                                 double *__restrict__ u) {
+  const int i = 2 + blockIdx.x * BX + threadIdx.x;
+  const int j = 2 + blockIdx.y * BY + threadIdx.y;
+  const int k0 = 2 + blockIdx.z * KCH;
+  if (i > nx - 1 || j > ny - 1) return;
+  for (int kk = 0; kk < KCH; kk++) {
+    const int k = k0 + kk;
+    if (k > nz - 1) break;
+    const size_t p = ix(nx, ny, i, j, k);
+    u[p] += (double)e[p];
+  }
+}
+template <class T>
+void add_correction(const Dim &d, const T *e, double *u, hipStream_t s) {
+  dim3 g(cdiv(d.nx - 2, BX), cdiv(d.ny - 2, BY), cdiv(d.nz - 2, KCH));
+  k_addcorr<T><<<g, dim3(BX, BY), 0, s>>>(d.nx, d.ny, d.nz, e, u);
+  check_launch("add_correction");
+}
+
+__global__ __launch_bounds__(256) void k_xpay(int nx, int ny, int nz,
+                                              const double *__restrict__ x,
+                                              double damp,
+                                              const double *__restrict__ dx,
+                                              double *__restrict__ out) {
+  const int i = 2 + blockIdx.x * BX + threadIdx.x;
+  const int j = 2 + blockIdx.y * BY + threadIdx.y;
+  const int k0 = 2 + blockIdx.z * KCH;
+  if (i > nx - 1 || j > ny - 1) return;
+  for (int kk = 0; kk < KCH; kk++) {
+    const int k = k0 + kk;
+    if (k > nz - 1) break;
+    const size_t p = ix(nx, ny, i, j, k);
+    double v = x[p];
+    v += damp * dx[p];
+    out[p] = v;
+  }
+}
+void xpay_interior(const Dim &d, const double *x, double damp,
+                   const double *dx, double *out, hipStream_t s) {
+  dim3 g(cdiv(d.nx - 2, BX), cdiv(d.ny - 2, BY), cdiv(d.nz - 2, KCH));
+  k_xpay<<<g, dim3(BX, BY), 0, s>>>(d.nx, d.ny, d.nz, x, damp, dx, out);
+  check_launch("xpay");
+}
+
+__global__ __launch_bounds__(256) void k_copyint(int nx, int ny, int nz,
+                                                 const double *__restrict__ x,
+                                                 double *__restrict__ y) {
+  const int i = 2 + blockIdx.x * BX + threadIdx.x;
+  const int j = 2 + blockIdx.y * BY + threadIdx.y;
+  const int k0 = 2 + blockIdx.z * KCH;
+  if (i > nx - 1 || j > ny - 1) return;
+  for (int kk = 0; kk < KCH; kk++) {
+    const int k = k0 + kk;
+    if (k > nz - 1) break;
+    const size_t p = ix(nx, ny, i, j, k);
+    y[p] = x[p];
+  }
+}
+void copy_interior(const Dim &d, const double *x, double *y, hipStream_t s) {
+  dim3 g(cdiv(d.nx - 2, BX), cdiv(d.ny - 2, BY), cdiv(d.nz - 2, KCH));
+  k_copyint<<<g, dim3(BX, BY), 0, s>>>(d.nx, d.ny, d.nz, x, y);
+  check_launch("copy_interior");
+}
+
+// Faces written i, then j, then k (VfboundPMG, mikpckd.cc:266-336, D4);
+// three launches keep the edge precedence of the reference.
+__global__ void k_face(int nx, int ny, int nz, int axis, double *x,
+                       const double *g) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  int na = axis == 0 ? ny : nx;
+  if (a >= na) return;
+  for (int side = 0; side < 2; side++) {
+    int i, j, k;
+    if (axis == 0) {
+      i = side ? nx : 1, j = a + 1, k = b + 1;
+    } else if (axis == 1) {
+      i = a + 1, j = side ? ny : 1, k = b + 1;
+    } else {
+      i = a + 1, j = b + 1, k = side ? nz : 1;
+    }
+    size_t plane = axis == 0 ? (size_t)ny * nz
+                             : (axis == 1 ? (size_t)nx * nz : (size_t)nx * ny);
+    size_t gi = (size_t)a + (size_t)b * na + (size_t)side * plane;
+    x[ix(nx, ny, i, j, k)] = g ? g[gi] : 0.0;
+  }
+}
+void fbound_dirichlet(const Dim &d, double *x, const double *gx,
+                      const double *gy, const double *gz, hipStream_t s) {
+  k_face<<<dim3(cdiv(d.ny, 64), d.nz), 64, 0, s>>>(d.nx, d.ny, d.nz, 0, x, gx);
+  k_face<<<dim3(cdiv(d.nx, 64), d.nz), 64, 0, s>>>(d.nx, d.ny, d.nz, 1, x, gy);
+  k_face<<<dim3(cdiv(d.nx, 64), d.ny), 64, 0, s>>>(d.nx, d.ny, d.nz, 2, x, gz);
+  check_launch("fbound");
+}
+void fbound00(const Dim &d, double *x, hipStream_t s) {
+  fbound_dirichlet(d, x, nullptr, nullptr, nullptr, s);
+}
+
+// ---------------------------------------------------------------------------
+// Setup: VbuildA_fv (buildAd.cc:62-212)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_buildA(
+    int nx, int ny, int nz, double *__restrict__ ac, double *__restrict__ cc,
+    double *__restrict__ fc, const double *__restrict__ xf,
+    const double *__restrict__ yf, const double *__restrict__ zf,
+    const double *__restrict__ gx, const double *__restrict__ gy,
+    const double *__restrict__ gz, const double *__restrict__ a1,
+    const double *__restrict__ a2, const double *__restrict__ a3,
+    const double *__restrict__ ccf, const double *__restrict__ fcf) {
+  const int i = 2 + blockIdx.x * BX + threadIdx.x;
+  const int j = 2 + blockIdx.y * BY + threadIdx.y;
+  const int k = 2 + blockIdx.z;
+  if (i > nx - 1 || j > ny - 1) return;
+  const size_t n = (size_t)nx * ny * nz;
+  double *oC = ac, *oE = ac + n, *oN = ac + 2 * n, *uC = ac + 3 * n;
+  const size_t p = ix(nx, ny, i, j, k);
+  const double hzm1 = zf[k - 1] - zf[k - 2], hz = zf[k] - zf[k - 1];
+  const double hym1 = yf[j - 1] - yf[j - 2], hy = yf[j] - yf[j - 1];
+  const double hxm1 = xf[i - 1] - xf[i - 2], hx = xf[i] - xf[i - 1];
+  const double cE = (hym1 + hy) * (hzm1 + hz) / (4.0 * hx);
+  const double cEm = (hym1 + hy) * (hzm1 + hz) / (4.0 * hxm1);
+  const double cN = (hxm1 + hx) * (hzm1 + hz) / (4.0 * hy);
+  const double cNm = (hxm1 + hx) * (hzm1 + hz) / (4.0 * hym1);
+  const double cU = (hxm1 + hx) * (hym1 + hy) / (4.0 * hz);
+  const double cUm = (hxm1 + hx) * (hym1 + hy) / (4.0 * hzm1);
+  const double cF = (hxm1 + hx) * (hym1 + hy) * (hzm1 + hz) / 8.0;
+  const size_t pxm = p - 1, pym = p - nx, pzm = p - (size_t)nx * ny;
+  double f = cF * fcf[p];
+  cc[p] = cF * ccf[p];
+  oC[p] = cE * a1[p] + cEm * a1[pxm] + cN * a2[p] + cNm * a2[pym] +
+          cU * a3[p] + cUm * a3[pzm];
+  const int ike = (i == nx - 1) ? 0 : 1;
+  oE[p] = ike * cE * a1[p];
+  f += (1 - ike) * cE * a1[p] * gx[(j - 1) + (size_t)(k - 1) * ny +
+                                   (size_t)ny * nz];
+  const int jke = (j == ny - 1) ? 0 : 1;
+  oN[p] = jke * cN * a2[p];
+  f += (1 - jke) * cN * a2[p] * gy[(i - 1) + (size_t)(k - 1) * nx +
+                                   (size_t)nx * nz];
+  const int kke = (k == nz - 1) ? 0 : 1;
+  uC[p] = kke * cU * a3[p];
+  f += (1 - kke) * cU * a3[p] * gz[(i - 1) + (size_t)(j - 1) * nx +
+                                   (size_t)nx * ny];
+  const int ikw = (i == 2) ? 0 : 1;
+  f += (1 - ikw) * cEm * a1[pxm] * gx[(j - 1) + (size_t)(k - 1) * ny];
+  const int jks = (j == 2) ? 0 : 1;
+  f += (1 - jks) * cNm * a2[pym] * gy[(i - 1) + (size_t)(k - 1) * nx];
+  const int kkd = (k == 2) ? 0 : 1;
+  f += (1 - kkd) * cUm * a3[pzm] * gz[(i - 1) + (size_t)(j - 1) * nx];
+  fc[p] = f;
+}
+
+void buildA(const Dim &d, double *ac, double *cc, double *fc,
+            const double *xf, const double *yf, const double *zf,
+            const double *gx, const double *gy, const double *gz,
+            const double *a1, const double *a2, const double *a3,
+            const double *ccf, const double *fcf, hipStream_t s) {
+  dim3 g(cdiv(d.nx - 2, BX), cdiv(d.ny - 2, BY), d.nz - 2);
+  k_buildA<<<g, dim3(BX, BY), 0, s>>>(d.nx, d.ny, d.nz, ac, cc, fc, xf, yf,
+                                      zf, gx, gy, gz, a1, a2, a3, ccf, fcf);
+  check_launch("buildA");
+}
+
+// ---------------------------------------------------------------------------
+// Setup: operator-based prolongation (VbuildPb_op7 / op27 / trilin)
+// ---------------------------------------------------------------------------
+// Symmetric 27-point storage: plane holding the coupling between p and p+e
+// for an upper direction e; -1 otherwise (plane order matvecd.cc:185-189).
+__host__ __device__ constexpr int upper_plane(int ex, int ey, int ez) {
+  return ez == 1
+             ? (ey == -1 ? (ex == -1 ? 13 : (ex == 0 ? 9 : 12))
+                         : (ey == 0 ? (ex == -1 ? 7 : (ex == 0 ? 3 : 6))
+                                    : (ex == -1 ? 11 : (ex == 0 ? 8 : 10))))
+         : (ez == 0 && ey == 1) ? (ex == 1 ? 4 : (ex == 0 ? 2 : 5))
+         : (ez == 0 && ey == 0 && ex == 1) ? 1
+                                           : -1;
+}
+
+template <int ND>
+__device__ inline double coupling(const double *__restrict__ ac, size_t n,
+                                  size_t p, long long off, int ex, int ey,
+                                  int ez) {
+  int pl = upper_plane(ex, ey, ez);
+  size_t at = p;
+  if (pl < 0) {
+    pl = upper_plane(-ex, -ey, -ez);
+    at = (size_t)((long long)p + off);
+  }
+  if (ND == 7 && pl > 3) return 0.0;
+  return ac[(size_t)pl * n + at];
+}
+
+__host__ __device__ constexpr int pplane(int dx, int dy, int dz) {
+  return 9 * (dz == 0 ? 0 : (dz == 1 ? 1 : 2)) +
+         (dy == -1 ? (dx == -1 ? 8 : (dx == 0 ? 2 : 7))
+                   : (dy == 0 ? (dx == -1 ? 4 : (dx == 0 ? 0 : 3))
+                              : (dx == -1 ? 6 : (dx == 0 ? 1 : 5))));
+}
+
+// Weights of the collapsed-operator rule: for F = C_f + d, axes with d_a = 0
+// are collapsed into the diagonal; neighbours towards C carry the known
+// weight of their own offset, neighbours away from C weight 0.
+template <int ND>
+__global__ __launch_bounds__(128) void k_buildP(int nxf, int nyf, int nzf,
+                                                int nxc, int nyc, int nzc,
+                                                const double *__restrict__ ac,
+                                                double *__restrict__ pc,
+                                                int mgprol) {
+  const int ii = 2 + blockIdx.x * 64 + threadIdx.x;
+  const int jj = 2 + blockIdx.y * 2 + threadIdx.y;
+  const int kk = 2 + blockIdx.z;
+  if (ii > nxc - 1 || jj > nyc - 1) return;
+  const size_t nf = (size_t)nxf * nyf * nzf, nc = (size_t)nxc * nyc * nzc;
+  const size_t pC = ix(nxc, nyc, ii, jj, kk);
+  const size_t pCf = ix(nxf, nyf, 2 * ii - 1, 2 * jj - 1, 2 * kk - 1);
+  const long long sx = 1, sy = nxf, sz = (long long)nxf * nyf;
+  double w[27];
+  w[13] = 1.0;  // index (dz+1)*9 + (dy+1)*3 + (dx+1)
+  if (mgprol == 0) {
+#pragma unroll
+    for (int a = 0; a < 27; a++) {
+      const int dx = a % 3 - 1, dy = (a / 3) % 3 - 1, dz = a / 9 - 1;
+      const int m = (dx != 0) + (dy != 0) + (dz != 0);
+      w[a] = 1.0 / (double)(1 << m);
+    }
+  } else {
+#pragma unroll
+    for (int nnz = 1; nnz <= 3; nnz++) {
+#pragma unroll
+      for (int a = 0; a < 27; a++) {
+        const int dx = a % 3 - 1, dy = (a / 3) % 3 - 1, dz = a / 9 - 1;
+        if ((dx != 0) + (dy != 0) + (dz != 0) != nnz) continue;
+        const size_t pF = (size_t)((long long)pCf + dx * sx + dy * sy + dz * sz);
+        double den = ac[pF], num = 0.0;
+#pragma unroll
+        for (int b = 0; b < 27; b++) {
+          const int ex = b % 3 - 1, ey = (b / 3) % 3 - 1, ez = b / 9 - 1;
+          if (b == 13) continue;
+          if (ND == 7 && (ex != 0) + (ey != 0) + (ez != 0) != 1) continue;
+          const int d3[3] = {dx, dy, dz}, e3[3] = {ex, ey, ez};
+          bool ok = true, toward = false;
+          int t3[3];
+#pragma unroll
+          for (int q = 0; q < 3; q++) {
+            if (d3[q] == 0) {
+              t3[q] = 0;
+            } else if (e3[q] == 0) {
+              t3[q] = d3[q];
+            } else if (e3[q] == -d3[q]) {
+              t3[q] = 0;
+              toward = true;
+            } else {
+              ok = false;
+            }
+          }
+          if (!ok) continue;
+          const double cpl = coupling<ND>(ac, nf, pF, ex * sx + ey * sy + ez * sz,
+                                          ex, ey, ez);
+          if (!toward)
+            den -= cpl;
+          else
+            num += cpl * w[(t3[2] + 1) * 9 + (t3[1] + 1) * 3 + (t3[0] + 1)];
+        }
+        w[a] = num / den;
+      }
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 27; a++) {
+    const int dx = a % 3 - 1, dy = (a / 3) % 3 - 1, dz = a / 9 - 1;
+    pc[(size_t)pplane(dx, dy, dz) * nc + pC] = w[a];
+  }
+}
+
+void buildP(const Dim &f, const Dim &c, int mgprol, int numdia,
+            const double *acf, double *pc, hipStream_t s) {
+  if (c.nx < 3 || c.ny < 3 || c.nz < 3) return;
+  dim3 g(cdiv(c.nx - 2, 64), cdiv(c.ny - 2, 2), c.nz - 2);
+  if (numdia == 7)
+    k_buildP<7><<<g, dim3(64, 2), 0, s>>>(f.nx, f.ny, f.nz, c.nx, c.ny, c.nz,
+                                          acf, pc, mgprol);
+  else
+    k_buildP<27><<<g, dim3(64, 2), 0, s>>>(f.nx, f.ny, f.nz, c.nx, c.ny, c.nz,
+                                           acf, pc, mgprol);
+  check_launch("buildP");
+}
+
+// ---------------------------------------------------------------------------
+// Setup: Galerkin coarse operator XA = P^T A P (VbuildG_7 / VbuildG_27)
+// ---------------------------------------------------------------------------
+// Stored direction delta of each of the 14 output planes.
+__constant__ signed char c_delta[14][3] = {
+    {0, 0, 0},  {1, 0, 0},  {0, 1, 0},  {0, 0, 1},   {1, 1, 0},
+    {-1, 1, 0}, {1, 0, 1},  {-1, 0, 1}, {0, 1, 1},   {0, -1, 1},
+    {1, 1, 1},  {-1, 1, 1}, {1, -1, 1}, {-1, -1, 1}};
+
+// For each coarse point C the product A P_C is formed on the 5x5x5 fine
+// box around C (in LDS), then each stored entry is the dot product of that
+// box with the weights of the neighbour C + delta:
+//   XA(C, C+delta) = sum_q P_{C+delta}(q) (A P_C)(q)   (A symmetric).
+template <int ND>
+__global__ __launch_bounds__(64) void k_buildG(int nxf, int nyf, int nzf,
+                                               int nxc, int nyc, int nzc,
+                                               const double *__restrict__ pc,
+                                               const double *__restrict__ ac,
+                                               double *__restrict__ acc) {
+  __shared__ double APs[125][64];
+  const int lane = threadIdx.x;
+  const int ii = 2 + blockIdx.x * 64 + lane;
+  const int jj = 2 + blockIdx.y;
+  const int kk = 2 + blockIdx.z;
+  if (ii > nxc - 1) return;
+  const size_t nf = (size_t)nxf * nyf * nzf, nc = (size_t)nxc * nyc * nzc;
+  const size_t pC = ix(nxc, nyc, ii, jj, kk);
+  const size_t pCf = ix(nxf, nyf, 2 * ii - 1, 2 * jj - 1, 2 * kk - 1);
+  const long long sx = 1, sy = nxf, sz = (long long)nxf * nyf;
+  const long long cx = 1, cy = nxc, cz = (long long)nxc * nyc;
+  double wC[27];
+#pragma unroll
+  for (int a = 0; a < 27; a++) {
+    const int dx = a % 3 - 1, dy = (a / 3) % 3 - 1, dz = a / 9 - 1;
+    wC[a] = pc[(size_t)pplane(dx, dy, dz) * nc + pC];
+  }
+  // (A P_C)(q) for q = C_f + (qx,qy,qz), |q| <= 2 per axis
+  for (int qa = 0; qa < 125; qa++) {
+    const int qx = qa % 5 - 2, qy = (qa / 5) % 5 - 2, qz = qa / 25 - 2;
+    const size_t pq = (size_t)((long long)pCf + qx * sx + qy * sy + qz * sz);
+    double v = 0.0;
+    // diagonal term (q inside the support of P_C)
+    if (qx >= -1 && qx <= 1 && qy >= -1 && qy <= 1 && qz >= -1 && qz <= 1)
+      v = ac[pq] * wC[(qz + 1) * 9 + (qy + 1) * 3 + (qx + 1)];
+#pragma unroll
+    for (int b = 0; b < 27; b++) {
+      if (b == 13) continue;
+      const int ex = b % 3 - 1, ey = (b / 3) % 3 - 1, ez = b / 9 - 1;
+      if (ND == 7 && (ex != 0) + (ey != 0) + (ez != 0) != 1) continue;
+      const int tx = qx + ex, ty = qy + ey, tz = qz + ez;
+      if (tx < -1 || tx > 1 || ty < -1 || ty > 1 || tz < -1 || tz > 1)
+        continue;
+      const double cpl =
+          coupling<ND>(ac, nf, pq, ex * sx + ey * sy + ez * sz, ex, ey, ez);
+      v -= cpl * wC[(tz + 1) * 9 + (ty + 1) * 3 + (tx + 1)];
+    }
+    APs[qa][lane] = v;
+  }
+  for (int s = 0; s < 14; s++) {
+    const int gx = c_delta[s][0], gy = c_delta[s][1], gz = c_delta[s][2];
+    const size_t pC2 = (size_t)((long long)pC + gx * cx + gy * cy + gz * cz);
+    double acc_v = 0.0;
+#pragma unroll
+    for (int a = 0; a < 27; a++) {
+      const int dx = a % 3 - 1, dy = (a / 3) % 3 - 1, dz = a / 9 - 1;
+      const int qx = 2 * gx + dx, qy = 2 * gy + dy, qz = 2 * gz + dz;
+      if (qx < -2 || qx > 2 || qy < -2 || qy > 2 || qz < -2 || qz > 2)
+        continue;
+      const double w2 = pc[(size_t)pplane(dx, dy, dz) * nc + pC2];
+      acc_v += w2 * APs[(qz + 2) * 25 + (qy + 2) * 5 + (qx + 2)][lane];
+    }
+    acc[(size_t)s * nc + pC] = s == 0 ? acc_v : -acc_v;
+  }
+}
+
+void buildG(const Dim &f, const Dim &c, int numdia, const double *pc,
+            const double *acf, double *acc, hipStream_t s) {
+  if (c.nx < 3 || c.ny < 3 || c.nz < 3) return;
+  dim3 g(cdiv(c.nx - 2, 64), c.ny - 2, c.nz - 2);
+  if (numdia == 7)
+    k_buildG<7><<<g, 64, 0, s>>>(f.nx, f.ny, f.nz, c.nx, c.ny, c.nz, pc, acf,
+                                 acc);
+  else
+    k_buildG<27><<<g, 64, 0, s>>>(f.nx, f.ny, f.nz, c.nx, c.ny, c.nz, pc, acf,
+                                  acc);
+  check_launch("buildG");
+}
+
+// ---------------------------------------------------------------------------
+// explicit instantiations
+// ---------------------------------------------------------------------------
+#define INST(T)                                                               \
+  template void gs7<T>(const Dim &, const T *, const T *, const T *, T *,     \
+                       int, int, hipStream_t);                                \
+  template void gs27<T>(const Dim &, const T *, const T *, const T *, T *,    \
+                        int, int, hipStream_t);                               \
+  template int matvec_dots<T>(const Dim &, int, const T *, const T *,         \
+                              const T *, const T *, T *, double *,            \
+                              hipStream_t);                                   \
+  template void restrict_<T>(const Dim &, const Dim &, const T *, T *,        \
+                             const T *, hipStream_t);                         \
+  template void interp<T>(const Dim &, const Dim &, const T *, const T *,     \
+                          T *, const double *, hipStream_t);                  \
+  template int nrm1_part<T>(const Dim &, const T *, double *, hipStream_t);   \
+  template int dot_part<T>(const Dim &, const T *, const T *, double *,       \
+                           hipStream_t);                                      \
+  template void cg_single<T>(const Dim &, int, const double *,                \
+                             const double *, const T *, T *, int, double,     \
+                             hipStream_t);                                    \
+  template void fill<T>(T *, size_t, T, hipStream_t);                         \
+  template void axpy_interior<T>(const Dim &, double, const T *, T *,         \
+                                 hipStream_t);                                \
+  template void add_correction<T>(const Dim &, const T *, double *,           \
+                                  hipStream_t);
+INST(double)
+INST(float)
+#undef INST
+template int resid<double, double>(const Dim &, int, const double *,
+                                   const double *, const double *,
+                                   const double *, double *, double *,
+                                   hipStream_t);
+template int resid<double, float>(const Dim &, int, const double *,
+                                  const double *, const double *,
+                                  const double *, float *, double *,
+                                  hipStream_t);
+template int resid<float, float>(const Dim &, int, const float *,
+                                 const float *, const float *, const float *,
+                                 float *, double *, hipStream_t);
+template void convert<double, float>(const double *, float *, size_t,
+                                     hipStream_t);
+template void convert<float, double>(const float *, double *, size_t,
+                                     hipStream_t);
+template void fill<int>(int *, size_t, int, hipStream_t);
+
+}  // namespace pmg

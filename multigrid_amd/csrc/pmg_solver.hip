@@ -1,0 +1,1103 @@
+// pmg_solver.hip -- PMG hierarchy, V-cycle driver and the C ABI.
+//
+// Host-side restatement of the reference's drivers with device-resident
+// state: Vmgdriv/Vmgdriv2 (pmgc/mgdrvd.cc:60-503), Vbuildstr/Vbuildops
+// (pmgc/mgsubd.cc:63-347), Vmvcs (pmgc/mgcsd.cc:60-530) and Vnewton
+// (pmgc/newtond.cc:140-486).  Everything runs on one HIP stream per
+// context; reductions finish on the device (xdamp never leaves HBM) and the
+// host reads back one scalar per V-cycle for the stopping test.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/pmg_mi355x.h"
+#include "pmg_internal.h"
+
+using namespace pmg;
+
+namespace {
+
+constexpr int MAXLEV = 16;
+
+struct Params {
+  int nrwk, niwk, nx, ny, nz, nlev, nu1, nu2, mgkey, itmax, istop, iinfo,
+      ipkey, mode, mgprol, mgcoar, mgdisc, mgsmoo, mgsolv, iperf;
+  double errtol;
+};
+
+int maxlev(int n1, int n2, int n3) {  // Vmaxlev (mgsubd.cc:456-484)
+  int lev = 0, done = 0;
+  do {
+    lev++;
+    int iden = 1 << (lev - 1);
+    int c1 = (n1 - 1) / iden + 1, c2 = (n2 - 1) / iden + 1,
+        c3 = (n3 - 1) / iden + 1;
+    if ((c1 - 1) * iden != n1 - 1 || c1 <= 2) done = 1;
+    if ((c2 - 1) * iden != n2 - 1 || c2 <= 2) done = 1;
+    if ((c3 - 1) * iden != n3 - 1 || c3 <= 2) done = 1;
+  } while (!done && lev < 30);
+  return lev - 1;
+}
+
+int mgsz(int mgcoar, int mgdisc, int mgsolv, int nx, int ny, int nz, int nlev,
+         int *out) {  // Vmgsz (mgdrvd.cc:505-601)
+  if (nlev < 1 || nlev > MAXLEV) return PMG_EBADPARM;
+  long long nf = (long long)nx * ny * nz, narr = nf;
+  int nxc = nx, nyc = ny, nzc = nz;
+  for (int l = 2; l <= nlev; l++) {
+    nxc = corsr(nxc), nyc = corsr(nyc), nzc = corsr(nzc);
+    narr += (long long)nxc * nyc * nzc;
+  }
+  long long nc = (long long)nxc * nyc * nzc, narrc = narr - nf;
+  int nfop, ncop;
+  long long nband = 0;
+  if (mgdisc == 0) nfop = 4;
+  else if (mgdisc == 1) nfop = 14;
+  else return PMG_EBADPARM;
+  if ((mgcoar == 0 || mgcoar == 1) && mgdisc == 0) ncop = 4;
+  else if (mgcoar == 2) ncop = 14;
+  else return PMG_EBADPARM;
+  if (mgsolv == 1) {
+    long long nb = ((mgcoar == 0 || mgcoar == 1) && mgdisc == 0)
+                       ? 1 + (long long)(nxc - 2) * (nyc - 2)
+                       : 1 + (long long)(nxc - 2) * (nyc - 2) + (nxc - 2) + 1;
+    nband = (long long)(nxc - 2) * (nyc - 2) * (nzc - 2) * nb;
+  } else if (mgsolv != 0) {
+    return PMG_EBADPARM;
+  }
+  long long nrpc = 100LL * (nlev + 1);
+  long long iretot = 2 * narr + nfop * nf + (27 + ncop) * narrc + nband + nrpc;
+  if (iretot > 0x7fffffffLL) return PMG_EBADPARM;  // int arrays of the ABI
+  out[0] = nxc, out[1] = nyc, out[2] = nzc;
+  out[3] = (int)nf, out[4] = (int)nc, out[5] = (int)narr, out[6] = (int)narrc;
+  out[7] = (int)nrpc, out[8] = 50 * (nlev + 1), out[9] = 100 * (nlev + 1);
+  out[10] = (int)iretot, out[11] = out[8] + out[9];
+  return PMG_OK;
+}
+
+// Decode iparm/rparm (mgdrvd.cc:111-132, 247-265) and reject what this
+// build does not provide.
+int decode(const int *iparm, const double *rparm, Params &p) {
+  p.nrwk = iparm[0], p.niwk = iparm[1], p.nx = iparm[2], p.ny = iparm[3],
+  p.nz = iparm[4], p.nlev = iparm[5], p.nu1 = iparm[6], p.nu2 = iparm[7],
+  p.mgkey = iparm[8], p.itmax = iparm[9], p.istop = iparm[10],
+  p.iinfo = iparm[11], p.ipkey = iparm[13], p.mode = iparm[15],
+  p.mgprol = iparm[16], p.mgcoar = iparm[17], p.mgdisc = iparm[18],
+  p.mgsmoo = iparm[19], p.mgsolv = iparm[20], p.iperf = iparm[21];
+  p.errtol = rparm[0];
+  if (p.nlev < 1 || p.nlev > MAXLEV || p.nx < 3 || p.ny < 3 || p.nz < 3)
+    return PMG_EBADPARM;
+  if (p.nlev > 1 && p.nlev > maxlev(p.nx, p.ny, p.nz)) return PMG_EBADPARM;
+  if (p.mgcoar != 2 || p.mgdisc != 0 || p.mgsmoo != 1 || p.mgsolv != 0 ||
+      (p.mgprol != 0 && p.mgprol != 1) || (p.istop != 0 && p.istop != 1) ||
+      p.iperf != 0 || p.ipkey != 0 || p.nu1 < 0 || p.nu2 < 0 || p.itmax < 0 ||
+      (p.mgkey != 0 && p.mgkey != 1))
+    return PMG_EBADPARM;
+  return PMG_OK;
+}
+
+struct Level {
+  Dim d;
+  int numdia = 7;
+  double *ac = nullptr, *cc = nullptr, *fc = nullptr, *pc = nullptr;
+  double *x = nullptr, *w0 = nullptr, *r = nullptr, *t = nullptr;
+  float *ac32 = nullptr, *cc32 = nullptr, *pc32 = nullptr, *x32 = nullptr,
+        *w032 = nullptr, *r32 = nullptr, *t32 = nullptr;
+  // Newton: Jacobian diagonal term and right-hand side per level
+  double *cpr = nullptr, *rhs = nullptr;
+  float *cpr32 = nullptr;
+};
+
+template <class T>
+struct LV {
+  Dim d;
+  int numdia;
+  const T *ac, *cc, *pc;
+  T *x, *w0, *r, *t;
+};
+
+}  // namespace
+
+struct pmg_ctx {
+  Params p{};
+  int nlev = 0;
+  Level L[MAXLEV];
+  hipStream_t s = nullptr;
+  std::vector<void *> allocs;
+  size_t bytes = 0;
+  bool ok = true;
+  // resident fine-level inputs
+  double *xf = nullptr, *yf = nullptr, *zf = nullptr, *gx = nullptr,
+         *gy = nullptr, *gz = nullptr, *a1 = nullptr, *a2 = nullptr,
+         *a3 = nullptr, *ccf = nullptr, *fcf = nullptr;
+  double *u = nullptr;       // fine solution (FP64)
+  double *work[4] = {};      // fine FP64 scratch (Newton)
+  double *part = nullptr;    // reduction partials
+  double *scal = nullptr;    // device scalars
+  double *hscal = nullptr;   // pinned host mirror
+  bool have32 = false, haveN = false;
+  int iters = 0, ierror = 0;
+  double relres = 0.0, setup_ms = 0.0, solve_ms = 0.0;
+  // optional per-kernel event timing of the fine smoothing sweeps
+  bool ktime = false;
+  std::vector<hipEvent_t> ev;
+  int nev = 0;
+  double ktime_ms = 0.0;
+  long long ktime_count = 0;
+
+  template <class T>
+  T *alloc(size_t n) {
+    void *ptr = nullptr;
+    if (!ok) return nullptr;
+    if (hipMalloc(&ptr, n * sizeof(T) + 256) != hipSuccess) {
+      ok = false;
+      return nullptr;
+    }
+    hipMemsetAsync(ptr, 0, n * sizeof(T) + 256, s);
+    allocs.push_back(ptr);
+    bytes += n * sizeof(T) + 256;
+    return (T *)ptr;
+  }
+  ~pmg_ctx() {
+    if (s) hipStreamSynchronize(s);
+    for (void *ptr : allocs) hipFree(ptr);
+    if (hscal) hipHostFree(hscal);
+    for (auto e : ev) hipEventDestroy(e);
+    if (s) hipStreamDestroy(s);
+  }
+};
+
+namespace {
+
+enum { S_RSDEN = 0, S_RSNRM = 1, S_XDAMP = 2, S_TMP = 4 };
+
+thread_local int t_iters = 0, t_ierror = 0;
+thread_local double t_relres = 0.0;
+
+int device_ok() {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n < 1) return 0;
+  return 1;
+}
+
+// reduce partials to a device scalar and read it back
+double host_scalar(pmg_ctx *c, int nb, int slot) {
+  reduce(c->part, nb, 1, 0, c->scal + slot, c->s);
+  hipMemcpyAsync(c->hscal + slot, c->scal + slot, sizeof(double),
+                 hipMemcpyDeviceToHost, c->s);
+  hipStreamSynchronize(c->s);
+  return c->hscal[slot];
+}
+
+template <class T>
+LV<T> view(Level &l);
+template <>
+LV<double> view<double>(Level &l) {
+  return {l.d, l.numdia, l.ac, l.cc, l.pc, l.x, l.w0, l.r, l.t};
+}
+template <>
+LV<float> view<float>(Level &l) {
+  return {l.d, l.numdia, l.ac32, l.cc32, l.pc32, l.x32, l.w032, l.r32, l.t32};
+}
+
+template <class T>
+void smooth(pmg_ctx *c, const LV<T> &l, const T *f, T *x, int nu, int iadj,
+            bool fine) {
+  if (nu <= 0) return;
+  bool timed = c->ktime && fine && c->nev + 2 <= (int)c->ev.size();
+  if (timed) hipEventRecord(c->ev[c->nev], c->s);
+  if (l.numdia == 7)
+    gs7<T>(l.d, l.ac, l.cc, f, x, nu, iadj, c->s);
+  else
+    gs27<T>(l.d, l.ac, l.cc, f, x, nu, iadj, c->s);
+  if (timed) {
+    hipEventRecord(c->ev[c->nev + 1], c->s);
+    c->nev += 2;
+    c->ktime_count += nu;
+  }
+}
+
+// One V-cycle (Vmvcs body, mgcsd.cc:299-480).  L[0].x is the iterate on
+// the fine level; frhs its right-hand side; cc64c the FP64 Helmholtz term of
+// the coarsest level used by the FP64 coarse solve.
+template <class T>
+void vcycle(pmg_ctx *c, LV<T> *L, const T *frhs, const double *ac64c,
+            const double *cc64c) {
+  const int nlev = c->nlev;
+  const Params &p = c->p;
+  hipStream_t s = c->s;
+  // nu1 pre-smoothings on the fine grid, then the residual (D1)
+  smooth<T>(c, L[0], frhs, L[0].x, p.nu1, 0, true);
+  resid<T, T>(L[0].d, L[0].numdia, L[0].ac, L[0].cc, frhs, L[0].x, L[0].r,
+              nullptr, s);
+  for (int l = 1; l < nlev; l++) {  // down (mgcsd.cc:325-360)
+    restrict_<T>(L[l - 1].d, L[l].d, L[l - 1].r, L[l].w0, L[l].pc, s);
+    if (l != nlev - 1) {
+      fill<T>(L[l].x, L[l].d.n(), (T)0, s);
+      smooth<T>(c, L[l], L[l].w0, L[l].x, p.nu1, 0, false);
+      resid<T, T>(L[l].d, L[l].numdia, L[l].ac, L[l].cc, L[l].w0, L[l].x,
+                  L[l].r, nullptr, s);
+    }
+  }
+  // coarsest: Vcghs, 100 iterations, errtol = epsiln = 0 (D5), FP64
+  {
+    const int l = nlev - 1;
+    cg_single<T>(L[l].d, L[l].numdia, ac64c, cc64c, L[l].w0, L[l].x, 100, 0.0,
+                 s);
+  }
+  for (int l = nlev - 2; l >= 0; l--) {  // up (mgcsd.cc:418-473)
+    const int cl = l + 1;
+    int nb = matvec_dots<T>(L[cl].d, L[cl].numdia, L[cl].ac, L[cl].cc,
+                            L[cl].x, L[cl].w0, L[cl].t, c->part, s);
+    reduce(c->part, nb, 2, 1, c->scal + S_XDAMP, s);
+    interp<T>(L[cl].d, L[l].d, L[cl].x, L[cl].pc, L[l].x, c->scal + S_XDAMP,
+              s);
+    smooth<T>(c, L[l], l == 0 ? frhs : L[l].w0, L[l].x, p.nu2, 1, l == 0);
+  }
+}
+
+// Linear solve with FP64 V-cycles.  L[0].x holds the iterate (zeroed by the
+// caller), frhs the fine right-hand side.  istop 0: absolute L1 residual,
+// 1: relative to ||frhs||_1 (mgcsd.cc:133-163, 483-527).
+int linear_fp64(pmg_ctx *c, LV<double> *L, const double *frhs,
+                const double *cc64c, int istop, double errtol, int itmax,
+                int *iters_out, double *rel_out) {
+  Level &F = c->L[0];
+  double rsden = 1.0;
+  if (istop == 1) {
+    int nb = nrm1_part<double>(F.d, frhs, c->part, c->s);
+    rsden = host_scalar(c, nb, S_RSDEN);
+  }
+  if (rsden == 0.0) rsden = 1.0;
+  double rsnrm = rsden;
+  const int nlev = c->nlev;
+  if (nlev == 1) {  // direct coarse solve on the only level (mgcsd.cc:170-291)
+    cg_single<double>(F.d, F.numdia, F.ac, L[0].cc, frhs, L[0].x, 100, 0.0,
+                      c->s);
+    int nb = resid<double, double>(F.d, F.numdia, F.ac, L[0].cc, frhs,
+                                   L[0].x, nullptr, c->part, c->s);
+    rsnrm = host_scalar(c, nb, S_RSNRM);
+    *iters_out = 1;
+    *rel_out = rsnrm / rsden;
+    return PMG_OK;
+  }
+  int iters = 0;
+  do {
+    vcycle<double>(c, L, frhs, c->L[nlev - 1].ac, cc64c);
+    iters++;
+    int nb = resid<double, double>(F.d, F.numdia, F.ac, L[0].cc, frhs, L[0].x,
+                                   nullptr, c->part, c->s);
+    rsnrm = host_scalar(c, nb, S_RSNRM);
+  } while (iters < itmax && rsnrm / rsden > errtol);
+  *iters_out = iters;
+  *rel_out = rsnrm / rsden;
+  return iters < itmax ? PMG_OK : PMG_ITMAX;
+}
+
+// Mixed precision: FP64 defect correction around FP32 V-cycles.
+//   r = f - A u (FP64, stored as the FP32 right-hand side of the cycle)
+//   e = V-cycle_FP32(0, r);  u += e (FP64)
+// The stopping test is the FP64 L1 residual, as in linear_fp64.
+int linear_mixed(pmg_ctx *c, LV<double> *L64, LV<float> *L32,
+                 const double *frhs, const double *cc64c, double *u,
+                 int istop, double errtol, int itmax, int *iters_out,
+                 double *rel_out) {
+  Level &F = c->L[0];
+  double rsden = 1.0;
+  if (istop == 1) {
+    int nb = nrm1_part<double>(F.d, frhs, c->part, c->s);
+    rsden = host_scalar(c, nb, S_RSDEN);
+  }
+  if (rsden == 0.0) rsden = 1.0;
+  float *r32 = F.w032;  // fine right-hand side of the FP32 cycle
+  int nb = resid<double, float>(F.d, F.numdia, F.ac, L64[0].cc, frhs, u, r32,
+                                c->part, c->s);
+  double rsnrm = host_scalar(c, nb, S_RSNRM);
+  int iters = 0;
+  const int nlev = c->nlev;
+  while (iters < itmax && rsnrm / rsden > errtol) {
+    fill<float>(L32[0].x, F.d.n(), 0.0f, c->s);
+    if (nlev == 1) {
+      cg_single<float>(F.d, F.numdia, F.ac, L64[0].cc, r32, L32[0].x, 100,
+                       0.0, c->s);
+    } else {
+      vcycle<float>(c, L32, r32, c->L[nlev - 1].ac, cc64c);
+    }
+    add_correction<float>(F.d, L32[0].x, u, c->s);
+    iters++;
+    nb = resid<double, float>(F.d, F.numdia, F.ac, L64[0].cc, frhs, u, r32,
+                              c->part, c->s);
+    rsnrm = host_scalar(c, nb, S_RSNRM);
+  }
+  *iters_out = iters;
+  *rel_out = rsnrm / rsden;
+  return (iters < itmax || rsnrm / rsden <= errtol) ? PMG_OK : PMG_ITMAX;
+}
+
+int alloc_hierarchy(pmg_ctx *c) {
+  const Params &p = c->p;
+  c->nlev = p.nlev;
+  Dim d{p.nx, p.ny, p.nz};
+  for (int l = 0; l < c->nlev; l++) {
+    Level &L = c->L[l];
+    L.d = d;
+    L.numdia = l == 0 ? 7 : 27;
+    size_t n = d.n();
+    L.ac = c->alloc<double>((l == 0 ? 4 : 14) * n);
+    L.cc = c->alloc<double>(n);
+    L.fc = c->alloc<double>(n);
+    if (l > 0) L.pc = c->alloc<double>(27 * n);
+    L.x = c->alloc<double>(n);
+    L.w0 = c->alloc<double>(n);
+    L.r = c->alloc<double>(n);
+    L.t = c->alloc<double>(n);
+    d = Dim{corsr(d.nx), corsr(d.ny), corsr(d.nz)};
+  }
+  size_t nf = c->L[0].d.n();
+  c->u = c->alloc<double>(nf);
+  c->part = c->alloc<double>(max_partials(c->L[0].d) * 2);
+  c->scal = c->alloc<double>(16);
+  if (hipHostMalloc((void **)&c->hscal, 16 * sizeof(double)) != hipSuccess)
+    c->ok = false;
+  return c->ok ? PMG_OK : PMG_ENOMEM;
+}
+
+int alloc_fp32(pmg_ctx *c) {
+  if (c->have32) return PMG_OK;
+  for (int l = 0; l < c->nlev; l++) {
+    Level &L = c->L[l];
+    size_t n = L.d.n();
+    L.ac32 = c->alloc<float>((l == 0 ? 4 : 14) * n);
+    L.cc32 = c->alloc<float>(n);
+    if (l > 0) L.pc32 = c->alloc<float>(27 * n);
+    L.x32 = c->alloc<float>(n);
+    L.w032 = c->alloc<float>(n);
+    L.r32 = c->alloc<float>(n);
+    L.t32 = c->alloc<float>(n);
+  }
+  if (!c->ok) return PMG_ENOMEM;
+  c->have32 = true;
+  return PMG_OK;
+}
+
+void convert_ops32(pmg_ctx *c) {
+  for (int l = 0; l < c->nlev; l++) {
+    Level &L = c->L[l];
+    size_t n = L.d.n();
+    convert<double, float>(L.ac, L.ac32, (l == 0 ? 4 : 14) * n, c->s);
+    convert<double, float>(L.cc, L.cc32, n, c->s);
+    if (l > 0) convert<double, float>(L.pc, L.pc32, 27 * n, c->s);
+  }
+}
+
+// Vbuildops ido = 0 and 1 (mgsubd.cc:63-264), Galerkin coarsening.
+void build_ops(pmg_ctx *c) {
+  hipStream_t s = c->s;
+  Level &F = c->L[0];
+  buildA(F.d, F.ac, F.cc, F.fc, c->xf, c->yf, c->zf, c->gx, c->gy, c->gz,
+         c->a1, c->a2, c->a3, c->ccf, c->fcf, s);
+  for (int l = 1; l < c->nlev; l++) {
+    Level &Lf = c->L[l - 1], &Lc = c->L[l];
+    buildP(Lf.d, Lc.d, c->p.mgprol, Lf.numdia, Lf.ac, Lc.pc, s);
+    buildG(Lf.d, Lc.d, Lf.numdia, Lc.pc, Lf.ac, Lc.ac, s);
+    restrict_<double>(Lf.d, Lc.d, Lf.cc, Lc.cc, Lc.pc, s);
+    restrict_<double>(Lf.d, Lc.d, Lf.fc, Lc.fc, Lc.pc, s);
+  }
+  if (c->have32) convert_ops32(c);
+}
+
+template <class T>
+void upload(pmg_ctx *c, T *dst, const T *src, size_t n) {
+  if (dst && src) hipMemcpyAsync(dst, src, n * sizeof(T),
+                                 hipMemcpyHostToDevice, c->s);
+}
+
+int ctx_create(pmg_ctx **out, const Params &p, const double *xf,
+               const double *yf, const double *zf, const double *gxcf,
+               const double *gycf, const double *gzcf, const double *a1cf,
+               const double *a2cf, const double *a3cf, const double *ccf,
+               const double *fcf) {
+  *out = nullptr;
+  if (!device_ok()) return PMG_EDEVICE;
+  pmg_ctx *c = new (std::nothrow) pmg_ctx;
+  if (!c) return PMG_ENOMEM;
+  c->p = p;
+  if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return PMG_EDEVICE;
+  }
+  int rc = alloc_hierarchy(c);
+  if (rc) {
+    delete c;
+    return rc;
+  }
+  const int nx = p.nx, ny = p.ny, nz = p.nz;
+  const size_t nf = (size_t)nx * ny * nz;
+  c->xf = c->alloc<double>(nx);
+  c->yf = c->alloc<double>(ny);
+  c->zf = c->alloc<double>(nz);
+  c->gx = c->alloc<double>(2 * (size_t)ny * nz);
+  c->gy = c->alloc<double>(2 * (size_t)nx * nz);
+  c->gz = c->alloc<double>(2 * (size_t)nx * ny);
+  c->a1 = c->alloc<double>(nf);
+  c->a2 = c->alloc<double>(nf);
+  c->a3 = c->alloc<double>(nf);
+  c->ccf = c->alloc<double>(nf);
+  c->fcf = c->alloc<double>(nf);
+  if (!c->ok) {
+    delete c;
+    return PMG_ENOMEM;
+  }
+  upload(c, c->xf, xf, nx);
+  upload(c, c->yf, yf, ny);
+  upload(c, c->zf, zf, nz);
+  upload(c, c->gx, gxcf, 2 * (size_t)ny * nz);
+  upload(c, c->gy, gycf, 2 * (size_t)nx * nz);
+  upload(c, c->gz, gzcf, 2 * (size_t)nx * ny);
+  upload(c, c->a1, a1cf, nf);
+  upload(c, c->a2, a2cf, nf);
+  upload(c, c->a3, a3cf, nf);
+  upload(c, c->ccf, ccf, nf);
+  upload(c, c->fcf, fcf, nf);
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  hipEventRecord(e0, c->s);
+  build_ops(c);
+  hipEventRecord(e1, c->s);
+  if (hipStreamSynchronize(c->s) != hipSuccess) {
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    delete c;
+    return PMG_EDEVICE;
+  }
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, e0, e1);
+  c->setup_ms = ms;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  *out = c;
+  return PMG_OK;
+}
+
+int ctx_solve(pmg_ctx *c, int mode) {
+  const Params &p = c->p;
+  Level &F = c->L[0];
+  if (mode == PMG_MIXED) {
+    int rc = alloc_fp32(c);
+    if (rc) return rc;
+    convert_ops32(c);
+  }
+  if (c->nlev > 1 || true) {
+    const Dim &cd = c->L[c->nlev - 1].d;
+    if (!cg_fits(cd)) return PMG_EBADPARM;
+  }
+  LV<double> L64[MAXLEV];
+  LV<float> L32[MAXLEV];
+  for (int l = 0; l < c->nlev; l++) {
+    L64[l] = view<double>(c->L[l]);
+    if (mode == PMG_MIXED) L32[l] = view<float>(c->L[l]);
+  }
+  const double *cc64c = c->L[c->nlev - 1].cc;
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  c->nev = 0;
+  hipEventRecord(e0, c->s);
+  // Vazeros + VfboundPMG00 on u (mgdrvd.cc:419, 439)
+  fill<double>(c->u, F.d.n(), 0.0, c->s);
+  int rc;
+  if (mode == PMG_MIXED) {
+    rc = linear_mixed(c, L64, L32, F.fc, cc64c, c->u, p.istop, p.errtol,
+                      p.itmax, &c->iters, &c->relres);
+  } else {
+    L64[0].x = c->u;
+    rc = linear_fp64(c, L64, F.fc, cc64c, p.istop, p.errtol, p.itmax,
+                     &c->iters, &c->relres);
+  }
+  // VfboundPMG(ibound=1): Dirichlet values on u (mgdrvd.cc:500-502)
+  fbound_dirichlet(F.d, c->u, c->gx, c->gy, c->gz, c->s);
+  hipEventRecord(e1, c->s);
+  hipError_t se = hipStreamSynchronize(c->s);
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, e0, e1);
+  c->solve_ms = ms;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (c->ktime) {
+    for (int q = 0; q + 1 < c->nev; q += 2) {
+      float kms = 0.f;
+      hipEventElapsedTime(&kms, c->ev[q], c->ev[q + 1]);
+      c->ktime_ms += kms;
+    }
+  }
+  if (se != hipSuccess) return PMG_EDEVICE;
+  c->ierror = rc == PMG_ITMAX ? 1 : 0;
+  return rc;
+}
+
+// ---- Newton (Vnewton, newtond.cc:140-486) ----
+int alloc_newton(pmg_ctx *c) {
+  if (c->haveN) return PMG_OK;
+  for (int l = 0; l < c->nlev; l++) {
+    Level &L = c->L[l];
+    L.cpr = c->alloc<double>(L.d.n());
+    L.rhs = c->alloc<double>(L.d.n());
+  }
+  for (int q = 0; q < 4; q++) c->work[q] = c->alloc<double>(c->L[0].d.n());
+  if (!c->ok) return PMG_ENOMEM;
+  c->haveN = true;
+  return PMG_OK;
+}
+
+double nm_norm(pmg_ctx *c, const double *x, double *r) {
+  Level &F = c->L[0];
+  int nb = nmresid7(F.d, F.ac, F.cc, F.fc, x, r, c->part, c->s);
+  return host_scalar(c, nb, S_TMP);
+}
+
+int ctx_newton(pmg_ctx *c, int mode) {
+  const Params &p = c->p;
+  int rc = alloc_newton(c);
+  if (rc) return rc;
+  if (mode == PMG_MIXED) {
+    rc = alloc_fp32(c);
+    if (rc) return rc;
+    convert_ops32(c);
+    for (int l = 0; l < c->nlev; l++)
+      if (!c->L[l].cpr32) c->L[l].cpr32 = c->alloc<float>(c->L[l].d.n());
+    if (!c->ok) return PMG_ENOMEM;
+  }
+  if (!cg_fits(c->L[c->nlev - 1].d)) return PMG_EBADPARM;
+  Level &F = c->L[0];
+  const Dim d = F.d;
+  const size_t nf = d.n();
+  hipStream_t s = c->s;
+  double *w0n = c->work[0], *w1 = c->work[1], *t1 = c->work[2],
+         *t2 = c->work[3];
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  hipEventRecord(e0, s);
+  fill<double>(c->u, nf, 0.0, s);
+  for (int q = 0; q < 4; q++) fill<double>(c->work[q], nf, 0.0, s);
+  double rsden = 1.0;
+  if (p.istop == 1) rsden = nm_norm(c, w1, t2);  // residual at u = 0
+  if (rsden == 0.0) rsden = 1.0;
+  double xold = nm_norm(c, c->u, w0n);
+  double xnew = xold, xmed = xold, rsnrm = rsden;
+  int idamp = 1, iters = 0;
+  LV<double> L64[MAXLEV];
+  LV<float> L32[MAXLEV];
+  for (int l = 0; l < c->nlev; l++) {
+    L64[l] = view<double>(c->L[l]);
+    L64[l].cc = c->L[l].cpr;
+    if (mode == PMG_MIXED) {
+      L32[l] = view<float>(c->L[l]);
+      L32[l].cc = c->L[l].cpr32;
+    }
+  }
+  double *xtmp = F.x;  // fine iterate of the inner linear solve
+  L64[0].x = xtmp;
+  for (;;) {
+    iters++;
+    // Vgetjac (newtond.cc:488-531)
+    copy_interior(d, w0n, F.rhs, s);
+    dc_vec(F.cc, c->u, F.cpr, nf, s);
+    for (int l = 1; l < c->nlev; l++) {
+      restrict_<double>(c->L[l - 1].d, c->L[l].d, c->L[l - 1].rhs,
+                        c->L[l].rhs, c->L[l].pc, s);
+      restrict_<double>(c->L[l - 1].d, c->L[l].d, c->L[l - 1].cpr,
+                        c->L[l].cpr, c->L[l].pc, s);
+    }
+    if (mode == PMG_MIXED)
+      for (int l = 0; l < c->nlev; l++)
+        convert<double, float>(c->L[l].cpr, c->L[l].cpr32, c->L[l].d.n(), s);
+    const double errtol_s = std::min(0.9 * xold, 1.0 * std::pow(xold, 2.0));
+    fill<double>(xtmp, nf, 0.0, s);
+    int its = 0;
+    double rel = 0.0;
+    const double *cc64c = c->L[c->nlev - 1].cpr;
+    if (mode == PMG_MIXED)
+      linear_mixed(c, L64, L32, F.rhs, cc64c, xtmp, 0, errtol_s, 1000, &its,
+                   &rel);
+    else
+      linear_fp64(c, L64, F.rhs, cc64c, 0, errtol_s, 1000, &its, &rel);
+    if (idamp == 1) {  // damping (newtond.cc:313-376)
+      xpay_interior(d, c->u, 1.0, xtmp, w1, s);
+      xnew = nm_norm(c, w1, w0n);
+      double damp = 1.0;
+      int iter_d = 0, dmode = 0;
+      while (iter_d < 10) {
+        if (dmode == 0) {
+          if (xnew < xold) dmode = 1;
+        } else if (xnew > xmed) {
+          break;
+        }
+        copy_interior(d, w1, t1, s);
+        copy_interior(d, w0n, t2, s);
+        xmed = xnew;
+        damp = damp / 2.0;
+        xpay_interior(d, c->u, damp, xtmp, w1, s);
+        xnew = nm_norm(c, w1, w0n);
+        iter_d++;
+      }
+      copy_interior(d, t1, c->u, s);
+      copy_interior(d, t2, w0n, s);
+      xnew = xmed;
+      xold = xnew;
+      if (iter_d - 1 == 0) idamp = 0;
+    } else {
+      axpy_interior<double>(d, 1.0, xtmp, c->u, s);
+      xnew = nm_norm(c, c->u, w0n);
+      xold = xnew;
+    }
+    rsnrm = xnew;
+    if (rsnrm / rsden <= p.errtol) break;
+    if (iters >= p.itmax) break;
+  }
+  fbound_dirichlet(d, c->u, c->gx, c->gy, c->gz, s);
+  hipEventRecord(e1, s);
+  hipError_t se = hipStreamSynchronize(s);
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, e0, e1);
+  c->solve_ms = ms;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (se != hipSuccess) return PMG_EDEVICE;
+  c->iters = iters;
+  c->relres = rsnrm / rsden;
+  c->ierror = c->relres <= p.errtol ? 0 : 1;
+  return c->ierror ? PMG_ITMAX : PMG_OK;
+}
+
+// Fill iwork/rwork as the reference leaves them (mgdrvd.cc:145-166,
+// mgsubd.cc:266-373): iz, ipc per level; rpc | cc | fc | pc | ac.
+int export_work(pmg_ctx *c, int *iwork, double *rwork, bool newton) {
+  int sz[12];
+  const Params &p = c->p;
+  mgsz(p.mgcoar, p.mgdisc, p.mgsolv, p.nx, p.ny, p.nz, p.nlev, sz);
+  const int nf = sz[3], narr = sz[5], narrc = sz[6], nrpc = sz[7], niz = sz[8];
+  std::memset(iwork, 0, sizeof(int) * (size_t)sz[11]);
+  int *iz = iwork, *ipc = iwork + niz;
+  auto IZ = [&](int a, int lev) -> int & { return iz[(lev - 1) * 50 + a - 1]; };
+  // Vbuildstr
+  Dim d{p.nx, p.ny, p.nz};
+  int n = (int)d.n();
+  for (int a = 1; a <= 11; a++) IZ(a, 1) = 1;
+  IZ(1, 2) = IZ(1, 1) + n;
+  IZ(2, 2) = IZ(2, 1) + 4 * d.ny * d.nz;
+  IZ(3, 2) = IZ(3, 1) + 4 * d.nx * d.nz;
+  IZ(4, 2) = IZ(4, 1) + 4 * d.nx * d.ny;
+  IZ(5, 2) = IZ(5, 1) + 100;
+  IZ(6, 2) = IZ(6, 1) + 100;
+  IZ(8, 2) = IZ(8, 1) + d.nx;
+  IZ(9, 2) = IZ(9, 1) + d.ny;
+  IZ(10, 2) = IZ(10, 1) + d.nz;
+  for (int lev = 2; lev <= p.nlev; lev++) {
+    d = Dim{corsr(d.nx), corsr(d.ny), corsr(d.nz)};
+    n = (int)d.n();
+    IZ(1, lev + 1) = IZ(1, lev) + n;
+    IZ(2, lev + 1) = IZ(2, lev) + 4 * d.ny * d.nz;
+    IZ(3, lev + 1) = IZ(3, lev) + 4 * d.nx * d.nz;
+    IZ(4, lev + 1) = IZ(4, lev) + 4 * d.nx * d.ny;
+    IZ(5, lev + 1) = IZ(5, lev) + 100;
+    IZ(6, lev + 1) = IZ(6, lev) + 100;
+    IZ(7, lev + 1) = IZ(7, lev) + 4 * n;
+    IZ(8, lev + 1) = IZ(8, lev) + d.nx;
+    IZ(9, lev + 1) = IZ(9, lev) + d.ny;
+    IZ(10, lev + 1) = IZ(10, lev) + d.nz;
+    IZ(11, lev) = IZ(11, lev - 1) + 27 * n;
+  }
+  // operator offsets as Vbuildops marks them
+  IZ(7, 2) = IZ(7, 1) + 4 * (int)c->L[0].d.n();
+  for (int lev = 2; lev <= p.nlev; lev++)
+    IZ(7, lev + 1) = IZ(7, lev) + 14 * (int)c->L[lev - 1].d.n();
+  for (int lev = 1; lev <= p.nlev; lev++) {
+    int *ipcl = ipc + IZ(5, lev) - 1;
+    ipcl[9] = p.ipkey;
+    ipcl[10] = lev == 1 ? 7 : 27;
+    if (lev == 1) ipcl[11] = 1;
+  }
+  size_t rtot = (size_t)sz[10] + (newton ? 2 * (size_t)nf : 0);
+  std::memset(rwork, 0, sizeof(double) * rtot);
+  double *ccb = rwork + nrpc, *fcb = ccb + narr;
+  double *after = fcb + narr + (newton ? 2 * (size_t)nf : 0);
+  double *pcb = after, *acb = pcb + 27 * (size_t)narrc;
+  for (int lev = 1; lev <= p.nlev; lev++) {
+    Level &L = c->L[lev - 1];
+    size_t nl = L.d.n();
+    hipMemcpyAsync(ccb + IZ(1, lev) - 1, L.cc, nl * sizeof(double),
+                   hipMemcpyDeviceToHost, c->s);
+    hipMemcpyAsync(fcb + IZ(1, lev) - 1, L.fc, nl * sizeof(double),
+                   hipMemcpyDeviceToHost, c->s);
+    hipMemcpyAsync(acb + IZ(7, lev) - 1, L.ac,
+                   (lev == 1 ? 4 : 14) * nl * sizeof(double),
+                   hipMemcpyDeviceToHost, c->s);
+    if (lev > 1)
+      hipMemcpyAsync(pcb + IZ(11, lev - 1) - 1, L.pc,
+                     27 * nl * sizeof(double), hipMemcpyDeviceToHost, c->s);
+  }
+  return hipStreamSynchronize(c->s) == hipSuccess ? PMG_OK : PMG_EDEVICE;
+}
+
+int check_sizes(const Params &p, bool newton) {
+  int sz[12];
+  int rc = mgsz(p.mgcoar, p.mgdisc, p.mgsolv, p.nx, p.ny, p.nz, p.nlev, sz);
+  if (rc) return rc;
+  long long need = (long long)sz[10] + (newton ? 2LL * sz[3] : 0);
+  if (p.nrwk < need || p.niwk < sz[11]) return PMG_EBADPARM;
+  return PMG_OK;
+}
+
+int driver(int *iparm, double *rparm, int *iwork, double *rwork, double *u,
+           const double *xf, const double *yf, const double *zf,
+           const double *gxcf, const double *gycf, const double *gzcf,
+           const double *a1cf, const double *a2cf, const double *a3cf,
+           const double *ccf, const double *fcf, bool newton) {
+  t_iters = 0, t_ierror = 0, t_relres = 0.0;
+  if (!iparm || !rparm || !iwork || !rwork || !u || !xf || !yf || !zf ||
+      !gxcf || !gycf || !gzcf || !a1cf || !a2cf || !a3cf || !ccf || !fcf)
+    return PMG_EBADPARM;
+  Params p;
+  int rc = decode(iparm, rparm, p);
+  if (rc) return rc;
+  if (!newton && p.mode != 0) return PMG_EBADPARM;  // FAS not provided
+  if (newton && p.mgkey != 0) return PMG_EBADPARM;  // Vfnewton not provided
+  rc = check_sizes(p, newton);
+  if (rc) return rc;
+  pmg_ctx *c = nullptr;
+  rc = ctx_create(&c, p, xf, yf, zf, gxcf, gycf, gzcf, a1cf, a2cf, a3cf, ccf,
+                  fcf);
+  if (rc) return rc;
+  int src = newton ? ctx_newton(c, PMG_FP64) : ctx_solve(c, PMG_FP64);
+  if (src == PMG_OK || src == PMG_ITMAX) {
+    hipMemcpyAsync(u, c->u, c->L[0].d.n() * sizeof(double),
+                   hipMemcpyDeviceToHost, c->s);
+    int erc = export_work(c, iwork, rwork, newton);
+    if (erc) src = erc;
+  }
+  t_iters = c->iters;
+  t_ierror = c->ierror;
+  t_relres = c->relres;
+  delete c;
+  return src;
+}
+
+// ---- single-kernel helpers on host arrays ----
+struct Scratch {
+  std::vector<void *> v;
+  bool ok = true;
+  template <class T>
+  T *put(const T *h, size_t n) {
+    void *d = nullptr;
+    if (hipMalloc(&d, n * sizeof(T) + 64) != hipSuccess) {
+      ok = false;
+      return nullptr;
+    }
+    v.push_back(d);
+    if (h)
+      hipMemcpy(d, h, n * sizeof(T), hipMemcpyHostToDevice);
+    else
+      hipMemset(d, 0, n * sizeof(T) + 64);
+    return (T *)d;
+  }
+  ~Scratch() {
+    for (void *d : v) hipFree(d);
+  }
+};
+
+int finish(Scratch &S, double *host, const double *dev, size_t n) {
+  if (!S.ok) return PMG_ENOMEM;
+  if (hipDeviceSynchronize() != hipSuccess) return PMG_EDEVICE;
+  if (hipMemcpy(host, dev, n * sizeof(double), hipMemcpyDeviceToHost) !=
+      hipSuccess)
+    return PMG_EDEVICE;
+  return PMG_OK;
+}
+
+bool dims_ok(int nx, int ny, int nz) { return nx >= 3 && ny >= 3 && nz >= 3; }
+bool coarse_of(int f, int c) { return f == 2 * c - 1; }
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+int pmg_mgsz(int mgcoar, int mgdisc, int mgsolv, int nx, int ny, int nz,
+             int nlev, int *out) {
+  if (!out) return PMG_EBADPARM;
+  return mgsz(mgcoar, mgdisc, mgsolv, nx, ny, nz, nlev, out);
+}
+
+int pmg_maxlev(int nx, int ny, int nz) { return maxlev(nx, ny, nz); }
+
+int pmg_mgdriv(int *iparm, double *rparm, int *iwork, double *rwork,
+               double *u, const double *xf, const double *yf, const double *zf,
+               const double *gxcf, const double *gycf, const double *gzcf,
+               double *a1cf, double *a2cf, double *a3cf, double *ccf,
+               double *fcf, double *tcf) {
+  (void)tcf;
+  return driver(iparm, rparm, iwork, rwork, u, xf, yf, zf, gxcf, gycf, gzcf,
+                a1cf, a2cf, a3cf, ccf, fcf, false);
+}
+
+int pmg_newdriv(int *iparm, double *rparm, int *iwork, double *rwork,
+                double *u, const double *xf, const double *yf,
+                const double *zf, const double *gxcf, const double *gycf,
+                const double *gzcf, double *a1cf, double *a2cf, double *a3cf,
+                double *ccf, double *fcf, double *tcf) {
+  (void)tcf;
+  return driver(iparm, rparm, iwork, rwork, u, xf, yf, zf, gxcf, gycf, gzcf,
+                a1cf, a2cf, a3cf, ccf, fcf, true);
+}
+
+int pmg_mypdefinit(int nion, const double *charge, const double *sconc) {
+  if (nion < 0 || (nion > 0 && (!charge || !sconc))) return PMG_EBADPARM;
+  if (!device_ok()) return PMG_EDEVICE;
+  set_ions(nion, charge, sconc);
+  return hipDeviceSynchronize() == hipSuccess ? PMG_OK : PMG_EDEVICE;
+}
+
+int pmg_last_stats(int *iters, int *ierror, double *relres) {
+  if (iters) *iters = t_iters;
+  if (ierror) *ierror = t_ierror;
+  if (relres) *relres = t_relres;
+  return PMG_OK;
+}
+
+int pmg_ctx_create(pmg_ctx **out, const int *iparm, const double *rparm,
+                   const double *xf, const double *yf, const double *zf,
+                   const double *gxcf, const double *gycf, const double *gzcf,
+                   const double *a1cf, const double *a2cf, const double *a3cf,
+                   const double *ccf, const double *fcf) {
+  if (!out || !iparm || !rparm) return PMG_EBADPARM;
+  Params p;
+  int rc = decode(iparm, rparm, p);
+  if (rc) return rc;
+  if (!xf || !yf || !zf || !gxcf || !gycf || !gzcf || !a1cf || !a2cf ||
+      !a3cf || !ccf || !fcf)
+    return PMG_EBADPARM;
+  return ctx_create(out, p, xf, yf, zf, gxcf, gycf, gzcf, a1cf, a2cf, a3cf,
+                    ccf, fcf);
+}
+
+int pmg_ctx_solve(pmg_ctx *ctx, int mode) {
+  if (!ctx || (mode != PMG_FP64 && mode != PMG_MIXED)) return PMG_EBADPARM;
+  if (ctx->p.mode == 1) return ctx_newton(ctx, mode);
+  return ctx_solve(ctx, mode);
+}
+
+int pmg_ctx_get_u(pmg_ctx *ctx, double *u) {
+  if (!ctx || !u) return PMG_EBADPARM;
+  if (hipMemcpyAsync(u, ctx->u, ctx->L[0].d.n() * sizeof(double),
+                     hipMemcpyDeviceToHost, ctx->s) != hipSuccess)
+    return PMG_EDEVICE;
+  return hipStreamSynchronize(ctx->s) == hipSuccess ? PMG_OK : PMG_EDEVICE;
+}
+
+int pmg_ctx_stats(pmg_ctx *ctx, int *iters, int *ierror, double *relres,
+                  double *setup_ms, double *solve_ms) {
+  if (!ctx) return PMG_EBADPARM;
+  if (iters) *iters = ctx->iters;
+  if (ierror) *ierror = ctx->ierror;
+  if (relres) *relres = ctx->relres;
+  if (setup_ms) *setup_ms = ctx->setup_ms;
+  if (solve_ms) *solve_ms = ctx->solve_ms;
+  return PMG_OK;
+}
+
+int pmg_ctx_rebuild(pmg_ctx *ctx) {
+  if (!ctx) return PMG_EBADPARM;
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  hipEventRecord(e0, ctx->s);
+  build_ops(ctx);
+  hipEventRecord(e1, ctx->s);
+  hipError_t se = hipStreamSynchronize(ctx->s);
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, e0, e1);
+  ctx->setup_ms = ms;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  return se == hipSuccess ? PMG_OK : PMG_EDEVICE;
+}
+
+size_t pmg_ctx_bytes(const pmg_ctx *ctx) { return ctx ? ctx->bytes : 0; }
+
+void pmg_ctx_destroy(pmg_ctx *ctx) { delete ctx; }
+
+int pmg_ctx_get_level(pmg_ctx *ctx, int lev, double *ac, double *cc,
+                      double *fc, double *pc) {
+  if (!ctx || lev < 1 || lev > ctx->nlev) return PMG_EBADPARM;
+  Level &L = ctx->L[lev - 1];
+  size_t n = L.d.n();
+  if (ac)
+    hipMemcpyAsync(ac, L.ac, (lev == 1 ? 4 : 14) * n * sizeof(double),
+                   hipMemcpyDeviceToHost, ctx->s);
+  if (cc)
+    hipMemcpyAsync(cc, L.cc, n * sizeof(double), hipMemcpyDeviceToHost,
+                   ctx->s);
+  if (fc)
+    hipMemcpyAsync(fc, L.fc, n * sizeof(double), hipMemcpyDeviceToHost,
+                   ctx->s);
+  if (pc && lev < ctx->nlev) {
+    Level &C = ctx->L[lev];
+    hipMemcpyAsync(pc, C.pc, 27 * C.d.n() * sizeof(double),
+                   hipMemcpyDeviceToHost, ctx->s);
+  }
+  return hipStreamSynchronize(ctx->s) == hipSuccess ? PMG_OK : PMG_EDEVICE;
+}
+
+int pmg_matvec(int nx, int ny, int nz, int numdia, const double *ac,
+               const double *cc, const double *x, double *y) {
+  if (!dims_ok(nx, ny, nz) || (numdia != 7 && numdia != 27) || !ac || !cc ||
+      !x || !y)
+    return PMG_EBADPARM;
+  if (!device_ok()) return PMG_EDEVICE;
+  Dim d{nx, ny, nz};
+  size_t n = d.n();
+  Scratch S;
+  double *dac = S.put(ac, (numdia == 7 ? 4 : 14) * n), *dcc = S.put(cc, n),
+         *dx = S.put(x, n), *dy = S.put(y, n);
+  if (!S.ok) return PMG_ENOMEM;
+  matvec_dots<double>(d, numdia, dac, dcc, dx, nullptr, dy, nullptr, 0);
+  return finish(S, y, dy, n);
+}
+
+int pmg_mresid(int nx, int ny, int nz, int numdia, const double *ac,
+               const double *cc, const double *fc, const double *x,
+               double *r) {
+  if (!dims_ok(nx, ny, nz) || (numdia != 7 && numdia != 27) || !ac || !cc ||
+      !fc || !x || !r)
+    return PMG_EBADPARM;
+  if (!device_ok()) return PMG_EDEVICE;
+  Dim d{nx, ny, nz};
+  size_t n = d.n();
+  Scratch S;
+  double *dac = S.put(ac, (numdia == 7 ? 4 : 14) * n), *dcc = S.put(cc, n),
+         *dfc = S.put(fc, n), *dx = S.put(x, n), *dr = S.put(r, n);
+  if (!S.ok) return PMG_ENOMEM;
+  resid<double, double>(d, numdia, dac, dcc, dfc, dx, dr, nullptr, 0);
+  return finish(S, r, dr, n);
+}
+
+int pmg_gsrb(int nx, int ny, int nz, int numdia, const double *ac,
+             const double *cc, const double *fc, double *x, int itmax,
+             int iadjoint) {
+  if (!dims_ok(nx, ny, nz) || (numdia != 7 && numdia != 27) || !ac || !cc ||
+      !fc || !x || itmax < 0 || (iadjoint != 0 && iadjoint != 1))
+    return PMG_EBADPARM;
+  if (!device_ok()) return PMG_EDEVICE;
+  Dim d{nx, ny, nz};
+  size_t n = d.n();
+  Scratch S;
+  double *dac = S.put(ac, (numdia == 7 ? 4 : 14) * n), *dcc = S.put(cc, n),
+         *dfc = S.put(fc, n), *dx = S.put(x, n);
+  if (!S.ok) return PMG_ENOMEM;
+  if (numdia == 7)
+    gs7<double>(d, dac, dcc, dfc, dx, itmax, iadjoint, 0);
+  else
+    gs27<double>(d, dac, dcc, dfc, dx, itmax, iadjoint, 0);
+  return finish(S, x, dx, n);
+}
+
+int pmg_restrc(int nxf, int nyf, int nzf, int nxc, int nyc, int nzc,
+               const double *xin, double *xout, const double *pc) {
+  if (!dims_ok(nxc, nyc, nzc) || !coarse_of(nxf, nxc) ||
+      !coarse_of(nyf, nyc) || !coarse_of(nzf, nzc) || !xin || !xout || !pc)
+    return PMG_EBADPARM;
+  if (!device_ok()) return PMG_EDEVICE;
+  Dim f{nxf, nyf, nzf}, c{nxc, nyc, nzc};
+  Scratch S;
+  double *dxi = S.put(xin, f.n()), *dxo = S.put(xout, c.n()),
+         *dpc = S.put(pc, 27 * c.n());
+  if (!S.ok) return PMG_ENOMEM;
+  restrict_<double>(f, c, dxi, dxo, dpc, 0);
+  return finish(S, xout, dxo, c.n());
+}
+
+int pmg_interp(int nxc, int nyc, int nzc, int nxf, int nyf, int nzf,
+               const double *xin, double *xout, const double *pc) {
+  if (!dims_ok(nxc, nyc, nzc) || !coarse_of(nxf, nxc) ||
+      !coarse_of(nyf, nyc) || !coarse_of(nzf, nzc) || !xin || !xout || !pc)
+    return PMG_EBADPARM;
+  if (!device_ok()) return PMG_EDEVICE;
+  Dim f{nxf, nyf, nzf}, c{nxc, nyc, nzc};
+  Scratch S;
+  double *dxi = S.put(xin, c.n()), *dxo = S.put(xout, f.n()),
+         *dpc = S.put(pc, 27 * c.n());
+  if (!S.ok) return PMG_ENOMEM;
+  interp<double>(c, f, dxi, dpc, dxo, nullptr, 0);
+  return finish(S, xout, dxo, f.n());
+}
+
+int pmg_buildP(int nxf, int nyf, int nzf, int nxc, int nyc, int nzc,
+               int mgprol, int numdia, const double *acf, double *pc) {
+  if (!dims_ok(nxc, nyc, nzc) || !coarse_of(nxf, nxc) ||
+      !coarse_of(nyf, nyc) || !coarse_of(nzf, nzc) ||
+      (numdia != 7 && numdia != 27) || (mgprol != 0 && mgprol != 1) || !acf ||
+      !pc)
+    return PMG_EBADPARM;
+  if (!device_ok()) return PMG_EDEVICE;
+  Dim f{nxf, nyf, nzf}, c{nxc, nyc, nzc};
+  Scratch S;
+  double *dac = S.put(acf, (numdia == 7 ? 4 : 14) * f.n()),
+         *dpc = S.put(pc, 27 * c.n());
+  if (!S.ok) return PMG_ENOMEM;
+  buildP(f, c, mgprol, numdia, dac, dpc, 0);
+  return finish(S, pc, dpc, 27 * c.n());
+}
+
+int pmg_buildG(int nxf, int nyf, int nzf, int nxc, int nyc, int nzc,
+               int numdia, const double *pc, const double *acf, double *acc) {
+  if (!dims_ok(nxc, nyc, nzc) || !coarse_of(nxf, nxc) ||
+      !coarse_of(nyf, nyc) || !coarse_of(nzf, nzc) ||
+      (numdia != 7 && numdia != 27) || !acf || !pc || !acc)
+    return PMG_EBADPARM;
+  if (!device_ok()) return PMG_EDEVICE;
+  Dim f{nxf, nyf, nzf}, c{nxc, nyc, nzc};
+  Scratch S;
+  double *dpc = S.put(pc, 27 * c.n()),
+         *dac = S.put(acf, (numdia == 7 ? 4 : 14) * f.n()),
+         *dacc = S.put(acc, 14 * c.n());
+  if (!S.ok) return PMG_ENOMEM;
+  buildG(f, c, numdia, dpc, dac, dacc, 0);
+  return finish(S, acc, dacc, 14 * c.n());
+}
+
+const char *pmg_version(void) { return "pmg_mi355x 0.1 (gfx950)"; }
+
+// Kernel timing hooks used by the benchmark: when enabled, the fine-level
+// smoothing launches of each solve are bracketed by HIP events on the
+// context stream; pmg_ctx_ktime returns their accumulated device time and
+// the number of full sweeps timed.
+int pmg_ctx_ktime_enable(pmg_ctx *ctx, int max_events) {
+  if (!ctx || max_events < 2) return PMG_EBADPARM;
+  for (auto e : ctx->ev) hipEventDestroy(e);
+  ctx->ev.assign((size_t)max_events, nullptr);
+  for (auto &e : ctx->ev) hipEventCreate(&e);
+  ctx->ktime = true;
+  ctx->ktime_ms = 0.0;
+  ctx->ktime_count = 0;
+  return PMG_OK;
+}
+
+int pmg_ctx_ktime(pmg_ctx *ctx, double *ms, long long *sweeps) {
+  if (!ctx) return PMG_EBADPARM;
+  if (ms) *ms = ctx->ktime_ms;
+  if (sweeps) *sweeps = ctx->ktime_count;
+  return PMG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,196 @@
+"""GPU parity: every HIP kernel and driver against the CPU oracle, through
+the C ABI.  Tolerances: single kernels 1e-13 relative (FMA contraction is
+the only difference); potential 1e-10 relative max-norm with identical
+V-cycle count; Galerkin coarse operators 1e-12 relative."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    s = max(np.max(np.abs(b)), 1e-300)
+    return np.max(np.abs(a - b)) / s
+
+
+def _call(mg, name, *args):
+    rc = getattr(mg.lib(), name)(*args)
+    assert rc == 0, f"{name} returned {rc}"
+
+
+dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+@pytest.mark.parametrize("numdia", [7, 27])
+@pytest.mark.parametrize("n", [9, 17])
+def test_matvec_resid(mg, orc, n, numdia):
+    print(f"matvec/resid n={n} numdia={numdia}", flush=True)
+    rng = np.random.default_rng(n + numdia)
+    ac, cc = H.random_op(n, numdia, rng)
+    x = H.from_int(n, rng.standard_normal((n - 2) ** 3))
+    f = H.from_int(n, rng.standard_normal((n - 2) ** 3))
+    y = np.zeros(n ** 3)
+    _call(mg, "pmg_matvec", n, n, n, numdia, dp(ac), dp(cc), dp(x), dp(y))
+    assert _rel(y, orc.matvec(n, numdia, ac, cc, x)) < 1e-13
+    r = np.zeros(n ** 3)
+    _call(mg, "pmg_mresid", n, n, n, numdia, dp(ac), dp(cc), dp(f), dp(x), dp(r))
+    assert _rel(r, orc.mresid(n, numdia, ac, cc, f, x)) < 1e-13
+
+
+@pytest.mark.parametrize("numdia", [7, 27])
+@pytest.mark.parametrize("iadj", [0, 1])
+def test_gsrb(mg, orc, numdia, iadj):
+    print(f"gsrb numdia={numdia} iadjoint={iadj}", flush=True)
+    n = 17
+    rng = np.random.default_rng(numdia * 10 + iadj)
+    ac, cc = H.random_op(n, numdia, rng)
+    f = H.from_int(n, rng.standard_normal((n - 2) ** 3))
+    x0 = H.from_int(n, rng.standard_normal((n - 2) ** 3))
+    x = x0.copy()
+    _call(mg, "pmg_gsrb", n, n, n, numdia, dp(ac), dp(cc), dp(f), dp(x), 3, iadj)
+    assert _rel(x, orc.gsrb(n, numdia, ac, cc, f, x0, 3, iadj)) < 1e-12
+
+
+@pytest.mark.parametrize("mgprol", [0, 1])
+@pytest.mark.parametrize("numdia", [7, 27])
+def test_buildP_buildG_transfer(mg, orc, numdia, mgprol):
+    print(f"buildP/buildG numdia={numdia} mgprol={mgprol}", flush=True)
+    nf, nc = 17, 9
+    rng = np.random.default_rng(numdia + 3 * mgprol)
+    ac, _ = H.random_op(nf, numdia, rng)
+    pc = np.zeros(27 * nc ** 3)
+    _call(mg, "pmg_buildP", nf, nf, nf, nc, nc, nc, mgprol, numdia, dp(ac), dp(pc))
+    pco = orc.buildP(nf, nc, mgprol, numdia, ac)
+    assert _rel(pc, pco) < 1e-13
+    acc = np.zeros(14 * nc ** 3)
+    _call(mg, "pmg_buildG", nf, nf, nf, nc, nc, nc, numdia, dp(pco), dp(ac), dp(acc))
+    assert _rel(acc, orc.buildG(nf, nc, numdia, pco, ac)) < 1e-12
+    r = H.from_int(nf, rng.standard_normal((nf - 2) ** 3))
+    xc = np.zeros(nc ** 3)
+    _call(mg, "pmg_restrc", nf, nf, nf, nc, nc, nc, dp(r), dp(xc), dp(pco))
+    assert _rel(xc, orc.restrc(nf, nc, r, pco)) < 1e-13
+    e = H.from_int(nc, rng.standard_normal((nc - 2) ** 3))
+    xf = np.zeros(nf ** 3)
+    _call(mg, "pmg_interp", nc, nc, nc, nf, nf, nf, dp(e), dp(xf), dp(pco))
+    assert _rel(xf, orc.interp(nc, nf, e, pco)) < 1e-13
+
+
+def _run_driver(mg, fn, inp, n, nlev, nonlin=False, **kw):
+    iparm, rparm = mg.make_params(n, nlev, nonlin=1 if nonlin else 0, **kw)
+    sz = mg.Vmgsz(2, 0, 0, n, n, n, nlev)
+    iw = np.zeros(sz["iintot"], np.int32)
+    rw = np.zeros(int(iparm[0]))
+    u = np.zeros(n ** 3)
+    a = {k: inp[k].copy() for k in ("xf", "yf", "zf", "gxcf", "gycf", "gzcf", "a1cf",
+                                     "a2cf", "a3cf", "ccf", "fcf", "tcf")}
+    ierr = fn(iparm, rparm, iw, rw, u, a["xf"], a["yf"], a["zf"], a["gxcf"], a["gycf"],
+              a["gzcf"], a["a1cf"], a["a2cf"], a["a3cf"], a["ccf"], a["fcf"], a["tcf"])
+    return ierr, u, iw, rw
+
+
+@pytest.mark.parametrize("n,nlev,natoms,R,seed", [(33, 3, 40, 6.0, 1), (65, 4, 200, 14.0, 20261202)])
+def test_mgdriv_parity(mg, orc, n, nlev, natoms, R, seed):
+    print(f"mgdriv parity n={n} nlev={nlev}", flush=True)
+    inp = mg.pbe_synth(n, 0.5, natoms, R, 0.15, 0, seed)
+    e1, u1, iw1, rw1 = _run_driver(mg, mg.Vmgdriv, inp, n, nlev)
+    it1 = mg.last_stats()[0]
+    e0, u0, iw0, rw0 = orc.solve(inp, n, nlev)
+    it0 = orc.last_stats()[0]
+    print(f"  iters gpu={it1} oracle={it0} potential rel err={_rel(u1, u0):.2e}", flush=True)
+    assert e1 == e0 == 0
+    assert it1 == it0
+    assert _rel(u1, u0) < 1e-10
+    assert np.array_equal(iw1, iw0)
+    # coarse operators (Galerkin) in the rwork layout: ac of levels >= 2
+    sz = mg.Vmgsz(2, 0, 0, n, n, n, nlev)
+    k_ac = sz["n_rpc"] + 2 * sz["narr"] + 27 * sz["narrc"]
+    fine = 4 * sz["nf"]
+    assert _rel(rw1[k_ac + fine:k_ac + fine + 14 * sz["narrc"]],
+                rw0[k_ac + fine:k_ac + fine + 14 * sz["narrc"]]) < 1e-12
+    assert _rel(rw1, rw0) < 1e-12
+
+
+def test_ctx_fp64_and_mixed(mg, orc):
+    print("ctx fp64 + mixed n=65", flush=True)
+    n, nlev = 65, 4
+    inp = mg.pbe_synth(n, 0.5, 200, 14.0, 0.15, 0, 20261202)
+    iparm, rparm = mg.make_params(n, nlev)
+    ctx = mg.Context(iparm, rparm, inp)
+    ctx.solve(mg.PMG_FP64)
+    u64, s64 = ctx.u(), ctx.stats()
+    ctx.solve(mg.PMG_MIXED)
+    umx, smx = ctx.u(), ctx.stats()
+    ctx.close()
+    _, u0, _, _ = orc.solve(inp, n, nlev)
+    it0 = orc.last_stats()[0]
+    print(f"  fp64 iters={s64['iters']} mixed iters={smx['iters']} oracle={it0} "
+          f"mixed err={_rel(umx, u0):.2e}", flush=True)
+    assert s64["iters"] == it0 and _rel(u64, u0) < 1e-10
+    assert smx["relres"] <= 1e-6 and smx["ierror"] == 0
+    assert _rel(umx, u0) < 1e-6
+
+
+def test_newdriv_parity(mg, orc):
+    print("newdriv parity n=33", flush=True)
+    n, nlev = 33, 3
+    ions = ([1.0, -1.0], [0.5, 0.5])
+    mg.Vmypdefinitnpbe(2, *ions)
+    orc.mypdefinit(*ions)
+    inp = mg.pbe_synth(n, 0.5, 40, 6.0, 0.3, 0, 5)
+    e1, u1, _, _ = _run_driver(mg, mg.Vnewdriv, inp, n, nlev, nonlin=True, errtol=1e-8)
+    it1 = mg.last_stats()[0]
+    e0, u0, _, _ = orc.solve(inp, n, nlev, nonlin=True, errtol=1e-8)
+    it0 = orc.last_stats()[0]
+    print(f"  newton iters gpu={it1} oracle={it0} err={_rel(u1, u0):.2e}", flush=True)
+    assert e1 == e0 == 0 and it1 == it0
+    assert _rel(u1, u0) < 1e-9
+
+
+def test_error_paths(mg):
+    print("error paths", flush=True)
+    n, nlev = 17, 3
+    inp = mg.pbe_synth(n, 0.5, 8, 2.0, 0.15, 0, 1)
+    # itmax exhausted -> ierror 1
+    e, _, _, _ = _run_driver(mg, mg.Vmgdriv, inp, n, nlev, itmax=1, errtol=1e-14)
+    assert e == 1 and mg.last_stats()[1] == 1
+    # unsupported smoother -> PMGError(EBADPARM)
+    with pytest.raises(mg.PMGError) as ex:
+        _run_driver(mg, mg.Vmgdriv, inp, n, nlev, mgsmoo=0)
+    assert ex.value.code == mg.PMG_EBADPARM
+    # zero right-hand side: rsden = 1 branch, u = 0 (pmgc/mgcsd.cc:154-157)
+    z = {k: (np.zeros_like(v) if k in ("fcf", "gxcf", "gycf", "gzcf") else v) for k, v in inp.items()}
+    e, u, _, _ = _run_driver(mg, mg.Vmgdriv, z, n, nlev)
+    assert e == 0 and np.all(u == 0.0)
+
+
+def test_full_size_c1_properties(mg, orc):
+    """161^3 / 5 levels: the FP64 GPU solution's L1 relative residual,
+    recomputed by the oracle on the host, meets 1e-6; the mixed-precision
+    solution agrees with it to 1e-6."""
+    print("c1 161^3 full-size properties", flush=True)
+    c, inp = mg.config_inputs("c1")
+    n, nlev = c["n"], c["nlev"]
+    iparm, rparm = mg.make_params(n, nlev)
+    ctx = mg.Context(iparm, rparm, inp)
+    ctx.solve(mg.PMG_FP64)
+    u64, s64 = ctx.u(), ctx.stats()
+    ctx.solve(mg.PMG_MIXED)
+    umx, smx = ctx.u(), ctx.stats()
+    ctx.close()
+    ac, cc, fc, _ = orc.buildA_fv(n, inp)
+    # residual of the interior unknowns (Dirichlet values are folded into fc)
+    ub = u64.copy().reshape(n, n, n)
+    ub[0, :, :] = ub[-1, :, :] = 0
+    ub[:, 0, :] = ub[:, -1, :] = 0
+    ub[:, :, 0] = ub[:, :, -1] = 0
+    r = orc.mresid(n, 7, ac, cc, fc, ub.ravel())
+    rel = np.abs(r).sum() / np.abs(fc).sum()
+    print(f"  fp64 iters={s64['iters']} relres={s64['relres']:.2e} host={rel:.2e} "
+          f"mixed iters={smx['iters']} diff={_rel(umx, u64):.2e}", flush=True)
+    assert s64["ierror"] == 0 and rel <= 1e-6 * 1.0001
+    assert abs(rel - s64["relres"]) <= 1e-9
+    assert _rel(umx, u64) < 1e-6
