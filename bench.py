@@ -91,15 +91,15 @@ def main():
     st = ctx.stats()
     err = float(np.max(np.abs(umx - u64)) / np.max(np.abs(u64)))
 
-    # roofline of the dominant kernel: fine-level red-black half-sweep (FP32
-    # in the mixed path).  Algorithmic bytes per fine interior point and
-    # half-sweep: own-colour oC, cc, fc (3 x 1/2) + oE, oN, uC (3) +
-    # other-colour x read (1/2) + own x write (1/2) = 5.5 arrays.
+    # roofline of the dominant kernel: the fine-level fused red-black sweep
+    # (FP32 in the mixed path), one launch per full sweep.  Algorithmic bytes
+    # per fine interior point of a two-colour fused sweep: oC, cc, fc, oE,
+    # oN, uC, x read, x write = 8 arrays.
     pts = (n - 2) ** 3
-    bytes_half = 5.5 * 4 * pts
-    launches = 2 * ksweeps
+    bytes_launch = 8 * 4 * pts
+    launches = ksweeps
     avg_launch_s = (kms / 1e3) / max(launches, 1)
-    achieved = bytes_half / avg_launch_s / 1e9 if launches else 0.0
+    achieved = bytes_launch / avg_launch_s / 1e9 if launches else 0.0
 
     ctx.close()
     cpu = None
@@ -133,7 +133,7 @@ def main():
         "setup_ms": st["setup_ms"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "fine RB-GS half-sweep (FP32)",
+                     "kernel": "k_gs7_sweep<float> (fine fused red-black sweep)",
                      "avg_launch_us": avg_launch_s * 1e6},
         "cpu_baseline": cpu,
     }

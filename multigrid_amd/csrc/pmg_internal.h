@@ -28,31 +28,48 @@ inline int corsr(int n) { return (n - 1) / 2 + 1; }
 // a level of dims d (the caller sizes the partials buffer with it).
 size_t max_partials(const Dim &d);
 
+// Destination of a reduction fused into the kernel that produces its terms:
+// per-block partials (2 per block at most), a block counter that is zero
+// between launches (the last block resets it), and the device scalar the
+// last block writes after summing the partials in block order.
+struct Red {
+  double *part = nullptr;
+  unsigned *ctr = nullptr;
+  double *out = nullptr;
+};
+
 // ---- smoothers (pmgc/gsd.cc) ----
 template <class T>
 void gs7(const Dim &d, const T *ac, const T *cc, const T *fc, T *x,
          int nsweeps, int iadjoint, hipStream_t s);
+// nsweeps full red-black sweeps, each one pass over memory (x and tmp
+// ping-pong; tmp must share x's boundary values).  Same result as gs7.
+template <class T>
+void gs7_fused(const Dim &d, const T *ac, const T *cc, const T *fc, T *x,
+               T *tmp, int nsweeps, int iadjoint, hipStream_t s);
+// zero_init: x starts from 0 (the caller need not clear it).
 template <class T>
 void gs27(const Dim &d, const T *ac, const T *cc, const T *fc, T *x,
-          int nsweeps, int iadjoint, hipStream_t s);
+          int nsweeps, int iadjoint, hipStream_t s, bool zero_init = false);
 
 // ---- residual r = f - A x (pmgc/matvecd.cc:381-528) ----
 // TO is the output type (FP64 operator may write an FP32 residual for the
-// mixed-precision inner solve).  If part != nullptr, per-block L1 partial
-// sums of the residual (in double) are written there; returns their count.
+// mixed-precision inner solve); r may be null.  If red != nullptr the L1
+// norm of the residual (in double) is written to red->out[0].
 template <class T, class TO>
-int resid(const Dim &d, int numdia, const T *ac, const T *cc, const T *fc,
-          const T *x, TO *r, double *part, hipStream_t s);
+void resid(const Dim &d, int numdia, const T *ac, const T *cc, const T *fc,
+           const T *x, TO *r, const Red *red, hipStream_t s);
 // Nonlinear residual r = f - A x - c_vec(cc, x) (pmgc/matvecd.cc:530-693),
 // 7-point fine operator only, FP64.
-int nmresid7(const Dim &d, const double *ac, const double *cc,
-             const double *fc, const double *x, double *r, double *part,
-             hipStream_t s);
+void nmresid7(const Dim &d, const double *ac, const double *cc,
+              const double *fc, const double *x, double *r, const Red *red,
+              hipStream_t s);
 
-// ---- y = A x, with partial sums of x.w0 and x.y (pmgc/mgcsd.cc:429-452) --
+// ---- y = A x; with red, red->out[0] = (x.w0) / (x.y), the Hackbusch-
+// Reusken damping of the coarse correction (pmgc/mgcsd.cc:429-452) ----
 template <class T>
-int matvec_dots(const Dim &d, int numdia, const T *ac, const T *cc,
-                const T *x, const T *w0, T *y, double *part, hipStream_t s);
+void matvec_dots(const Dim &d, int numdia, const T *ac, const T *cc,
+                 const T *x, const T *w0, T *y, const Red *red, hipStream_t s);
 
 // ---- grid transfer (pmgc/matvecd.cc:695-983) ----
 template <class T>
@@ -65,16 +82,9 @@ void interp(const Dim &c, const Dim &f, const T *xc, const T *pc, T *xf,
             const double *xdamp, hipStream_t s);
 
 // ---- reductions ----
-// mode 0: out[0] = sum part[b*stride]; mode 1: out[0] = sum part[b*stride]
-// / sum part[b*stride+1]; mode 2: out[0], out[1] = both sums.
-void reduce(const double *part, int nblk, int stride, int mode, double *out,
-            hipStream_t s);
-// Deterministic interior L1 norm / dot into part (returns block count).
+// Deterministic interior L1 norm into red.out[0].
 template <class T>
-int nrm1_part(const Dim &d, const T *x, double *part, hipStream_t s);
-template <class T>
-int dot_part(const Dim &d, const T *x, const T *y, double *part,
-             hipStream_t s);
+void nrm1(const Dim &d, const T *x, const Red &red, hipStream_t s);
 
 // ---- coarsest-level CG (pmgc/cgd.cc:60-124), one workgroup, FP64 ----
 // Returns false if the level is too large for the single-workgroup kernel.

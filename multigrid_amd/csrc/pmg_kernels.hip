@@ -52,6 +52,46 @@ __device__ inline double block_sum(double v, double *red) {
   return t;  // valid in thread 0
 }
 
+// Fused reduction epilogue.  v0 (and v1 when NV == 2) are this block's sums,
+// valid in thread 0.  The last block to arrive sums all partials in block
+// order (deterministic) and writes out[0] = sum0 (MODE 0) or sum0 / sum1
+// (MODE 1; 0 when sum1 == 0, a zero coarse correction), then resets the
+// counter so the next launch starts from zero.
+template <int NT, int NV, int MODE>
+__device__ inline void red_finish(const Red &r, double v0, double v1,
+                                  double *lds) {
+  __shared__ int last;
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  const unsigned nblk = gridDim.x * gridDim.y * gridDim.z;
+  const unsigned b =
+      blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  if (tid == 0) {
+    r.part[(size_t)b * NV] = v0;
+    if (NV == 2) r.part[(size_t)b * NV + 1] = v1;
+    __threadfence();
+    const unsigned t = atomicAdd(r.ctr, 1u);
+    last = (t == nblk - 1) ? 1 : 0;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  const volatile double *vp = r.part;
+  double a = 0.0, c = 0.0;
+  for (unsigned q = tid; q < nblk; q += NT) {
+    a += vp[(size_t)q * NV];
+    if (NV == 2) c += vp[(size_t)q * NV + 1];
+  }
+  const double sa = block_sum<NT>(a, lds);
+  const double sc = NV == 2 ? block_sum<NT>(c, lds) : 0.0;
+  if (tid == 0) {
+    if (MODE == 1)
+      r.out[0] = sc != 0.0 ? sa / sc : 0.0;
+    else
+      r.out[0] = sa;
+    atomicExch(r.ctr, 0u);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // 27-point neighbour sums in the reference's grouping (matvecd.cc:137-170)
 // ---------------------------------------------------------------------------
@@ -146,6 +186,96 @@ __global__ __launch_bounds__(256) void k_gs27(int nx, int ny, int nz,
   x[p] = (fc[p] + (T)((T)((T)sO + (T)sU) + (T)sD)) / (ac[p] + cc[p]);
 }
 
+// One full red-black sweep in a single pass: xout = black(red(xin)), the
+// two half-sweeps of Vgsrb7x (gsd.cc:81-110) in the reference colour order.
+// First-colour updates read only xin, so a workgroup can recompute them on
+// the one-point halo of its (i, j) tile; second-colour updates then read the
+// new first-colour values from LDS (in-plane) and registers (k +- 1).  The
+// workgroup marches a chunk of k planes; each thread owns one (i, j) column
+// and keeps the first-colour ("mid") values of planes k-1, k, k+1.  Three
+// rotating LDS planes need one barrier per plane.
+constexpr int SW_TX = 64, SW_TY = 8, SW_KC = 16;
+template <class T>
+__global__ __launch_bounds__(SW_TX *SW_TY) void k_gs7_sweep(
+    int nx, int ny, int nz, const T *__restrict__ ac, const T *__restrict__ cc,
+    const T *__restrict__ fc, const T *__restrict__ xin, T *__restrict__ xout,
+    int c1) {
+  __shared__ T M[3][SW_TY][SW_TX];
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int i = 1 + blockIdx.x * (SW_TX - 2) + tx;
+  const int j = 1 + blockIdx.y * (SW_TY - 2) + ty;
+  const int kb = 2 + blockIdx.z * SW_KC;
+  const int ke = min(kb + SW_KC - 1, nz - 1);
+  const size_t n = (size_t)nx * ny * nz, sy = nx, sz = (size_t)nx * ny;
+  const T *oE = ac + n, *oN = ac + 2 * n, *uC = ac + 3 * n;
+  const bool col = i <= nx && j <= ny;
+  const bool inner = i >= 2 && i <= nx - 1 && j >= 2 && j <= ny - 1;
+  const bool owned = inner && tx >= 1 && tx <= SW_TX - 2 && ty >= 1 &&
+                     ty <= SW_TY - 2;
+  const int c2 = 1 - c1;
+  const size_t pc0 = col ? ix(nx, ny, i, j, 1) : 0;  // column base, k = 1
+  auto X = [&](int k) -> T {  // xin on this column, 0 outside the grid
+    return (col && k >= 1 && k <= nz) ? xin[pc0 + (size_t)(k - 1) * sz] : (T)0;
+  };
+  // first-colour update of (i, j, k) from xin; xm/xp = xin at k-1 / k+1
+  auto mid = [&](int k, T xk, T xm, T xp) -> T {
+    if (!inner || k < 2 || k > nz - 1 || ((i + j + k) & 1) != c1) return xk;
+    const size_t p = pc0 + (size_t)(k - 1) * sz;
+    const T s = oN[p] * xin[p + sy] + oN[p - sy] * xin[p - sy] +
+                oE[p] * xin[p + 1] + oE[p - 1] * xin[p - 1] +
+                uC[p - sz] * xm + uC[p] * xp;
+    return (fc[p] + s) / (ac[p] + cc[p]);
+  };
+  T x0 = X(kb - 2), x1 = X(kb - 1), x2 = X(kb), x3 = X(kb + 1);
+  T mprev = mid(kb - 1, x1, x0, x2);  // plane kb-1
+  T mcur = mid(kb, x2, x1, x3);       // plane kb
+  M[kb % 3][ty][tx] = mcur;
+  for (int k = kb; k <= ke; k++) {
+    const T x4 = X(k + 2);
+    const T mnext = mid(k + 1, x3, x2, x4);
+    M[(k + 1) % 3][ty][tx] = mnext;
+    __syncthreads();
+    if (owned) {
+      T v = mcur;
+      if (((i + j + k) & 1) == c2) {
+        const size_t p = pc0 + (size_t)(k - 1) * sz;
+        const T(*L)[SW_TX] = M[k % 3];
+        const T s = oN[p] * L[ty + 1][tx] + oN[p - sy] * L[ty - 1][tx] +
+                    oE[p] * L[ty][tx + 1] + oE[p - 1] * L[ty][tx - 1] +
+                    uC[p - sz] * mprev + uC[p] * mnext;
+        v = (fc[p] + s) / (ac[p] + cc[p]);
+      }
+      xout[pc0 + (size_t)(k - 1) * sz] = v;
+    }
+    mprev = mcur;
+    mcur = mnext;
+    x2 = x3;
+    x3 = x4;
+  }
+}
+
+// nsweeps full sweeps, ping-ponging between x and tmp (same boundary
+// values); the result is left in x.
+template <class T>
+void gs7_fused(const Dim &d, const T *ac, const T *cc, const T *fc, T *x,
+               T *tmp, int nsweeps, int iadjoint, hipStream_t s) {
+  if (d.nx < 3 || d.ny < 3 || d.nz < 3 || nsweeps <= 0) return;
+  dim3 g(cdiv(d.nx - 2, SW_TX - 2), cdiv(d.ny - 2, SW_TY - 2),
+         cdiv(d.nz - 2, SW_KC));
+  dim3 b(SW_TX, SW_TY);
+  const int c1 = iadjoint ? 1 : 0;
+  T *src = x, *dst = tmp;
+  for (int it = 0; it < nsweeps; it++) {
+    k_gs7_sweep<T><<<g, b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, fc, src, dst, c1);
+    T *t = src;
+    src = dst;
+    dst = t;
+  }
+  if (src != x)
+    hipMemcpyAsync(x, src, d.n() * sizeof(T), hipMemcpyDeviceToDevice, s);
+  check_launch("gs7_sweep");
+}
+
 template <class T>
 void gs7(const Dim &d, const T *ac, const T *cc, const T *fc, T *x,
          int nsweeps, int iadjoint, hipStream_t s) {
@@ -161,10 +291,100 @@ void gs7(const Dim &d, const T *ac, const T *cc, const T *fc, T *x,
   check_launch("gs7");
 }
 
+// A whole 8-colour sweep in two launches.  Colours 0-3 live on even k
+// planes and 4-7 on odd ones ((i%2)+2(j%2)+4(k%2), gsd.cc:168), and a 27-point
+// stencil couples a plane only to itself and to k+-1.  So every plane of one
+// parity can be swept independently: one workgroup per plane keeps the plane
+// in LDS, runs its four colours in the reference order with a barrier after
+// each, and reads the (fixed) neighbouring planes from global memory.  Each
+// point sees exactly the neighbour values of the one-launch-per-colour
+// order, so the result is identical.
+// zero_stage: the plane starts from x = 0 (Vazeros before the sweep);
+// zero_nb: the k+-1 planes are also still zero (first phase of that sweep).
+template <class T, int NT>
+__global__ __launch_bounds__(NT) void k_gs27_plane(
+    int nx, int ny, int nz, const T *__restrict__ ac, const T *__restrict__ cc,
+    const T *__restrict__ fc, T *__restrict__ x, int pz, int iadj,
+    int zero_stage, int zero_nb) {
+  extern __shared__ unsigned char smem_raw[];
+  T *X = reinterpret_cast<T *>(smem_raw);
+  const int k = 2 + pz + 2 * (int)blockIdx.x;
+  if (k > nz - 1) return;
+  const int tid = threadIdx.x;
+  const size_t n = (size_t)nx * ny * nz, sy = nx, sz = (size_t)nx * ny;
+  const size_t plane = (size_t)nx * ny, base = (size_t)(k - 1) * plane;
+  for (size_t q = tid; q < plane; q += NT)
+    X[q] = zero_stage ? (T)0 : x[base + q];
+  __syncthreads();
+  const T *oE = ac + n, *oN = ac + 2 * n, *uC = ac + 3 * n, *oNE = ac + 4 * n,
+          *oNW = ac + 5 * n, *uE = ac + 6 * n, *uW = ac + 7 * n,
+          *uN = ac + 8 * n, *uS = ac + 9 * n, *uNE = ac + 10 * n,
+          *uNW = ac + 11 * n, *uSE = ac + 12 * n, *uSW = ac + 13 * n;
+  for (int st = 0; st < 4; st++) {
+    const int c = iadj ? 3 - st : st;
+    const int px = c & 1, py = (c >> 1) & 1;
+    const int ni = (nx - 2 - px + 1) / 2, nj = (ny - 2 - py + 1) / 2;
+    for (int t = tid; t < ni * nj; t += NT) {
+      const int i = 2 + px + 2 * (t % ni), j = 2 + py + 2 * (t / ni);
+      const size_t q = (size_t)(j - 1) * nx + (i - 1), p = base + q;
+      const T o = oN[p] * X[q + sy] + oN[p - sy] * X[q - sy] +
+                  oE[p] * X[q + 1] + oE[p - 1] * X[q - 1] +
+                  oNE[p] * X[q + 1 + sy] + oNW[p] * X[q - 1 + sy] +
+                  oNW[p + 1 - sy] * X[q + 1 - sy] +
+                  oNE[p - 1 - sy] * X[q - 1 - sy];
+      T u = (T)0, dd = (T)0;
+      if (!zero_nb) {
+        u = uC[p] * x[p + sz] + uN[p] * x[p + sy + sz] +
+            uS[p] * x[p - sy + sz] + uE[p] * x[p + 1 + sz] +
+            uW[p] * x[p - 1 + sz] + uNE[p] * x[p + 1 + sy + sz] +
+            uNW[p] * x[p - 1 + sy + sz] + uSE[p] * x[p + 1 - sy + sz] +
+            uSW[p] * x[p - 1 - sy + sz];
+        dd = uC[p - sz] * x[p - sz] + uS[p + sy - sz] * x[p + sy - sz] +
+             uN[p - sy - sz] * x[p - sy - sz] +
+             uW[p + 1 - sz] * x[p + 1 - sz] +
+             uE[p - 1 - sz] * x[p - 1 - sz] +
+             uSW[p + 1 + sy - sz] * x[p + 1 + sy - sz] +
+             uSE[p - 1 + sy - sz] * x[p - 1 + sy - sz] +
+             uNW[p + 1 - sy - sz] * x[p + 1 - sy - sz] +
+             uNE[p - 1 - sy - sz] * x[p - 1 - sy - sz];
+      }
+      X[q] = (fc[p] + ((o + u) + dd)) / (ac[p] + cc[p]);
+    }
+    __syncthreads();
+  }
+  for (size_t q = tid; q < plane; q += NT) {
+    const int i = 1 + (int)(q % nx), j = 1 + (int)(q / nx);
+    if (i >= 2 && i <= nx - 1 && j >= 2 && j <= ny - 1) x[base + q] = X[q];
+  }
+}
+
+constexpr size_t GS27_LDS_MAX = 64 * 1024;
+
 template <class T>
 void gs27(const Dim &d, const T *ac, const T *cc, const T *fc, T *x,
-          int nsweeps, int iadjoint, hipStream_t s) {
+          int nsweeps, int iadjoint, hipStream_t s, bool zero_init) {
   if (d.nx < 3 || d.ny < 3 || d.nz < 3) return;
+  const size_t lds = (size_t)d.nx * d.ny * sizeof(T);
+  if (lds <= GS27_LDS_MAX) {
+    const int per_colour = (d.nx - 2) * (d.ny - 2) / 4;
+    for (int it = 0; it < nsweeps; it++)
+      for (int ph = 0; ph < 2; ph++) {
+        const int pz = iadjoint ? 1 - ph : ph;
+        const int nplanes = pz == 0 ? (d.nz - 1) / 2 : (d.nz - 2) / 2;
+        if (nplanes <= 0) continue;
+        const int zs = (zero_init && it == 0) ? 1 : 0;
+        const int zn = (zs && ph == 0) ? 1 : 0;
+        if (per_colour > 256)
+          k_gs27_plane<T, 1024><<<nplanes, 1024, lds, s>>>(
+              d.nx, d.ny, d.nz, ac, cc, fc, x, pz, iadjoint, zs, zn);
+        else
+          k_gs27_plane<T, 256><<<nplanes, 256, lds, s>>>(
+              d.nx, d.ny, d.nz, ac, cc, fc, x, pz, iadjoint, zs, zn);
+      }
+    check_launch("gs27_plane");
+    return;
+  }
+  if (zero_init) fill<T>(x, d.n(), (T)0, s);
   dim3 b(BX, BY);
   for (int it = 0; it < nsweeps; it++)
     for (int c = 0; c < 8; c++) {
@@ -187,7 +407,7 @@ template <class T, class TO, int ND>
 __global__ __launch_bounds__(256) void k_resid(
     int nx, int ny, int nz, const T *__restrict__ ac,
     const T *__restrict__ cc, const T *__restrict__ fc,
-    const T *__restrict__ x, TO *__restrict__ r, double *__restrict__ part) {
+    const T *__restrict__ x, TO *__restrict__ r, Red rd, int do_red) {
   __shared__ double red[4];
   const size_t n = (size_t)nx * ny * nz, sy = nx, sz = (size_t)nx * ny;
   const int i = 2 + blockIdx.x * BX + threadIdx.x;
@@ -216,27 +436,25 @@ __global__ __launch_bounds__(256) void k_resid(
       acc += fabs((double)v);
     }
   }
-  if (part) {
-    double t = block_sum<256>(acc, red);
-    if (threadIdx.x == 0 && threadIdx.y == 0)
-      part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] =
-          t;
+  if (do_red) {
+    const double t = block_sum<256>(acc, red);
+    red_finish<256, 1, 0>(rd, t, 0.0, red);
   }
 }
 
 template <class T, class TO>
-int resid(const Dim &d, int numdia, const T *ac, const T *cc, const T *fc,
-          const T *x, TO *r, double *part, hipStream_t s) {
+void resid(const Dim &d, int numdia, const T *ac, const T *cc, const T *fc,
+           const T *x, TO *r, const Red *red, hipStream_t s) {
   dim3 g(cdiv(d.nx - 2, BX), cdiv(d.ny - 2, BY), cdiv(d.nz - 2, KCH));
   dim3 b(BX, BY);
+  const Red rd = red ? *red : Red{};
   if (numdia == 7)
-    k_resid<T, TO, 7><<<g, b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, fc, x, r,
-                                      part);
+    k_resid<T, TO, 7><<<g, b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, fc, x, r, rd,
+                                      red ? 1 : 0);
   else
-    k_resid<T, TO, 27><<<g, b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, fc, x, r,
-                                       part);
+    k_resid<T, TO, 27><<<g, b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, fc, x, r, rd,
+                                       red ? 1 : 0);
   check_launch("resid");
-  return (int)(g.x * g.y * g.z);
 }
 
 // ---------------------------------------------------------------------------
@@ -291,8 +509,8 @@ void dc_vec(const double *coef, const double *u, double *out, size_t n,
 __global__ __launch_bounds__(256) void k_nmresid7(
     int nx, int ny, int nz, const double *__restrict__ ac,
     const double *__restrict__ cc, const double *__restrict__ fc,
-    const double *__restrict__ x, double *__restrict__ r,
-    double *__restrict__ part) {
+    const double *__restrict__ x, double *__restrict__ r, Red rd,
+    int do_red) {
   __shared__ double red[4];
   const size_t n = (size_t)nx * ny * nz, sy = nx, sz = (size_t)nx * ny;
   const int i = 2 + blockIdx.x * BX + threadIdx.x;
@@ -314,22 +532,20 @@ __global__ __launch_bounds__(256) void k_nmresid7(
       acc += fabs(v);
     }
   }
-  if (part) {
-    double t = block_sum<256>(acc, red);
-    if (threadIdx.x == 0 && threadIdx.y == 0)
-      part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] =
-          t;
+  if (do_red) {
+    const double t = block_sum<256>(acc, red);
+    red_finish<256, 1, 0>(rd, t, 0.0, red);
   }
 }
 
-int nmresid7(const Dim &d, const double *ac, const double *cc,
-             const double *fc, const double *x, double *r, double *part,
-             hipStream_t s) {
+void nmresid7(const Dim &d, const double *ac, const double *cc,
+              const double *fc, const double *x, double *r, const Red *red,
+              hipStream_t s) {
   dim3 g(cdiv(d.nx - 2, BX), cdiv(d.ny - 2, BY), cdiv(d.nz - 2, KCH));
+  const Red rd = red ? *red : Red{};
   k_nmresid7<<<g, dim3(BX, BY), 0, s>>>(d.nx, d.ny, d.nz, ac, cc, fc, x, r,
-                                        part);
+                                        rd, red ? 1 : 0);
   check_launch("nmresid7");
-  return (int)(g.x * g.y * g.z);
 }
 
 // ---------------------------------------------------------------------------
@@ -339,7 +555,7 @@ template <class T, int ND>
 __global__ __launch_bounds__(256) void k_matvec_dots(
     int nx, int ny, int nz, const T *__restrict__ ac,
     const T *__restrict__ cc, const T *__restrict__ x,
-    const T *__restrict__ w0, T *__restrict__ y, double *__restrict__ part) {
+    const T *__restrict__ w0, T *__restrict__ y, Red rd, int do_red) {
   __shared__ double red[4];
   const size_t n = (size_t)nx * ny * nz, sy = nx, sz = (size_t)nx * ny;
   const int i = 2 + blockIdx.x * BX + threadIdx.x;
@@ -367,30 +583,27 @@ __global__ __launch_bounds__(256) void k_matvec_dots(
       a1 += (double)x[p] * (double)v;
     }
   }
-  if (part) {
-    double t0 = block_sum<256>(a0, red);
-    double t1 = block_sum<256>(a1, red);
-    if (threadIdx.x == 0 && threadIdx.y == 0) {
-      size_t b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-      part[2 * b] = t0;
-      part[2 * b + 1] = t1;
-    }
+  if (do_red) {
+    const double t0 = block_sum<256>(a0, red);
+    const double t1 = block_sum<256>(a1, red);
+    red_finish<256, 2, 1>(rd, t0, t1, red);
   }
 }
 
 template <class T>
-int matvec_dots(const Dim &d, int numdia, const T *ac, const T *cc,
-                const T *x, const T *w0, T *y, double *part, hipStream_t s) {
+void matvec_dots(const Dim &d, int numdia, const T *ac, const T *cc,
+                 const T *x, const T *w0, T *y, const Red *red,
+                 hipStream_t s) {
   dim3 g(cdiv(d.nx - 2, BX), cdiv(d.ny - 2, BY), cdiv(d.nz - 2, KCH));
   dim3 b(BX, BY);
+  const Red rd = red ? *red : Red{};
   if (numdia == 7)
     k_matvec_dots<T, 7><<<g, b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, x, w0, y,
-                                        part);
+                                        rd, red ? 1 : 0);
   else
     k_matvec_dots<T, 27><<<g, b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, x, w0, y,
-                                         part);
+                                         rd, red ? 1 : 0);
   check_launch("matvec_dots");
-  return (int)(g.x * g.y * g.z);
 }
 
 // ---------------------------------------------------------------------------
@@ -531,39 +744,10 @@ void interp(const Dim &c, const Dim &f, const T *xc, const T *pc, T *xf,
 // ---------------------------------------------------------------------------
 // Reductions
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void k_reduce(const double *part, int nblk,
-                                                 int stride, int mode,
-                                                 double *out) {
-  __shared__ double red[16];
-  double a = 0.0, b = 0.0;
-  for (int t = threadIdx.x; t < nblk; t += 1024) {
-    a += part[(size_t)t * stride];
-    if (mode) b += part[(size_t)t * stride + 1];
-  }
-  // block_sum indexes by threadIdx.x + threadIdx.y*blockDim.x
-  double sa = block_sum<1024>(a, red);
-  double sb = mode ? block_sum<1024>(b, red) : 0.0;
-  if (threadIdx.x == 0) {
-    if (mode == 1) {
-      out[0] = sb != 0.0 ? sa / sb : 0.0;  // zero correction: xdamp = 0
-    } else {
-      out[0] = sa;
-      if (mode == 2) out[1] = sb;
-    }
-  }
-}
-
-void reduce(const double *part, int nblk, int stride, int mode, double *out,
-            hipStream_t s) {
-  k_reduce<<<1, 1024, 0, s>>>(part, nblk, stride, mode, out);
-  check_launch("reduce");
-}
-
-template <class T, int MODE>
-__global__ __launch_bounds__(256) void k_norm_part(int nx, int ny, int nz,
-                                                   const T *__restrict__ x,
-                                                   const T *__restrict__ y,
-                                                   double *part) {
+template <class T>
+__global__ __launch_bounds__(256) void k_nrm1(int nx, int ny, int nz,
+                                              const T *__restrict__ x,
+                                              Red rd) {
   __shared__ double red[4];
   const int i = 2 + blockIdx.x * BX + threadIdx.x;
   const int j = 2 + blockIdx.y * BY + threadIdx.y;
@@ -573,32 +757,17 @@ __global__ __launch_bounds__(256) void k_norm_part(int nx, int ny, int nz,
     for (int kk = 0; kk < KCH; kk++) {
       const int k = k0 + kk;
       if (k > nz - 1) break;
-      const size_t p = ix(nx, ny, i, j, k);
-      if (MODE == 0)
-        acc += fabs((double)x[p]);
-      else
-        acc += (double)x[p] * (double)y[p];
+      acc += fabs((double)x[ix(nx, ny, i, j, k)]);
     }
-  double t = block_sum<256>(acc, red);
-  if (threadIdx.x == 0 && threadIdx.y == 0)
-    part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = t;
+  const double t = block_sum<256>(acc, red);
+  red_finish<256, 1, 0>(rd, t, 0.0, red);
 }
 
 template <class T>
-int nrm1_part(const Dim &d, const T *x, double *part, hipStream_t s) {
+void nrm1(const Dim &d, const T *x, const Red &red, hipStream_t s) {
   dim3 g(cdiv(d.nx - 2, BX), cdiv(d.ny - 2, BY), cdiv(d.nz - 2, KCH));
-  k_norm_part<T, 0><<<g, dim3(BX, BY), 0, s>>>(d.nx, d.ny, d.nz, x, x, part);
+  k_nrm1<T><<<g, dim3(BX, BY), 0, s>>>(d.nx, d.ny, d.nz, x, red);
   check_launch("nrm1");
-  return (int)(g.x * g.y * g.z);
-}
-
-template <class T>
-int dot_part(const Dim &d, const T *x, const T *y, double *part,
-             hipStream_t s) {
-  dim3 g(cdiv(d.nx - 2, BX), cdiv(d.ny - 2, BY), cdiv(d.nz - 2, KCH));
-  k_norm_part<T, 1><<<g, dim3(BX, BY), 0, s>>>(d.nx, d.ny, d.nz, x, y, part);
-  check_launch("dot");
-  return (int)(g.x * g.y * g.z);
 }
 
 // ---------------------------------------------------------------------------
@@ -1187,18 +1356,18 @@ void buildG(const Dim &f, const Dim &c, int numdia, const double *pc,
 #define INST(T)                                                               \
   template void gs7<T>(const Dim &, const T *, const T *, const T *, T *,     \
                        int, int, hipStream_t);                                \
+  template void gs7_fused<T>(const Dim &, const T *, const T *, const T *,    \
+                             T *, T *, int, int, hipStream_t);                \
   template void gs27<T>(const Dim &, const T *, const T *, const T *, T *,    \
-                        int, int, hipStream_t);                               \
-  template int matvec_dots<T>(const Dim &, int, const T *, const T *,         \
-                              const T *, const T *, T *, double *,            \
-                              hipStream_t);                                   \
+                        int, int, hipStream_t, bool);                         \
+  template void matvec_dots<T>(const Dim &, int, const T *, const T *,        \
+                               const T *, const T *, T *, const Red *,        \
+                               hipStream_t);                                  \
   template void restrict_<T>(const Dim &, const Dim &, const T *, T *,        \
                              const T *, hipStream_t);                         \
   template void interp<T>(const Dim &, const Dim &, const T *, const T *,     \
                           T *, const double *, hipStream_t);                  \
-  template int nrm1_part<T>(const Dim &, const T *, double *, hipStream_t);   \
-  template int dot_part<T>(const Dim &, const T *, const T *, double *,       \
-                           hipStream_t);                                      \
+  template void nrm1<T>(const Dim &, const T *, const Red &, hipStream_t);     \
   template void cg_single<T>(const Dim &, int, const double *,                \
                              const double *, const T *, T *, int, double,     \
                              hipStream_t);                                    \
@@ -1210,17 +1379,17 @@ void buildG(const Dim &f, const Dim &c, int numdia, const double *pc,
 INST(double)
 INST(float)
 #undef INST
-template int resid<double, double>(const Dim &, int, const double *,
+template void resid<double, double>(const Dim &, int, const double *,
+                                    const double *, const double *,
+                                    const double *, double *, const Red *,
+                                    hipStream_t);
+template void resid<double, float>(const Dim &, int, const double *,
                                    const double *, const double *,
-                                   const double *, double *, double *,
+                                   const double *, float *, const Red *,
                                    hipStream_t);
-template int resid<double, float>(const Dim &, int, const double *,
-                                  const double *, const double *,
-                                  const double *, float *, double *,
-                                  hipStream_t);
-template int resid<float, float>(const Dim &, int, const float *,
-                                 const float *, const float *, const float *,
-                                 float *, double *, hipStream_t);
+template void resid<float, float>(const Dim &, int, const float *,
+                                  const float *, const float *, const float *,
+                                  float *, const Red *, hipStream_t);
 template void convert<double, float>(const double *, float *, size_t,
                                      hipStream_t);
 template void convert<float, double>(const float *, double *, size_t,

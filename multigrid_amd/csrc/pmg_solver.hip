@@ -140,6 +140,7 @@ struct pmg_ctx {
   double *u = nullptr;       // fine solution (FP64)
   double *work[4] = {};      // fine FP64 scratch (Newton)
   double *part = nullptr;    // reduction partials
+  unsigned *ctr = nullptr;   // reduction block counter (zero between uses)
   double *scal = nullptr;    // device scalars
   double *hscal = nullptr;   // pinned host mirror
   bool have32 = false, haveN = false;
@@ -187,9 +188,16 @@ int device_ok() {
   return 1;
 }
 
-// reduce partials to a device scalar and read it back
-double host_scalar(pmg_ctx *c, int nb, int slot) {
-  reduce(c->part, nb, 1, 0, c->scal + slot, c->s);
+Red red_to(pmg_ctx *c, int slot) {
+  Red r;
+  r.part = c->part;
+  r.ctr = c->ctr;
+  r.out = c->scal + slot;
+  return r;
+}
+
+// read a device scalar back (waits for the stream)
+double host_scalar(pmg_ctx *c, int slot) {
   hipMemcpyAsync(c->hscal + slot, c->scal + slot, sizeof(double),
                  hipMemcpyDeviceToHost, c->s);
   hipStreamSynchronize(c->s);
@@ -209,14 +217,22 @@ LV<float> view<float>(Level &l) {
 
 template <class T>
 void smooth(pmg_ctx *c, const LV<T> &l, const T *f, T *x, int nu, int iadj,
-            bool fine) {
-  if (nu <= 0) return;
+            bool fine, bool zero_init = false) {
+  if (nu <= 0) {
+    if (zero_init) fill<T>(x, l.d.n(), (T)0, c->s);
+    return;
+  }
   bool timed = c->ktime && fine && c->nev + 2 <= (int)c->ev.size();
   if (timed) hipEventRecord(c->ev[c->nev], c->s);
-  if (l.numdia == 7)
-    gs7<T>(l.d, l.ac, l.cc, f, x, nu, iadj, c->s);
-  else
-    gs27<T>(l.d, l.ac, l.cc, f, x, nu, iadj, c->s);
+  if (l.numdia == 7) {
+    if (zero_init) fill<T>(x, l.d.n(), (T)0, c->s);
+    if (l.t && l.t != x)
+      gs7_fused<T>(l.d, l.ac, l.cc, f, x, l.t, nu, iadj, c->s);
+    else
+      gs7<T>(l.d, l.ac, l.cc, f, x, nu, iadj, c->s);
+  } else {
+    gs27<T>(l.d, l.ac, l.cc, f, x, nu, iadj, c->s, zero_init);
+  }
   if (timed) {
     hipEventRecord(c->ev[c->nev + 1], c->s);
     c->nev += 2;
@@ -240,8 +256,8 @@ void vcycle(pmg_ctx *c, LV<T> *L, const T *frhs, const double *ac64c,
   for (int l = 1; l < nlev; l++) {  // down (mgcsd.cc:325-360)
     restrict_<T>(L[l - 1].d, L[l].d, L[l - 1].r, L[l].w0, L[l].pc, s);
     if (l != nlev - 1) {
-      fill<T>(L[l].x, L[l].d.n(), (T)0, s);
-      smooth<T>(c, L[l], L[l].w0, L[l].x, p.nu1, 0, false);
+      // Vazeros + nu1 smoothings (zero start folded into the sweep)
+      smooth<T>(c, L[l], L[l].w0, L[l].x, p.nu1, 0, false, true);
       resid<T, T>(L[l].d, L[l].numdia, L[l].ac, L[l].cc, L[l].w0, L[l].x,
                   L[l].r, nullptr, s);
     }
@@ -254,9 +270,9 @@ void vcycle(pmg_ctx *c, LV<T> *L, const T *frhs, const double *ac64c,
   }
   for (int l = nlev - 2; l >= 0; l--) {  // up (mgcsd.cc:418-473)
     const int cl = l + 1;
-    int nb = matvec_dots<T>(L[cl].d, L[cl].numdia, L[cl].ac, L[cl].cc,
-                            L[cl].x, L[cl].w0, L[cl].t, c->part, s);
-    reduce(c->part, nb, 2, 1, c->scal + S_XDAMP, s);
+    const Red rd = red_to(c, S_XDAMP);
+    matvec_dots<T>(L[cl].d, L[cl].numdia, L[cl].ac, L[cl].cc, L[cl].x,
+                   L[cl].w0, L[cl].t, &rd, s);
     interp<T>(L[cl].d, L[l].d, L[cl].x, L[cl].pc, L[l].x, c->scal + S_XDAMP,
               s);
     smooth<T>(c, L[l], l == 0 ? frhs : L[l].w0, L[l].x, p.nu2, 1, l == 0);
@@ -271,9 +287,10 @@ int linear_fp64(pmg_ctx *c, LV<double> *L, const double *frhs,
                 int *iters_out, double *rel_out) {
   Level &F = c->L[0];
   double rsden = 1.0;
+  const Red rden = red_to(c, S_RSDEN), rnrm = red_to(c, S_RSNRM);
   if (istop == 1) {
-    int nb = nrm1_part<double>(F.d, frhs, c->part, c->s);
-    rsden = host_scalar(c, nb, S_RSDEN);
+    nrm1<double>(F.d, frhs, rden, c->s);
+    rsden = host_scalar(c, S_RSDEN);
   }
   if (rsden == 0.0) rsden = 1.0;
   double rsnrm = rsden;
@@ -281,9 +298,9 @@ int linear_fp64(pmg_ctx *c, LV<double> *L, const double *frhs,
   if (nlev == 1) {  // direct coarse solve on the only level (mgcsd.cc:170-291)
     cg_single<double>(F.d, F.numdia, F.ac, L[0].cc, frhs, L[0].x, 100, 0.0,
                       c->s);
-    int nb = resid<double, double>(F.d, F.numdia, F.ac, L[0].cc, frhs,
-                                   L[0].x, nullptr, c->part, c->s);
-    rsnrm = host_scalar(c, nb, S_RSNRM);
+    resid<double, double>(F.d, F.numdia, F.ac, L[0].cc, frhs, L[0].x,
+                          (double *)nullptr, &rnrm, c->s);
+    rsnrm = host_scalar(c, S_RSNRM);
     *iters_out = 1;
     *rel_out = rsnrm / rsden;
     return PMG_OK;
@@ -292,9 +309,9 @@ int linear_fp64(pmg_ctx *c, LV<double> *L, const double *frhs,
   do {
     vcycle<double>(c, L, frhs, c->L[nlev - 1].ac, cc64c);
     iters++;
-    int nb = resid<double, double>(F.d, F.numdia, F.ac, L[0].cc, frhs, L[0].x,
-                                   nullptr, c->part, c->s);
-    rsnrm = host_scalar(c, nb, S_RSNRM);
+    resid<double, double>(F.d, F.numdia, F.ac, L[0].cc, frhs, L[0].x,
+                          (double *)nullptr, &rnrm, c->s);
+    rsnrm = host_scalar(c, S_RSNRM);
   } while (iters < itmax && rsnrm / rsden > errtol);
   *iters_out = iters;
   *rel_out = rsnrm / rsden;
@@ -311,15 +328,16 @@ int linear_mixed(pmg_ctx *c, LV<double> *L64, LV<float> *L32,
                  double *rel_out) {
   Level &F = c->L[0];
   double rsden = 1.0;
+  const Red rden = red_to(c, S_RSDEN), rnrm = red_to(c, S_RSNRM);
   if (istop == 1) {
-    int nb = nrm1_part<double>(F.d, frhs, c->part, c->s);
-    rsden = host_scalar(c, nb, S_RSDEN);
+    nrm1<double>(F.d, frhs, rden, c->s);
+    rsden = host_scalar(c, S_RSDEN);
   }
   if (rsden == 0.0) rsden = 1.0;
   float *r32 = F.w032;  // fine right-hand side of the FP32 cycle
-  int nb = resid<double, float>(F.d, F.numdia, F.ac, L64[0].cc, frhs, u, r32,
-                                c->part, c->s);
-  double rsnrm = host_scalar(c, nb, S_RSNRM);
+  resid<double, float>(F.d, F.numdia, F.ac, L64[0].cc, frhs, u, r32, &rnrm,
+                       c->s);
+  double rsnrm = host_scalar(c, S_RSNRM);
   int iters = 0;
   const int nlev = c->nlev;
   while (iters < itmax && rsnrm / rsden > errtol) {
@@ -332,9 +350,9 @@ int linear_mixed(pmg_ctx *c, LV<double> *L64, LV<float> *L32,
     }
     add_correction<float>(F.d, L32[0].x, u, c->s);
     iters++;
-    nb = resid<double, float>(F.d, F.numdia, F.ac, L64[0].cc, frhs, u, r32,
-                              c->part, c->s);
-    rsnrm = host_scalar(c, nb, S_RSNRM);
+    resid<double, float>(F.d, F.numdia, F.ac, L64[0].cc, frhs, u, r32, &rnrm,
+                         c->s);
+    rsnrm = host_scalar(c, S_RSNRM);
   }
   *iters_out = iters;
   *rel_out = rsnrm / rsden;
@@ -363,6 +381,7 @@ int alloc_hierarchy(pmg_ctx *c) {
   size_t nf = c->L[0].d.n();
   c->u = c->alloc<double>(nf);
   c->part = c->alloc<double>(max_partials(c->L[0].d) * 2);
+  c->ctr = c->alloc<unsigned>(16);
   c->scal = c->alloc<double>(16);
   if (hipHostMalloc((void **)&c->hscal, 16 * sizeof(double)) != hipSuccess)
     c->ok = false;
@@ -559,8 +578,9 @@ int alloc_newton(pmg_ctx *c) {
 
 double nm_norm(pmg_ctx *c, const double *x, double *r) {
   Level &F = c->L[0];
-  int nb = nmresid7(F.d, F.ac, F.cc, F.fc, x, r, c->part, c->s);
-  return host_scalar(c, nb, S_TMP);
+  const Red rd = red_to(c, S_TMP);
+  nmresid7(F.d, F.ac, F.cc, F.fc, x, r, &rd, c->s);
+  return host_scalar(c, S_TMP);
 }
 
 int ctx_newton(pmg_ctx *c, int mode) {
@@ -1005,9 +1025,11 @@ int pmg_gsrb(int nx, int ny, int nz, int numdia, const double *ac,
   double *dac = S.put(ac, (numdia == 7 ? 4 : 14) * n), *dcc = S.put(cc, n),
          *dfc = S.put(fc, n), *dx = S.put(x, n);
   if (!S.ok) return PMG_ENOMEM;
-  if (numdia == 7)
-    gs7<double>(d, dac, dcc, dfc, dx, itmax, iadjoint, 0);
-  else
+  if (numdia == 7) {
+    double *tmp = S.put(x, n);  // same boundary values as x
+    if (!S.ok) return PMG_ENOMEM;
+    gs7_fused<double>(d, dac, dcc, dfc, dx, tmp, itmax, iadjoint, 0);
+  } else
     gs27<double>(d, dac, dcc, dfc, dx, itmax, iadjoint, 0);
   return finish(S, x, dx, n);
 }

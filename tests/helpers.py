@@ -75,3 +75,37 @@ def random_op(n, numdia, rng, zero_boundary=True):
     for a, (i, j, k) in enumerate(pts):
         ac[IX(n, i, j, k)] = offsum[a] + rng.uniform(0.5, 1.5)
     return ac, cc
+
+
+def _interior_mask(n):
+    m = np.zeros((n, n, n), bool)
+    m[1:-1, 1:-1, 1:-1] = True
+    return m
+
+
+def _shift_from(a, ex, ey, ez):
+    """out[k, j, i] = a[k - ez, j - ey, i - ex] (zero outside)."""
+    n = a.shape[0]
+    out = np.zeros_like(a)
+    sl = lambda d: (slice(d, n), slice(0, n - d)) if d >= 0 else (slice(0, n + d), slice(-d, n))
+    (ko, ki), (jo, ji), (io, ii) = sl(ez), sl(ey), sl(ex)
+    out[ko, jo, io] = a[ki, ji, ii]
+    return out
+
+
+def random_op_vec(n, numdia, rng):
+    """Vectorised random_op for large grids (arrays indexed [k, j, i])."""
+    nplanes = 4 if numdia == 7 else 14
+    mask = _interior_mask(n)
+    ac = np.zeros((nplanes, n, n, n))
+    for pl in range(1, nplanes):
+        ex, ey, ez = UPPER27[pl]
+        nb = _shift_from(mask.astype(float), -ex, -ey, -ez) > 0  # p+e interior
+        ac[pl] = rng.uniform(0.1, 1.0, (n, n, n)) * (1.0 if pl <= 3 else 0.3) * (mask & nb)
+    diag = np.zeros((n, n, n))
+    for pl in range(1, nplanes):
+        ex, ey, ez = UPPER27[pl]
+        diag += ac[pl] + _shift_from(ac[pl], ex, ey, ez)
+    ac[0] = (diag + rng.uniform(0.5, 1.5, (n, n, n))) * mask
+    cc = rng.uniform(0.0, 0.5, (n, n, n)) * mask
+    return ac.reshape(-1), cc.reshape(-1)

@@ -55,6 +55,23 @@ def test_gsrb(mg, orc, numdia, iadj):
     assert _rel(x, orc.gsrb(n, numdia, ac, cc, f, x0, 3, iadj)) < 1e-12
 
 
+@pytest.mark.parametrize("numdia", [7, 27])
+def test_gsrb_large(mg, orc, numdia):
+    """97^3: several workgroups and k-chunks for the fused 7-point sweep; the
+    per-colour 27-point path (an FP64 97^2 plane exceeds the LDS budget)."""
+    print(f"gsrb large n=97 numdia={numdia}", flush=True)
+    n = 97
+    rng = np.random.default_rng(97 + numdia)
+    ac, cc = H.random_op_vec(n, numdia, rng)
+    mask = np.zeros((n, n, n)); mask[1:-1, 1:-1, 1:-1] = 1
+    f = (rng.standard_normal((n, n, n)) * mask).ravel()
+    x0 = (rng.standard_normal((n, n, n)) * mask).ravel()
+    for iadj, sweeps in ((0, 2), (1, 1)):
+        x = x0.copy()
+        _call(mg, "pmg_gsrb", n, n, n, numdia, dp(ac), dp(cc), dp(f), dp(x), sweeps, iadj)
+        assert _rel(x, orc.gsrb(n, numdia, ac, cc, f, x0, sweeps, iadj)) < 1e-12
+
+
 @pytest.mark.parametrize("mgprol", [0, 1])
 @pytest.mark.parametrize("numdia", [7, 27])
 def test_buildP_buildG_transfer(mg, orc, numdia, mgprol):
