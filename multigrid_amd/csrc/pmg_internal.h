@@ -28,13 +28,11 @@ inline int corsr(int n) { return (n - 1) / 2 + 1; }
 // a level of dims d (the caller sizes the partials buffer with it).
 size_t max_partials(const Dim &d);
 
-// Destination of a reduction fused into the kernel that produces its terms:
-// per-block partials (2 per block at most), a block counter that is zero
-// between launches (the last block resets it), and the device scalar the
-// last block writes after summing the partials in block order.
+// Destination of a reduction computed alongside the kernel that produces
+// its terms: per-block partials (2 per block at most) and the device scalar
+// a final single-workgroup pass writes after summing them in block order.
 struct Red {
   double *part = nullptr;
-  unsigned *ctr = nullptr;
   double *out = nullptr;
 };
 

@@ -52,44 +52,47 @@ __device__ inline double block_sum(double v, double *red) {
   return t;  // valid in thread 0
 }
 
-// Fused reduction epilogue.  v0 (and v1 when NV == 2) are this block's sums,
-// valid in thread 0.  The last block to arrive sums all partials in block
-// order (deterministic) and writes out[0] = sum0 (MODE 0) or sum0 / sum1
-// (MODE 1; 0 when sum1 == 0, a zero coarse correction), then resets the
-// counter so the next launch starts from zero.
-template <int NT, int NV, int MODE>
-__device__ inline void red_finish(const Red &r, double v0, double v1,
-                                  double *lds) {
-  __shared__ int last;
+// Reduction epilogue: each block stores its sums (valid in thread 0) as a
+// partial; red_final() then sums the partials in block order in a second,
+// single-workgroup kernel.  (A "last block finishes" scheme needs one
+// device-scope atomic per block; across the eight XCDs, whose L2 caches are
+// not coherent, those serialise and cost more than the extra launch.)
+template <int NV>
+__device__ inline void red_store(const Red &r, double v0, double v1) {
   const int tid = threadIdx.x + threadIdx.y * blockDim.x;
-  const unsigned nblk = gridDim.x * gridDim.y * gridDim.z;
-  const unsigned b =
-      blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
   if (tid == 0) {
-    r.part[(size_t)b * NV] = v0;
-    if (NV == 2) r.part[(size_t)b * NV + 1] = v1;
-    __threadfence();
-    const unsigned t = atomicAdd(r.ctr, 1u);
-    last = (t == nblk - 1) ? 1 : 0;
+    const size_t b =
+        blockIdx.x + gridDim.x * (blockIdx.y + (size_t)gridDim.y * blockIdx.z);
+    r.part[b * NV] = v0;
+    if (NV == 2) r.part[b * NV + 1] = v1;
   }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  const volatile double *vp = r.part;
+}
+
+// out[0] = sum0 (MODE 0) or sum0 / sum1 (MODE 1; 0 when sum1 == 0, a zero
+// coarse correction).  Fixed order: thread t sums blocks t, t+1024, ...
+template <int NV, int MODE>
+__global__ __launch_bounds__(1024) void k_red_final(const double *part,
+                                                    unsigned nblk,
+                                                    double *out) {
+  __shared__ double lds[16];
   double a = 0.0, c = 0.0;
-  for (unsigned q = tid; q < nblk; q += NT) {
-    a += vp[(size_t)q * NV];
-    if (NV == 2) c += vp[(size_t)q * NV + 1];
+  for (unsigned q = threadIdx.x; q < nblk; q += 1024) {
+    a += part[(size_t)q * NV];
+    if (NV == 2) c += part[(size_t)q * NV + 1];
   }
-  const double sa = block_sum<NT>(a, lds);
-  const double sc = NV == 2 ? block_sum<NT>(c, lds) : 0.0;
-  if (tid == 0) {
+  const double sa = block_sum<1024>(a, lds);
+  const double sc = NV == 2 ? block_sum<1024>(c, lds) : 0.0;
+  if (threadIdx.x == 0) {
     if (MODE == 1)
-      r.out[0] = sc != 0.0 ? sa / sc : 0.0;
+      out[0] = sc != 0.0 ? sa / sc : 0.0;
     else
-      r.out[0] = sa;
-    atomicExch(r.ctr, 0u);
+      out[0] = sa;
   }
+}
+
+template <int NV, int MODE>
+static void red_final(const Red &r, const dim3 &g, hipStream_t s) {
+  k_red_final<NV, MODE><<<1, 1024, 0, s>>>(r.part, g.x * g.y * g.z, r.out);
 }
 
 // ---------------------------------------------------------------------------
@@ -195,6 +198,17 @@ __global__ __launch_bounds__(256) void k_gs27(int nx, int ny, int nz,
 // and keeps the first-colour ("mid") values of planes k-1, k, k+1.  Three
 // rotating LDS planes need one barrier per plane.
 constexpr int SW_TX = 64, SW_TY = 8, SW_KC = 16;
+// Operands of one first-colour update (coefficients and in-plane xin
+// neighbours) and of one second-colour update (coefficients).
+template <class T>
+struct SwRed {
+  T oC, cc, fc, oE, oEm, oN, oNm, uC, uCm, xe, xw, xn, xs;
+};
+template <class T>
+struct SwBlk {
+  T oC, cc, fc, oE, oEm, oN, oNm, uC, uCm;
+};
+
 template <class T>
 __global__ __launch_bounds__(SW_TX *SW_TY) void k_gs7_sweep(
     int nx, int ny, int nz, const T *__restrict__ ac, const T *__restrict__ cc,
@@ -212,38 +226,65 @@ __global__ __launch_bounds__(SW_TX *SW_TY) void k_gs7_sweep(
   const bool inner = i >= 2 && i <= nx - 1 && j >= 2 && j <= ny - 1;
   const bool owned = inner && tx >= 1 && tx <= SW_TX - 2 && ty >= 1 &&
                      ty <= SW_TY - 2;
-  const int c2 = 1 - c1;
   const size_t pc0 = col ? ix(nx, ny, i, j, 1) : 0;  // column base, k = 1
   auto X = [&](int k) -> T {  // xin on this column, 0 outside the grid
     return (col && k >= 1 && k <= nz) ? xin[pc0 + (size_t)(k - 1) * sz] : (T)0;
   };
-  // first-colour update of (i, j, k) from xin; xm/xp = xin at k-1 / k+1
-  auto mid = [&](int k, T xk, T xm, T xp) -> T {
-    if (!inner || k < 2 || k > nz - 1 || ((i + j + k) & 1) != c1) return xk;
-    const size_t p = pc0 + (size_t)(k - 1) * sz;
-    const T s = oN[p] * xin[p + sy] + oN[p - sy] * xin[p - sy] +
-                oE[p] * xin[p + 1] + oE[p - 1] * xin[p - 1] +
-                uC[p - sz] * xm + uC[p] * xp;
-    return (fc[p] + s) / (ac[p] + cc[p]);
+  // first colour at plane k / second colour at plane k on this column?
+  auto first = [&](int k) {
+    return inner && k >= 2 && k <= nz - 1 && ((i + j + k) & 1) == c1;
   };
-  T x0 = X(kb - 2), x1 = X(kb - 1), x2 = X(kb), x3 = X(kb + 1);
-  T mprev = mid(kb - 1, x1, x0, x2);  // plane kb-1
-  T mcur = mid(kb, x2, x1, x3);       // plane kb
+  auto second = [&](int k) {
+    return owned && k <= ke && ((i + j + k) & 1) != c1;
+  };
+  auto load_red = [&](int k, SwRed<T> &R) {
+    if (!first(k)) return;
+    const size_t p = pc0 + (size_t)(k - 1) * sz;
+    R.oC = ac[p], R.cc = cc[p], R.fc = fc[p], R.oE = oE[p], R.oEm = oE[p - 1],
+    R.oN = oN[p], R.oNm = oN[p - sy], R.uC = uC[p], R.uCm = uC[p - sz];
+    R.xe = xin[p + 1], R.xw = xin[p - 1], R.xn = xin[p + sy],
+    R.xs = xin[p - sy];
+  };
+  auto load_blk = [&](int k, SwBlk<T> &B) {
+    if (!second(k)) return;
+    const size_t p = pc0 + (size_t)(k - 1) * sz;
+    B.oC = ac[p], B.cc = cc[p], B.fc = fc[p], B.oE = oE[p], B.oEm = oE[p - 1],
+    B.oN = oN[p], B.oNm = oN[p - sy], B.uC = uC[p], B.uCm = uC[p - sz];
+  };
+  // first-colour value at plane k (reference term order, gsd.cc:98-105)
+  auto mid = [&](int k, const SwRed<T> &R, T xk, T xm, T xp) -> T {
+    if (!first(k)) return xk;
+    const T s = R.oN * R.xn + R.oNm * R.xs + R.oE * R.xe + R.oEm * R.xw +
+                R.uCm * xm + R.uC * xp;
+    return (R.fc + s) / (R.oC + R.cc);
+  };
+  SwRed<T> R0{}, R1{};
+  SwBlk<T> B0{}, B1{};
+  T x0 = X(kb - 2), x1 = X(kb - 1), x2 = X(kb), x3 = X(kb + 1), x4 = X(kb + 2);
+  load_red(kb - 1, R0);
+  T mprev = mid(kb - 1, R0, x1, x0, x2);  // plane kb-1
+  load_red(kb, R0);
+  T mcur = mid(kb, R0, x2, x1, x3);  // plane kb
   M[kb % 3][ty][tx] = mcur;
+  load_red(kb + 1, R0);
+  load_blk(kb, B0);
   for (int k = kb; k <= ke; k++) {
-    const T x4 = X(k + 2);
-    const T mnext = mid(k + 1, x3, x2, x4);
+    // x2..x4 = xin at k..k+2; R0 / B0 hold the operands of this step
+    const T mnext = mid(k + 1, R0, x3, x2, x4);
     M[(k + 1) % 3][ty][tx] = mnext;
+    // operands of the next step travel while this step waits and computes
+    load_red(k + 2, R1);
+    load_blk(k + 1, B1);
+    const T x5 = X(k + 3);
     __syncthreads();
     if (owned) {
       T v = mcur;
-      if (((i + j + k) & 1) == c2) {
-        const size_t p = pc0 + (size_t)(k - 1) * sz;
+      if (second(k)) {
         const T(*L)[SW_TX] = M[k % 3];
-        const T s = oN[p] * L[ty + 1][tx] + oN[p - sy] * L[ty - 1][tx] +
-                    oE[p] * L[ty][tx + 1] + oE[p - 1] * L[ty][tx - 1] +
-                    uC[p - sz] * mprev + uC[p] * mnext;
-        v = (fc[p] + s) / (ac[p] + cc[p]);
+        const T s = B0.oN * L[ty + 1][tx] + B0.oNm * L[ty - 1][tx] +
+                    B0.oE * L[ty][tx + 1] + B0.oEm * L[ty][tx - 1] +
+                    B0.uCm * mprev + B0.uC * mnext;
+        v = (B0.fc + s) / (B0.oC + B0.cc);
       }
       xout[pc0 + (size_t)(k - 1) * sz] = v;
     }
@@ -251,6 +292,9 @@ __global__ __launch_bounds__(SW_TX *SW_TY) void k_gs7_sweep(
     mcur = mnext;
     x2 = x3;
     x3 = x4;
+    x4 = x5;
+    R0 = R1;
+    B0 = B1;
   }
 }
 
@@ -438,7 +482,7 @@ __global__ __launch_bounds__(256) void k_resid(
   }
   if (do_red) {
     const double t = block_sum<256>(acc, red);
-    red_finish<256, 1, 0>(rd, t, 0.0, red);
+    red_store<1>(rd, t, 0.0);
   }
 }
 
@@ -454,6 +498,7 @@ void resid(const Dim &d, int numdia, const T *ac, const T *cc, const T *fc,
   else
     k_resid<T, TO, 27><<<g, b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, fc, x, r, rd,
                                        red ? 1 : 0);
+  if (red) red_final<1, 0>(rd, g, s);
   check_launch("resid");
 }
 
@@ -534,7 +579,7 @@ __global__ __launch_bounds__(256) void k_nmresid7(
   }
   if (do_red) {
     const double t = block_sum<256>(acc, red);
-    red_finish<256, 1, 0>(rd, t, 0.0, red);
+    red_store<1>(rd, t, 0.0);
   }
 }
 
@@ -545,6 +590,7 @@ void nmresid7(const Dim &d, const double *ac, const double *cc,
   const Red rd = red ? *red : Red{};
   k_nmresid7<<<g, dim3(BX, BY), 0, s>>>(d.nx, d.ny, d.nz, ac, cc, fc, x, r,
                                         rd, red ? 1 : 0);
+  if (red) red_final<1, 0>(rd, g, s);
   check_launch("nmresid7");
 }
 
@@ -586,7 +632,7 @@ __global__ __launch_bounds__(256) void k_matvec_dots(
   if (do_red) {
     const double t0 = block_sum<256>(a0, red);
     const double t1 = block_sum<256>(a1, red);
-    red_finish<256, 2, 1>(rd, t0, t1, red);
+    red_store<2>(rd, t0, t1);
   }
 }
 
@@ -603,6 +649,7 @@ void matvec_dots(const Dim &d, int numdia, const T *ac, const T *cc,
   else
     k_matvec_dots<T, 27><<<g, b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, x, w0, y,
                                          rd, red ? 1 : 0);
+  if (red) red_final<2, 1>(rd, g, s);
   check_launch("matvec_dots");
 }
 
@@ -760,13 +807,14 @@ __global__ __launch_bounds__(256) void k_nrm1(int nx, int ny, int nz,
       acc += fabs((double)x[ix(nx, ny, i, j, k)]);
     }
   const double t = block_sum<256>(acc, red);
-  red_finish<256, 1, 0>(rd, t, 0.0, red);
+  red_store<1>(rd, t, 0.0);
 }
 
 template <class T>
 void nrm1(const Dim &d, const T *x, const Red &red, hipStream_t s) {
   dim3 g(cdiv(d.nx - 2, BX), cdiv(d.ny - 2, BY), cdiv(d.nz - 2, KCH));
   k_nrm1<T><<<g, dim3(BX, BY), 0, s>>>(d.nx, d.ny, d.nz, x, red);
+  red_final<1, 0>(red, g, s);
   check_launch("nrm1");
 }
 
@@ -780,27 +828,75 @@ constexpr size_t CG_LDS_MAX = 60 * 1024;
 
 bool cg_fits(const Dim &d) {
   return d.n() * sizeof(double) <= CG_LDS_MAX &&
-         d.interior() <= (size_t)CG_NT * 8;
+         d.interior() <= (size_t)CG_NT * 2;
 }
 
-template <class T, int NPT>
+// Sum over the workgroup with a single barrier: every thread reads the 16
+// per-wavefront partials in the same order, so all get the same value.
+// Consecutive calls must alternate buf (see the barrier argument in the
+// CG loop below).
+__device__ inline double cg_allsum(double v, double *buf, int nw) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if ((threadIdx.x & 63) == 0) buf[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int w = 0; w < nw; w++) t += buf[w];
+  return t;
+}
+
+// 26 neighbour offsets in the reference's term order (in-plane 8, up 9,
+// down 9; matvecd.cc:137-170) and the plane / position holding each
+// coupling (upper directions at p, lower ones at the neighbour).
+template <int ND>
+__device__ inline void cg_load_row(const double *__restrict__ ac, size_t n,
+                                   long long sy, long long sz, size_t p,
+                                   double *c, long long *off) {
+  const long long o[26] = {sy,           -sy,          1,
+                           -1,           1 + sy,       -1 + sy,
+                           1 - sy,       -1 - sy,      sz,
+                           sy + sz,      -sy + sz,     1 + sz,
+                           -1 + sz,      1 + sy + sz,  -1 + sy + sz,
+                           1 - sy + sz,  -1 - sy + sz, -sz,
+                           sy - sz,      -sy - sz,     1 - sz,
+                           -1 - sz,      1 + sy - sz,  -1 + sy - sz,
+                           1 - sy - sz,  -1 - sy - sz};
+  // plane of each term, and whether it is stored at the neighbour
+  const int pl[26] = {2, 2, 1, 1, 4, 5, 5, 4, 3, 8, 9, 6, 7,
+                      10, 11, 12, 13, 3, 9, 8, 7, 6, 13, 12, 11, 10};
+  const bool atn[26] = {0, 1, 0, 1, 0, 0, 1, 1, 0, 0, 0, 0, 0,
+                        0, 0, 0, 0, 1, 1, 1, 1, 1, 1, 1, 1, 1};
+#pragma unroll
+  for (int q = 0; q < 26; q++) {
+    off[q] = o[q];
+    const bool use = ND == 27 || pl[q] <= 3;
+    c[q] = use ? ac[(size_t)pl[q] * n +
+                    (atn[q] ? (size_t)((long long)p + o[q]) : p)]
+               : 0.0;
+  }
+}
+
+template <class T, int NPT, int ND>
 __global__ __launch_bounds__(CG_NT) void k_cg_single(
-    int nx, int ny, int nz, int numdia, const double *__restrict__ ac,
+    int nx, int ny, int nz, const double *__restrict__ ac,
     const double *__restrict__ cc, const T *__restrict__ f,
     T *__restrict__ xout, int itmax, double errtol) {
   extern __shared__ double P[];
-  __shared__ double red[16];
-  const int tid = threadIdx.x;
-  const size_t n = (size_t)nx * ny * nz, sy = nx, sz = (size_t)nx * ny;
+  __shared__ double rbuf[2][CG_NT / 64];
+  const int tid = threadIdx.x, NTB = blockDim.x, nw = NTB / 64;
+  const size_t n = (size_t)nx * ny * nz;
+  const long long sy = nx, sz = (long long)nx * ny;
   const int ni = nx - 2, nj = ny - 2;
   const int nint = ni * nj * (nz - 2);
-  for (size_t q = tid; q < n; q += CG_NT) P[q] = 0.0;
+  for (size_t q = tid; q < n; q += NTB) P[q] = 0.0;
   size_t pidx[NPT];
   bool val[NPT];
-  double xr[NPT], rr[NPT], apv[NPT];
+  double xr[NPT], rr[NPT], apv[NPT], dg[NPT];
+  double cf[NPT][26];
+  long long off[26];
 #pragma unroll
   for (int m = 0; m < NPT; m++) {
-    const int t = tid + m * CG_NT;
+    const int t = tid + m * NTB;
     val[m] = t < nint;
     const int tt = val[m] ? t : 0;
     const int i = 2 + tt % ni, j = 2 + (tt / ni) % nj, k = 2 + tt / (ni * nj);
@@ -808,30 +904,24 @@ __global__ __launch_bounds__(CG_NT) void k_cg_single(
     xr[m] = 0.0;
     rr[m] = val[m] ? (double)f[pidx[m]] : 0.0;  // r = f - A*0
     apv[m] = 0.0;
+    dg[m] = ac[pidx[m]] + cc[pidx[m]];
+    cg_load_row<ND>(ac, n, sy, sz, pidx[m], cf[m], off);
   }
   __syncthreads();
   double s = 0.0;
 #pragma unroll
   for (int m = 0; m < NPT; m++) s += rr[m] * rr[m];
-  double t0 = block_sum<CG_NT>(s, red);
-  if (tid == 0) red[8] = t0;
-  __syncthreads();
-  const double denom = sqrt(red[8]);
-  __syncthreads();
+  double rhok2 = cg_allsum(s, rbuf[0], nw);
+  int b = 1;
+  const double denom = sqrt(rhok2);
   int iters = 0;
   double rhok1 = 0.0;
   if (denom != 0.0) {
     for (;;) {
-      s = 0.0;
-#pragma unroll
-      for (int m = 0; m < NPT; m++) s += rr[m] * rr[m];
-      double t1 = block_sum<CG_NT>(s, red);
-      if (tid == 0) red[8] = t1;
-      __syncthreads();
-      const double rhok2 = red[8];
-      __syncthreads();
       if (sqrt(rhok2) / denom <= errtol) break;
       if (iters >= itmax) break;
+      // the previous reduction's barrier ordered every read of P before
+      // these writes
       if (iters == 0) {
 #pragma unroll
         for (int m = 0; m < NPT; m++)
@@ -847,40 +937,37 @@ __global__ __launch_bounds__(CG_NT) void k_cg_single(
       s = 0.0;
 #pragma unroll
       for (int m = 0; m < NPT; m++) {
-        if (!val[m]) continue;
         const size_t p = pidx[m];
-        double v;
-        if (numdia == 7) {
-          const double *oE = ac + n, *oN = ac + 2 * n, *uC = ac + 3 * n;
-          v = -oN[p] * P[p + sy] - oN[p - sy] * P[p - sy] - oE[p] * P[p + 1] -
-              oE[p - 1] * P[p - 1] - uC[p - sz] * P[p - sz] -
-              uC[p] * P[p + sz] + (ac[p] + cc[p]) * P[p];
-        } else {
-          double sO, sU, sD;
-          sum27<double, double>(ac, n, sy, sz, p, P, sO, sU, sD);
-          v = -sO + -sU + -sD + (ac[p] + cc[p]) * P[p];
-        }
-        apv[m] = v;
-        s += P[p] * v;
+        double o = 0.0, u = 0.0, dd = 0.0;
+#pragma unroll
+        for (int q = 0; q < 8; q++) o += cf[m][q] * P[(size_t)((long long)p + off[q])];
+#pragma unroll
+        for (int q = 8; q < 17; q++) u += cf[m][q] * P[(size_t)((long long)p + off[q])];
+#pragma unroll
+        for (int q = 17; q < 26; q++) dd += cf[m][q] * P[(size_t)((long long)p + off[q])];
+        const double v = -o + -u + -dd + dg[m] * P[p];
+        apv[m] = val[m] ? v : 0.0;
+        if (val[m]) s += P[p] * v;
       }
-      double t2 = block_sum<CG_NT>(s, red);
-      if (tid == 0) red[8] = t2;
-      __syncthreads();
-      const double pAp = red[8];
-      __syncthreads();
+      const double pAp = cg_allsum(s, rbuf[b], nw);
+      b ^= 1;
       const double alpha = rhok2 / pAp;
       rhok1 = rhok2;
+      s = 0.0;
 #pragma unroll
       for (int m = 0; m < NPT; m++) {
         if (!val[m]) continue;
         xr[m] += alpha * P[pidx[m]];
         rr[m] += -alpha * apv[m];
+        s += rr[m] * rr[m];
       }
       iters++;
+      rhok2 = cg_allsum(s, rbuf[b], nw);
+      b ^= 1;
     }
   }
   // x = 0 on the boundary (Vazeros before the solve), solution inside
-  for (size_t q = tid; q < n; q += CG_NT) {
+  for (size_t q = tid; q < n; q += NTB) {
     const int i = 1 + (int)(q % nx), j = 1 + (int)((q / nx) % ny),
               k = 1 + (int)(q / ((size_t)nx * ny));
     if (i == 1 || i == nx || j == 1 || j == ny || k == 1 || k == nz)
@@ -891,6 +978,24 @@ __global__ __launch_bounds__(CG_NT) void k_cg_single(
     if (val[m]) xout[pidx[m]] = (T)xr[m];
 }
 
+template <class T, int ND>
+static void cg_launch(const Dim &d, const double *ac, const double *cc,
+                      const T *f, T *x, int itmax, double errtol,
+                      hipStream_t s) {
+  const size_t lds = d.n() * sizeof(double);
+  const size_t ni = d.interior();
+  // one interior point per thread when they fit: as few wavefronts as the
+  // level needs (the loop is LDS- and barrier-bound, idle waves cost)
+  if (ni <= CG_NT) {
+    const int nt = (int)((ni + 63) / 64) * 64;
+    k_cg_single<T, 1, ND><<<1, nt, lds, s>>>(d.nx, d.ny, d.nz, ac, cc, f, x,
+                                             itmax, errtol);
+  } else {
+    k_cg_single<T, 2, ND><<<1, CG_NT, lds, s>>>(d.nx, d.ny, d.nz, ac, cc, f,
+                                                x, itmax, errtol);
+  }
+}
+
 template <class T>
 void cg_single(const Dim &d, int numdia, const double *ac, const double *cc,
                const T *f, T *x, int itmax, double errtol, hipStream_t s) {
@@ -898,20 +1003,10 @@ void cg_single(const Dim &d, int numdia, const double *ac, const double *cc,
     fprintf(stderr, "pmg: coarsest level too large for cg_single\n");
     return;
   }
-  const size_t lds = d.n() * sizeof(double);
-  const size_t ni = d.interior();
-  if (ni <= CG_NT)
-    k_cg_single<T, 1><<<1, CG_NT, lds, s>>>(d.nx, d.ny, d.nz, numdia, ac, cc,
-                                            f, x, itmax, errtol);
-  else if (ni <= 2 * CG_NT)
-    k_cg_single<T, 2><<<1, CG_NT, lds, s>>>(d.nx, d.ny, d.nz, numdia, ac, cc,
-                                            f, x, itmax, errtol);
-  else if (ni <= 4 * CG_NT)
-    k_cg_single<T, 4><<<1, CG_NT, lds, s>>>(d.nx, d.ny, d.nz, numdia, ac, cc,
-                                            f, x, itmax, errtol);
+  if (numdia == 7)
+    cg_launch<T, 7>(d, ac, cc, f, x, itmax, errtol, s);
   else
-    k_cg_single<T, 8><<<1, CG_NT, lds, s>>>(d.nx, d.ny, d.nz, numdia, ac, cc,
-                                            f, x, itmax, errtol);
+    cg_launch<T, 27>(d, ac, cc, f, x, itmax, errtol, s);
   check_launch("cg_single");
 }
 
@@ -1265,88 +1360,85 @@ void buildP(const Dim &f, const Dim &c, int mgprol, int numdia,
 // ---------------------------------------------------------------------------
 // Setup: Galerkin coarse operator XA = P^T A P (VbuildG_7 / VbuildG_27)
 // ---------------------------------------------------------------------------
-// Stored direction delta of each of the 14 output planes.
-__constant__ signed char c_delta[14][3] = {
-    {0, 0, 0},  {1, 0, 0},  {0, 1, 0},  {0, 0, 1},   {1, 1, 0},
-    {-1, 1, 0}, {1, 0, 1},  {-1, 0, 1}, {0, 1, 1},   {0, -1, 1},
-    {1, 1, 1},  {-1, 1, 1}, {1, -1, 1}, {-1, -1, 1}};
-
-// For each coarse point C the product A P_C is formed on the 5x5x5 fine
-// box around C (in LDS), then each stored entry is the dot product of that
-// box with the weights of the neighbour C + delta:
-//   XA(C, C+delta) = sum_q P_{C+delta}(q) (A P_C)(q)   (A symmetric).
+// One thread per coarse point C, no LDS: for every fine point F = C_f + d of
+// the support of P_C and every stencil neighbour G = F + e, the product
+// P_C(d) * A(F, G) is scattered into the stored entries whose neighbour
+// C + delta has G in its support:
+//   XA(C, C+delta) += P_C(d) A(F, G) P_{C+delta}(d + e - 2 delta).
+// All index arithmetic is compile-time after unrolling; 14 accumulators.
 template <int ND>
-__global__ __launch_bounds__(64) void k_buildG(int nxf, int nyf, int nzf,
-                                               int nxc, int nyc, int nzc,
-                                               const double *__restrict__ pc,
-                                               const double *__restrict__ ac,
-                                               double *__restrict__ acc) {
-  __shared__ double APs[125][64];
-  const int lane = threadIdx.x;
-  const int ii = 2 + blockIdx.x * 64 + lane;
-  const int jj = 2 + blockIdx.y;
+__global__ __launch_bounds__(256) void k_buildG(int nxf, int nyf, int nzf,
+                                                int nxc, int nyc, int nzc,
+                                                const double *__restrict__ pc,
+                                                const double *__restrict__ ac,
+                                                double *__restrict__ acc) {
+  const int ii = 2 + blockIdx.x * BX + threadIdx.x;
+  const int jj = 2 + blockIdx.y * BY + threadIdx.y;
   const int kk = 2 + blockIdx.z;
-  if (ii > nxc - 1) return;
+  if (ii > nxc - 1 || jj > nyc - 1) return;
   const size_t nf = (size_t)nxf * nyf * nzf, nc = (size_t)nxc * nyc * nzc;
   const size_t pC = ix(nxc, nyc, ii, jj, kk);
   const size_t pCf = ix(nxf, nyf, 2 * ii - 1, 2 * jj - 1, 2 * kk - 1);
   const long long sx = 1, sy = nxf, sz = (long long)nxf * nyf;
-  const long long cx = 1, cy = nxc, cz = (long long)nxc * nyc;
-  double wC[27];
+  const long long cy = nxc, cz = (long long)nxc * nyc;
+  constexpr signed char DL[14][3] = {
+      {0, 0, 0},  {1, 0, 0},  {0, 1, 0},  {0, 0, 1},   {1, 1, 0},
+      {-1, 1, 0}, {1, 0, 1},  {-1, 0, 1}, {0, 1, 1},   {0, -1, 1},
+      {1, 1, 1},  {-1, 1, 1}, {1, -1, 1}, {-1, -1, 1}};
+  double xa[14];
+#pragma unroll
+  for (int s = 0; s < 14; s++) xa[s] = 0.0;
 #pragma unroll
   for (int a = 0; a < 27; a++) {
     const int dx = a % 3 - 1, dy = (a / 3) % 3 - 1, dz = a / 9 - 1;
-    wC[a] = pc[(size_t)pplane(dx, dy, dz) * nc + pC];
-  }
-  // (A P_C)(q) for q = C_f + (qx,qy,qz), |q| <= 2 per axis
-  for (int qa = 0; qa < 125; qa++) {
-    const int qx = qa % 5 - 2, qy = (qa / 5) % 5 - 2, qz = qa / 25 - 2;
-    const size_t pq = (size_t)((long long)pCf + qx * sx + qy * sy + qz * sz);
-    double v = 0.0;
-    // diagonal term (q inside the support of P_C)
-    if (qx >= -1 && qx <= 1 && qy >= -1 && qy <= 1 && qz >= -1 && qz <= 1)
-      v = ac[pq] * wC[(qz + 1) * 9 + (qy + 1) * 3 + (qx + 1)];
+    const double wC = pc[(size_t)pplane(dx, dy, dz) * nc + pC];
+    const size_t pF = (size_t)((long long)pCf + dx * sx + dy * sy + dz * sz);
 #pragma unroll
     for (int b = 0; b < 27; b++) {
-      if (b == 13) continue;
       const int ex = b % 3 - 1, ey = (b / 3) % 3 - 1, ez = b / 9 - 1;
-      if (ND == 7 && (ex != 0) + (ey != 0) + (ez != 0) != 1) continue;
-      const int tx = qx + ex, ty = qy + ey, tz = qz + ez;
-      if (tx < -1 || tx > 1 || ty < -1 || ty > 1 || tz < -1 || tz > 1)
-        continue;
-      const double cpl =
-          coupling<ND>(ac, nf, pq, ex * sx + ey * sy + ez * sz, ex, ey, ez);
-      v -= cpl * wC[(tz + 1) * 9 + (ty + 1) * 3 + (tx + 1)];
-    }
-    APs[qa][lane] = v;
-  }
-  for (int s = 0; s < 14; s++) {
-    const int gx = c_delta[s][0], gy = c_delta[s][1], gz = c_delta[s][2];
-    const size_t pC2 = (size_t)((long long)pC + gx * cx + gy * cy + gz * cz);
-    double acc_v = 0.0;
+      const int ne = (ex != 0) + (ey != 0) + (ez != 0);
+      if (ND == 7 && ne > 1) continue;
+      const int qx = dx + ex, qy = dy + ey, qz = dz + ez;  // G - C_f
+      bool any = false;
 #pragma unroll
-    for (int a = 0; a < 27; a++) {
-      const int dx = a % 3 - 1, dy = (a / 3) % 3 - 1, dz = a / 9 - 1;
-      const int qx = 2 * gx + dx, qy = 2 * gy + dy, qz = 2 * gz + dz;
-      if (qx < -2 || qx > 2 || qy < -2 || qy > 2 || qz < -2 || qz > 2)
-        continue;
-      const double w2 = pc[(size_t)pplane(dx, dy, dz) * nc + pC2];
-      acc_v += w2 * APs[(qz + 2) * 25 + (qy + 2) * 5 + (qx + 2)][lane];
+      for (int s = 0; s < 14; s++) {
+        const int tx = qx - 2 * DL[s][0], ty = qy - 2 * DL[s][1],
+                  tz = qz - 2 * DL[s][2];
+        any |= (tx >= -1 && tx <= 1 && ty >= -1 && ty <= 1 && tz >= -1 &&
+                tz <= 1);
+      }
+      if (!any) continue;
+      const double A = ne == 0 ? ac[pF]
+                               : -coupling<ND>(ac, nf, pF,
+                                               ex * sx + ey * sy + ez * sz,
+                                               ex, ey, ez);
+      const double wa = wC * A;
+#pragma unroll
+      for (int s = 0; s < 14; s++) {
+        const int tx = qx - 2 * DL[s][0], ty = qy - 2 * DL[s][1],
+                  tz = qz - 2 * DL[s][2];
+        if (tx < -1 || tx > 1 || ty < -1 || ty > 1 || tz < -1 || tz > 1)
+          continue;
+        const size_t pC2 =
+            (size_t)((long long)pC + DL[s][0] + DL[s][1] * cy + DL[s][2] * cz);
+        xa[s] += wa * pc[(size_t)pplane(tx, ty, tz) * nc + pC2];
+      }
     }
-    acc[(size_t)s * nc + pC] = s == 0 ? acc_v : -acc_v;
   }
+#pragma unroll
+  for (int s = 0; s < 14; s++) acc[(size_t)s * nc + pC] = s == 0 ? xa[s] : -xa[s];
 }
 
 void buildG(const Dim &f, const Dim &c, int numdia, const double *pc,
             const double *acf, double *acc, hipStream_t s) {
   if (c.nx < 3 || c.ny < 3 || c.nz < 3) return;
-  dim3 g(cdiv(c.nx - 2, 64), c.ny - 2, c.nz - 2);
+  dim3 g(cdiv(c.nx - 2, BX), cdiv(c.ny - 2, BY), c.nz - 2);
   if (numdia == 7)
-    k_buildG<7><<<g, 64, 0, s>>>(f.nx, f.ny, f.nz, c.nx, c.ny, c.nz, pc, acf,
-                                 acc);
+    k_buildG<7><<<g, dim3(BX, BY), 0, s>>>(f.nx, f.ny, f.nz, c.nx, c.ny, c.nz,
+                                           pc, acf, acc);
   else
-    k_buildG<27><<<g, 64, 0, s>>>(f.nx, f.ny, f.nz, c.nx, c.ny, c.nz, pc, acf,
-                                  acc);
+    k_buildG<27><<<g, dim3(BX, BY), 0, s>>>(f.nx, f.ny, f.nz, c.nx, c.ny,
+                                            c.nz, pc, acf, acc);
   check_launch("buildG");
 }
 

@@ -140,7 +140,6 @@ struct pmg_ctx {
   double *u = nullptr;       // fine solution (FP64)
   double *work[4] = {};      // fine FP64 scratch (Newton)
   double *part = nullptr;    // reduction partials
-  unsigned *ctr = nullptr;   // reduction block counter (zero between uses)
   double *scal = nullptr;    // device scalars
   double *hscal = nullptr;   // pinned host mirror
   bool have32 = false, haveN = false;
@@ -191,7 +190,6 @@ int device_ok() {
 Red red_to(pmg_ctx *c, int slot) {
   Red r;
   r.part = c->part;
-  r.ctr = c->ctr;
   r.out = c->scal + slot;
   return r;
 }
@@ -381,7 +379,6 @@ int alloc_hierarchy(pmg_ctx *c) {
   size_t nf = c->L[0].d.n();
   c->u = c->alloc<double>(nf);
   c->part = c->alloc<double>(max_partials(c->L[0].d) * 2);
-  c->ctr = c->alloc<unsigned>(16);
   c->scal = c->alloc<double>(16);
   if (hipHostMalloc((void **)&c->hscal, 16 * sizeof(double)) != hipSuccess)
     c->ok = false;
