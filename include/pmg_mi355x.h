@@ -120,6 +120,9 @@ int pmg_ctx_get_level(pmg_ctx *ctx, int lev, double *ac, double *cc,
  * pmg_ctx_ktime returns the accumulated device milliseconds and the number
  * of full red-black sweeps they cover. */
 int pmg_ctx_ktime_enable(pmg_ctx *ctx, int max_events);
+/* Replay each solve iteration from a captured HIP graph (default on; event
+ * timing of the smoothing kernels uses the direct launches instead). */
+int pmg_ctx_set_graphs(pmg_ctx *ctx, int on);
 int pmg_ctx_ktime(pmg_ctx *ctx, double *ms, long long *sweeps);
 
 /* --- single-kernel entry points on host arrays (parity checking of the

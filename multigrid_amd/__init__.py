@@ -75,6 +75,7 @@ def _declare(L):
         "pmg_ctx_get_level": (i, [vp, i, dp, dp, dp, dp]),
         "pmg_ctx_ktime_enable": (i, [vp, i]),
         "pmg_ctx_ktime": (i, [vp, dp, ctypes.POINTER(ctypes.c_longlong)]),
+        "pmg_ctx_set_graphs": (i, [vp, i]),
         "pmg_matvec": (i, [i, i, i, i, dp, dp, dp, dp]),
         "pmg_mresid": (i, [i, i, i, i, dp, dp, dp, dp, dp]),
         "pmg_gsrb": (i, [i, i, i, i, dp, dp, dp, dp, i, i]),
@@ -252,6 +253,9 @@ class Context:
 
     def ktime_enable(self, max_events=8192):
         return _check(lib().pmg_ctx_ktime_enable(self._h, max_events), "ktime")
+
+    def set_graphs(self, on):
+        return _check(lib().pmg_ctx_set_graphs(self._h, 1 if on else 0), "set_graphs")
 
     def ktime(self):
         ms, cnt = ctypes.c_double(), ctypes.c_longlong()

@@ -335,97 +335,125 @@ void gs7(const Dim &d, const T *ac, const T *cc, const T *fc, T *x,
   check_launch("gs7");
 }
 
-// A whole 8-colour sweep in two launches.  Colours 0-3 live on even k
-// planes and 4-7 on odd ones ((i%2)+2(j%2)+4(k%2), gsd.cc:168), and a 27-point
-// stencil couples a plane only to itself and to k+-1.  So every plane of one
-// parity can be swept independently: one workgroup per plane keeps the plane
-// in LDS, runs its four colours in the reference order with a barrier after
-// each, and reads the (fixed) neighbouring planes from global memory.  Each
-// point sees exactly the neighbour values of the one-launch-per-colour
-// order, so the result is identical.
-// zero_stage: the plane starts from x = 0 (Vazeros before the sweep);
-// zero_nb: the k+-1 planes are also still zero (first phase of that sweep).
-template <class T, int NT>
-__global__ __launch_bounds__(NT) void k_gs27_plane(
+// A whole 8-colour sweep (Vgsrb27x, gsd.cc:160-211) as four row-parallel
+// launches.  Colour (i%2) + 2(j%2) + 4(k%2) in order 0..7 visits the row
+// classes (k%2, j%2) = (0,0), (0,1), (1,0), (1,1), each holding the colours
+// i%2 = 0 then 1.  A 27-point stencil couples a row only to rows of other
+// classes (fixed while this class is updated) and to itself, so the rows of
+// one class are independent; inside a row the second colour reads its i+-1
+// neighbours of the first colour, carried between adjacent lanes.  One
+// wavefront per row, lane l owns the points i0 = 2+2l and i0+1.  Every point
+// sees the neighbour values of the one-launch-per-colour order: identical
+// results.  iadjoint reverses the class order and the in-row order.
+// zmask: row classes (bit 2*(k%2) + (j%2)) whose values are still zero in
+// this sweep (zero start, Vazeros before the smoothing): read as 0.
+constexpr int ROWS_MAXN = 129;  // 64 lanes of point pairs
+template <class T, int ADJ>
+__global__ __launch_bounds__(256) void k_gs27_rows(
     int nx, int ny, int nz, const T *__restrict__ ac, const T *__restrict__ cc,
-    const T *__restrict__ fc, T *__restrict__ x, int pz, int iadj,
-    int zero_stage, int zero_nb) {
-  extern __shared__ unsigned char smem_raw[];
-  T *X = reinterpret_cast<T *>(smem_raw);
-  const int k = 2 + pz + 2 * (int)blockIdx.x;
-  if (k > nz - 1) return;
-  const int tid = threadIdx.x;
+    const T *__restrict__ fc, T *__restrict__ x, int pz, int py, int nj,
+    int nrows, int zmask) {
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= nrows) return;  // whole wavefront; no block barrier is used
+  const int j = 2 + py + 2 * (row % nj), k = 2 + pz + 2 * (row / nj);
   const size_t n = (size_t)nx * ny * nz, sy = nx, sz = (size_t)nx * ny;
-  const size_t plane = (size_t)nx * ny, base = (size_t)(k - 1) * plane;
-  for (size_t q = tid; q < plane; q += NT)
-    X[q] = zero_stage ? (T)0 : x[base + q];
-  __syncthreads();
   const T *oE = ac + n, *oN = ac + 2 * n, *uC = ac + 3 * n, *oNE = ac + 4 * n,
           *oNW = ac + 5 * n, *uE = ac + 6 * n, *uW = ac + 7 * n,
           *uN = ac + 8 * n, *uS = ac + 9 * n, *uNE = ac + 10 * n,
           *uNW = ac + 11 * n, *uSE = ac + 12 * n, *uSW = ac + 13 * n;
-  for (int st = 0; st < 4; st++) {
-    const int c = iadj ? 3 - st : st;
-    const int px = c & 1, py = (c >> 1) & 1;
-    const int ni = (nx - 2 - px + 1) / 2, nj = (ny - 2 - py + 1) / 2;
-    for (int t = tid; t < ni * nj; t += NT) {
-      const int i = 2 + px + 2 * (t % ni), j = 2 + py + 2 * (t / ni);
-      const size_t q = (size_t)(j - 1) * nx + (i - 1), p = base + q;
-      const T o = oN[p] * X[q + sy] + oN[p - sy] * X[q - sy] +
-                  oE[p] * X[q + 1] + oE[p - 1] * X[q - 1] +
-                  oNE[p] * X[q + 1 + sy] + oNW[p] * X[q - 1 + sy] +
-                  oNW[p + 1 - sy] * X[q + 1 - sy] +
-                  oNE[p - 1 - sy] * X[q - 1 - sy];
-      T u = (T)0, dd = (T)0;
-      if (!zero_nb) {
-        u = uC[p] * x[p + sz] + uN[p] * x[p + sy + sz] +
-            uS[p] * x[p - sy + sz] + uE[p] * x[p + 1 + sz] +
-            uW[p] * x[p - 1 + sz] + uNE[p] * x[p + 1 + sy + sz] +
-            uNW[p] * x[p - 1 + sy + sz] + uSE[p] * x[p + 1 - sy + sz] +
-            uSW[p] * x[p - 1 - sy + sz];
-        dd = uC[p - sz] * x[p - sz] + uS[p + sy - sz] * x[p + sy - sz] +
-             uN[p - sy - sz] * x[p - sy - sz] +
-             uW[p + 1 - sz] * x[p + 1 - sz] +
-             uE[p - 1 - sz] * x[p - 1 - sz] +
-             uSW[p + 1 + sy - sz] * x[p + 1 + sy - sz] +
-             uSE[p - 1 + sy - sz] * x[p - 1 + sy - sz] +
-             uNW[p + 1 - sy - sz] * x[p + 1 - sy - sz] +
-             uNE[p - 1 - sy - sz] * x[p - 1 - sy - sz];
-      }
-      X[q] = (fc[p] + ((o + u) + dd)) / (ac[p] + cc[p]);
-    }
-    __syncthreads();
+  // neighbour row classes: (dy, dz) -> zero?
+  auto zr = [&](int dy, int dz) -> bool {
+    const int cls = 2 * (pz ^ (dz != 0)) + (py ^ (dy != 0));
+    return (zmask >> cls) & 1;
+  };
+  const bool zN = zr(1, 0), zU = zr(0, 1), zNU = zr(1, 1);  // classes
+  const bool zown = (zmask >> (2 * pz + py)) & 1;
+  const int i0 = 2 + 2 * lane, i1 = i0 + 1;
+  const size_t rb = ix(nx, ny, 1, j, k);  // row base (i = 1)
+  // current values of the own row, boundary included (i <= nx)
+  const T a0 = (i0 <= nx) ? (zown && i0 <= nx - 1 ? (T)0 : x[rb + i0 - 1])
+                          : (T)0;
+  const T a1 = (i1 <= nx) ? (zown && i1 <= nx - 1 ? (T)0 : x[rb + i1 - 1])
+                          : (T)0;
+  const T xb1 = x[rb];  // i = 1 boundary
+  auto X = [&](size_t q, bool z) -> T { return z ? (T)0 : x[q]; };
+  // update of point i with in-row neighbours xw (i-1) and xe (i+1)
+  auto upd = [&](int i, T xw, T xe) -> T {
+    const size_t p = rb + i - 1;
+    const T o = oN[p] * X(p + sy, zN) + oN[p - sy] * X(p - sy, zN) +
+                oE[p] * xe + oE[p - 1] * xw + oNE[p] * X(p + 1 + sy, zN) +
+                oNW[p] * X(p - 1 + sy, zN) +
+                oNW[p + 1 - sy] * X(p + 1 - sy, zN) +
+                oNE[p - 1 - sy] * X(p - 1 - sy, zN);
+    const T u = uC[p] * X(p + sz, zU) + uN[p] * X(p + sy + sz, zNU) +
+                uS[p] * X(p - sy + sz, zNU) + uE[p] * X(p + 1 + sz, zU) +
+                uW[p] * X(p - 1 + sz, zU) + uNE[p] * X(p + 1 + sy + sz, zNU) +
+                uNW[p] * X(p - 1 + sy + sz, zNU) +
+                uSE[p] * X(p + 1 - sy + sz, zNU) +
+                uSW[p] * X(p - 1 - sy + sz, zNU);
+    const T dd = uC[p - sz] * X(p - sz, zU) +
+                 uS[p + sy - sz] * X(p + sy - sz, zNU) +
+                 uN[p - sy - sz] * X(p - sy - sz, zNU) +
+                 uW[p + 1 - sz] * X(p + 1 - sz, zU) +
+                 uE[p - 1 - sz] * X(p - 1 - sz, zU) +
+                 uSW[p + 1 + sy - sz] * X(p + 1 + sy - sz, zNU) +
+                 uSE[p - 1 + sy - sz] * X(p - 1 + sy - sz, zNU) +
+                 uNW[p + 1 - sy - sz] * X(p + 1 - sy - sz, zNU) +
+                 uNE[p - 1 - sy - sz] * X(p - 1 - sy - sz, zNU);
+    return (fc[p] + ((o + u) + dd)) / (ac[p] + cc[p]);
+  };
+  const bool v0 = i0 <= nx - 1, v1 = i1 <= nx - 1;
+  T n0 = a0, n1 = a1;
+  if (ADJ == 0) {
+    // colour i%2 = 0 (points i0): neighbours i0-1 (lane-1's a1), i0+1 (a1)
+    T w = __shfl_up(a1, 1, 64);
+    if (lane == 0) w = xb1;
+    if (v0) n0 = upd(i0, w, a1);
+    // colour i%2 = 1 (points i1): neighbours i1-1 = n0, i1+1 = lane+1's n0
+    const T e = __shfl_down(n0, 1, 64);
+    if (v1) n1 = upd(i1, n0, e);
+  } else {
+    const T e = __shfl_down(a0, 1, 64);
+    if (v1) n1 = upd(i1, a0, e);
+    T w = __shfl_up(n1, 1, 64);
+    if (lane == 0) w = xb1;
+    if (v0) n0 = upd(i0, w, n1);
   }
-  for (size_t q = tid; q < plane; q += NT) {
-    const int i = 1 + (int)(q % nx), j = 1 + (int)(q / nx);
-    if (i >= 2 && i <= nx - 1 && j >= 2 && j <= ny - 1) x[base + q] = X[q];
-  }
+  if (v0) x[rb + i0 - 1] = n0;
+  if (v1) x[rb + i1 - 1] = n1;
 }
 
-constexpr size_t GS27_LDS_MAX = 64 * 1024;
+static int count_parity(int n, int par) {  // #{v in [2, n-1] : v%2 == par}
+  const int first = 2 + par;
+  return first > n - 1 ? 0 : (n - 1 - first) / 2 + 1;
+}
 
 template <class T>
 void gs27(const Dim &d, const T *ac, const T *cc, const T *fc, T *x,
           int nsweeps, int iadjoint, hipStream_t s, bool zero_init) {
   if (d.nx < 3 || d.ny < 3 || d.nz < 3) return;
-  const size_t lds = (size_t)d.nx * d.ny * sizeof(T);
-  if (lds <= GS27_LDS_MAX) {
-    const int per_colour = (d.nx - 2) * (d.ny - 2) / 4;
-    for (int it = 0; it < nsweeps; it++)
-      for (int ph = 0; ph < 2; ph++) {
-        const int pz = iadjoint ? 1 - ph : ph;
-        const int nplanes = pz == 0 ? (d.nz - 1) / 2 : (d.nz - 2) / 2;
-        if (nplanes <= 0) continue;
-        const int zs = (zero_init && it == 0) ? 1 : 0;
-        const int zn = (zs && ph == 0) ? 1 : 0;
-        if (per_colour > 256)
-          k_gs27_plane<T, 1024><<<nplanes, 1024, lds, s>>>(
-              d.nx, d.ny, d.nz, ac, cc, fc, x, pz, iadjoint, zs, zn);
-        else
-          k_gs27_plane<T, 256><<<nplanes, 256, lds, s>>>(
-              d.nx, d.ny, d.nz, ac, cc, fc, x, pz, iadjoint, zs, zn);
+  if (d.nx <= ROWS_MAXN) {
+    for (int it = 0; it < nsweeps; it++) {
+      int zmask = (zero_init && it == 0) ? 0xF : 0;
+      for (int ph = 0; ph < 4; ph++) {
+        const int cls = iadjoint ? 3 - ph : ph;
+        const int pz = cls >> 1, py = cls & 1;
+        const int nj = count_parity(d.ny, py), nk = count_parity(d.nz, pz);
+        const int nrows = nj * nk;
+        if (nrows > 0) {
+          const int nb = (nrows + 3) / 4;
+          if (iadjoint)
+            k_gs27_rows<T, 1><<<nb, 256, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, fc,
+                                                 x, pz, py, nj, nrows, zmask);
+          else
+            k_gs27_rows<T, 0><<<nb, 256, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, fc,
+                                                 x, pz, py, nj, nrows, zmask);
+        }
+        zmask &= ~(1 << cls);
       }
-    check_launch("gs27_plane");
+    }
+    check_launch("gs27_rows");
     return;
   }
   if (zero_init) fill<T>(x, d.n(), (T)0, s);
@@ -1360,85 +1388,118 @@ void buildP(const Dim &f, const Dim &c, int mgprol, int numdia,
 // ---------------------------------------------------------------------------
 // Setup: Galerkin coarse operator XA = P^T A P (VbuildG_7 / VbuildG_27)
 // ---------------------------------------------------------------------------
-// One thread per coarse point C, no LDS: for every fine point F = C_f + d of
-// the support of P_C and every stencil neighbour G = F + e, the product
-// P_C(d) * A(F, G) is scattered into the stored entries whose neighbour
-// C + delta has G in its support:
-//   XA(C, C+delta) += P_C(d) A(F, G) P_{C+delta}(d + e - 2 delta).
-// All index arithmetic is compile-time after unrolling; 14 accumulators.
-template <int ND>
-__global__ __launch_bounds__(256) void k_buildG(int nxf, int nyf, int nzf,
-                                                int nxc, int nyc, int nzc,
-                                                const double *__restrict__ pc,
-                                                const double *__restrict__ ac,
-                                                double *__restrict__ acc) {
-  const int ii = 2 + blockIdx.x * BX + threadIdx.x;
-  const int jj = 2 + blockIdx.y * BY + threadIdx.y;
-  const int kk = 2 + blockIdx.z;
-  if (ii > nxc - 1 || jj > nyc - 1) return;
-  const size_t nf = (size_t)nxf * nyf * nzf, nc = (size_t)nxc * nyc * nzc;
-  const size_t pC = ix(nxc, nyc, ii, jj, kk);
-  const size_t pCf = ix(nxf, nyf, 2 * ii - 1, 2 * jj - 1, 2 * kk - 1);
-  const long long sx = 1, sy = nxf, sz = (long long)nxf * nyf;
-  const long long cy = nxc, cz = (long long)nxc * nyc;
-  constexpr signed char DL[14][3] = {
-      {0, 0, 0},  {1, 0, 0},  {0, 1, 0},  {0, 0, 1},   {1, 1, 0},
-      {-1, 1, 0}, {1, 0, 1},  {-1, 0, 1}, {0, 1, 1},   {0, -1, 1},
-      {1, 1, 1},  {-1, 1, 1}, {1, -1, 1}, {-1, -1, 1}};
-  double xa[14];
-#pragma unroll
-  for (int s = 0; s < 14; s++) xa[s] = 0.0;
+// Stored direction delta of each of the 14 output planes.
+__host__ __device__ constexpr int gdelta(int s, int a) {
+  return s == 0   ? 0
+         : s == 1 ? (a == 0 ? 1 : 0)
+         : s == 2 ? (a == 1 ? 1 : 0)
+         : s == 3 ? (a == 2 ? 1 : 0)
+         : s == 4 ? (a == 2 ? 0 : 1)
+         : s == 5 ? (a == 0 ? -1 : (a == 1 ? 1 : 0))
+         : s == 6 ? (a == 1 ? 0 : 1)
+         : s == 7 ? (a == 0 ? -1 : (a == 1 ? 0 : 1))
+         : s == 8 ? (a == 0 ? 0 : 1)
+         : s == 9 ? (a == 0 ? 0 : (a == 1 ? -1 : 1))
+         : s == 10 ? 1
+         : s == 11 ? (a == 0 ? -1 : 1)
+         : s == 12 ? (a == 1 ? -1 : 1)
+                   : (a == 2 ? 1 : -1);
+}
+
+// One stored entry XA(C, C + delta) (delta = output plane S):
+//   sum over fine F = C_f + d in the support of P_C and stencil neighbours
+//   G = F + e in the support of P_{C+delta} of P_C(d) A(F, G) P_{C+delta}(q),
+//   q = d + e - 2 delta.
+// All (d, e) pairs are pruned at compile time.
+template <int ND, int S>
+__device__ inline double rap_entry(const double *__restrict__ pc,
+                                   const double *__restrict__ ac, size_t nf,
+                                   size_t nc, size_t pC, size_t pC2,
+                                   size_t pCf, long long sy, long long sz) {
+  double acc = 0.0;
 #pragma unroll
   for (int a = 0; a < 27; a++) {
     const int dx = a % 3 - 1, dy = (a / 3) % 3 - 1, dz = a / 9 - 1;
+    bool anyd = false;
+#pragma unroll
+    for (int b = 0; b < 27; b++) {
+      const int ex = b % 3 - 1, ey = (b / 3) % 3 - 1, ez = b / 9 - 1;
+      const int qx = dx + ex - 2 * gdelta(S, 0), qy = dy + ey - 2 * gdelta(S, 1),
+                qz = dz + ez - 2 * gdelta(S, 2);
+      const bool ok = (ND == 27 || (ex != 0) + (ey != 0) + (ez != 0) <= 1) &&
+                      qx >= -1 && qx <= 1 && qy >= -1 && qy <= 1 &&
+                      qz >= -1 && qz <= 1;
+      anyd |= ok;
+    }
+    if (!anyd) continue;
     const double wC = pc[(size_t)pplane(dx, dy, dz) * nc + pC];
-    const size_t pF = (size_t)((long long)pCf + dx * sx + dy * sy + dz * sz);
+    const size_t pF = (size_t)((long long)pCf + dx + dy * sy + dz * sz);
+    double part = 0.0;
 #pragma unroll
     for (int b = 0; b < 27; b++) {
       const int ex = b % 3 - 1, ey = (b / 3) % 3 - 1, ez = b / 9 - 1;
       const int ne = (ex != 0) + (ey != 0) + (ez != 0);
       if (ND == 7 && ne > 1) continue;
-      const int qx = dx + ex, qy = dy + ey, qz = dz + ez;  // G - C_f
-      bool any = false;
-#pragma unroll
-      for (int s = 0; s < 14; s++) {
-        const int tx = qx - 2 * DL[s][0], ty = qy - 2 * DL[s][1],
-                  tz = qz - 2 * DL[s][2];
-        any |= (tx >= -1 && tx <= 1 && ty >= -1 && ty <= 1 && tz >= -1 &&
-                tz <= 1);
-      }
-      if (!any) continue;
-      const double A = ne == 0 ? ac[pF]
-                               : -coupling<ND>(ac, nf, pF,
-                                               ex * sx + ey * sy + ez * sz,
-                                               ex, ey, ez);
-      const double wa = wC * A;
-#pragma unroll
-      for (int s = 0; s < 14; s++) {
-        const int tx = qx - 2 * DL[s][0], ty = qy - 2 * DL[s][1],
-                  tz = qz - 2 * DL[s][2];
-        if (tx < -1 || tx > 1 || ty < -1 || ty > 1 || tz < -1 || tz > 1)
-          continue;
-        const size_t pC2 =
-            (size_t)((long long)pC + DL[s][0] + DL[s][1] * cy + DL[s][2] * cz);
-        xa[s] += wa * pc[(size_t)pplane(tx, ty, tz) * nc + pC2];
-      }
+      const int qx = dx + ex - 2 * gdelta(S, 0), qy = dy + ey - 2 * gdelta(S, 1),
+                qz = dz + ez - 2 * gdelta(S, 2);
+      if (qx < -1 || qx > 1 || qy < -1 || qy > 1 || qz < -1 || qz > 1)
+        continue;
+      const double A =
+          ne == 0 ? ac[pF]
+                  : -coupling<ND>(ac, nf, pF, ex + ey * sy + ez * sz, ex, ey, ez);
+      part += A * pc[(size_t)pplane(qx, qy, qz) * nc + pC2];
     }
+    acc += wC * part;
   }
-#pragma unroll
-  for (int s = 0; s < 14; s++) acc[(size_t)s * nc + pC] = s == 0 ? xa[s] : -xa[s];
+  return acc;
+}
+
+// Grid: (coarse i tiles, coarse j, coarse k x 14 entries); one thread per
+// (coarse point, stored entry), so each coarse point's 14 entries run in 14
+// different wavefronts (enough of them in flight to hide the gather latency).
+template <int ND>
+__global__ __launch_bounds__(64) void k_buildG(int nxf, int nyf, int nzf,
+                                               int nxc, int nyc, int nzc,
+                                               const double *__restrict__ pc,
+                                               const double *__restrict__ ac,
+                                               double *__restrict__ acc) {
+  const int ii = 2 + blockIdx.x * 64 + threadIdx.x;
+  const int jj = 2 + blockIdx.y;
+  const int kk = 2 + blockIdx.z / 14;
+  const int sidx = blockIdx.z % 14;
+  if (ii > nxc - 1) return;
+  const size_t nf = (size_t)nxf * nyf * nzf, nc = (size_t)nxc * nyc * nzc;
+  const size_t pC = ix(nxc, nyc, ii, jj, kk);
+  const size_t pCf = ix(nxf, nyf, 2 * ii - 1, 2 * jj - 1, 2 * kk - 1);
+  const long long sy = nxf, sz = (long long)nxf * nyf;
+  const long long cy = nxc, cz = (long long)nxc * nyc;
+  double v = 0.0;
+#define RAP_CASE(S)                                                          \
+  case S: {                                                                  \
+    const size_t pC2 = (size_t)((long long)pC + gdelta(S, 0) +              \
+                                gdelta(S, 1) * cy + gdelta(S, 2) * cz);      \
+    v = rap_entry<ND, S>(pc, ac, nf, nc, pC, pC2, pCf, sy, sz);              \
+    break;                                                                   \
+  }
+  switch (sidx) {
+    RAP_CASE(0) RAP_CASE(1) RAP_CASE(2) RAP_CASE(3) RAP_CASE(4) RAP_CASE(5)
+    RAP_CASE(6) RAP_CASE(7) RAP_CASE(8) RAP_CASE(9) RAP_CASE(10)
+    RAP_CASE(11) RAP_CASE(12) RAP_CASE(13)
+  }
+#undef RAP_CASE
+  acc[(size_t)sidx * nc + pC] = sidx == 0 ? v : -v;
 }
 
 void buildG(const Dim &f, const Dim &c, int numdia, const double *pc,
             const double *acf, double *acc, hipStream_t s) {
   if (c.nx < 3 || c.ny < 3 || c.nz < 3) return;
-  dim3 g(cdiv(c.nx - 2, BX), cdiv(c.ny - 2, BY), c.nz - 2);
+  dim3 g(cdiv(c.nx - 2, 64), c.ny - 2, (c.nz - 2) * 14);
   if (numdia == 7)
-    k_buildG<7><<<g, dim3(BX, BY), 0, s>>>(f.nx, f.ny, f.nz, c.nx, c.ny, c.nz,
-                                           pc, acf, acc);
+    k_buildG<7><<<g, 64, 0, s>>>(f.nx, f.ny, f.nz, c.nx, c.ny, c.nz, pc, acf,
+                                 acc);
   else
-    k_buildG<27><<<g, dim3(BX, BY), 0, s>>>(f.nx, f.ny, f.nz, c.nx, c.ny,
-                                            c.nz, pc, acf, acc);
+    k_buildG<27><<<g, 64, 0, s>>>(f.nx, f.ny, f.nz, c.nx, c.ny, c.nz, pc, acf,
+                                  acc);
   check_launch("buildG");
 }
 

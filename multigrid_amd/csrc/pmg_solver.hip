@@ -145,6 +145,17 @@ struct pmg_ctx {
   bool have32 = false, haveN = false;
   int iters = 0, ierror = 0;
   double relres = 0.0, setup_ms = 0.0, solve_ms = 0.0;
+  // linear-solve session (pmg_ctx_solve and the serving loop)
+  struct Session {
+    int mode = PMG_FP64, iters = 0, rc = PMG_OK;
+    bool done = true, pending = false;
+    double rsden = 1.0, rsnrm = 0.0;
+  } ses;
+  hipEvent_t cyc_ev = nullptr, t0_ev = nullptr, t1_ev = nullptr;
+  // V-cycle bodies captured as HIP graphs, one per precision mode
+  bool graphs = true, capturing = false;
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  hipGraph_t graph[2] = {nullptr, nullptr};
   // optional per-kernel event timing of the fine smoothing sweeps
   bool ktime = false;
   std::vector<hipEvent_t> ev;
@@ -167,6 +178,13 @@ struct pmg_ctx {
   }
   ~pmg_ctx() {
     if (s) hipStreamSynchronize(s);
+    for (int m = 0; m < 2; m++) {
+      if (gexec[m]) hipGraphExecDestroy(gexec[m]);
+      if (graph[m]) hipGraphDestroy(graph[m]);
+    }
+    if (cyc_ev) hipEventDestroy(cyc_ev);
+    if (t0_ev) hipEventDestroy(t0_ev);
+    if (t1_ev) hipEventDestroy(t1_ev);
     for (void *ptr : allocs) hipFree(ptr);
     if (hscal) hipHostFree(hscal);
     for (auto e : ev) hipEventDestroy(e);
@@ -220,7 +238,8 @@ void smooth(pmg_ctx *c, const LV<T> &l, const T *f, T *x, int nu, int iadj,
     if (zero_init) fill<T>(x, l.d.n(), (T)0, c->s);
     return;
   }
-  bool timed = c->ktime && fine && c->nev + 2 <= (int)c->ev.size();
+  bool timed = c->ktime && !c->capturing && fine &&
+               c->nev + 2 <= (int)c->ev.size();
   if (timed) hipEventRecord(c->ev[c->nev], c->s);
   if (l.numdia == 7) {
     if (zero_init) fill<T>(x, l.d.n(), (T)0, c->s);
@@ -503,7 +522,74 @@ int ctx_create(pmg_ctx **out, const Params &p, const double *xf,
   return PMG_OK;
 }
 
-int ctx_solve(pmg_ctx *c, int mode) {
+// ---- linear solve as a session of enqueue / poll steps ----
+// One step enqueues one iteration of Vmvcs's loop (mgcsd.cc:299-527): the
+// V-cycle and the FP64 L1 stopping norm (for PMG_MIXED: the FP32 V-cycle on
+// the FP64 residual, the FP64 update, the new FP64 residual and its norm),
+// then an 8-byte copy of the norm to pinned host memory and an event.  The
+// host decides whether to continue when the event has completed.  The step
+// body is captured once per mode as a HIP graph and replayed.
+
+void views(pmg_ctx *c, LV<double> *L64, LV<float> *L32, int mode) {
+  for (int l = 0; l < c->nlev; l++) {
+    L64[l] = view<double>(c->L[l]);
+    if (mode == PMG_MIXED) L32[l] = view<float>(c->L[l]);
+  }
+  L64[0].x = c->u;
+}
+
+void step_body(pmg_ctx *c, int mode) {
+  LV<double> L64[MAXLEV];
+  LV<float> L32[MAXLEV];
+  views(c, L64, L32, mode);
+  Level &F = c->L[0];
+  const int nlev = c->nlev;
+  const double *cc64c = c->L[nlev - 1].cc;
+  const Red rnrm = red_to(c, S_RSNRM);
+  if (mode == PMG_MIXED) {
+    float *r32 = F.w032;
+    fill<float>(L32[0].x, F.d.n(), 0.0f, c->s);
+    if (nlev == 1)
+      cg_single<float>(F.d, F.numdia, F.ac, F.cc, r32, L32[0].x, 100, 0.0,
+                       c->s);
+    else
+      vcycle<float>(c, L32, r32, c->L[nlev - 1].ac, cc64c);
+    add_correction<float>(F.d, L32[0].x, c->u, c->s);
+    resid<double, float>(F.d, F.numdia, F.ac, F.cc, F.fc, c->u, r32, &rnrm,
+                         c->s);
+  } else {
+    if (nlev == 1)
+      cg_single<double>(F.d, F.numdia, F.ac, F.cc, F.fc, c->u, 100, 0.0, c->s);
+    else
+      vcycle<double>(c, L64, F.fc, c->L[nlev - 1].ac, cc64c);
+    resid<double, double>(F.d, F.numdia, F.ac, F.cc, F.fc, c->u,
+                          (double *)nullptr, &rnrm, c->s);
+  }
+}
+
+int capture(pmg_ctx *c, int mode) {
+  if (c->gexec[mode]) return PMG_OK;
+  if (hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal) !=
+      hipSuccess)
+    return PMG_EDEVICE;
+  c->capturing = true;
+  step_body(c, mode);
+  c->capturing = false;
+  hipGraph_t g = nullptr;
+  if (hipStreamEndCapture(c->s, &g) != hipSuccess || !g) return PMG_EDEVICE;
+  if (hipGraphInstantiate(&c->gexec[mode], g, nullptr, nullptr, 0) !=
+      hipSuccess) {
+    hipGraphDestroy(g);
+    c->gexec[mode] = nullptr;
+    return PMG_EDEVICE;
+  }
+  c->graph[mode] = g;
+  return PMG_OK;
+}
+
+// Start a solve: Vazeros + VfboundPMG00 on u (mgdrvd.cc:419, 439), the
+// stopping-test denominator (mgcsd.cc:133-163).  Blocks for two scalars.
+int lin_begin(pmg_ctx *c, int mode) {
   const Params &p = c->p;
   Level &F = c->L[0];
   if (mode == PMG_MIXED) {
@@ -511,42 +597,90 @@ int ctx_solve(pmg_ctx *c, int mode) {
     if (rc) return rc;
     convert_ops32(c);
   }
-  if (c->nlev > 1 || true) {
-    const Dim &cd = c->L[c->nlev - 1].d;
-    if (!cg_fits(cd)) return PMG_EBADPARM;
+  if (!cg_fits(c->L[c->nlev - 1].d)) return PMG_EBADPARM;
+  if (!c->cyc_ev) {
+    hipEventCreateWithFlags(&c->cyc_ev, hipEventDisableTiming);
+    hipEventCreate(&c->t0_ev);
+    hipEventCreate(&c->t1_ev);
   }
-  LV<double> L64[MAXLEV];
-  LV<float> L32[MAXLEV];
-  for (int l = 0; l < c->nlev; l++) {
-    L64[l] = view<double>(c->L[l]);
-    if (mode == PMG_MIXED) L32[l] = view<float>(c->L[l]);
-  }
-  const double *cc64c = c->L[c->nlev - 1].cc;
-  hipEvent_t e0, e1;
-  hipEventCreate(&e0);
-  hipEventCreate(&e1);
+  auto &S = c->ses;
+  S = pmg_ctx::Session{};
+  S.mode = mode;
+  S.done = false;
   c->nev = 0;
-  hipEventRecord(e0, c->s);
-  // Vazeros + VfboundPMG00 on u (mgdrvd.cc:419, 439)
+  hipEventRecord(c->t0_ev, c->s);
   fill<double>(c->u, F.d.n(), 0.0, c->s);
-  int rc;
-  if (mode == PMG_MIXED) {
-    rc = linear_mixed(c, L64, L32, F.fc, cc64c, c->u, p.istop, p.errtol,
-                      p.itmax, &c->iters, &c->relres);
-  } else {
-    L64[0].x = c->u;
-    rc = linear_fp64(c, L64, F.fc, cc64c, p.istop, p.errtol, p.itmax,
-                     &c->iters, &c->relres);
+  if (p.istop == 1) {
+    const Red rden = red_to(c, S_RSDEN);
+    nrm1<double>(F.d, F.fc, rden, c->s);
+    S.rsden = host_scalar(c, S_RSDEN);
   }
-  // VfboundPMG(ibound=1): Dirichlet values on u (mgdrvd.cc:500-502)
+  if (S.rsden == 0.0) S.rsden = 1.0;  // mgcsd.cc:154-157
+  S.rsnrm = S.rsden;
+  if (mode == PMG_MIXED) {
+    // residual of u = 0, the first right-hand side of the FP32 cycle
+    const Red rnrm = red_to(c, S_RSNRM);
+    resid<double, float>(F.d, F.numdia, F.ac, F.cc, F.fc, c->u, F.w032, &rnrm,
+                         c->s);
+    S.rsnrm = host_scalar(c, S_RSNRM);
+    // while (iters < itmax && relres > errtol): maybe nothing to do
+    if (!(S.iters < p.itmax && S.rsnrm / S.rsden > p.errtol)) S.done = true;
+  }
+  if (c->graphs && !c->ktime && !S.done) {
+    int rc = capture(c, mode);
+    if (rc) return rc;
+  }
+  return PMG_OK;
+}
+
+void lin_enqueue(pmg_ctx *c) {
+  auto &S = c->ses;
+  if (S.done || S.pending) return;
+  if (c->gexec[S.mode] && !c->ktime)
+    hipGraphLaunch(c->gexec[S.mode], c->s);
+  else
+    step_body(c, S.mode);
+  hipMemcpyAsync(c->hscal + S_RSNRM, c->scal + S_RSNRM, sizeof(double),
+                 hipMemcpyDeviceToHost, c->s);
+  hipEventRecord(c->cyc_ev, c->s);
+  S.pending = true;
+}
+
+// Returns 1 when the session has finished (after this poll), 0 otherwise.
+int lin_poll(pmg_ctx *c, bool wait) {
+  auto &S = c->ses;
+  const Params &p = c->p;
+  if (S.done) return 1;
+  if (!S.pending) return 0;
+  if (wait) {
+    hipEventSynchronize(c->cyc_ev);
+  } else if (hipEventQuery(c->cyc_ev) != hipSuccess) {
+    return 0;
+  }
+  S.pending = false;
+  S.iters++;
+  S.rsnrm = c->hscal[S_RSNRM];
+  const bool more = (c->nlev > 1 || S.mode == PMG_MIXED) && S.iters < p.itmax &&
+                    S.rsnrm / S.rsden > p.errtol;
+  if (!more) {
+    S.done = true;
+    const bool conv = S.rsnrm / S.rsden <= p.errtol;
+    S.rc = (S.iters < p.itmax || conv) ? PMG_OK : PMG_ITMAX;
+    if (c->nlev == 1 && S.mode == PMG_FP64) S.rc = PMG_OK;
+  }
+  return S.done ? 1 : 0;
+}
+
+// VfboundPMG(ibound=1): Dirichlet values on u (mgdrvd.cc:500-502); stats.
+int lin_end(pmg_ctx *c) {
+  auto &S = c->ses;
+  Level &F = c->L[0];
   fbound_dirichlet(F.d, c->u, c->gx, c->gy, c->gz, c->s);
-  hipEventRecord(e1, c->s);
+  hipEventRecord(c->t1_ev, c->s);
   hipError_t se = hipStreamSynchronize(c->s);
   float ms = 0.f;
-  hipEventElapsedTime(&ms, e0, e1);
+  hipEventElapsedTime(&ms, c->t0_ev, c->t1_ev);
   c->solve_ms = ms;
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
   if (c->ktime) {
     for (int q = 0; q + 1 < c->nev; q += 2) {
       float kms = 0.f;
@@ -555,8 +689,20 @@ int ctx_solve(pmg_ctx *c, int mode) {
     }
   }
   if (se != hipSuccess) return PMG_EDEVICE;
-  c->ierror = rc == PMG_ITMAX ? 1 : 0;
-  return rc;
+  c->iters = S.iters;
+  c->relres = S.rsnrm / S.rsden;
+  c->ierror = S.rc == PMG_ITMAX ? 1 : 0;
+  return S.rc;
+}
+
+int ctx_solve(pmg_ctx *c, int mode) {
+  int rc = lin_begin(c, mode);
+  if (rc) return rc;
+  while (!c->ses.done) {
+    lin_enqueue(c);
+    lin_poll(c, true);
+  }
+  return lin_end(c);
 }
 
 // ---- Newton (Vnewton, newtond.cc:140-486) ----
@@ -1109,6 +1255,12 @@ int pmg_ctx_ktime_enable(pmg_ctx *ctx, int max_events) {
   ctx->ktime = true;
   ctx->ktime_ms = 0.0;
   ctx->ktime_count = 0;
+  return PMG_OK;
+}
+
+int pmg_ctx_set_graphs(pmg_ctx *ctx, int on) {
+  if (!ctx) return PMG_EBADPARM;
+  ctx->graphs = on != 0;
   return PMG_OK;
 }
 

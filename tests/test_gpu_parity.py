@@ -55,13 +55,13 @@ def test_gsrb(mg, orc, numdia, iadj):
     assert _rel(x, orc.gsrb(n, numdia, ac, cc, f, x0, 3, iadj)) < 1e-12
 
 
-@pytest.mark.parametrize("numdia", [7, 27])
-def test_gsrb_large(mg, orc, numdia):
-    """97^3: several workgroups and k-chunks for the fused 7-point sweep; the
-    per-colour 27-point path (an FP64 97^2 plane exceeds the LDS budget)."""
-    print(f"gsrb large n=97 numdia={numdia}", flush=True)
-    n = 97
-    rng = np.random.default_rng(97 + numdia)
+@pytest.mark.parametrize("numdia,n", [(7, 97), (27, 97), (27, 131)])
+def test_gsrb_large(mg, orc, numdia, n):
+    """Several workgroups and k-chunks for the fused 7-point sweep (97^3);
+    the row-parallel 27-point sweep at 97^3 and its per-colour fallback for
+    rows longer than 129 points (131^3)."""
+    print(f"gsrb large n={n} numdia={numdia}", flush=True)
+    rng = np.random.default_rng(n + numdia)
     ac, cc = H.random_op_vec(n, numdia, rng)
     mask = np.zeros((n, n, n)); mask[1:-1, 1:-1, 1:-1] = 1
     f = (rng.standard_normal((n, n, n)) * mask).ravel()
