@@ -149,18 +149,49 @@ int pmg_buildP(int nxf, int nyf, int nzf, int nxc, int nyc, int nzc,
 int pmg_buildG(int nxf, int nyf, int nzf, int nxc, int nyc, int nzc,
                int numdia, const double *pc, const double *acf, double *acc);
 
-/* --- synthetic Poisson-Boltzmann inputs (no reference counterpart: the
- * reference assembles them from PQR files in PBE_pmgc_class.h:191-344) ---
- * Fills the fine-level Vmgdriv inputs for a seeded synthetic molecule.
- * charge_mode 0: +-1 with net zero; 1: uniform over {-1,-0.5,0,0.5,1}.
- * gxcf/gycf/gzcf need 4*ny*nz, 4*nx*nz, 4*nx*ny entries; others n^3,
- * xf/yf/zf n.  Host-only; needs no GPU. */
+/* --- Poisson-Boltzmann inputs from an atom list (no reference
+ * counterpart as a C call: the reference assembles them inside its solver
+ * class, PBE_pmgc_class.h:191-344, from atoms read out of PQR files) --- */
+
+/* Seeded synthetic molecule: natoms x {x, y, z, radius, charge} into atoms.
+ * Atoms uniform in a ball of radius sphere_r, radii U[1.2, 2.0] A;
+ * charge_mode 0: +-1 with net zero, 1: uniform over {-1,-0.5,0,0.5,1}.
+ * Host-only; needs no GPU. */
+int pmg_pbe_molecule(int natoms, double sphere_r, int charge_mode,
+                     uint64_t seed, double *atoms);
+
+/* Assemble the fine-level Vmgdriv inputs of an n^3 grid of spacing h
+ * (centred on the origin) on the GPU: dielectric faces (2 inside the atom
+ * spheres, 78.54 outside, harmonic averages), kappa^2 map for a 1:1 salt of
+ * ionic_strength mol/L with a 2 A ion-exclusion shell, trilinear charge
+ * source 4*pi*lambda_B*q/h^3, Debye-Hueckel boundary faces.  Host arrays:
+ * xf/yf/zf n; gxcf/gycf/gzcf the two planes (low, high) of each face,
+ * 2*n*n entries (the (n,n,4) arrays of the driver hold them first);
+ * a1cf..tcf n^3 (tcf zeroed).  Any output may be NULL. */
+int pmg_pbe_assemble(int n, double h, double ionic_strength, int natoms,
+                     const double *atoms, double *xf, double *yf, double *zf,
+                     double *gxcf, double *gycf, double *gzcf, double *a1cf,
+                     double *a2cf, double *a3cf, double *ccf, double *fcf,
+                     double *tcf);
+
+/* pmg_pbe_molecule + pmg_pbe_assemble; atoms (5*natoms) may be NULL. */
 int pmg_pbe_synth(int n, double h, int natoms, double sphere_r,
                   double ionic_strength, int charge_mode, uint64_t seed,
                   double *xf, double *yf, double *zf, double *gxcf,
                   double *gycf, double *gzcf, double *a1cf, double *a2cf,
                   double *a3cf, double *ccf, double *fcf, double *tcf,
-                  double *atoms /* NULL or 5*natoms: x,y,z,radius,charge */);
+                  double *atoms);
+
+/* Create a context whose fine-level inputs are assembled on the GPU from
+ * atoms (as pmg_pbe_assemble; nx = ny = nz required), then build the
+ * hierarchy.  Only the atom list crosses PCIe. */
+int pmg_ctx_create_pbe(pmg_ctx **out, const int *iparm, const double *rparm,
+                       double h, double ionic_strength, int natoms,
+                       const double *atoms);
+/* Re-assemble a context's inputs for another molecule on the same grid
+ * size and rebuild its hierarchy (graphs and buffers are kept). */
+int pmg_ctx_load_pbe(pmg_ctx *ctx, double h, double ionic_strength,
+                     int natoms, const double *atoms);
 
 /* Version string of the library build. */
 const char *pmg_version(void);

@@ -84,6 +84,10 @@ def _declare(L):
         "pmg_buildP": (i, [i, i, i, i, i, i, i, i, dp, dp]),
         "pmg_buildG": (i, [i, i, i, i, i, i, i, dp, dp, dp]),
         "pmg_pbe_synth": (i, [i, d, i, d, d, i, ctypes.c_uint64] + [dp] * 13),
+        "pmg_pbe_molecule": (i, [i, d, i, ctypes.c_uint64, dp]),
+        "pmg_pbe_assemble": (i, [i, d, d, i, dp] + [dp] * 12),
+        "pmg_ctx_create_pbe": (i, [ctypes.POINTER(vp), ip, dp, d, d, i, dp]),
+        "pmg_ctx_load_pbe": (i, [vp, d, d, i, dp]),
         "pmg_version": (ctypes.c_char_p, []),
     }
     for name, (res, args) in sig.items():
@@ -199,17 +203,33 @@ def last_stats():
 class Context:
     """A solve whose inputs and hierarchy stay resident in HBM."""
 
-    def __init__(self, iparm, rparm, inp):
+    def __init__(self, iparm, rparm, inp=None, pbe=None):
+        """inp: dict of host input arrays (pmg_ctx_create), or
+        pbe=(h, ionic_strength, atoms): inputs assembled on the GPU from an
+        (natoms, 5) atom array (pmg_ctx_create_pbe)."""
         self._h = ctypes.c_void_p()
         L = lib()
+        self.n = int(iparm[2]), int(iparm[3]), int(iparm[4])
+        self.nlev = int(iparm[5])
+        if pbe is not None:
+            h, ionic, atoms = pbe
+            at = np.ascontiguousarray(atoms, np.float64).reshape(-1)
+            rc = L.pmg_ctx_create_pbe(ctypes.byref(self._h), _ip(iparm), _dp(rparm),
+                                      h, ionic, at.size // 5, _dp(at))
+            _check(rc, "pmg_ctx_create_pbe")
+            return
         rc = L.pmg_ctx_create(ctypes.byref(self._h), _ip(iparm), _dp(rparm),
                               _dp(inp["xf"]), _dp(inp["yf"]), _dp(inp["zf"]),
                               _dp(inp["gxcf"]), _dp(inp["gycf"]), _dp(inp["gzcf"]),
                               _dp(inp["a1cf"]), _dp(inp["a2cf"]), _dp(inp["a3cf"]),
                               _dp(inp["ccf"]), _dp(inp["fcf"]))
         _check(rc, "pmg_ctx_create")
-        self.n = int(iparm[2]), int(iparm[3]), int(iparm[4])
-        self.nlev = int(iparm[5])
+
+    def load_pbe(self, h, ionic_strength, atoms):
+        """Re-assemble the inputs for another molecule and rebuild."""
+        at = np.ascontiguousarray(atoms, np.float64).reshape(-1)
+        return _check(lib().pmg_ctx_load_pbe(self._h, h, ionic_strength, at.size // 5,
+                                             _dp(at)), "pmg_ctx_load_pbe")
 
     def solve(self, mode=PMG_FP64):
         return _check(lib().pmg_ctx_solve(self._h, mode), "pmg_ctx_solve")
@@ -275,24 +295,44 @@ class Context:
 
 
 # ---------------------------------------------------------------------------
-# Synthetic PB inputs (host, no GPU needed)
+# Poisson-Boltzmann inputs: synthetic molecules (host) and their assembly
+# into grid inputs (GPU)
 # ---------------------------------------------------------------------------
-def pbe_synth(n, h, natoms, sphere_r, ionic_strength=0.15, charge_mode=0, seed=20261202):
-    """Fine-level Vmgdriv inputs for a seeded synthetic molecule."""
-    N = n * n * n
+def pbe_molecule(natoms, sphere_r, charge_mode=0, seed=20261202):
+    """Seeded synthetic molecule, (natoms, 5) = x, y, z, radius, charge."""
+    atoms = np.zeros(5 * max(natoms, 1))
+    _check(lib().pmg_pbe_molecule(natoms, sphere_r, charge_mode, ctypes.c_uint64(seed),
+                                  _dp(atoms)), "pbe_molecule")
+    return atoms[: 5 * natoms].reshape(natoms, 5)
+
+
+def _pbe_arrays(n):
     out = {k: np.zeros(n) for k in ("xf", "yf", "zf")}
     for k in ("gxcf", "gycf", "gzcf"):
         out[k] = np.zeros(4 * n * n)
     for k in ("a1cf", "a2cf", "a3cf", "ccf", "fcf", "tcf"):
-        out[k] = np.zeros(N)
-    atoms = np.zeros(5 * max(natoms, 1))
-    rc = lib().pmg_pbe_synth(n, h, natoms, sphere_r, ionic_strength, charge_mode,
-                             ctypes.c_uint64(seed), *(_dp(out[k]) for k in (
-                                 "xf", "yf", "zf", "gxcf", "gycf", "gzcf", "a1cf",
-                                 "a2cf", "a3cf", "ccf", "fcf", "tcf")), _dp(atoms))
-    _check(rc, "pbe_synth")
-    out["atoms"] = atoms[: 5 * natoms].reshape(natoms, 5)
+        out[k] = np.zeros(n * n * n)
     return out
+
+
+PBE_KEYS = ("xf", "yf", "zf", "gxcf", "gycf", "gzcf", "a1cf", "a2cf", "a3cf",
+            "ccf", "fcf", "tcf")
+
+
+def pbe_assemble(n, h, ionic_strength, atoms):
+    """Fine-level Vmgdriv inputs assembled on the GPU from an atom array."""
+    at = np.ascontiguousarray(atoms, np.float64).reshape(-1)
+    out = _pbe_arrays(n)
+    rc = lib().pmg_pbe_assemble(n, h, ionic_strength, at.size // 5, _dp(at),
+                                *(_dp(out[k]) for k in PBE_KEYS))
+    _check(rc, "pbe_assemble")
+    out["atoms"] = at.reshape(-1, 5)
+    return out
+
+
+def pbe_synth(n, h, natoms, sphere_r, ionic_strength=0.15, charge_mode=0, seed=20261202):
+    """Fine-level inputs for a seeded synthetic molecule (GPU assembly)."""
+    return pbe_assemble(n, h, ionic_strength, pbe_molecule(natoms, sphere_r, charge_mode, seed))
 
 
 # Benchmark configurations (sizes and molecule parameters of the five

@@ -126,4 +126,17 @@ void buildP(const Dim &f, const Dim &c, int mgprol, int numdia,
 void buildG(const Dim &f, const Dim &c, int numdia, const double *pc,
             const double *acf, double *acc, hipStream_t s);
 
+// ---- Poisson-Boltzmann input assembly from atoms (PBE_pmgc_class.h:191-344)
+// atoms: device array natoms x {x, y, z, radius, charge}; next: natoms ints;
+// scratch: pbe_scratch_bytes(n).  Writes the fine-level inputs of an n^3
+// grid of spacing h centred on the origin; gx/gy/gz get their two planes
+// (low, high).  Returns 0, or 2 for invalid arguments.
+void pbe_constants(double ionic, double *lambda_b, double *kappa2);
+size_t pbe_scratch_bytes(int n);
+int pbe_assemble(int n, double h, double ionic, int natoms,
+                 const double *atoms, int *next, void *scratch, double *xf,
+                 double *yf, double *zf, double *gx, double *gy, double *gz,
+                 double *a1, double *a2, double *a3, double *ccf, double *fcf,
+                 hipStream_t s);
+
 }  // namespace pmg

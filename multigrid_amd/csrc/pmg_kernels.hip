@@ -32,21 +32,47 @@ static void check_launch(const char *what) {
 }
 
 // ---------------------------------------------------------------------------
-// Block reduction (deterministic for a fixed block shape)
+// Wavefront and block reductions (deterministic for a fixed block shape)
 // ---------------------------------------------------------------------------
+// One DPP lane move of a double (both 32-bit halves with the same control);
+// lanes without a source, or in rows outside ROWMASK, read 0.
+template <int CTRL, int ROWMASK>
+__device__ inline double dpp_f64(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL,
+                                             ROWMASK, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL,
+                                             ROWMASK, 0xF, true);
+  return __hiloint2double(hi, lo);
+}
+
+// Sum over the 64 lanes of a wavefront, returned in every lane.  Row
+// prefix sums by row_shr 1, 2, 4, 8 leave each 16-lane row's total in its
+// lane 15; row_bcast15 and row_bcast31 fold the four rows into lane 63,
+// which is then read out.  All moves stay in the VALU (no LDS round trips
+// as with lane shuffles), and the order of additions is fixed.
+__device__ inline double wave_sum(double v) {
+  v += dpp_f64<0x111, 0xF>(v);  // row_shr:1
+  v += dpp_f64<0x112, 0xF>(v);  // row_shr:2
+  v += dpp_f64<0x114, 0xF>(v);  // row_shr:4
+  v += dpp_f64<0x118, 0xF>(v);  // row_shr:8
+  v += dpp_f64<0x142, 0xA>(v);  // row_bcast:15 into rows 1, 3
+  v += dpp_f64<0x143, 0xC>(v);  // row_bcast:31 into rows 2, 3
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+  return __hiloint2double(hi, lo);
+}
+
 template <int NT>
 __device__ inline double block_sum(double v, double *red) {
   const int tid = threadIdx.x + threadIdx.y * blockDim.x;
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  v = wave_sum(v);
   const int lane = tid & 63, wid = tid >> 6;
   if (lane == 0) red[wid] = v;
   __syncthreads();
   double t = 0.0;
   if (wid == 0) {
     t = lane < NT / 64 ? red[lane] : 0.0;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
+    t = wave_sum(t);
   }
   __syncthreads();
   return t;  // valid in thread 0
@@ -864,8 +890,7 @@ bool cg_fits(const Dim &d) {
 // Consecutive calls must alternate buf (see the barrier argument in the
 // CG loop below).
 __device__ inline double cg_allsum(double v, double *buf, int nw) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  v = wave_sum(v);
   if ((threadIdx.x & 63) == 0) buf[threadIdx.x >> 6] = v;
   __syncthreads();
   double t = 0.0;

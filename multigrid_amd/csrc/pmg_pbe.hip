@@ -1,0 +1,348 @@
+// pmg_pbe.hip -- Poisson-Boltzmann input assembly on the GPU.
+//
+// Turns an atom list {x, y, z, radius, charge} into the fine-level
+// arguments of the PMG driver (a1cf, a2cf, a3cf, ccf, fcf, the six
+// Debye-Hueckel faces and the node coordinates), as the reference does in
+// PBE_pmgc_class.h:191-344 with the benchmark's molecule conventions:
+//   - nodes x_i = c0 + i*h with c0 = -(n-1)h/2;
+//   - a node is inside an atom when |x-a|^2 <= r^2 and inside its ion
+//     exclusion shell when |x-a|^2 <= (r+2)^2 (the inclusive sphere test of
+//     dot_finder.h:24-87; shell as PBE_pmgc_class.h:276-281);
+//   - eps = 2 inside, 78.54 outside; a1cf(i) = harmonic_average(eps(i),
+//     eps(i+1)) (scientific_quantities.h:30-32), the node value on the last
+//     plane;
+//   - ccf = 0 inside the shell, eps_w*kappa^2 outside (PBE_pmgc_class.h:293-297);
+//   - fcf: trilinear spreading of 4*pi*lambda_B*q over the atom's cell
+//     corners, summed over atoms in list order
+//     (create_charge_distribution.h:16-61, PBE_pmgc_class.h:331-339);
+//   - faces: Debye-Hueckel spheres summed in list order
+//     (PBE_pmgc_class.h:173-189, Debye_Hueckel_functions.h:10-14).
+//
+// Kernels.  The reference marks spheres per atom and spreads charges in a
+// serial single_task.  Here:
+//   k_pb_init   clears the per-node cell-list heads and the two marks;
+//   k_pb_mark   one workgroup per atom writes the marks of the nodes in its
+//               shell's bounding box (idempotent byte stores, no atomics)
+//               and pushes the atom onto the list of its cell's lower corner;
+//   k_pb_nodes  one thread per node writes a1cf..fcf in one coalesced pass;
+//               the charge of a node is gathered from the lists of its (up to)
+//               8 cells, taken in increasing atom index, so the sum has the
+//               reference's serial order and the result is deterministic;
+//   k_pb_faces  one thread per boundary node sums all atoms, staged through
+//               LDS in chunks.
+// Floating-point contraction is off in these kernels so that every
+// coordinate, weight and mark is the same double as the serial restatement
+// computes.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+
+#include "pmg_internal.h"
+
+namespace pmg {
+
+namespace {
+
+constexpr double EPS_IN = 2.0, EPS_OUT = 78.54, ION_SHELL = 2.0;
+constexpr double PI_D = 3.14159265358979323846;
+
+inline int cdivp(long long a, int b) { return (int)((a + b - 1) / b); }
+
+__device__ inline double harmonic_average(double x, double y) {
+#pragma clang fp contract(off)
+  return 2 * x * y / (x + y);
+}
+
+__global__ void k_pb_init(size_t N, int *head, uint8_t *inside,
+                          uint8_t *excl) {
+  for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < N;
+       p += (size_t)gridDim.x * blockDim.x) {
+    head[p] = -1;
+    inside[p] = 0;
+    excl[p] = 0;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_pb_mark(
+    int n, double c0, double h, int natoms, const double *__restrict__ atoms,
+    int *head, int *next, uint8_t *inside, uint8_t *excl) {
+#pragma clang fp contract(off)
+  const int a = blockIdx.x;
+  if (a >= natoms) return;
+  const double *A = atoms + 5 * (size_t)a;
+  const double ax = A[0], ay = A[1], az = A[2], r = A[3];
+  const double re = r + ION_SHELL;
+  int lo[3], hi[3];
+  const double P[3] = {ax, ay, az};
+#pragma unroll
+  for (int d = 0; d < 3; d++) {  // a superset of the shell's nodes
+    double l = floor((P[d] - re - c0) / h) - 1.0;
+    double u = ceil((P[d] + re - c0) / h) + 1.0;
+    if (!(l >= 0.0)) l = 0.0;
+    if (!(u <= n - 1.0)) u = n - 1.0;
+    lo[d] = (int)l, hi[d] = (int)u;
+  }
+  if (threadIdx.x == 0) {  // push onto the list of the cell's lower corner
+    int c[3];
+    bool ok = true;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      double fl = floor((P[d] - c0) / h);
+      if (!(fl >= 0.0 && fl <= (double)(n - 2))) ok = false;
+      c[d] = ok ? (int)fl : 0;
+    }
+    if (ok) {
+      size_t q = ((size_t)c[2] * n + c[1]) * n + c[0];
+      next[a] = atomicExch(&head[q], a);
+    }
+  }
+  const int bx = hi[0] - lo[0] + 1, by = hi[1] - lo[1] + 1,
+            bz = hi[2] - lo[2] + 1;
+  if (bx <= 0 || by <= 0 || bz <= 0) return;
+  const int vol = bx * by * bz;
+  const double re2 = re * re, r2 = r * r;
+  for (int t = threadIdx.x; t < vol; t += blockDim.x) {
+    const int i = lo[0] + t % bx, rem = t / bx;
+    const int j = lo[1] + rem % by, k = lo[2] + rem / by;
+    const double dx = (c0 + i * h) - ax, dy = (c0 + j * h) - ay,
+                 dz = (c0 + k * h) - az;
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    const size_t p = ((size_t)k * n + j) * n + i;
+    if (d2 <= re2) excl[p] = 1;
+    if (d2 <= r2) inside[p] = 1;
+  }
+}
+
+// Charge atom a puts on corner (di,dj,dk) of its cell.
+__device__ inline double corner_charge(const double *__restrict__ atoms, int a,
+                                       int corner, double c0, double h,
+                                       double q4, double h3) {
+#pragma clang fp contract(off)
+  const double *A = atoms + 5 * (size_t)a;
+  double w = 1.0;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const double g = (A[d] - c0) / h;
+    const int c = (int)floor(g);
+    const double wl = (c + 1) - g, wu = g - c;
+    w *= ((corner >> d) & 1) ? wu : wl;
+  }
+  const double cq = q4 * A[4];
+  return cq * w / h3;
+}
+
+__global__ __launch_bounds__(256) void k_pb_nodes(
+    int n, double c0, double h, double q4, double h3, double kap2e,
+    const double *__restrict__ atoms, const int *__restrict__ head,
+    const int *__restrict__ next, const uint8_t *__restrict__ inside,
+    const uint8_t *__restrict__ excl, double *xf, double *yf, double *zf,
+    double *a1, double *a2, double *a3, double *ccf, double *fcf) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  const int j = blockIdx.y * 4 + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i >= n || j >= n) return;
+  const size_t nn = (size_t)n * n;
+  const size_t p = (size_t)k * nn + (size_t)j * n + i;
+  if (j == 0 && k == 0) xf[i] = yf[i] = zf[i] = c0 + i * h;
+  const double e0 = inside[p] ? EPS_IN : EPS_OUT;
+  a1[p] = i + 1 < n ? harmonic_average(e0, inside[p + 1] ? EPS_IN : EPS_OUT)
+                    : e0;
+  a2[p] = j + 1 < n ? harmonic_average(e0, inside[p + n] ? EPS_IN : EPS_OUT)
+                    : e0;
+  a3[p] = k + 1 < n ? harmonic_average(e0, inside[p + nn] ? EPS_IN : EPS_OUT)
+                    : e0;
+  ccf[p] = excl[p] ? 0.0 : kap2e;
+  // charge: atoms of the cells whose corner (di,dj,dk) is this node, in
+  // increasing index (selection over the unordered lists)
+  double acc = 0.0;
+  int last = -1;
+  for (;;) {
+    int best = INT_MAX, bc = 0;
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+      const int qi = i - (c & 1), qj = j - ((c >> 1) & 1),
+                qk = k - ((c >> 2) & 1);
+      if (qi < 0 || qj < 0 || qk < 0 || qi > n - 2 || qj > n - 2 ||
+          qk > n - 2)
+        continue;
+      const size_t q = (size_t)qk * nn + (size_t)qj * n + qi;
+      for (int a = head[q]; a >= 0; a = next[a])
+        if (a > last && a < best) best = a, bc = c;
+    }
+    if (best == INT_MAX) break;
+    acc += corner_charge(atoms, best, bc, c0, h, q4, h3);
+    last = best;
+  }
+  fcf[p] = acc;
+}
+
+constexpr int FACE_NT = 256;
+
+__global__ __launch_bounds__(FACE_NT) void k_pb_faces(
+    int n, double c0, double h, double lb, double kap, int natoms,
+    const double *__restrict__ atoms, double *gx, double *gy, double *gz) {
+#pragma clang fp contract(off)
+  __shared__ double sx[FACE_NT], sy[FACE_NT], sz[FACE_NT], sr[FACE_NT],
+      sc[FACE_NT];
+  const int f = blockIdx.y;
+  const long long nn = (long long)n * n;
+  const long long idx = (long long)blockIdx.x * FACE_NT + threadIdx.x;
+  const bool valid = idx < nn;
+  const int a = valid ? (int)(idx % n) : 0, b = valid ? (int)(idx / n) : 0;
+  const double lo = c0, hi = c0 + (n - 1) * h;
+  const double va = c0 + a * h, vb = c0 + b * h;
+  const double side = (f & 1) ? hi : lo;
+  double X, Y, Z;
+  if ((f >> 1) == 0) X = side, Y = va, Z = vb;
+  else if ((f >> 1) == 1) X = va, Y = side, Z = vb;
+  else X = va, Y = vb, Z = side;
+  double s = 0.0;
+  for (int base = 0; base < natoms; base += FACE_NT) {
+    const int m = min(FACE_NT, natoms - base);
+    __syncthreads();
+    if ((int)threadIdx.x < m) {
+      const double *A = atoms + 5 * (size_t)(base + threadIdx.x);
+      sx[threadIdx.x] = A[0];
+      sy[threadIdx.x] = A[1];
+      sz[threadIdx.x] = A[2];
+      sr[threadIdx.x] = A[3];
+      // lambda_B q / (eps_w (1 + kappa r)): the atom's factor of the
+      // Debye-Hueckel term, the same double for every boundary node
+      sc[threadIdx.x] = lb * A[4] / (EPS_OUT * (1.0 + kap * A[3]));
+    }
+    __syncthreads();
+    if (valid)
+      for (int t = 0; t < m; t++) {
+        const double dx = X - sx[t], dy = Y - sy[t], dz = Z - sz[t];
+        const double d = sqrt(dx * dx + dy * dy + dz * dz);
+        s += sc[t] * exp(-kap * (d - sr[t])) / d;
+      }
+  }
+  if (!valid) return;
+  double *dst = (f >> 1) == 0 ? gx : ((f >> 1) == 1 ? gy : gz);
+  dst[(size_t)(f & 1) * nn + (size_t)b * n + a] = s;
+}
+
+void check(const char *what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    fprintf(stderr, "pmg: launch of %s failed: %s\n", what,
+            hipGetErrorString(e));
+}
+
+}  // namespace
+
+void pbe_constants(double ionic, double *lb, double *kap2) {
+  const double kB = 1.380649e-23, qe = 1.602176634e-19,
+               eps0 = 8.8541878128e-12, NA = 6.02214076e23, Tk = 298.15;
+  *lb = qe * qe / (4.0 * PI_D * eps0 * kB * Tk) * 1e10;
+  *kap2 = 2.0 * ionic * 1000.0 * NA * qe * qe / (eps0 * EPS_OUT * kB * Tk) *
+          1e-20;
+}
+
+size_t pbe_scratch_bytes(int n) {
+  const size_t N = (size_t)n * n * n;
+  return N * sizeof(int) + 2 * N;
+}
+
+int pbe_assemble(int n, double h, double ionic, int natoms,
+                 const double *atoms, int *next, void *scratch, double *xf,
+                 double *yf, double *zf, double *gx, double *gy, double *gz,
+                 double *a1, double *a2, double *a3, double *ccf, double *fcf,
+                 hipStream_t s) {
+  if (n < 3 || !(h > 0.0) || !(ionic >= 0.0) || natoms < 0 ||
+      (long long)n * n * n > (1LL << 31) - 1)
+    return 2;
+  double lb, kap2;
+  pbe_constants(ionic, &lb, &kap2);
+  const double kap = sqrt(kap2), kap2e = EPS_OUT * kap2;
+  const double c0 = -(n - 1) * h / 2.0, h3 = h * h * h;
+  const double q4 = 4.0 * PI_D * lb;
+  const size_t N = (size_t)n * n * n;
+  int *head = (int *)scratch;
+  uint8_t *inside = (uint8_t *)scratch + N * sizeof(int);
+  uint8_t *excl = inside + N;
+  k_pb_init<<<(unsigned)std::min<size_t>((N + 255) / 256, 8192), 256, 0, s>>>(
+      N, head, inside, excl);
+  check("k_pb_init");
+  if (natoms > 0) {
+    k_pb_mark<<<natoms, 256, 0, s>>>(n, c0, h, natoms, atoms, head, next,
+                                     inside, excl);
+    check("k_pb_mark");
+  }
+  dim3 gn(cdivp(n, 64), cdivp(n, 4), n);
+  k_pb_nodes<<<gn, dim3(64, 4), 0, s>>>(n, c0, h, q4, h3, kap2e, atoms, head,
+                                        next, inside, excl, xf, yf, zf, a1,
+                                        a2, a3, ccf, fcf);
+  check("k_pb_nodes");
+  dim3 gf(cdivp((long long)n * n, FACE_NT), 6);
+  k_pb_faces<<<gf, FACE_NT, 0, s>>>(n, c0, h, lb, kap, natoms, atoms, gx, gy,
+                                    gz);
+  check("k_pb_faces");
+  return 0;
+}
+
+}  // namespace pmg
+
+// ---- C ABI on host arrays ----
+#include "../../include/pmg_mi355x.h"
+
+extern "C" int pmg_pbe_assemble(int n, double h, double ionic_strength,
+                                int natoms, const double *atoms, double *xf,
+                                double *yf, double *zf, double *gxcf,
+                                double *gycf, double *gzcf, double *a1cf,
+                                double *a2cf, double *a3cf, double *ccf,
+                                double *fcf, double *tcf) {
+  if (n < 3 || !(h > 0.0) || !(ionic_strength >= 0.0) || natoms < 0 ||
+      (natoms > 0 && !atoms) || (long long)n * n * n > (1LL << 31) - 1)
+    return PMG_EBADPARM;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return PMG_EDEVICE;
+  const size_t N = (size_t)n * n * n, nn = (size_t)n * n;
+  // device layout: 5 N-arrays, 3 coordinate arrays, 3 face pairs, atoms,
+  // cell-list links, assembly scratch
+  const size_t nd = 5 * N + 3 * (size_t)n + 6 * nn + 5 * (size_t)natoms;
+  double *d = nullptr;
+  void *scr = nullptr;
+  int *nxt = nullptr;
+  hipStream_t s = nullptr;
+  int rc = PMG_OK;
+  if (hipMalloc(&d, nd * sizeof(double)) != hipSuccess ||
+      hipMalloc(&scr, pmg::pbe_scratch_bytes(n)) != hipSuccess ||
+      hipMalloc(&nxt, sizeof(int) * (natoms > 0 ? natoms : 1)) != hipSuccess)
+    rc = PMG_ENOMEM;
+  if (!rc && hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess)
+    rc = PMG_EDEVICE;
+  if (!rc) {
+    double *A1 = d, *A2 = A1 + N, *A3 = A2 + N, *CC = A3 + N, *FC = CC + N;
+    double *X = FC + N, *Y = X + n, *Z = Y + n;
+    double *GX = Z + n, *GY = GX + 2 * nn, *GZ = GY + 2 * nn;
+    double *AT = GZ + 2 * nn;
+    if (natoms > 0)
+      hipMemcpyAsync(AT, atoms, sizeof(double) * 5 * natoms,
+                     hipMemcpyHostToDevice, s);
+    rc = pmg::pbe_assemble(n, h, ionic_strength, natoms, AT, nxt, scr, X, Y,
+                           Z, GX, GY, GZ, A1, A2, A3, CC, FC, s);
+    auto get = [&](double *dst, const double *src, size_t cnt) {
+      if (dst)
+        hipMemcpyAsync(dst, src, cnt * sizeof(double), hipMemcpyDeviceToHost,
+                       s);
+    };
+    get(a1cf, A1, N), get(a2cf, A2, N), get(a3cf, A3, N), get(ccf, CC, N);
+    get(fcf, FC, N), get(xf, X, n), get(yf, Y, n), get(zf, Z, n);
+    get(gxcf, GX, 2 * nn), get(gycf, GY, 2 * nn), get(gzcf, GZ, 2 * nn);
+    if (hipStreamSynchronize(s) != hipSuccess) rc = PMG_EDEVICE;
+  }
+  if (!rc && tcf)
+    for (size_t p = 0; p < N; p++) tcf[p] = 0.0;
+  if (s) hipStreamDestroy(s);
+  hipFree(d);
+  hipFree(scr);
+  hipFree(nxt);
+  return rc;
+}

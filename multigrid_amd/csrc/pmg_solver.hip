@@ -143,6 +143,7 @@ struct pmg_ctx {
   double *scal = nullptr;    // device scalars
   double *hscal = nullptr;   // pinned host mirror
   bool have32 = false, haveN = false;
+  bool ops32_ok = false;  // FP32 operator copies match the FP64 hierarchy
   int iters = 0, ierror = 0;
   double relres = 0.0, setup_ms = 0.0, solve_ms = 0.0;
   // linear-solve session (pmg_ctx_solve and the serving loop)
@@ -152,6 +153,11 @@ struct pmg_ctx {
     double rsden = 1.0, rsnrm = 0.0;
   } ses;
   hipEvent_t cyc_ev = nullptr, t0_ev = nullptr, t1_ev = nullptr;
+  hipEvent_t su0_ev = nullptr, su1_ev = nullptr, stage_ev = nullptr;
+  // atom list of the GPU input assembly: device copy (+ links) and pinned
+  // staging, capacity in atoms
+  double *atoms_d = nullptr, *atoms_h = nullptr;
+  int atoms_cap = 0;
   // V-cycle bodies captured as HIP graphs, one per precision mode
   bool graphs = true, capturing = false;
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
@@ -185,6 +191,10 @@ struct pmg_ctx {
     if (cyc_ev) hipEventDestroy(cyc_ev);
     if (t0_ev) hipEventDestroy(t0_ev);
     if (t1_ev) hipEventDestroy(t1_ev);
+    for (hipEvent_t e : {su0_ev, su1_ev, stage_ev})
+      if (e) hipEventDestroy(e);
+    if (atoms_d) hipFree(atoms_d);
+    if (atoms_h) hipHostFree(atoms_h);
     for (void *ptr : allocs) hipFree(ptr);
     if (hscal) hipHostFree(hscal);
     for (auto e : ev) hipEventDestroy(e);
@@ -445,7 +455,11 @@ void build_ops(pmg_ctx *c) {
     restrict_<double>(Lf.d, Lc.d, Lf.cc, Lc.cc, Lc.pc, s);
     restrict_<double>(Lf.d, Lc.d, Lf.fc, Lc.fc, Lc.pc, s);
   }
-  if (c->have32) convert_ops32(c);
+  c->ops32_ok = false;
+  if (c->have32) {
+    convert_ops32(c);
+    c->ops32_ok = true;
+  }
 }
 
 template <class T>
@@ -454,11 +468,9 @@ void upload(pmg_ctx *c, T *dst, const T *src, size_t n) {
                                  hipMemcpyHostToDevice, c->s);
 }
 
-int ctx_create(pmg_ctx **out, const Params &p, const double *xf,
-               const double *yf, const double *zf, const double *gxcf,
-               const double *gycf, const double *gzcf, const double *a1cf,
-               const double *a2cf, const double *a3cf, const double *ccf,
-               const double *fcf) {
+// Allocate a context for p: stream, hierarchy and the resident fine-level
+// inputs (zeroed, no data yet).
+int ctx_new(pmg_ctx **out, const Params &p) {
   *out = nullptr;
   if (!device_ok()) return PMG_EDEVICE;
   pmg_ctx *c = new (std::nothrow) pmg_ctx;
@@ -490,6 +502,45 @@ int ctx_create(pmg_ctx **out, const Params &p, const double *xf,
     delete c;
     return PMG_ENOMEM;
   }
+  *out = c;
+  return PMG_OK;
+}
+
+// Enqueue the hierarchy build (and, for a mixed-precision context, the FP32
+// operator copies) bracketed by the setup events.
+void setup_start(pmg_ctx *c) {
+  if (!c->su0_ev) {
+    hipEventCreate(&c->su0_ev);
+    hipEventCreate(&c->su1_ev);
+  }
+  hipEventRecord(c->su0_ev, c->s);
+}
+
+void setup_enqueue(pmg_ctx *c, bool started = false) {
+  if (!started) setup_start(c);
+  build_ops(c);
+  hipEventRecord(c->su1_ev, c->s);
+}
+
+// Wait for the setup and record its device time.
+int setup_wait(pmg_ctx *c) {
+  if (hipStreamSynchronize(c->s) != hipSuccess) return PMG_EDEVICE;
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, c->su0_ev, c->su1_ev);
+  c->setup_ms = ms;
+  return PMG_OK;
+}
+
+int ctx_create(pmg_ctx **out, const Params &p, const double *xf,
+               const double *yf, const double *zf, const double *gxcf,
+               const double *gycf, const double *gzcf, const double *a1cf,
+               const double *a2cf, const double *a3cf, const double *ccf,
+               const double *fcf) {
+  pmg_ctx *c = nullptr;
+  int rc = ctx_new(&c, p);
+  if (rc) return rc;
+  const int nx = p.nx, ny = p.ny, nz = p.nz;
+  const size_t nf = (size_t)nx * ny * nz;
   upload(c, c->xf, xf, nx);
   upload(c, c->yf, yf, ny);
   upload(c, c->zf, zf, nz);
@@ -501,23 +552,69 @@ int ctx_create(pmg_ctx **out, const Params &p, const double *xf,
   upload(c, c->a3, a3cf, nf);
   upload(c, c->ccf, ccf, nf);
   upload(c, c->fcf, fcf, nf);
-  hipEvent_t e0, e1;
-  hipEventCreate(&e0);
-  hipEventCreate(&e1);
-  hipEventRecord(e0, c->s);
-  build_ops(c);
-  hipEventRecord(e1, c->s);
-  if (hipStreamSynchronize(c->s) != hipSuccess) {
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+  setup_enqueue(c);
+  rc = setup_wait(c);
+  if (rc) {
     delete c;
-    return PMG_EDEVICE;
+    return rc;
   }
-  float ms = 0.f;
-  hipEventElapsedTime(&ms, e0, e1);
-  c->setup_ms = ms;
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  *out = c;
+  return PMG_OK;
+}
+
+// Stage an atom list and enqueue the GPU assembly of the fine-level inputs
+// (pmg_pbe.hip) into the context's resident input arrays.  The assembly
+// scratch is the fine residual buffer, which no solve is using yet.
+int pbe_enqueue(pmg_ctx *c, double h, double ionic, int natoms,
+                const double *atoms) {
+  const Params &p = c->p;
+  if (p.nx != p.ny || p.nx != p.nz || !(h > 0.0) || !(ionic >= 0.0) ||
+      natoms < 0 || (natoms > 0 && !atoms))
+    return PMG_EBADPARM;
+  if (c->stage_ev) hipEventSynchronize(c->stage_ev);  // staging buffer free
+  if (natoms > c->atoms_cap) {
+    hipStreamSynchronize(c->s);
+    if (c->atoms_d) hipFree(c->atoms_d);
+    if (c->atoms_h) hipHostFree(c->atoms_h);
+    c->atoms_d = nullptr, c->atoms_h = nullptr, c->atoms_cap = 0;
+    int cap = std::max(natoms, 1024);
+    // atoms (5 doubles) and the cell-list links (1 int) per atom
+    if (hipMalloc((void **)&c->atoms_d, (size_t)cap * 6 * sizeof(double)) !=
+            hipSuccess ||
+        hipHostMalloc((void **)&c->atoms_h,
+                      (size_t)cap * 5 * sizeof(double)) != hipSuccess)
+      return PMG_ENOMEM;
+    c->atoms_cap = cap;
+  }
+  if (!c->stage_ev) hipEventCreateWithFlags(&c->stage_ev, hipEventDisableTiming);
+  if (natoms > 0) {
+    memcpy(c->atoms_h, atoms, (size_t)natoms * 5 * sizeof(double));
+    hipMemcpyAsync(c->atoms_d, c->atoms_h, (size_t)natoms * 5 * sizeof(double),
+                   hipMemcpyHostToDevice, c->s);
+  }
+  hipEventRecord(c->stage_ev, c->s);
+  int *next = (int *)(c->atoms_d + 5 * (size_t)c->atoms_cap);
+  int rc = pbe_assemble(p.nx, h, ionic, natoms, c->atoms_d, next, c->L[0].r,
+                        c->xf, c->yf, c->zf, c->gx, c->gy, c->gz, c->a1,
+                        c->a2, c->a3, c->ccf, c->fcf, c->s);
+  return rc ? PMG_EBADPARM : PMG_OK;
+}
+
+int ctx_create_pbe(pmg_ctx **out, const Params &p, double h, double ionic,
+                   int natoms, const double *atoms) {
+  pmg_ctx *c = nullptr;
+  int rc = ctx_new(&c, p);
+  if (rc) return rc;
+  setup_start(c);  // setup time includes the input assembly
+  rc = pbe_enqueue(c, h, ionic, natoms, atoms);
+  if (!rc) {
+    setup_enqueue(c, true);
+    rc = setup_wait(c);
+  }
+  if (rc) {
+    delete c;
+    return rc;
+  }
   *out = c;
   return PMG_OK;
 }
@@ -595,7 +692,8 @@ int lin_begin(pmg_ctx *c, int mode) {
   if (mode == PMG_MIXED) {
     int rc = alloc_fp32(c);
     if (rc) return rc;
-    convert_ops32(c);
+    if (!c->ops32_ok) convert_ops32(c);
+    c->ops32_ok = true;
   }
   if (!cg_fits(c->L[c->nlev - 1].d)) return PMG_EBADPARM;
   if (!c->cyc_ev) {
@@ -733,7 +831,8 @@ int ctx_newton(pmg_ctx *c, int mode) {
   if (mode == PMG_MIXED) {
     rc = alloc_fp32(c);
     if (rc) return rc;
-    convert_ops32(c);
+    if (!c->ops32_ok) convert_ops32(c);
+    c->ops32_ok = true;
     for (int l = 0; l < c->nlev; l++)
       if (!c->L[l].cpr32) c->L[l].cpr32 = c->alloc<float>(c->L[l].d.n());
     if (!c->ok) return PMG_ENOMEM;
@@ -1081,19 +1180,35 @@ int pmg_ctx_stats(pmg_ctx *ctx, int *iters, int *ierror, double *relres,
 
 int pmg_ctx_rebuild(pmg_ctx *ctx) {
   if (!ctx) return PMG_EBADPARM;
-  hipEvent_t e0, e1;
-  hipEventCreate(&e0);
-  hipEventCreate(&e1);
-  hipEventRecord(e0, ctx->s);
-  build_ops(ctx);
-  hipEventRecord(e1, ctx->s);
-  hipError_t se = hipStreamSynchronize(ctx->s);
-  float ms = 0.f;
-  hipEventElapsedTime(&ms, e0, e1);
-  ctx->setup_ms = ms;
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
-  return se == hipSuccess ? PMG_OK : PMG_EDEVICE;
+  setup_enqueue(ctx);
+  return setup_wait(ctx);
+}
+
+int pmg_ctx_create_pbe(pmg_ctx **out, const int *iparm, const double *rparm,
+                       double h, double ionic_strength, int natoms,
+                       const double *atoms) {
+  if (!out || !iparm || !rparm) return PMG_EBADPARM;
+  *out = nullptr;
+  Params p;
+  int rc = decode(iparm, rparm, p);
+  if (rc) return rc;
+  if (p.nx != p.ny || p.nx != p.nz || !(h > 0.0) || !(ionic_strength >= 0.0) ||
+      natoms < 0 || (natoms > 0 && !atoms))
+    return PMG_EBADPARM;
+  return ctx_create_pbe(out, p, h, ionic_strength, natoms, atoms);
+}
+
+int pmg_ctx_load_pbe(pmg_ctx *ctx, double h, double ionic_strength,
+                     int natoms, const double *atoms) {
+  if (!ctx) return PMG_EBADPARM;
+  setup_start(ctx);
+  int rc = pbe_enqueue(ctx, h, ionic_strength, natoms, atoms);
+  if (rc) {
+    hipStreamSynchronize(ctx->s);
+    return rc;
+  }
+  setup_enqueue(ctx, true);
+  return setup_wait(ctx);
 }
 
 size_t pmg_ctx_bytes(const pmg_ctx *ctx) { return ctx ? ctx->bytes : 0; }

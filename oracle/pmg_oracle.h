@@ -131,6 +131,20 @@ int pmgo_newdriv(int *iparm, double *rparm, int *iwork, double *rwork,
 int pmgo_last_stats(int *iters, int *ierror, double *relres, double *hist,
                     int hist_cap);
 
+/* Poisson-Boltzmann input assembly from an atom list (pbe_oracle.c;
+ * PBE_pmgc_class.h:191-344).  atoms: natoms x {x, y, z, radius, charge}.
+ * Writes the Vmgdriv fine-level inputs of an n^3 grid of spacing h centred
+ * on the origin; gxcf/gycf/gzcf planes 1-2 of the (n,n,4) face arrays.
+ * Pinned: charge conservation, the closed-form dielectric/kappa values,
+ * the exclusion-shell geometry and the Debye-Hueckel sum evaluated
+ * independently in tests/test_pbinput.py. */
+void pbo_constants(double ionic, double *lambda_b, double *kappa2);
+void pbo_assemble(int n, double h, double ionic, int natoms,
+                  const double *atoms, double *xf, double *yf, double *zf,
+                  double *gxcf, double *gycf, double *gzcf, double *a1cf,
+                  double *a2cf, double *a3cf, double *ccf, double *fcf,
+                  double *tcf);
+
 #ifdef __cplusplus
 }
 #endif

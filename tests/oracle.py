@@ -48,6 +48,8 @@ def olib():
             "pmgo_mgdriv": (i, [ip, dp, ip, dp, dp] + [dp] * 12),
             "pmgo_newdriv": (i, [ip, dp, ip, dp, dp] + [dp] * 12),
             "pmgo_last_stats": (i, [ip, ip, dp, dp, i]),
+            "pbo_constants": (None, [d, dp, dp]),
+            "pbo_assemble": (None, [i, d, d, i, dp] + [dp] * 12),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -194,3 +196,41 @@ def solve(inp, n, nlev, driver=None, nonlin=False, **kw):
               a["gycf"], a["gzcf"], a["a1cf"], a["a2cf"], a["a3cf"], a["ccf"],
               a["fcf"], a["tcf"])
     return ierr, u, iw, rw
+
+
+PBE_KEYS = ("xf", "yf", "zf", "gxcf", "gycf", "gzcf", "a1cf", "a2cf", "a3cf",
+            "ccf", "fcf", "tcf")
+
+
+def pbe_constants(ionic):
+    lb, k2 = ctypes.c_double(), ctypes.c_double()
+    olib().pbo_constants(ionic, ctypes.byref(lb), ctypes.byref(k2))
+    return lb.value, k2.value
+
+
+def pbe_assemble(n, h, ionic, atoms):
+    """Oracle assembly of the fine-level inputs from an (natoms, 5) array."""
+    at = np.ascontiguousarray(atoms, np.float64).reshape(-1)
+    out = {k: np.zeros(n) for k in ("xf", "yf", "zf")}
+    for k in ("gxcf", "gycf", "gzcf"):
+        out[k] = np.zeros(4 * n * n)
+    for k in ("a1cf", "a2cf", "a3cf", "ccf", "fcf", "tcf"):
+        out[k] = np.zeros(n * n * n)
+    olib().pbo_assemble(n, h, ionic, at.size // 5, dp(at), *(dp(out[k]) for k in PBE_KEYS))
+    out["atoms"] = at.reshape(-1, 5)
+    return out
+
+
+def pbe_synth(n, h, natoms, sphere_r, ionic=0.15, charge_mode=0, seed=20261202):
+    """Synthetic molecule from the library's (host-only) generator, assembled
+    by the oracle: the inputs CPU-only tests and the CPU baseline use."""
+    import multigrid_amd as mg
+    return pbe_assemble(n, h, ionic, mg.pbe_molecule(natoms, sphere_r, charge_mode, seed))
+
+
+def config_inputs(name, **over):
+    import multigrid_amd as mg
+    c = dict(mg.CONFIGS[name])
+    c.update(over)
+    return c, pbe_synth(c["n"], c["h"], c["natoms"], c["sphere_r"], c["ionic"],
+                        c["charge_mode"], c["seed"])

@@ -45,9 +45,12 @@ def test_packmg_layout(mg):
 
 
 @pytest.mark.skipif(have_gpu(), reason="checks the no-GPU error path")
-def test_no_gpu_is_reported(mg):
+def test_no_gpu_is_reported(mg, orc):
     n, nlev = 9, 2
-    inp = mg.pbe_synth(n, 0.5, 4, 1.0)
+    inp = orc.pbe_synth(n, 0.5, 4, 1.0)
+    with pytest.raises(mg.PMGError) as e:
+        mg.pbe_assemble(n, 0.5, 0.15, inp["atoms"])
+    assert e.value.code == mg.PMG_EDEVICE
     iparm, rparm = mg.make_params(n, nlev)
     sz = mg.Vmgsz(2, 0, 0, n, n, n, nlev)
     with pytest.raises(mg.PMGError) as e:

@@ -112,7 +112,7 @@ def _run_driver(mg, fn, inp, n, nlev, nonlin=False, **kw):
 @pytest.mark.parametrize("n,nlev,natoms,R,seed", [(33, 3, 40, 6.0, 1), (65, 4, 200, 14.0, 20261202)])
 def test_mgdriv_parity(mg, orc, n, nlev, natoms, R, seed):
     print(f"mgdriv parity n={n} nlev={nlev}", flush=True)
-    inp = mg.pbe_synth(n, 0.5, natoms, R, 0.15, 0, seed)
+    inp = orc.pbe_synth(n, 0.5, natoms, R, 0.15, 0, seed)
     e1, u1, iw1, rw1 = _run_driver(mg, mg.Vmgdriv, inp, n, nlev)
     it1 = mg.last_stats()[0]
     e0, u0, iw0, rw0 = orc.solve(inp, n, nlev)
@@ -134,7 +134,7 @@ def test_mgdriv_parity(mg, orc, n, nlev, natoms, R, seed):
 def test_ctx_fp64_and_mixed(mg, orc):
     print("ctx fp64 + mixed n=65", flush=True)
     n, nlev = 65, 4
-    inp = mg.pbe_synth(n, 0.5, 200, 14.0, 0.15, 0, 20261202)
+    inp = orc.pbe_synth(n, 0.5, 200, 14.0, 0.15, 0, 20261202)
     iparm, rparm = mg.make_params(n, nlev)
     ctx = mg.Context(iparm, rparm, inp)
     ctx.solve(mg.PMG_FP64)
@@ -157,7 +157,7 @@ def test_newdriv_parity(mg, orc):
     ions = ([1.0, -1.0], [0.5, 0.5])
     mg.Vmypdefinitnpbe(2, *ions)
     orc.mypdefinit(*ions)
-    inp = mg.pbe_synth(n, 0.5, 40, 6.0, 0.3, 0, 5)
+    inp = orc.pbe_synth(n, 0.5, 40, 6.0, 0.3, 0, 5)
     e1, u1, _, _ = _run_driver(mg, mg.Vnewdriv, inp, n, nlev, nonlin=True, errtol=1e-8)
     it1 = mg.last_stats()[0]
     e0, u0, _, _ = orc.solve(inp, n, nlev, nonlin=True, errtol=1e-8)
@@ -167,10 +167,10 @@ def test_newdriv_parity(mg, orc):
     assert _rel(u1, u0) < 1e-9
 
 
-def test_error_paths(mg):
+def test_error_paths(mg, orc):
     print("error paths", flush=True)
     n, nlev = 17, 3
-    inp = mg.pbe_synth(n, 0.5, 8, 2.0, 0.15, 0, 1)
+    inp = orc.pbe_synth(n, 0.5, 8, 2.0, 0.15, 0, 1)
     # itmax exhausted -> ierror 1
     e, _, _, _ = _run_driver(mg, mg.Vmgdriv, inp, n, nlev, itmax=1, errtol=1e-14)
     assert e == 1 and mg.last_stats()[1] == 1
@@ -189,10 +189,13 @@ def test_full_size_c1_properties(mg, orc):
     recomputed by the oracle on the host, meets 1e-6; the mixed-precision
     solution agrees with it to 1e-6."""
     print("c1 161^3 full-size properties", flush=True)
-    c, inp = mg.config_inputs("c1")
+    c = mg.CONFIGS["c1"]
     n, nlev = c["n"], c["nlev"]
+    atoms = mg.pbe_molecule(c["natoms"], c["sphere_r"], c["charge_mode"], c["seed"])
+    inp = orc.pbe_assemble(n, c["h"], c["ionic"], atoms)
     iparm, rparm = mg.make_params(n, nlev)
-    ctx = mg.Context(iparm, rparm, inp)
+    # inputs assembled on the GPU from the atom list
+    ctx = mg.Context(iparm, rparm, pbe=(c["h"], c["ionic"], atoms))
     ctx.solve(mg.PMG_FP64)
     u64, s64 = ctx.u(), ctx.stats()
     ctx.solve(mg.PMG_MIXED)

@@ -113,7 +113,7 @@ def test_newton_manufactured(orc):
 
 def test_vcycle_contracts_on_pb_problem(orc, mg):
     n, nlev = 33, 4
-    inp = mg.pbe_synth(n, 0.5, 40, 6.0, 0.15, 0, 3)
+    inp = orc.pbe_synth(n, 0.5, 40, 6.0, 0.15, 0, 3)
     ierr, u, _, _ = orc.solve(inp, n, nlev)
     it, ie, rel, hist = orc.last_stats()
     assert ierr == 0 and rel <= 1e-6 and 2 <= it <= 30
