@@ -69,6 +69,13 @@ template <class T>
 void matvec_dots(const Dim &d, int numdia, const T *ac, const T *cc,
                  const T *x, const T *w0, T *y, const Red *red, hipStream_t s);
 
+// y = A x and the per-block partials of (x.w0, x.y) into part (2 per
+// block), for interp_damped; returns the number of blocks.
+template <class T>
+unsigned matvec_dots_part(const Dim &d, int numdia, const T *ac, const T *cc,
+                          const T *x, const T *w0, T *y, double *part,
+                          hipStream_t s);
+
 // ---- grid transfer (pmgc/matvecd.cc:695-983) ----
 template <class T>
 void restrict_(const Dim &f, const Dim &c, const T *xin, T *xout, const T *pc,
@@ -78,6 +85,13 @@ void restrict_(const Dim &f, const Dim &c, const T *xin, T *xout, const T *pc,
 template <class T>
 void interp(const Dim &c, const Dim &f, const T *xc, const T *pc, T *xf,
             const double *xdamp, hipStream_t s);
+
+// xf(interior) += xdamp * (P xc) with xdamp = sum(part[2b]) / sum(part[2b+1])
+// over the nblk partials of matvec_dots_part (0 when the denominator is 0),
+// summed by every workgroup in one fixed order.
+template <class T>
+void interp_damped(const Dim &c, const Dim &f, const T *xc, const T *pc,
+                   T *xf, const double *part, unsigned nblk, hipStream_t s);
 
 // ---- reductions ----
 // Deterministic interior L1 norm into red.out[0].

@@ -8,6 +8,7 @@
 // order, one partial per block, and a single-block final pass.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 
@@ -217,126 +218,147 @@ __global__ __launch_bounds__(256) void k_gs27(int nx, int ny, int nz,
 
 // One full red-black sweep in a single pass: xout = black(red(xin)), the
 // two half-sweeps of Vgsrb7x (gsd.cc:81-110) in the reference colour order.
-// First-colour updates read only xin, so a workgroup can recompute them on
-// the one-point halo of its (i, j) tile; second-colour updates then read the
-// new first-colour values from LDS (in-plane) and registers (k +- 1).  The
-// workgroup marches a chunk of k planes; each thread owns one (i, j) column
-// and keeps the first-colour ("mid") values of planes k-1, k, k+1.  Three
-// rotating LDS planes need one barrier per plane.
-constexpr int SW_TX = 64, SW_TY = 8, SW_KC = 16;
-// Operands of one first-colour update (coefficients and in-plane xin
-// neighbours) and of one second-colour update (coefficients).
+//
+// A workgroup owns a (TX-4) x (TY-4) tile of (i, j) columns and marches a
+// chunk of KC planes in k; each thread holds one column of a TX x TY tile
+// that includes two rings of halo.  Per plane each thread loads only its own
+// point of the six operator arrays and of xin (prefetched three planes
+// ahead into registers) and shares the in-plane operands through LDS:
+//   - first-colour values ("mid") are computed on the tile minus the outer
+//     ring, reading xin and the west/south couplings oE(i-1), oN(j-1) of
+//     their neighbours from LDS;
+//   - second-colour values are computed on the owned tile (minus two rings)
+//     from the neighbours' mid values in LDS and the column's mid values of
+//     planes k-1 and k+1 in registers.
+// So every array element is fetched once per tile (plus the halo), all
+// loads are coalesced rows, and one barrier per plane orders the LDS
+// traffic: xin planes rotate over 2 slots, couplings and mid values over 3
+// (a plane's slot is rewritten two planes later, after a barrier every
+// reader has passed).
+constexpr int SW_TX = 64, SW_TY = 16;
 template <class T>
-struct SwRed {
-  T oC, cc, fc, oE, oEm, oN, oNm, uC, uCm, xe, xw, xn, xs;
-};
-template <class T>
-struct SwBlk {
-  T oC, cc, fc, oE, oEm, oN, oNm, uC, uCm;
+struct SwPl {  // own-point operands of one plane
+  T oC, cc, fc, oE, oN, uC, x;
 };
 
 template <class T>
 __global__ __launch_bounds__(SW_TX *SW_TY) void k_gs7_sweep(
     int nx, int ny, int nz, const T *__restrict__ ac, const T *__restrict__ cc,
     const T *__restrict__ fc, const T *__restrict__ xin, T *__restrict__ xout,
-    int c1) {
-  __shared__ T M[3][SW_TY][SW_TX];
+    int c1, int kc) {
+  __shared__ T SX[2][SW_TY][SW_TX], SE[3][SW_TY][SW_TX], SN[3][SW_TY][SW_TX],
+      SM[3][SW_TY][SW_TX];
   const int tx = threadIdx.x, ty = threadIdx.y;
-  const int i = 1 + blockIdx.x * (SW_TX - 2) + tx;
-  const int j = 1 + blockIdx.y * (SW_TY - 2) + ty;
-  const int kb = 2 + blockIdx.z * SW_KC;
-  const int ke = min(kb + SW_KC - 1, nz - 1);
-  const size_t n = (size_t)nx * ny * nz, sy = nx, sz = (size_t)nx * ny;
+  const int i = blockIdx.x * (SW_TX - 4) + tx;
+  const int j = blockIdx.y * (SW_TY - 4) + ty;
+  const int kb = 2 + blockIdx.z * kc;
+  const int ke = min(kb + kc - 1, nz - 1);
+  const size_t n = (size_t)nx * ny * nz, sz = (size_t)nx * ny;
   const T *oE = ac + n, *oN = ac + 2 * n, *uC = ac + 3 * n;
-  const bool col = i <= nx && j <= ny;
   const bool inner = i >= 2 && i <= nx - 1 && j >= 2 && j <= ny - 1;
-  const bool owned = inner && tx >= 1 && tx <= SW_TX - 2 && ty >= 1 &&
-                     ty <= SW_TY - 2;
-  const size_t pc0 = col ? ix(nx, ny, i, j, 1) : 0;  // column base, k = 1
-  auto X = [&](int k) -> T {  // xin on this column, 0 outside the grid
-    return (col && k >= 1 && k <= nz) ? xin[pc0 + (size_t)(k - 1) * sz] : (T)0;
+  const bool midr = inner && tx >= 1 && tx <= SW_TX - 2 && ty >= 1 &&
+                    ty <= SW_TY - 2;
+  const bool owned = inner && tx >= 2 && tx <= SW_TX - 3 && ty >= 2 &&
+                     ty <= SW_TY - 3;
+  // column base, clamped into the grid: loads stay unconditional
+  const size_t p0 = ix(nx, ny, min(max(i, 1), nx), min(max(j, 1), ny), 1);
+  auto load = [&](int q, SwPl<T> &P) {
+    const int qc = min(max(q, 1), nz);
+    const size_t p = p0 + (size_t)(qc - 1) * sz;
+    P.oC = ac[p], P.cc = cc[p], P.fc = fc[p], P.oE = oE[p], P.oN = oN[p],
+    P.uC = uC[p], P.x = xin[p];
   };
-  // first colour at plane k / second colour at plane k on this column?
-  auto first = [&](int k) {
-    return inner && k >= 2 && k <= nz - 1 && ((i + j + k) & 1) == c1;
+  auto first = [&](int q) {
+    return midr && q >= 2 && q <= nz - 1 && ((i + j + q) & 1) == c1;
   };
-  auto second = [&](int k) {
-    return owned && k <= ke && ((i + j + k) & 1) != c1;
+  // first-colour value at plane q (reference term order, gsd.cc:98-105)
+  auto fval = [&](int q, const SwPl<T> &P, T xm, T xp, T uCm) -> T {
+    const int s = q % 3, sx = q & 1;
+    const T sm = P.oN * SX[sx][ty + 1][tx] + SN[s][ty - 1][tx] * SX[sx][ty - 1][tx] +
+                 P.oE * SX[sx][ty][tx + 1] + SE[s][ty][tx - 1] * SX[sx][ty][tx - 1] +
+                 uCm * xm + P.uC * xp;
+    return (P.fc + sm) / (P.oC + P.cc);
   };
-  auto load_red = [&](int k, SwRed<T> &R) {
-    if (!first(k)) return;
-    const size_t p = pc0 + (size_t)(k - 1) * sz;
-    R.oC = ac[p], R.cc = cc[p], R.fc = fc[p], R.oE = oE[p], R.oEm = oE[p - 1],
-    R.oN = oN[p], R.oNm = oN[p - sy], R.uC = uC[p], R.uCm = uC[p - sz];
-    R.xe = xin[p + 1], R.xw = xin[p - 1], R.xn = xin[p + sy],
-    R.xs = xin[p - sy];
-  };
-  auto load_blk = [&](int k, SwBlk<T> &B) {
-    if (!second(k)) return;
-    const size_t p = pc0 + (size_t)(k - 1) * sz;
-    B.oC = ac[p], B.cc = cc[p], B.fc = fc[p], B.oE = oE[p], B.oEm = oE[p - 1],
-    B.oN = oN[p], B.oNm = oN[p - sy], B.uC = uC[p], B.uCm = uC[p - sz];
-  };
-  // first-colour value at plane k (reference term order, gsd.cc:98-105)
-  auto mid = [&](int k, const SwRed<T> &R, T xk, T xm, T xp) -> T {
-    if (!first(k)) return xk;
-    const T s = R.oN * R.xn + R.oNm * R.xs + R.oE * R.xe + R.oEm * R.xw +
-                R.uCm * xm + R.uC * xp;
-    return (R.fc + s) / (R.oC + R.cc);
-  };
-  SwRed<T> R0{}, R1{};
-  SwBlk<T> B0{}, B1{};
-  T x0 = X(kb - 2), x1 = X(kb - 1), x2 = X(kb), x3 = X(kb + 1), x4 = X(kb + 2);
-  load_red(kb - 1, R0);
-  T mprev = mid(kb - 1, R0, x1, x0, x2);  // plane kb-1
-  load_red(kb, R0);
-  T mcur = mid(kb, R0, x2, x1, x3);  // plane kb
-  M[kb % 3][ty][tx] = mcur;
-  load_red(kb + 1, R0);
-  load_blk(kb, B0);
+  SwPl<T> Pm1, P0, P1, P2, P3;  // planes k-1 .. k+3
+  load(kb - 2, P3);
+  const T uCm2 = P3.uC, xm2 = P3.x;
+  load(kb - 1, Pm1);
+  load(kb, P0);
+  load(kb + 1, P1);
+  load(kb + 2, P2);
+  SX[(kb - 1) & 1][ty][tx] = Pm1.x;
+  SE[(kb - 1) % 3][ty][tx] = Pm1.oE;
+  SN[(kb - 1) % 3][ty][tx] = Pm1.oN;
+  SX[kb & 1][ty][tx] = P0.x;
+  SE[kb % 3][ty][tx] = P0.oE;
+  SN[kb % 3][ty][tx] = P0.oN;
+  __syncthreads();
+  T mprev = first(kb - 1) ? fval(kb - 1, Pm1, xm2, P0.x, uCm2) : Pm1.x;
+  T mcur = first(kb) ? fval(kb, P0, Pm1.x, P1.x, Pm1.uC) : P0.x;
+  SM[kb % 3][ty][tx] = mcur;
   for (int k = kb; k <= ke; k++) {
-    // x2..x4 = xin at k..k+2; R0 / B0 hold the operands of this step
-    const T mnext = mid(k + 1, R0, x3, x2, x4);
-    M[(k + 1) % 3][ty][tx] = mnext;
-    // operands of the next step travel while this step waits and computes
-    load_red(k + 2, R1);
-    load_blk(k + 1, B1);
-    const T x5 = X(k + 3);
+    load(k + 3, P3);
+    const int s1 = (k + 1) % 3, s0 = k % 3;
+    SX[(k + 1) & 1][ty][tx] = P1.x;
+    SE[s1][ty][tx] = P1.oE;
+    SN[s1][ty][tx] = P1.oN;
     __syncthreads();
+    const T mnext = first(k + 1) ? fval(k + 1, P1, P0.x, P2.x, P0.uC) : P1.x;
+    SM[s1][ty][tx] = mnext;
     if (owned) {
       T v = mcur;
-      if (second(k)) {
-        const T(*L)[SW_TX] = M[k % 3];
-        const T s = B0.oN * L[ty + 1][tx] + B0.oNm * L[ty - 1][tx] +
-                    B0.oE * L[ty][tx + 1] + B0.oEm * L[ty][tx - 1] +
-                    B0.uCm * mprev + B0.uC * mnext;
-        v = (B0.fc + s) / (B0.oC + B0.cc);
+      if (((i + j + k) & 1) != c1) {
+        const T sm = P0.oN * SM[s0][ty + 1][tx] +
+                     SN[s0][ty - 1][tx] * SM[s0][ty - 1][tx] +
+                     P0.oE * SM[s0][ty][tx + 1] +
+                     SE[s0][ty][tx - 1] * SM[s0][ty][tx - 1] +
+                     Pm1.uC * mprev + P0.uC * mnext;
+        v = (P0.fc + sm) / (P0.oC + P0.cc);
       }
-      xout[pc0 + (size_t)(k - 1) * sz] = v;
+      xout[p0 + (size_t)(k - 1) * sz] = v;
     }
     mprev = mcur;
     mcur = mnext;
-    x2 = x3;
-    x3 = x4;
-    x4 = x5;
-    R0 = R1;
-    B0 = B1;
+    Pm1 = P0;
+    P0 = P1;
+    P1 = P2;
+    P2 = P3;
   }
 }
 
+// Compute units of the current device (cached per device).
+static int device_cus() {
+  static int cus[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (!cus[dev]) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount,
+                              dev) != hipSuccess || v <= 0)
+      v = 256;
+    cus[dev] = v;
+  }
+  return cus[dev];
+}
+
 // nsweeps full sweeps, ping-ponging between x and tmp (same boundary
-// values); the result is left in x.
+// values); the result is left in x.  The k chunk is sized so that one round
+// of workgroups (one per compute unit: the LDS and registers of a
+// 1024-thread workgroup fill a CU) covers the grid.
 template <class T>
 void gs7_fused(const Dim &d, const T *ac, const T *cc, const T *fc, T *x,
                T *tmp, int nsweeps, int iadjoint, hipStream_t s) {
   if (d.nx < 3 || d.ny < 3 || d.nz < 3 || nsweeps <= 0) return;
-  dim3 g(cdiv(d.nx - 2, SW_TX - 2), cdiv(d.ny - 2, SW_TY - 2),
-         cdiv(d.nz - 2, SW_KC));
+  const int gx = cdiv(d.nx - 2, SW_TX - 4), gy = cdiv(d.ny - 2, SW_TY - 4);
+  const int zb = std::max(1, device_cus() / (gx * gy));
+  const int kc = std::max(8, cdiv(d.nz - 2, zb));
+  dim3 g(gx, gy, cdiv(d.nz - 2, kc));
   dim3 b(SW_TX, SW_TY);
   const int c1 = iadjoint ? 1 : 0;
   T *src = x, *dst = tmp;
   for (int it = 0; it < nsweeps; it++) {
-    k_gs7_sweep<T><<<g, b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, fc, src, dst, c1);
+    k_gs7_sweep<T><<<g, b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, fc, src, dst, c1,
+                                   kc);
     T *t = src;
     src = dst;
     dst = t;
@@ -707,6 +729,24 @@ void matvec_dots(const Dim &d, int numdia, const T *ac, const T *cc,
   check_launch("matvec_dots");
 }
 
+template <class T>
+unsigned matvec_dots_part(const Dim &d, int numdia, const T *ac, const T *cc,
+                          const T *x, const T *w0, T *y, double *part,
+                          hipStream_t s) {
+  dim3 g(cdiv(d.nx - 2, BX), cdiv(d.ny - 2, BY), cdiv(d.nz - 2, KCH));
+  dim3 b(BX, BY);
+  Red rd;
+  rd.part = part;
+  if (numdia == 7)
+    k_matvec_dots<T, 7><<<g, b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, x, w0, y,
+                                        rd, 1);
+  else
+    k_matvec_dots<T, 27><<<g, b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, x, w0, y,
+                                         rd, 1);
+  check_launch("matvec_dots");
+  return g.x * g.y * g.z;
+}
+
 // ---------------------------------------------------------------------------
 // Grid transfer: Vrestrc2 (matvecd.cc:755-795), VinterpPMG2 (:878-960)
 // ---------------------------------------------------------------------------
@@ -775,10 +815,28 @@ __global__ __launch_bounds__(256) void k_interp(int nxc, int nyc, int nzc,
                                                 const T *__restrict__ xc,
                                                 const T *__restrict__ pc,
                                                 T *__restrict__ xf,
-                                                const double *xdamp) {
+                                                const double *xdamp,
+                                                const double *part,
+                                                unsigned nblk) {
   const int ii = 1 + blockIdx.x * BX + threadIdx.x;
   const int jj = 1 + blockIdx.y * BY + threadIdx.y;
   const int kk = 1 + blockIdx.z;
+  // xdamp from the (x.w0, x.y) partials of the coarse matvec, summed by
+  // every block in the same fixed order (mgcsd.cc:429-452)
+  __shared__ double red[8];
+  __shared__ double dsh;
+  if (ADD && part) {
+    const int tid = threadIdx.x + threadIdx.y * BX;
+    double a = 0.0, c = 0.0;
+    for (unsigned q = tid; q < nblk; q += BX * BY) {
+      a += part[2 * (size_t)q];
+      c += part[2 * (size_t)q + 1];
+    }
+    a = block_sum<BX * BY>(a, red);
+    c = block_sum<BX * BY>(c, red);
+    if (tid == 0) dsh = c != 0.0 ? a / c : 0.0;
+    __syncthreads();
+  }
   if (ii > nxc - 1 || jj > nyc - 1) return;
   const size_t nc = (size_t)nxc * nyc * nzc;
   const size_t c0 = ix(nxc, nyc, ii, jj, kk);
@@ -807,7 +865,7 @@ __global__ __launch_bounds__(256) void k_interp(int nxc, int nyc, int nzc,
 #undef XC
   const int i = 2 * ii - 1, j = 2 * jj - 1, k = 2 * kk - 1;
   T damp = (T)0;
-  if (ADD) damp = (T)(*xdamp);
+  if (ADD) damp = (T)(part ? dsh : *xdamp);
 #pragma unroll
   for (int q = 0; q < 8; q++) {
     const int fi = i + (q & 1), fj = j + ((q >> 1) & 1), fk = k + (q >> 2);
@@ -832,14 +890,29 @@ void interp(const Dim &c, const Dim &f, const T *xc, const T *pc, T *xf,
   dim3 g(cdiv(c.nx - 1, BX), cdiv(c.ny - 1, BY), c.nz - 1);
   if (xdamp) {
     k_interp<T, true><<<g, dim3(BX, BY), 0, s>>>(c.nx, c.ny, c.nz, f.nx, f.ny,
-                                                 f.nz, xc, pc, xf, xdamp);
+                                                 f.nz, xc, pc, xf, xdamp,
+                                                 nullptr, 0u);
   } else {
     k_interp<T, false><<<g, dim3(BX, BY), 0, s>>>(c.nx, c.ny, c.nz, f.nx,
                                                   f.ny, f.nz, xc, pc, xf,
-                                                  nullptr);
+                                                  nullptr, nullptr, 0u);
     if constexpr (sizeof(T) == 8) fbound00(f, (double *)xf, s);
   }
   check_launch("interp");
+}
+
+template <class T>
+void interp_damped(const Dim &c, const Dim &f, const T *xc, const T *pc,
+                   T *xf, const double *part, unsigned nblk, hipStream_t s) {
+  if (f.nx != 2 * c.nx - 1 || f.ny != 2 * c.ny - 1 || f.nz != 2 * c.nz - 1) {
+    fprintf(stderr, "pmg: interp shape mismatch\n");
+    return;
+  }
+  dim3 g(cdiv(c.nx - 1, BX), cdiv(c.ny - 1, BY), c.nz - 1);
+  k_interp<T, true><<<g, dim3(BX, BY), 0, s>>>(c.nx, c.ny, c.nz, f.nx, f.ny,
+                                               f.nz, xc, pc, xf, nullptr, part,
+                                               nblk);
+  check_launch("interp_damped");
 }
 
 // ---------------------------------------------------------------------------
@@ -942,7 +1015,9 @@ __global__ __launch_bounds__(CG_NT) void k_cg_single(
   const int ni = nx - 2, nj = ny - 2;
   const int nint = ni * nj * (nz - 2);
   for (size_t q = tid; q < n; q += NTB) P[q] = 0.0;
-  size_t pidx[NPT];
+  // per point: LDS indices of the point and of its 26 neighbours (the
+  // level fits the LDS, so 32-bit), coefficients in registers
+  int pidx[NPT], nb[NPT == 1 ? 26 : 1][26];  // indices kept for NPT 1
   bool val[NPT];
   double xr[NPT], rr[NPT], apv[NPT], dg[NPT];
   double cf[NPT][26];
@@ -953,12 +1028,17 @@ __global__ __launch_bounds__(CG_NT) void k_cg_single(
     val[m] = t < nint;
     const int tt = val[m] ? t : 0;
     const int i = 2 + tt % ni, j = 2 + (tt / ni) % nj, k = 2 + tt / (ni * nj);
-    pidx[m] = ix(nx, ny, i, j, k);
+    const size_t p = ix(nx, ny, i, j, k);
+    pidx[m] = (int)p;
     xr[m] = 0.0;
-    rr[m] = val[m] ? (double)f[pidx[m]] : 0.0;  // r = f - A*0
+    rr[m] = val[m] ? (double)f[p] : 0.0;  // r = f - A*0
     apv[m] = 0.0;
-    dg[m] = ac[pidx[m]] + cc[pidx[m]];
-    cg_load_row<ND>(ac, n, sy, sz, pidx[m], cf[m], off);
+    dg[m] = ac[p] + cc[p];
+    cg_load_row<ND>(ac, n, sy, sz, p, cf[m], off);
+    if constexpr (NPT == 1) {
+#pragma unroll
+      for (int q = 0; q < 26; q++) nb[0][q] = (int)((long long)p + off[q]);
+    }
   }
   __syncthreads();
   double s = 0.0;
@@ -971,7 +1051,10 @@ __global__ __launch_bounds__(CG_NT) void k_cg_single(
   double rhok1 = 0.0;
   if (denom != 0.0) {
     for (;;) {
-      if (sqrt(rhok2) / denom <= errtol) break;
+      // stopping test sqrt(rhok2)/denom <= errtol; for errtol = 0 (the
+      // multigrid coarse solve) that is rhok2 == 0, without the divide
+      if (errtol == 0.0 ? rhok2 == 0.0 : sqrt(rhok2) / denom <= errtol)
+        break;
       if (iters >= itmax) break;
       // the previous reduction's barrier ordered every read of P before
       // these writes
@@ -990,17 +1073,21 @@ __global__ __launch_bounds__(CG_NT) void k_cg_single(
       s = 0.0;
 #pragma unroll
       for (int m = 0; m < NPT; m++) {
-        const size_t p = pidx[m];
+        auto NB = [&](int q) -> int {
+          if constexpr (NPT == 1) return nb[0][q];
+          else return pidx[m] + (int)off[q];
+        };
         double o = 0.0, u = 0.0, dd = 0.0;
 #pragma unroll
-        for (int q = 0; q < 8; q++) o += cf[m][q] * P[(size_t)((long long)p + off[q])];
+        for (int q = 0; q < 8; q++) o += cf[m][q] * P[NB(q)];
 #pragma unroll
-        for (int q = 8; q < 17; q++) u += cf[m][q] * P[(size_t)((long long)p + off[q])];
+        for (int q = 8; q < 17; q++) u += cf[m][q] * P[NB(q)];
 #pragma unroll
-        for (int q = 17; q < 26; q++) dd += cf[m][q] * P[(size_t)((long long)p + off[q])];
-        const double v = -o + -u + -dd + dg[m] * P[p];
+        for (int q = 17; q < 26; q++) dd += cf[m][q] * P[NB(q)];
+        const double pp = P[pidx[m]];
+        const double v = -o + -u + -dd + dg[m] * pp;
         apv[m] = val[m] ? v : 0.0;
-        if (val[m]) s += P[p] * v;
+        if (val[m]) s += pp * v;
       }
       const double pAp = cg_allsum(s, rbuf[b], nw);
       b ^= 1;
@@ -1545,6 +1632,12 @@ void buildG(const Dim &f, const Dim &c, int numdia, const double *pc,
                              const T *, hipStream_t);                         \
   template void interp<T>(const Dim &, const Dim &, const T *, const T *,     \
                           T *, const double *, hipStream_t);                  \
+  template void interp_damped<T>(const Dim &, const Dim &, const T *,         \
+                                 const T *, T *, const double *, unsigned,    \
+                                 hipStream_t);                                \
+  template unsigned matvec_dots_part<T>(const Dim &, int, const T *,          \
+                                        const T *, const T *, const T *, T *, \
+                                        double *, hipStream_t);               \
   template void nrm1<T>(const Dim &, const T *, const Red &, hipStream_t);     \
   template void cg_single<T>(const Dim &, int, const double *,                \
                              const double *, const T *, T *, int, double,     \

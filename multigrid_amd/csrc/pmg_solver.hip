@@ -297,11 +297,11 @@ void vcycle(pmg_ctx *c, LV<T> *L, const T *frhs, const double *ac64c,
   }
   for (int l = nlev - 2; l >= 0; l--) {  // up (mgcsd.cc:418-473)
     const int cl = l + 1;
-    const Red rd = red_to(c, S_XDAMP);
-    matvec_dots<T>(L[cl].d, L[cl].numdia, L[cl].ac, L[cl].cc, L[cl].x,
-                   L[cl].w0, L[cl].t, &rd, s);
-    interp<T>(L[cl].d, L[l].d, L[cl].x, L[cl].pc, L[l].x, c->scal + S_XDAMP,
-              s);
+    const unsigned nb =
+        matvec_dots_part<T>(L[cl].d, L[cl].numdia, L[cl].ac, L[cl].cc,
+                            L[cl].x, L[cl].w0, L[cl].t, c->part, s);
+    interp_damped<T>(L[cl].d, L[l].d, L[cl].x, L[cl].pc, L[l].x, c->part, nb,
+                     s);
     smooth<T>(c, L[l], l == 0 ? frhs : L[l].w0, L[l].x, p.nu2, 1, l == 0);
   }
 }
