@@ -1118,12 +1118,164 @@ __global__ __launch_bounds__(CG_NT) void k_cg_single(
     if (val[m]) xout[pidx[m]] = (T)xr[m];
 }
 
+// Row-segment variant for 27-point levels: each thread owns CG_R
+// consecutive points of an i-row.  Per iteration it reads the R+2 values of
+// each of the 9 neighbouring rows once from LDS (15 reads per point instead
+// of 27) and keeps the 26 couplings of its points in registers; fewer
+// wavefronts make the per-iteration reductions and barriers cheaper.  Terms
+// are summed in the reference's three groups (in-plane, up, down), each
+// group row by row.
+constexpr int CG_R = 3, CG_RNT = 512;
+template <class T>
+__global__ __launch_bounds__(CG_RNT) void k_cg_rows(
+    int nx, int ny, int nz, const double *__restrict__ ac,
+    const double *__restrict__ cc, const T *__restrict__ f,
+    T *__restrict__ xout, int itmax, double errtol) {
+  constexpr int R = CG_R;
+  extern __shared__ double P[];
+  __shared__ double rbuf[2][CG_RNT / 64];
+  const int tid = threadIdx.x, NTB = blockDim.x, nw = NTB / 64;
+  const size_t n = (size_t)nx * ny * nz;
+  const long long sy = nx, sz = (long long)nx * ny;
+  const int ni = nx - 2, nj = ny - 2, nk = nz - 2;
+  const int nseg = (ni + R - 1) / R;
+  const bool act = tid < nseg * nj * nk;
+  const int tt = act ? tid : 0;
+  const int j = 2 + (tt / nseg) % nj, k = 2 + tt / (nseg * nj);
+  const int i0 = 2 + (tt % nseg) * R;
+  // n + R + 2 entries: a tail segment may read a few values past the grid
+  for (size_t q = tid; q < n + R + 2; q += NTB) P[q] = 0.0;
+  int pidx[R];
+  bool val[R];
+  double xr[R], rr[R], apv[R], dg[R];
+  double cf[R][26];
+  long long off[26];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    val[r] = act && i0 + r <= nx - 1;
+    const size_t p = ix(nx, ny, val[r] ? i0 + r : 2, j, k);
+    pidx[r] = (int)p;
+    xr[r] = 0.0;
+    rr[r] = val[r] ? (double)f[p] : 0.0;  // r = f - A*0
+    apv[r] = 0.0;
+    dg[r] = val[r] ? ac[p] + cc[p] : 0.0;
+    cg_load_row<27>(ac, n, sy, sz, p, cf[r], off);
+    if (!val[r]) {
+#pragma unroll
+      for (int q = 0; q < 26; q++) cf[r][q] = 0.0;
+    }
+  }
+  // LDS index of (i0-1, j+dj, k+dk), rows in (dk, dj) order
+  int rb[9];
+#pragma unroll
+  for (int a = 0; a < 9; a++)
+    rb[a] = (int)ix(nx, ny, i0 - 1, j + (a % 3) - 1, k + (a / 3) - 1);
+  __syncthreads();
+  double s = 0.0;
+#pragma unroll
+  for (int r = 0; r < R; r++) s += rr[r] * rr[r];
+  double rhok2 = cg_allsum(s, rbuf[0], nw);
+  int b = 1;
+  const double denom = sqrt(rhok2);
+  int iters = 0;
+  double rhok1 = 0.0;
+  // offset q of cg_load_row -> (di, dj, dk)
+  constexpr signed char DI[26] = {0, 0, 1, -1, 1, -1, 1, -1, 0, 0, 0, 1, -1,
+                                  1, -1, 1, -1, 0, 0, 0, 1, -1, 1, -1, 1, -1};
+  constexpr signed char DJ[26] = {1, -1, 0, 0, 1, 1, -1, -1, 0, 1, -1, 0, 0,
+                                  1, 1, -1, -1, 0, 1, -1, 0, 0, 1, 1, -1, -1};
+  constexpr signed char DK[26] = {0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1,
+                                  1, 1, 1, 1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
+  if (denom != 0.0) {
+    for (;;) {
+      if (errtol == 0.0 ? rhok2 == 0.0 : sqrt(rhok2) / denom <= errtol)
+        break;
+      if (iters >= itmax) break;
+      if (iters == 0) {
+#pragma unroll
+        for (int r = 0; r < R; r++)
+          if (val[r]) P[pidx[r]] = rr[r];
+      } else {
+        const double beta = rhok2 / rhok1;
+        const double ib = 1.0 / beta;
+#pragma unroll
+        for (int r = 0; r < R; r++)
+          if (val[r]) P[pidx[r]] = (P[pidx[r]] + ib * rr[r]) * beta;
+      }
+      __syncthreads();
+      double o[R], u[R], dd[R], pp[R];
+#pragma unroll
+      for (int r = 0; r < R; r++) o[r] = u[r] = dd[r] = 0.0;
+#pragma unroll
+      for (int a = 0; a < 9; a++) {
+        const int dj = (a % 3) - 1, dk = (a / 3) - 1;
+        double w[R + 2];
+#pragma unroll
+        for (int c = 0; c < R + 2; c++) w[c] = P[rb[a] + c];
+        if (dj == 0 && dk == 0) {
+#pragma unroll
+          for (int r = 0; r < R; r++) pp[r] = w[r + 1];
+        }
+#pragma unroll
+        for (int q = 0; q < 26; q++) {
+          if (DJ[q] != dj || DK[q] != dk) continue;
+#pragma unroll
+          for (int r = 0; r < R; r++) {
+            const double t = cf[r][q] * w[r + 1 + DI[q]];
+            if (DK[q] == 0) o[r] += t;
+            else if (DK[q] > 0) u[r] += t;
+            else dd[r] += t;
+          }
+        }
+      }
+      s = 0.0;
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const double v = -o[r] + -u[r] + -dd[r] + dg[r] * pp[r];
+        apv[r] = val[r] ? v : 0.0;
+        if (val[r]) s += pp[r] * v;
+      }
+      const double pAp = cg_allsum(s, rbuf[b], nw);
+      b ^= 1;
+      const double alpha = rhok2 / pAp;
+      rhok1 = rhok2;
+      s = 0.0;
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        if (!val[r]) continue;
+        xr[r] += alpha * pp[r];
+        rr[r] += -alpha * apv[r];
+        s += rr[r] * rr[r];
+      }
+      iters++;
+      rhok2 = cg_allsum(s, rbuf[b], nw);
+      b ^= 1;
+    }
+  }
+  for (size_t q = tid; q < n; q += NTB) {
+    const int i = 1 + (int)(q % nx), jj = 1 + (int)((q / nx) % ny),
+              kk = 1 + (int)(q / ((size_t)nx * ny));
+    if (i == 1 || i == nx || jj == 1 || jj == ny || kk == 1 || kk == nz)
+      xout[q] = (T)0;
+  }
+#pragma unroll
+  for (int r = 0; r < R; r++)
+    if (val[r]) xout[pidx[r]] = (T)xr[r];
+}
+
 template <class T, int ND>
 static void cg_launch(const Dim &d, const double *ac, const double *cc,
                       const T *f, T *x, int itmax, double errtol,
                       hipStream_t s) {
   const size_t lds = d.n() * sizeof(double);
   const size_t ni = d.interior();
+  const int nrow = (int)((d.nx - 2 + CG_R - 1) / CG_R) * (d.ny - 2) * (d.nz - 2);
+  if (ND == 27 && nrow <= CG_RNT) {
+    const int nt = (nrow + 63) / 64 * 64;
+    k_cg_rows<T><<<1, nt, lds + (CG_R + 2) * sizeof(double), s>>>(
+        d.nx, d.ny, d.nz, ac, cc, f, x, itmax, errtol);
+    return;
+  }
   // one interior point per thread when they fit: as few wavefronts as the
   // level needs (the loop is LDS- and barrier-bound, idle waves cost)
   if (ni <= CG_NT) {
