@@ -32,6 +32,9 @@ def parse():
     ap.add_argument("--config", default="c1")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="first run a plain device copy of this many MiB (a reference "
+                         "kernel of known traffic for hardware-counter runs)")
     return ap.parse_args()
 
 
@@ -82,6 +85,16 @@ def main():
               "(the solver runs on a single MI355X).", file=sys.stderr)
         sys.exit(2)
     import multigrid_amd as mg
+
+    if args.copy_probe_mib:
+        import torch
+        nel = args.copy_probe_mib * (1 << 20) // 4
+        a = torch.ones(nel, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
 
     cfg = dict(mg.CONFIGS[args.config])
     n, nlev = cfg["n"], cfg["nlev"]

@@ -245,13 +245,20 @@ template <class T>
 __global__ __launch_bounds__(SW_TX *SW_TY) void k_gs7_sweep(
     int nx, int ny, int nz, const T *__restrict__ ac, const T *__restrict__ cc,
     const T *__restrict__ fc, const T *__restrict__ xin, T *__restrict__ xout,
-    int c1, int kc) {
+    int c1, int kc, int gx, int gy, int gz) {
   __shared__ T SX[2][SW_TY][SW_TX], SE[3][SW_TY][SW_TX], SN[3][SW_TY][SW_TX],
       SM[3][SW_TY][SW_TX];
+  // XCD-aware tile order: workgroups are dispatched round-robin over the 8
+  // XCDs, so workgroup b runs on XCD b % 8.  Each XCD gets a contiguous run
+  // of tiles (x fastest, then y, then k chunk): neighbouring tiles, which
+  // read each other's halo rows and columns, share that XCD's L2.
+  const int ntile = gx * gy * gz, per = (ntile + 7) / 8;
+  const int t = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+  if (t >= ntile) return;  // whole workgroup, before any barrier
   const int tx = threadIdx.x, ty = threadIdx.y;
-  const int i = blockIdx.x * (SW_TX - 4) + tx;
-  const int j = blockIdx.y * (SW_TY - 4) + ty;
-  const int kb = 2 + blockIdx.z * kc;
+  const int i = (t % gx) * (SW_TX - 4) + tx;
+  const int j = ((t / gx) % gy) * (SW_TY - 4) + ty;
+  const int kb = 2 + (t / (gx * gy)) * kc;
   const int ke = min(kb + kc - 1, nz - 1);
   const size_t n = (size_t)nx * ny * nz, sz = (size_t)nx * ny;
   const T *oE = ac + n, *oN = ac + 2 * n, *uC = ac + 3 * n;
@@ -297,7 +304,9 @@ __global__ __launch_bounds__(SW_TX *SW_TY) void k_gs7_sweep(
   T mcur = first(kb) ? fval(kb, P0, Pm1.x, P1.x, Pm1.uC) : P0.x;
   SM[kb % 3][ty][tx] = mcur;
   for (int k = kb; k <= ke; k++) {
-    load(k + 3, P3);
+    // plane ke+3 is never used: the last step re-reads plane ke+2 (an L2
+    // hit) so that the load stays unconditional
+    load(min(k + 3, ke + 2), P3);
     const int s1 = (k + 1) % 3, s0 = k % 3;
     SX[(k + 1) & 1][ty][tx] = P1.x;
     SE[s1][ty][tx] = P1.oE;
@@ -352,13 +361,14 @@ void gs7_fused(const Dim &d, const T *ac, const T *cc, const T *fc, T *x,
   const int gx = cdiv(d.nx - 2, SW_TX - 4), gy = cdiv(d.ny - 2, SW_TY - 4);
   const int zb = std::max(1, device_cus() / (gx * gy));
   const int kc = std::max(8, cdiv(d.nz - 2, zb));
-  dim3 g(gx, gy, cdiv(d.nz - 2, kc));
+  const int gz = cdiv(d.nz - 2, kc);
+  const int nblk = 8 * cdiv(gx * gy * gz, 8);
   dim3 b(SW_TX, SW_TY);
   const int c1 = iadjoint ? 1 : 0;
   T *src = x, *dst = tmp;
   for (int it = 0; it < nsweeps; it++) {
-    k_gs7_sweep<T><<<g, b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, fc, src, dst, c1,
-                                   kc);
+    k_gs7_sweep<T><<<nblk, b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, fc, src, dst,
+                                      c1, kc, gx, gy, gz);
     T *t = src;
     src = dst;
     dst = t;
