@@ -25,6 +25,30 @@ __host__ __device__ inline size_t ix(int nx, int ny, int i, int j, int k) {
 
 inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
+// XCD-aware tiling.  Workgroups are dispatched round-robin over the eight
+// XCDs (workgroup b runs on XCD b % 8), each with its own L2.  Stencil
+// kernels are launched as 1-D grids of xcd_blocks(g) workgroups and map
+// workgroup b to tile (b % 8) * per + b / 8 of the logical g.x * g.y * g.z
+// tile grid (x fastest): every XCD works on one contiguous slab of tiles, so
+// the halo rows and planes that neighbouring tiles share are fetched into
+// one L2 once.  Workgroups past the last tile do no work (reductions still
+// store a zero partial).
+struct XG {
+  int gx, gy, gz;
+};
+inline XG xg_of(const dim3 &g) { return XG{(int)g.x, (int)g.y, (int)g.z}; }
+inline unsigned xcd_blocks(const dim3 &g) {
+  return 8u * ((g.x * g.y * g.z + 7u) / 8u);
+}
+__device__ inline bool xcd_tile(const XG &g, int &bx, int &by, int &bz) {
+  const int nt = g.gx * g.gy * g.gz, per = (nt + 7) / 8;
+  const int t = (int)(blockIdx.x % 8u) * per + (int)(blockIdx.x / 8u);
+  bx = t % g.gx;
+  by = (t / g.gx) % g.gy;
+  bz = t / (g.gx * g.gy);
+  return t < nt;
+}
+
 static void check_launch(const char *what) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess)
@@ -412,8 +436,12 @@ __global__ __launch_bounds__(256) void k_gs27_rows(
     const T *__restrict__ fc, T *__restrict__ x, int pz, int py, int nj,
     int nrows, int zmask) {
   const int lane = threadIdx.x & 63;
-  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (row >= nrows) return;  // whole wavefront; no block barrier is used
+  // XCD-contiguous runs of rows (see xcd_tile): the row blocks of one XCD
+  // are neighbours in (j, k), in every class launch of the sweep
+  const int nrb = (nrows + 3) / 4, per = (nrb + 7) / 8;
+  const int rblk = (int)(blockIdx.x % 8u) * per + (int)(blockIdx.x / 8u);
+  const int row = rblk * 4 + (threadIdx.x >> 6);
+  if (rblk >= nrb || row >= nrows) return;  // whole wavefronts; no barrier
   const int j = 2 + py + 2 * (row % nj), k = 2 + pz + 2 * (row / nj);
   const size_t n = (size_t)nx * ny * nz, sy = nx, sz = (size_t)nx * ny;
   const T *oE = ac + n, *oN = ac + 2 * n, *uC = ac + 3 * n, *oNE = ac + 4 * n,
@@ -500,7 +528,7 @@ void gs27(const Dim &d, const T *ac, const T *cc, const T *fc, T *x,
         const int nj = count_parity(d.ny, py), nk = count_parity(d.nz, pz);
         const int nrows = nj * nk;
         if (nrows > 0) {
-          const int nb = (nrows + 3) / 4;
+          const int nb = 8 * (((nrows + 3) / 4 + 7) / 8);
           if (iadjoint)
             k_gs27_rows<T, 1><<<nb, 256, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, fc,
                                                  x, pz, py, nj, nrows, zmask);
@@ -537,14 +565,16 @@ template <class T, class TO, int ND>
 __global__ __launch_bounds__(256) void k_resid(
     int nx, int ny, int nz, const T *__restrict__ ac,
     const T *__restrict__ cc, const T *__restrict__ fc,
-    const T *__restrict__ x, TO *__restrict__ r, Red rd, int do_red) {
+    const T *__restrict__ x, TO *__restrict__ r, Red rd, int do_red, XG xg) {
   __shared__ double red[4];
   const size_t n = (size_t)nx * ny * nz, sy = nx, sz = (size_t)nx * ny;
-  const int i = 2 + blockIdx.x * BX + threadIdx.x;
-  const int j = 2 + blockIdx.y * BY + threadIdx.y;
-  const int k0 = 2 + blockIdx.z * KCH;
+  int bx, by, bz;
+  const bool tok = xcd_tile(xg, bx, by, bz);
+  const int i = 2 + bx * BX + threadIdx.x;
+  const int j = 2 + by * BY + threadIdx.y;
+  const int k0 = 2 + bz * KCH;
   double acc = 0.0;
-  if (i <= nx - 1 && j <= ny - 1) {
+  if (tok && i <= nx - 1 && j <= ny - 1) {
     for (int kk = 0; kk < KCH; kk++) {
       const int k = k0 + kk;
       if (k > nz - 1) break;
@@ -579,12 +609,12 @@ void resid(const Dim &d, int numdia, const T *ac, const T *cc, const T *fc,
   dim3 b(BX, BY);
   const Red rd = red ? *red : Red{};
   if (numdia == 7)
-    k_resid<T, TO, 7><<<g, b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, fc, x, r, rd,
-                                      red ? 1 : 0);
+    k_resid<T, TO, 7><<<xcd_blocks(g), b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, fc, x, r, rd,
+                                      red ? 1 : 0, xg_of(g));
   else
-    k_resid<T, TO, 27><<<g, b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, fc, x, r, rd,
-                                       red ? 1 : 0);
-  if (red) red_final<1, 0>(rd, g, s);
+    k_resid<T, TO, 27><<<xcd_blocks(g), b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, fc, x, r, rd,
+                                       red ? 1 : 0, xg_of(g));
+  if (red) red_final<1, 0>(rd, dim3(xcd_blocks(g)), s);
   check_launch("resid");
 }
 
@@ -641,14 +671,16 @@ __global__ __launch_bounds__(256) void k_nmresid7(
     int nx, int ny, int nz, const double *__restrict__ ac,
     const double *__restrict__ cc, const double *__restrict__ fc,
     const double *__restrict__ x, double *__restrict__ r, Red rd,
-    int do_red) {
+    int do_red, XG xg) {
   __shared__ double red[4];
   const size_t n = (size_t)nx * ny * nz, sy = nx, sz = (size_t)nx * ny;
-  const int i = 2 + blockIdx.x * BX + threadIdx.x;
-  const int j = 2 + blockIdx.y * BY + threadIdx.y;
-  const int k0 = 2 + blockIdx.z * KCH;
+  int bx, by, bz;
+  const bool tok = xcd_tile(xg, bx, by, bz);
+  const int i = 2 + bx * BX + threadIdx.x;
+  const int j = 2 + by * BY + threadIdx.y;
+  const int k0 = 2 + bz * KCH;
   double acc = 0.0;
-  if (i <= nx - 1 && j <= ny - 1) {
+  if (tok && i <= nx - 1 && j <= ny - 1) {
     const double *oE = ac + n, *oN = ac + 2 * n, *uC = ac + 3 * n;
     for (int kk = 0; kk < KCH; kk++) {
       const int k = k0 + kk;
@@ -674,9 +706,9 @@ void nmresid7(const Dim &d, const double *ac, const double *cc,
               hipStream_t s) {
   dim3 g(cdiv(d.nx - 2, BX), cdiv(d.ny - 2, BY), cdiv(d.nz - 2, KCH));
   const Red rd = red ? *red : Red{};
-  k_nmresid7<<<g, dim3(BX, BY), 0, s>>>(d.nx, d.ny, d.nz, ac, cc, fc, x, r,
-                                        rd, red ? 1 : 0);
-  if (red) red_final<1, 0>(rd, g, s);
+  k_nmresid7<<<xcd_blocks(g), dim3(BX, BY), 0, s>>>(d.nx, d.ny, d.nz, ac, cc, fc, x, r,
+                                        rd, red ? 1 : 0, xg_of(g));
+  if (red) red_final<1, 0>(rd, dim3(xcd_blocks(g)), s);
   check_launch("nmresid7");
 }
 
@@ -687,14 +719,16 @@ template <class T, int ND>
 __global__ __launch_bounds__(256) void k_matvec_dots(
     int nx, int ny, int nz, const T *__restrict__ ac,
     const T *__restrict__ cc, const T *__restrict__ x,
-    const T *__restrict__ w0, T *__restrict__ y, Red rd, int do_red) {
+    const T *__restrict__ w0, T *__restrict__ y, Red rd, int do_red, XG xg) {
   __shared__ double red[4];
   const size_t n = (size_t)nx * ny * nz, sy = nx, sz = (size_t)nx * ny;
-  const int i = 2 + blockIdx.x * BX + threadIdx.x;
-  const int j = 2 + blockIdx.y * BY + threadIdx.y;
-  const int k0 = 2 + blockIdx.z * KCH;
+  int bx, by, bz;
+  const bool tok = xcd_tile(xg, bx, by, bz);
+  const int i = 2 + bx * BX + threadIdx.x;
+  const int j = 2 + by * BY + threadIdx.y;
+  const int k0 = 2 + bz * KCH;
   double a0 = 0.0, a1 = 0.0;
-  if (i <= nx - 1 && j <= ny - 1) {
+  if (tok && i <= nx - 1 && j <= ny - 1) {
     for (int kk = 0; kk < KCH; kk++) {
       const int k = k0 + kk;
       if (k > nz - 1) break;
@@ -730,12 +764,12 @@ void matvec_dots(const Dim &d, int numdia, const T *ac, const T *cc,
   dim3 b(BX, BY);
   const Red rd = red ? *red : Red{};
   if (numdia == 7)
-    k_matvec_dots<T, 7><<<g, b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, x, w0, y,
-                                        rd, red ? 1 : 0);
+    k_matvec_dots<T, 7><<<xcd_blocks(g), b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, x, w0, y,
+                                        rd, red ? 1 : 0, xg_of(g));
   else
-    k_matvec_dots<T, 27><<<g, b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, x, w0, y,
-                                         rd, red ? 1 : 0);
-  if (red) red_final<2, 1>(rd, g, s);
+    k_matvec_dots<T, 27><<<xcd_blocks(g), b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, x, w0, y,
+                                         rd, red ? 1 : 0, xg_of(g));
+  if (red) red_final<2, 1>(rd, dim3(xcd_blocks(g)), s);
   check_launch("matvec_dots");
 }
 
@@ -748,13 +782,13 @@ unsigned matvec_dots_part(const Dim &d, int numdia, const T *ac, const T *cc,
   Red rd;
   rd.part = part;
   if (numdia == 7)
-    k_matvec_dots<T, 7><<<g, b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, x, w0, y,
-                                        rd, 1);
+    k_matvec_dots<T, 7><<<xcd_blocks(g), b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, x, w0, y,
+                                        rd, 1, xg_of(g));
   else
-    k_matvec_dots<T, 27><<<g, b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, x, w0, y,
-                                         rd, 1);
+    k_matvec_dots<T, 27><<<xcd_blocks(g), b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, x, w0, y,
+                                         rd, 1, xg_of(g));
   check_launch("matvec_dots");
-  return g.x * g.y * g.z;
+  return xcd_blocks(g);
 }
 
 // ---------------------------------------------------------------------------
@@ -765,10 +799,12 @@ __global__ __launch_bounds__(256) void k_restrict(int nxf, int nyf, int nxc,
                                                   int nyc, int nzc,
                                                   const T *__restrict__ xin,
                                                   T *__restrict__ xout,
-                                                  const T *__restrict__ pc) {
-  const int ii = 2 + blockIdx.x * BX + threadIdx.x;
-  const int jj = 2 + blockIdx.y * BY + threadIdx.y;
-  const int kk = 2 + blockIdx.z;
+                                                  const T *__restrict__ pc, XG xg) {
+  int bx, by, bz;
+  if (!xcd_tile(xg, bx, by, bz)) return;
+  const int ii = 2 + bx * BX + threadIdx.x;
+  const int jj = 2 + by * BY + threadIdx.y;
+  const int kk = 2 + bz;
   if (ii > nxc - 1 || jj > nyc - 1) return;
   const size_t nc = (size_t)nxc * nyc * nzc, pC = ix(nxc, nyc, ii, jj, kk);
   const size_t sy = nxf, sz = (size_t)nxf * nyf;
@@ -805,8 +841,8 @@ void restrict_(const Dim &f, const Dim &c, const T *xin, T *xout, const T *pc,
     return;
   }
   dim3 g(cdiv(c.nx - 2, BX), cdiv(c.ny - 2, BY), c.nz - 2);
-  k_restrict<T><<<g, dim3(BX, BY), 0, s>>>(f.nx, f.ny, c.nx, c.ny, c.nz, xin,
-                                           xout, pc);
+  k_restrict<T><<<xcd_blocks(g), dim3(BX, BY), 0, s>>>(f.nx, f.ny, c.nx, c.ny, c.nz, xin,
+                                           xout, pc, xg_of(g));
   check_launch("restrict");
 }
 
@@ -827,10 +863,12 @@ __global__ __launch_bounds__(256) void k_interp(int nxc, int nyc, int nzc,
                                                 T *__restrict__ xf,
                                                 const double *xdamp,
                                                 const double *part,
-                                                unsigned nblk) {
-  const int ii = 1 + blockIdx.x * BX + threadIdx.x;
-  const int jj = 1 + blockIdx.y * BY + threadIdx.y;
-  const int kk = 1 + blockIdx.z;
+                                                unsigned nblk, XG xg) {
+  int bx, by, bz;
+  const bool tok = xcd_tile(xg, bx, by, bz);
+  const int ii = 1 + bx * BX + threadIdx.x;
+  const int jj = 1 + by * BY + threadIdx.y;
+  const int kk = 1 + bz;
   // xdamp from the (x.w0, x.y) partials of the coarse matvec, summed by
   // every block in the same fixed order (mgcsd.cc:429-452)
   __shared__ double red[8];
@@ -847,7 +885,7 @@ __global__ __launch_bounds__(256) void k_interp(int nxc, int nyc, int nzc,
     if (tid == 0) dsh = c != 0.0 ? a / c : 0.0;
     __syncthreads();
   }
-  if (ii > nxc - 1 || jj > nyc - 1) return;
+  if (!tok || ii > nxc - 1 || jj > nyc - 1) return;
   const size_t nc = (size_t)nxc * nyc * nzc;
   const size_t c0 = ix(nxc, nyc, ii, jj, kk);
   const size_t cx = 1, cy = nxc, cz = (size_t)nxc * nyc;
@@ -899,13 +937,13 @@ void interp(const Dim &c, const Dim &f, const T *xc, const T *pc, T *xf,
   }
   dim3 g(cdiv(c.nx - 1, BX), cdiv(c.ny - 1, BY), c.nz - 1);
   if (xdamp) {
-    k_interp<T, true><<<g, dim3(BX, BY), 0, s>>>(c.nx, c.ny, c.nz, f.nx, f.ny,
+    k_interp<T, true><<<xcd_blocks(g), dim3(BX, BY), 0, s>>>(c.nx, c.ny, c.nz, f.nx, f.ny,
                                                  f.nz, xc, pc, xf, xdamp,
-                                                 nullptr, 0u);
+                                                 nullptr, 0u, xg_of(g));
   } else {
-    k_interp<T, false><<<g, dim3(BX, BY), 0, s>>>(c.nx, c.ny, c.nz, f.nx,
+    k_interp<T, false><<<xcd_blocks(g), dim3(BX, BY), 0, s>>>(c.nx, c.ny, c.nz, f.nx,
                                                   f.ny, f.nz, xc, pc, xf,
-                                                  nullptr, nullptr, 0u);
+                                                  nullptr, nullptr, 0u, xg_of(g));
     if constexpr (sizeof(T) == 8) fbound00(f, (double *)xf, s);
   }
   check_launch("interp");
@@ -919,9 +957,9 @@ void interp_damped(const Dim &c, const Dim &f, const T *xc, const T *pc,
     return;
   }
   dim3 g(cdiv(c.nx - 1, BX), cdiv(c.ny - 1, BY), c.nz - 1);
-  k_interp<T, true><<<g, dim3(BX, BY), 0, s>>>(c.nx, c.ny, c.nz, f.nx, f.ny,
+  k_interp<T, true><<<xcd_blocks(g), dim3(BX, BY), 0, s>>>(c.nx, c.ny, c.nz, f.nx, f.ny,
                                                f.nz, xc, pc, xf, nullptr, part,
-                                               nblk);
+                                               nblk, xg_of(g));
   check_launch("interp_damped");
 }
 
