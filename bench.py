@@ -55,6 +55,20 @@ def timed_steps(ctx, mol, mode, steps, warmup):
     return wall, np.array(lat), iters
 
 
+def measured_traffic(n):
+    """HBM bytes per launch of the fine sweep for an n^3 level, as measured
+    with rocprofv3 FETCH_SIZE / WRITE_SIZE counters and recorded in
+    profiles/pb<n>_hbm_traffic.md (None when no such measurement exists)."""
+    path = os.path.join(ROOT, "profiles", f"pb{n}_hbm_traffic.md")
+    try:
+        for line in open(path):
+            if line.startswith(f"traffic_bytes_per_launch k_gs7_sweep<float> n={n}:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(cfg, atoms, threads):
     """The CPU oracle (C restatement of the reference pmgc, OpenMP) timed on
     this host on one complete solve of the same molecule: the solve
@@ -166,7 +180,7 @@ def main():
         **err_oracle,
         "setup_ms": st["setup_ms"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(n),
                      "kernel": "k_gs7_sweep<float> (fine fused red-black sweep)",
                      "avg_launch_us": avg_launch_s * 1e6, "timed_launches": launches},
         "cpu_baseline": cpu,
