@@ -193,6 +193,41 @@ int pmg_ctx_create_pbe(pmg_ctx **out, const int *iparm, const double *rparm,
 int pmg_ctx_load_pbe(pmg_ctx *ctx, double h, double ionic_strength,
                      int natoms, const double *atoms);
 
+/* --- serving loop (no reference counterpart: the reference solves one
+ * problem per process).  Continuous batching of linear PB solves of
+ * uneven grid sizes: requests are admitted as solve slots and device
+ * memory free up, their contexts are pooled by solve shape, and the
+ * iterations of all admitted solves overlap on the device (one HIP stream
+ * each, graph-captured V-cycle iterations).  Results are those of the same
+ * solve run alone. --- */
+typedef struct pmg_server pmg_server;
+
+/* Start a server on the current GPU: at most max_active solves in flight,
+ * contexts within mem_budget bytes of HBM (0: 90% of the free memory). */
+int pmg_server_create(pmg_server **out, size_t mem_budget, int max_active);
+/* Queue the solve of an atom list (natoms x {x, y, z, radius, charge},
+ * copied) on the cubic grid iparm/rparm describe (linear: iparm(16) = 0),
+ * spacing h, 1:1 salt of ionic_strength mol/L, mode PMG_FP64 or PMG_MIXED.
+ * u_out (n^3, may be NULL) receives the potential and must stay valid until
+ * pmg_server_wait returns for the ticket.  PMG_EBUSY: the solve can never
+ * fit the memory budget. */
+int pmg_server_submit(pmg_server *srv, const int *iparm, const double *rparm,
+                      int mode, double h, double ionic_strength, int natoms,
+                      const double *atoms, double *u_out, int64_t *ticket);
+/* Wait for a ticket (timeout_ms < 0: no limit; PMG_EBUSY on timeout, the
+ * ticket stays valid); returns the solve's status code, releases the ticket.
+ * latency_ms: submission to completion, queueing included. */
+int pmg_server_wait(pmg_server *srv, int64_t ticket, int timeout_ms,
+                    int *iters, double *relres, double *latency_ms);
+/* Completed solves, latency percentiles (ms), completed solves per second
+ * from the first submission to the last completion, budgeted bytes held,
+ * most solves seen in flight at once. */
+int pmg_server_stats(pmg_server *srv, long long *completed, double *p50_ms,
+                     double *p99_ms, double *solves_per_s, size_t *mem_used,
+                     int *max_concurrent);
+/* Finish every submitted request, then free the server. */
+void pmg_server_destroy(pmg_server *srv);
+
 /* Version string of the library build. */
 const char *pmg_version(void);
 

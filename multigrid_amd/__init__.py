@@ -88,6 +88,13 @@ def _declare(L):
         "pmg_pbe_assemble": (i, [i, d, d, i, dp] + [dp] * 12),
         "pmg_ctx_create_pbe": (i, [ctypes.POINTER(vp), ip, dp, d, d, i, dp]),
         "pmg_ctx_load_pbe": (i, [vp, d, d, i, dp]),
+        "pmg_server_create": (i, [ctypes.POINTER(vp), ctypes.c_size_t, i]),
+        "pmg_server_submit": (i, [vp, ip, dp, i, d, d, i, dp, dp,
+                                  ctypes.POINTER(ctypes.c_int64)]),
+        "pmg_server_wait": (i, [vp, ctypes.c_int64, i, ip, dp, dp]),
+        "pmg_server_stats": (i, [vp, ctypes.POINTER(ctypes.c_longlong), dp, dp, dp,
+                                 ctypes.POINTER(ctypes.c_size_t), ip]),
+        "pmg_server_destroy": (None, [vp]),
         "pmg_version": (ctypes.c_char_p, []),
     }
     for name, (res, args) in sig.items():
@@ -292,6 +299,66 @@ class Context:
             self.close()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------------------
+# Serving loop
+# ---------------------------------------------------------------------------
+class Server:
+    """Continuous-batching PB solve server (pmg_server_*)."""
+
+    def __init__(self, mem_budget=0, max_active=8):
+        self._h = ctypes.c_void_p()
+        _check(lib().pmg_server_create(ctypes.byref(self._h), mem_budget, max_active),
+               "pmg_server_create")
+        self._out = {}
+
+    def submit(self, iparm, rparm, mode, h, ionic_strength, atoms, want_u=True):
+        """Queue a solve; returns the ticket.  The potential (if want_u)
+        comes back from wait()."""
+        at = np.ascontiguousarray(atoms, np.float64).reshape(-1)
+        n = int(iparm[2]) * int(iparm[3]) * int(iparm[4])
+        u = np.empty(n) if want_u else None
+        t = ctypes.c_int64()
+        rc = lib().pmg_server_submit(self._h, _ip(iparm), _dp(rparm), mode, h,
+                                     ionic_strength, at.size // 5, _dp(at), _dp(u),
+                                     ctypes.byref(t))
+        _check(rc, "pmg_server_submit")
+        self._out[t.value] = u
+        return t.value
+
+    def wait(self, ticket, timeout_ms=-1):
+        it, rr, lat = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+        rc = lib().pmg_server_wait(self._h, ticket, timeout_ms, ctypes.byref(it),
+                                   ctypes.byref(rr), ctypes.byref(lat))
+        _check(rc, "pmg_server_wait")
+        return dict(rc=rc, iters=it.value, relres=rr.value, latency_ms=lat.value,
+                    u=self._out.pop(ticket))
+
+    def stats(self):
+        c, mem, mx = ctypes.c_longlong(), ctypes.c_size_t(), ctypes.c_int()
+        p50, p99, sps = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        lib().pmg_server_stats(self._h, ctypes.byref(c), ctypes.byref(p50),
+                               ctypes.byref(p99), ctypes.byref(sps), ctypes.byref(mem),
+                               ctypes.byref(mx))
+        return dict(completed=c.value, p50_ms=p50.value, p99_ms=p99.value,
+                    solves_per_s=sps.value, mem_used=mem.value, max_concurrent=mx.value)
+
+    def close(self):
+        if self._h:
+            lib().pmg_server_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# Solve shapes of the serving workload: grid size -> levels (coarsest 5^3
+# to 11^3), all at h = 0.5 A.
+SERVE_LEVELS = {33: 3, 65: 4, 97: 5, 129: 5, 161: 5}
 
 
 # ---------------------------------------------------------------------------

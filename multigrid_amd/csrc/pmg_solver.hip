@@ -19,18 +19,14 @@
 
 #include "../../include/pmg_mi355x.h"
 #include "pmg_internal.h"
+#include "pmg_ctx.h"
 
 using namespace pmg;
 
-namespace {
+namespace pmg {
+namespace drv {
 
-constexpr int MAXLEV = 16;
 
-struct Params {
-  int nrwk, niwk, nx, ny, nz, nlev, nu1, nu2, mgkey, itmax, istop, iinfo,
-      ipkey, mode, mgprol, mgcoar, mgdisc, mgsmoo, mgsolv, iperf;
-  double errtol;
-};
 
 int maxlev(int n1, int n2, int n3) {  // Vmaxlev (mgsubd.cc:456-484)
   int lev = 0, done = 0;
@@ -103,106 +99,6 @@ int decode(const int *iparm, const double *rparm, Params &p) {
   return PMG_OK;
 }
 
-struct Level {
-  Dim d;
-  int numdia = 7;
-  double *ac = nullptr, *cc = nullptr, *fc = nullptr, *pc = nullptr;
-  double *x = nullptr, *w0 = nullptr, *r = nullptr, *t = nullptr;
-  float *ac32 = nullptr, *cc32 = nullptr, *pc32 = nullptr, *x32 = nullptr,
-        *w032 = nullptr, *r32 = nullptr, *t32 = nullptr;
-  // Newton: Jacobian diagonal term and right-hand side per level
-  double *cpr = nullptr, *rhs = nullptr;
-  float *cpr32 = nullptr;
-};
-
-template <class T>
-struct LV {
-  Dim d;
-  int numdia;
-  const T *ac, *cc, *pc;
-  T *x, *w0, *r, *t;
-};
-
-}  // namespace
-
-struct pmg_ctx {
-  Params p{};
-  int nlev = 0;
-  Level L[MAXLEV];
-  hipStream_t s = nullptr;
-  std::vector<void *> allocs;
-  size_t bytes = 0;
-  bool ok = true;
-  // resident fine-level inputs
-  double *xf = nullptr, *yf = nullptr, *zf = nullptr, *gx = nullptr,
-         *gy = nullptr, *gz = nullptr, *a1 = nullptr, *a2 = nullptr,
-         *a3 = nullptr, *ccf = nullptr, *fcf = nullptr;
-  double *u = nullptr;       // fine solution (FP64)
-  double *work[4] = {};      // fine FP64 scratch (Newton)
-  double *part = nullptr;    // reduction partials
-  double *scal = nullptr;    // device scalars
-  double *hscal = nullptr;   // pinned host mirror
-  bool have32 = false, haveN = false;
-  bool ops32_ok = false;  // FP32 operator copies match the FP64 hierarchy
-  int iters = 0, ierror = 0;
-  double relres = 0.0, setup_ms = 0.0, solve_ms = 0.0;
-  // linear-solve session (pmg_ctx_solve and the serving loop)
-  struct Session {
-    int mode = PMG_FP64, iters = 0, rc = PMG_OK;
-    bool done = true, pending = false;
-    double rsden = 1.0, rsnrm = 0.0;
-  } ses;
-  hipEvent_t cyc_ev = nullptr, t0_ev = nullptr, t1_ev = nullptr;
-  hipEvent_t su0_ev = nullptr, su1_ev = nullptr, stage_ev = nullptr;
-  // atom list of the GPU input assembly: device copy (+ links) and pinned
-  // staging, capacity in atoms
-  double *atoms_d = nullptr, *atoms_h = nullptr;
-  int atoms_cap = 0;
-  // V-cycle bodies captured as HIP graphs, one per precision mode
-  bool graphs = true, capturing = false;
-  hipGraphExec_t gexec[2] = {nullptr, nullptr};
-  hipGraph_t graph[2] = {nullptr, nullptr};
-  // optional per-kernel event timing of the fine smoothing sweeps
-  bool ktime = false;
-  std::vector<hipEvent_t> ev;
-  int nev = 0;
-  double ktime_ms = 0.0;
-  long long ktime_count = 0;
-
-  template <class T>
-  T *alloc(size_t n) {
-    void *ptr = nullptr;
-    if (!ok) return nullptr;
-    if (hipMalloc(&ptr, n * sizeof(T) + 256) != hipSuccess) {
-      ok = false;
-      return nullptr;
-    }
-    hipMemsetAsync(ptr, 0, n * sizeof(T) + 256, s);
-    allocs.push_back(ptr);
-    bytes += n * sizeof(T) + 256;
-    return (T *)ptr;
-  }
-  ~pmg_ctx() {
-    if (s) hipStreamSynchronize(s);
-    for (int m = 0; m < 2; m++) {
-      if (gexec[m]) hipGraphExecDestroy(gexec[m]);
-      if (graph[m]) hipGraphDestroy(graph[m]);
-    }
-    if (cyc_ev) hipEventDestroy(cyc_ev);
-    if (t0_ev) hipEventDestroy(t0_ev);
-    if (t1_ev) hipEventDestroy(t1_ev);
-    for (hipEvent_t e : {su0_ev, su1_ev, stage_ev})
-      if (e) hipEventDestroy(e);
-    if (atoms_d) hipFree(atoms_d);
-    if (atoms_h) hipHostFree(atoms_h);
-    for (void *ptr : allocs) hipFree(ptr);
-    if (hscal) hipHostFree(hscal);
-    for (auto e : ev) hipEventDestroy(e);
-    if (s) hipStreamDestroy(s);
-  }
-};
-
-namespace {
 
 enum { S_RSDEN = 0, S_RSNRM = 1, S_XDAMP = 2, S_TMP = 4 };
 
@@ -516,7 +412,7 @@ void setup_start(pmg_ctx *c) {
   hipEventRecord(c->su0_ev, c->s);
 }
 
-void setup_enqueue(pmg_ctx *c, bool started = false) {
+void setup_enqueue(pmg_ctx *c, bool started) {
   if (!started) setup_start(c);
   build_ops(c);
   hipEventRecord(c->su1_ev, c->s);
@@ -685,7 +581,9 @@ int capture(pmg_ctx *c, int mode) {
 }
 
 // Start a solve: Vazeros + VfboundPMG00 on u (mgdrvd.cc:419, 439), the
-// stopping-test denominator (mgcsd.cc:133-163).  Blocks for two scalars.
+// stopping-test denominator (mgcsd.cc:133-163) and, for PMG_MIXED, the
+// first residual; the two scalars come back with the start event, which
+// lin_poll reads.  Never blocks.
 int lin_begin(pmg_ctx *c, int mode) {
   const Params &p = c->p;
   Level &F = c->L[0];
@@ -698,42 +596,44 @@ int lin_begin(pmg_ctx *c, int mode) {
   if (!cg_fits(c->L[c->nlev - 1].d)) return PMG_EBADPARM;
   if (!c->cyc_ev) {
     hipEventCreateWithFlags(&c->cyc_ev, hipEventDisableTiming);
+    hipEventCreateWithFlags(&c->fin_ev, hipEventDisableTiming);
     hipEventCreate(&c->t0_ev);
     hipEventCreate(&c->t1_ev);
+  }
+  // the iteration graph is captured before the start is enqueued (capture
+  // records, it does not run, the work issued while it lasts)
+  if (c->graphs && !c->ktime) {
+    int rc = capture(c, mode);
+    if (rc) return rc;
   }
   auto &S = c->ses;
   S = pmg_ctx::Session{};
   S.mode = mode;
   S.done = false;
+  S.phase = 0;
   c->nev = 0;
   hipEventRecord(c->t0_ev, c->s);
   fill<double>(c->u, F.d.n(), 0.0, c->s);
   if (p.istop == 1) {
     const Red rden = red_to(c, S_RSDEN);
     nrm1<double>(F.d, F.fc, rden, c->s);
-    S.rsden = host_scalar(c, S_RSDEN);
   }
-  if (S.rsden == 0.0) S.rsden = 1.0;  // mgcsd.cc:154-157
-  S.rsnrm = S.rsden;
   if (mode == PMG_MIXED) {
     // residual of u = 0, the first right-hand side of the FP32 cycle
     const Red rnrm = red_to(c, S_RSNRM);
     resid<double, float>(F.d, F.numdia, F.ac, F.cc, F.fc, c->u, F.w032, &rnrm,
                          c->s);
-    S.rsnrm = host_scalar(c, S_RSNRM);
-    // while (iters < itmax && relres > errtol): maybe nothing to do
-    if (!(S.iters < p.itmax && S.rsnrm / S.rsden > p.errtol)) S.done = true;
   }
-  if (c->graphs && !c->ktime && !S.done) {
-    int rc = capture(c, mode);
-    if (rc) return rc;
-  }
+  hipMemcpyAsync(c->hscal, c->scal, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                 c->s);
+  hipEventRecord(c->cyc_ev, c->s);
+  S.pending = true;
   return PMG_OK;
 }
 
 void lin_enqueue(pmg_ctx *c) {
   auto &S = c->ses;
-  if (S.done || S.pending) return;
+  if (S.done || S.pending || S.phase != 1) return;
   if (c->gexec[S.mode] && !c->ktime)
     hipGraphLaunch(c->gexec[S.mode], c->s);
   else
@@ -756,6 +656,20 @@ int lin_poll(pmg_ctx *c, bool wait) {
     return 0;
   }
   S.pending = false;
+  if (S.phase == 0) {  // the start has landed
+    S.phase = 1;
+    if (p.istop == 1) S.rsden = c->hscal[S_RSDEN];
+    if (S.rsden == 0.0) S.rsden = 1.0;  // mgcsd.cc:154-157
+    S.rsnrm = S.mode == PMG_MIXED ? c->hscal[S_RSNRM] : S.rsden;
+    // mixed: while (iters < itmax && relres > errtol) may do nothing
+    if (S.mode == PMG_MIXED &&
+        !(S.iters < p.itmax && S.rsnrm / S.rsden > p.errtol)) {
+      S.done = true;
+      S.rc = PMG_OK;
+      return 1;
+    }
+    return 0;
+  }
   S.iters++;
   S.rsnrm = c->hscal[S_RSNRM];
   const bool more = (c->nlev > 1 || S.mode == PMG_MIXED) && S.iters < p.itmax &&
@@ -769,13 +683,29 @@ int lin_poll(pmg_ctx *c, bool wait) {
   return S.done ? 1 : 0;
 }
 
-// VfboundPMG(ibound=1): Dirichlet values on u (mgdrvd.cc:500-502); stats.
-int lin_end(pmg_ctx *c) {
-  auto &S = c->ses;
+// VfboundPMG(ibound=1): Dirichlet values on u (mgdrvd.cc:500-502), then the
+// optional copy of the solution to pinned host memory.
+void lin_finish(pmg_ctx *c, double *host_u) {
   Level &F = c->L[0];
   fbound_dirichlet(F.d, c->u, c->gx, c->gy, c->gz, c->s);
   hipEventRecord(c->t1_ev, c->s);
-  hipError_t se = hipStreamSynchronize(c->s);
+  if (host_u)
+    hipMemcpyAsync(host_u, c->u, F.d.n() * sizeof(double),
+                   hipMemcpyDeviceToHost, c->s);
+  hipEventRecord(c->fin_ev, c->s);
+  c->ses.phase = 2;
+}
+
+int lin_finished(pmg_ctx *c, bool wait) {
+  if (wait) return hipEventSynchronize(c->fin_ev) == hipSuccess ? 1 : -1;
+  const hipError_t e = hipEventQuery(c->fin_ev);
+  if (e == hipSuccess) return 1;
+  return e == hipErrorNotReady ? 0 : -1;
+}
+
+int lin_end(pmg_ctx *c) {
+  auto &S = c->ses;
+  hipError_t se = hipEventSynchronize(c->fin_ev);
   float ms = 0.f;
   hipEventElapsedTime(&ms, c->t0_ev, c->t1_ev);
   c->solve_ms = ms;
@@ -786,7 +716,7 @@ int lin_end(pmg_ctx *c) {
       c->ktime_ms += kms;
     }
   }
-  if (se != hipSuccess) return PMG_EDEVICE;
+  if (se != hipSuccess || hipGetLastError() != hipSuccess) return PMG_EDEVICE;
   c->iters = S.iters;
   c->relres = S.rsnrm / S.rsden;
   c->ierror = S.rc == PMG_ITMAX ? 1 : 0;
@@ -796,11 +726,31 @@ int lin_end(pmg_ctx *c) {
 int ctx_solve(pmg_ctx *c, int mode) {
   int rc = lin_begin(c, mode);
   if (rc) return rc;
-  while (!c->ses.done) {
-    lin_enqueue(c);
-    lin_poll(c, true);
-  }
+  while (!lin_poll(c, true)) lin_enqueue(c);
+  lin_finish(c, nullptr);
   return lin_end(c);
+}
+
+// HBM bytes of a context for p, as allocated by ctx_new (+ alloc_fp32 for
+// PMG_MIXED): per level the operator planes, cc, fc, P and four work
+// vectors; the resident fine inputs; padding of every allocation.
+size_t ctx_bytes_estimate(const Params &p, int mode) {
+  size_t b64 = 0, b32 = 0, nalloc = 0;
+  Dim d{p.nx, p.ny, p.nz};
+  for (int l = 0; l < p.nlev; l++) {
+    const size_t n = d.n();
+    b64 += ((l == 0 ? 4 : 14) + 2 + (l ? 27 : 0) + 4) * n;
+    b32 += ((l == 0 ? 4 : 14) + 1 + (l ? 27 : 0) + 4) * n;
+    nalloc += 16;
+    d = Dim{corsr(d.nx), corsr(d.ny), corsr(d.nz)};
+  }
+  const size_t nf = (size_t)p.nx * p.ny * p.nz;
+  b64 += nf + 2 * max_partials(Dim{p.nx, p.ny, p.nz}) + 16;  // u, partials
+  b64 += (size_t)p.nx + p.ny + p.nz + 2 * ((size_t)p.ny * p.nz +
+         (size_t)p.nx * p.nz + (size_t)p.nx * p.ny) + 5 * nf;  // inputs
+  nalloc += 16;
+  size_t bytes = 8 * b64 + (mode == PMG_MIXED ? 4 * b32 : 0) + 256 * nalloc;
+  return bytes + 6 * sizeof(double) * 1024 + (1 << 20);  // atoms, slack
 }
 
 // ---- Newton (Vnewton, newtond.cc:140-486) ----
@@ -1088,7 +1038,10 @@ int finish(Scratch &S, double *host, const double *dev, size_t n) {
 bool dims_ok(int nx, int ny, int nz) { return nx >= 3 && ny >= 3 && nz >= 3; }
 bool coarse_of(int f, int c) { return f == 2 * c - 1; }
 
-}  // namespace
+}  // namespace drv
+}  // namespace pmg
+
+using namespace pmg::drv;
 
 // ===========================================================================
 // C ABI

@@ -59,3 +59,10 @@ def test_no_gpu_is_reported(mg, orc):
                                                         "gzcf", "a1cf", "a2cf", "a3cf",
                                                         "ccf", "fcf", "tcf")))
     assert e.value.code == mg.PMG_EDEVICE
+
+
+@pytest.mark.skipif(have_gpu(), reason="checks the no-GPU error path")
+def test_server_without_gpu_is_reported(mg):
+    with pytest.raises(mg.PMGError) as e:
+        mg.Server(0, 2)
+    assert e.value.code == mg.PMG_EDEVICE
