@@ -32,6 +32,10 @@ def parse():
     ap.add_argument("--config", default="c1")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--serve-requests", type=int, default=2000,
+                    help="requests of the serving stream (0: skip the serving leg)")
+    ap.add_argument("--serve-active", type=int, default=8,
+                    help="solves in flight at once in the serving loop")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (a reference "
                          "kernel of known traffic for hardware-counter runs)")
@@ -90,6 +94,81 @@ def cpu_baseline(cfg, atoms, threads):
     return {"value": 1.0 / dt, "unit": "solves/s", "cores": nthreads, "kind": "port",
             "sample": f"1 full {n}^3 solve (setup + V-cycles to 1e-6), {dt:.2f} s; "
                       f"serial input assembly {ta:.2f} s not included"}, u
+
+
+def serve_requests(mg, count, seed=20261206):
+    """The serving workload: grid sizes uniform over {65, 97, 129, 161}^3
+    (h = 0.5 A), atom counts U[100, 3000] in a ball whose radius keeps the
+    atom density of the 65^3 workload (14 A for 200 atoms), capped to fit
+    the box; charges +-1, 0.15 M salt; V(2,2) cycles to 1e-6."""
+    rng = np.random.default_rng(seed)
+    sizes = [65, 97, 129, 161]
+    out = []
+    for q in range(count):
+        n = sizes[int(rng.integers(0, 4))]
+        na = int(rng.integers(100, 3001))
+        half = (n - 1) * 0.5 / 2
+        R = min(14.0 * (na / 200.0) ** (1.0 / 3.0), 0.75 * half - 2.0)
+        atoms = mg.pbe_molecule(na, R, 0, seed * 7919 + q)
+        out.append((n, mg.SERVE_LEVELS[n], atoms))
+    return out
+
+
+def serve_stream(mg, reqs, mode, rate, active, budget):
+    """Submit the requests with Poisson arrivals at `rate` per second (all
+    at once when rate is None) and wait for all; returns the server stats."""
+    srv = mg.Server(budget, active)
+    params = {n: mg.make_params(n, nlev) for n, nlev in set((r[0], r[1]) for r in reqs)}
+    rng = np.random.default_rng(7)
+    tickets = []
+    t0 = time.perf_counter()
+    t_next = t0
+    for (n, nlev, atoms) in reqs:
+        if rate:
+            t_next += rng.exponential(1.0 / rate)
+            d = t_next - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+        iparm, rparm = params[n]
+        tickets.append(srv.submit(iparm, rparm, mode, 0.5, 0.15, atoms, want_u=False))
+    iters = [srv.wait(t)["iters"] for t in tickets]
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    st["wall_s"] = wall
+    st["mean_iters"] = float(np.mean(iters))
+    print(f"serving: {len(reqs)} requests mode={mode} rate={rate} -> "
+          f"{st['solves_per_s']:.1f} solves/s p50={st['p50_ms']:.2f} ms "
+          f"p99={st['p99_ms']:.2f} ms", file=sys.stderr, flush=True)
+    return st
+
+
+def serving_leg(mg, count, active):
+    reqs = serve_requests(mg, count)
+    budget = 200 * (1 << 30)
+    # warm the context pool (allocation and graph capture of each shape)
+    for n in (65, 97, 129, 161):
+        first = [r for r in reqs if r[0] == n][: active]
+        serve_stream(mg, first, mg.PMG_FP64, None, active, budget)
+        serve_stream(mg, first, mg.PMG_MIXED, None, active, budget)
+    # FP64 capacity: the whole stream submitted at once
+    cap = serve_stream(mg, reqs, mg.PMG_FP64, None, active, budget)
+    rate = 0.8 * cap["solves_per_s"]
+    mixed = serve_stream(mg, reqs, mg.PMG_MIXED, rate, active, budget)
+    fp64 = serve_stream(mg, reqs, mg.PMG_FP64, rate, active, budget)
+    cap_mx = serve_stream(mg, reqs, mg.PMG_MIXED, None, active, budget)
+    keep = ("solves_per_s", "p50_ms", "p99_ms", "max_concurrent", "mean_iters", "wall_s")
+    return {
+        "workload": f"{count} linearised PB requests, n uniform over {{65,97,129,161}}^3, "
+                    "h=0.5, 100-3000 atoms, stop 1e-6, Poisson arrivals",
+        "arrival_rate_per_s": rate,
+        "max_active": active,
+        "budget_bytes": budget,
+        "results_on_device": True,
+        "mixed": {k: mixed[k] for k in keep},
+        "fp64": {k: fp64[k] for k in keep},
+        "capacity_solves_per_s": {"fp64": cap["solves_per_s"], "mixed": cap_mx["solves_per_s"]},
+    }
 
 
 def main():
@@ -153,6 +232,9 @@ def main():
         err_oracle = {"fp64_potential_rel_err_vs_oracle": float(np.max(np.abs(u64 - u_or))) / den,
                       "mixed_potential_rel_err_vs_oracle": float(np.max(np.abs(umx - u_or))) / den}
 
+    serving = serving_leg(mg, args.serve_requests, args.serve_active) \
+        if args.serve_requests > 0 else None
+
     value = args.steps / wallmx
     out = {
         "metric": "PB solves/sec + p99 latency at 1e-6 rel residual: mixed-precision vs FP64 GPU",
@@ -184,6 +266,7 @@ def main():
                      "kernel": "k_gs7_sweep<float> (fine fused red-black sweep)",
                      "avg_launch_us": avg_launch_s * 1e6, "timed_launches": launches},
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
 
