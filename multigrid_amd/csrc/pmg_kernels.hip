@@ -1617,10 +1617,12 @@ __global__ __launch_bounds__(128) void k_buildP(int nxf, int nyf, int nzf,
                                                 int nxc, int nyc, int nzc,
                                                 const double *__restrict__ ac,
                                                 double *__restrict__ pc,
-                                                int mgprol) {
-  const int ii = 2 + blockIdx.x * 64 + threadIdx.x;
-  const int jj = 2 + blockIdx.y * 2 + threadIdx.y;
-  const int kk = 2 + blockIdx.z;
+                                                int mgprol, XG xg) {
+  int bx, by, bz;
+  if (!xcd_tile(xg, bx, by, bz)) return;
+  const int ii = 2 + bx * 64 + threadIdx.x;
+  const int jj = 2 + by * 2 + threadIdx.y;
+  const int kk = 2 + bz;
   if (ii > nxc - 1 || jj > nyc - 1) return;
   const size_t nf = (size_t)nxf * nyf * nzf, nc = (size_t)nxc * nyc * nzc;
   const size_t pC = ix(nxc, nyc, ii, jj, kk);
@@ -1689,11 +1691,11 @@ void buildP(const Dim &f, const Dim &c, int mgprol, int numdia,
   if (c.nx < 3 || c.ny < 3 || c.nz < 3) return;
   dim3 g(cdiv(c.nx - 2, 64), cdiv(c.ny - 2, 2), c.nz - 2);
   if (numdia == 7)
-    k_buildP<7><<<g, dim3(64, 2), 0, s>>>(f.nx, f.ny, f.nz, c.nx, c.ny, c.nz,
-                                          acf, pc, mgprol);
+    k_buildP<7><<<xcd_blocks(g), dim3(64, 2), 0, s>>>(f.nx, f.ny, f.nz, c.nx, c.ny, c.nz,
+                                          acf, pc, mgprol, xg_of(g));
   else
-    k_buildP<27><<<g, dim3(64, 2), 0, s>>>(f.nx, f.ny, f.nz, c.nx, c.ny, c.nz,
-                                           acf, pc, mgprol);
+    k_buildP<27><<<xcd_blocks(g), dim3(64, 2), 0, s>>>(f.nx, f.ny, f.nz, c.nx, c.ny, c.nz,
+                                           acf, pc, mgprol, xg_of(g));
   check_launch("buildP");
 }
 

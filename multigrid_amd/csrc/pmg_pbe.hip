@@ -28,11 +28,12 @@
 //               the charge of a node is gathered from the lists of its (up to)
 //               8 cells, taken in increasing atom index, so the sum has the
 //               reference's serial order and the result is deterministic;
-//   k_pb_faces  one thread per boundary node sums all atoms, staged through
-//               LDS in chunks.
+//   k_pb_faces  four wavefronts per 64 boundary nodes split the atoms,
+//               staged through LDS in chunks.
 // Floating-point contraction is off in these kernels so that every
 // coordinate, weight and mark is the same double as the serial restatement
-// computes.
+// computes; only the boundary values (exp, reordered sums) differ from it
+// by rounding.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -181,17 +182,25 @@ __global__ __launch_bounds__(256) void k_pb_nodes(
   fcf[p] = acc;
 }
 
-constexpr int FACE_NT = 256;
+constexpr int FACE_NT = 256, FACE_NODES = 64, FACE_SPLIT = FACE_NT / FACE_NODES;
 
+// Debye-Hueckel faces.  A workgroup takes 64 boundary nodes of one face;
+// its four wavefronts each sum a quarter of every staged chunk of atoms
+// (all lanes of a wavefront read the same atom: an LDS broadcast), and the
+// four partial sums are added in a fixed order.  Per term the atom factor
+// lambda_B q exp(kappa r) / (eps_w (1 + kappa r)) is staged once, leaving
+// one exp and one reciprocal square root per (node, atom).  The sum is
+// deterministic; its order differs from the serial one by rounding only.
 __global__ __launch_bounds__(FACE_NT) void k_pb_faces(
     int n, double c0, double h, double lb, double kap, int natoms,
     const double *__restrict__ atoms, double *gx, double *gy, double *gz) {
 #pragma clang fp contract(off)
-  __shared__ double sx[FACE_NT], sy[FACE_NT], sz[FACE_NT], sr[FACE_NT],
-      sc[FACE_NT];
+  __shared__ double sx[FACE_NT], sy[FACE_NT], sz[FACE_NT], sc[FACE_NT];
+  __shared__ double part[FACE_SPLIT][FACE_NODES];
   const int f = blockIdx.y;
+  const int lane = threadIdx.x % FACE_NODES, split = threadIdx.x / FACE_NODES;
   const long long nn = (long long)n * n;
-  const long long idx = (long long)blockIdx.x * FACE_NT + threadIdx.x;
+  const long long idx = (long long)blockIdx.x * FACE_NODES + lane;
   const bool valid = idx < nn;
   const int a = valid ? (int)(idx % n) : 0, b = valid ? (int)(idx / n) : 0;
   const double lo = c0, hi = c0 + (n - 1) * h;
@@ -210,22 +219,27 @@ __global__ __launch_bounds__(FACE_NT) void k_pb_faces(
       sx[threadIdx.x] = A[0];
       sy[threadIdx.x] = A[1];
       sz[threadIdx.x] = A[2];
-      sr[threadIdx.x] = A[3];
-      // lambda_B q / (eps_w (1 + kappa r)): the atom's factor of the
-      // Debye-Hueckel term, the same double for every boundary node
-      sc[threadIdx.x] = lb * A[4] / (EPS_OUT * (1.0 + kap * A[3]));
+      sc[threadIdx.x] = lb * A[4] / (EPS_OUT * (1.0 + kap * A[3])) *
+                        exp(kap * A[3]);
     }
     __syncthreads();
-    if (valid)
-      for (int t = 0; t < m; t++) {
-        const double dx = X - sx[t], dy = Y - sy[t], dz = Z - sz[t];
-        const double d = sqrt(dx * dx + dy * dy + dz * dz);
-        s += sc[t] * exp(-kap * (d - sr[t])) / d;
-      }
+    const int per = (m + FACE_SPLIT - 1) / FACE_SPLIT;
+    const int t0 = split * per, t1 = min(m, t0 + per);
+    for (int t = t0; t < t1; t++) {
+      const double dx = X - sx[t], dy = Y - sy[t], dz = Z - sz[t];
+      const double d2 = dx * dx + dy * dy + dz * dz;
+      const double ri = rsqrt(d2);
+      s += sc[t] * exp(-kap * (d2 * ri)) * ri;
+    }
   }
-  if (!valid) return;
+  part[split][lane] = s;
+  __syncthreads();
+  if (split != 0 || !valid) return;
+  double tot = part[0][lane];
+#pragma unroll
+  for (int q = 1; q < FACE_SPLIT; q++) tot += part[q][lane];
   double *dst = (f >> 1) == 0 ? gx : ((f >> 1) == 1 ? gy : gz);
-  dst[(size_t)(f & 1) * nn + (size_t)b * n + a] = s;
+  dst[(size_t)(f & 1) * nn + (size_t)b * n + a] = tot;
 }
 
 void check(const char *what) {
@@ -280,7 +294,7 @@ int pbe_assemble(int n, double h, double ionic, int natoms,
                                         next, inside, excl, xf, yf, zf, a1,
                                         a2, a3, ccf, fcf);
   check("k_pb_nodes");
-  dim3 gf(cdivp((long long)n * n, FACE_NT), 6);
+  dim3 gf(cdivp((long long)n * n, FACE_NODES), 6);
   k_pb_faces<<<gf, FACE_NT, 0, s>>>(n, c0, h, lb, kap, natoms, atoms, gx, gy,
                                     gz);
   check("k_pb_faces");
