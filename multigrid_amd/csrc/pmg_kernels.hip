@@ -1014,8 +1014,14 @@ __device__ inline double cg_allsum(double v, double *buf, int nw) {
   v = wave_sum(v);
   if ((threadIdx.x & 63) == 0) buf[threadIdx.x >> 6] = v;
   __syncthreads();
+  // all per-wavefront values are requested before any is added (a loop of
+  // dependent LDS reads would pay the LDS latency once per wavefront)
+  double p[CG_NT / 64];
+#pragma unroll
+  for (int w = 0; w < CG_NT / 64; w++) p[w] = w < nw ? buf[w] : 0.0;
   double t = 0.0;
-  for (int w = 0; w < nw; w++) t += buf[w];
+#pragma unroll
+  for (int w = 0; w < CG_NT / 64; w++) t += p[w];
   return t;
 }
 
@@ -1181,7 +1187,7 @@ __global__ __launch_bounds__(CG_RNT) void k_cg_rows(
     T *__restrict__ xout, int itmax, double errtol) {
   constexpr int R = CG_R;
   extern __shared__ double P[];
-  __shared__ double rbuf[2][CG_RNT / 64];
+  __shared__ double rbuf[2][CG_NT / 64];  // cg_allsum reads CG_NT / 64
   const int tid = threadIdx.x, NTB = blockDim.x, nw = NTB / 64;
   const size_t n = (size_t)nx * ny * nz;
   const long long sy = nx, sz = (long long)nx * ny;
