@@ -36,6 +36,8 @@ def parse():
                     help="requests of the serving stream (0: skip the serving leg)")
     ap.add_argument("--serve-active", type=int, default=8,
                     help="solves in flight at once in the serving loop")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the 289^3 mixed-precision and 129^3 Newton legs")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (a reference "
                          "kernel of known traffic for hardware-counter runs)")
@@ -171,6 +173,85 @@ def serving_leg(mg, count, active):
     }
 
 
+def rel_err(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+def large_leg(mg, cpu):
+    """The 289^3 workload (h = 0.35 A, 6 levels, 5000 atoms, charges over
+    {-1,-0.5,0,0.5,1}): one FP64 and one mixed-precision solve (assembly +
+    build + V-cycles), the FP32 fine sweep's bandwidth, and the error of the
+    mixed potential against the FP64 GPU solve and (with the CPU leg) the
+    FP64 oracle."""
+    cfg = mg.CONFIGS["c2"]
+    n, nlev = cfg["n"], cfg["nlev"]
+    atoms = mg.pbe_molecule(cfg["natoms"], cfg["sphere_r"], cfg["charge_mode"], cfg["seed"])
+    mol = (cfg["h"], cfg["ionic"], atoms)
+    iparm, rparm = mg.make_params(n, nlev)
+    ctx = mg.Context(iparm, rparm, pbe=mol)
+    out = {"workload": f"linearised PBE {n}^3 h={cfg['h']} nlev={nlev} {cfg['natoms']} atoms"}
+    for name, mode in (("fp64", mg.PMG_FP64), ("mixed", mg.PMG_MIXED)):
+        ctx.load_pbe(*mol)
+        ctx.solve(mode)  # warm (graph capture)
+        t0 = time.perf_counter()
+        ctx.load_pbe(*mol)
+        ctx.solve(mode)
+        dt = time.perf_counter() - t0
+        st = ctx.stats()
+        out[name] = {"ms": dt * 1e3, "iters": st["iters"], "relres": st["relres"],
+                     "setup_ms": st["setup_ms"]}
+        out[name + "_u"] = ctx.u()
+    ctx.ktime_enable(4096)
+    ctx.solve(mg.PMG_MIXED)
+    kms, sweeps = ctx.ktime()
+    ctx.close()
+    out["fp32_sweep_gbs"] = 8 * 4 * (n - 2) ** 3 / (kms / 1e3 / max(sweeps, 1)) / 1e9
+    u64, umx = out.pop("fp64_u"), out.pop("mixed_u")
+    out["mixed_rel_err_vs_fp64_gpu"] = rel_err(umx, u64)
+    if cpu:
+        from tests import oracle as orc
+        inp = orc.pbe_assemble(n, cfg["h"], cfg["ionic"], atoms)
+        _, u0, _, _ = orc.solve(inp, n, nlev)
+        out["fp64_rel_err_vs_oracle"] = rel_err(u64, u0)
+        out["mixed_rel_err_vs_oracle"] = rel_err(umx, u0)
+    return out
+
+
+def newton_leg(mg, cpu):
+    """The 129^3 nonlinear PBE (sinh ionic term, 0.3 M 1:1 salt, 800 atoms):
+    Newton with FP64 and with mixed-precision inner V-cycles to 1e-8, and
+    (with the CPU leg) the error against the FP64 oracle's Newton."""
+    cfg = mg.CONFIGS["c3"]
+    n, nlev = cfg["n"], cfg["nlev"]
+    atoms = mg.pbe_molecule(cfg["natoms"], cfg["sphere_r"], cfg["charge_mode"], cfg["seed"])
+    ions = ([1.0, -1.0], [0.5, 0.5])
+    mg.Vmypdefinitnpbe(2, *ions)
+    iparm, rparm = mg.make_params(n, nlev, nonlin=1, errtol=1e-8)
+    ctx = mg.Context(iparm, rparm, pbe=(cfg["h"], cfg["ionic"], atoms))
+    out = {"workload": f"nonlinear PBE {n}^3 h={cfg['h']} nlev={nlev} {cfg['natoms']} atoms, "
+                       f"{cfg['ionic']} M, Newton to 1e-8"}
+    us = {}
+    for name, mode in (("fp64", mg.PMG_FP64), ("mixed", mg.PMG_MIXED)):
+        ctx.solve(mode)  # warm
+        t0 = time.perf_counter()
+        ctx.solve(mode)
+        dt = time.perf_counter() - t0
+        st = ctx.stats()
+        out[name] = {"ms": dt * 1e3, "newton_iters": st["iters"], "relres": st["relres"]}
+        us[name] = ctx.u()
+    ctx.close()
+    out["mixed_rel_err_vs_fp64_gpu"] = rel_err(us["mixed"], us["fp64"])
+    if cpu:
+        from tests import oracle as orc
+        orc.mypdefinit(*ions)
+        inp = orc.pbe_assemble(n, cfg["h"], cfg["ionic"], atoms)
+        _, u0, _, _ = orc.solve(inp, n, nlev, nonlin=True, errtol=1e-8)
+        out["oracle_newton_iters"] = orc.last_stats()[0]
+        out["fp64_rel_err_vs_oracle"] = rel_err(us["fp64"], u0)
+        out["mixed_rel_err_vs_oracle"] = rel_err(us["mixed"], u0)
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -234,6 +315,12 @@ def main():
 
     serving = serving_leg(mg, args.serve_requests, args.serve_active) \
         if args.serve_requests > 0 else None
+    extra = {}
+    if not args.no_extra:
+        extra["large_289"] = large_leg(mg, not args.no_cpu)
+        print(f"large: {extra['large_289']}", file=sys.stderr, flush=True)
+        extra["newton_129"] = newton_leg(mg, not args.no_cpu)
+        print(f"newton: {extra['newton_129']}", file=sys.stderr, flush=True)
 
     value = args.steps / wallmx
     out = {
@@ -267,6 +354,7 @@ def main():
                      "avg_launch_us": avg_launch_s * 1e6, "timed_launches": launches},
         "cpu_baseline": cpu,
         "serving": serving,
+        **extra,
     }
     print(json.dumps(out), flush=True)
 

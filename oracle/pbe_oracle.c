@@ -89,6 +89,7 @@ void pbo_assemble(int n, double h, double ionic, int natoms,
           if (d2 <= r * r) inside[p] = 1;
         }
   }
+#pragma omp parallel for schedule(static)
   for (int k = 0; k < n; k++)
     for (int j = 0; j < n; j++)
       for (int i = 0; i < n; i++) {
@@ -139,6 +140,9 @@ void pbo_assemble(int n, double h, double ionic, int natoms,
 
   /* faces: gxcf(j,k,1|2), gycf(i,k,1|2), gzcf(i,j,1|2) */
   const double lo = c0, hi = c0 + (n - 1) * h;
+  /* every node's sum runs over the atoms in list order, whatever thread
+   * takes the node */
+#pragma omp parallel for collapse(2) schedule(static)
   for (int f = 0; f < 6; f++)
     for (int b = 0; b < n; b++)
       for (int a = 0; a < n; a++) {
