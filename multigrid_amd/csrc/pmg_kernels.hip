@@ -327,6 +327,9 @@ __global__ __launch_bounds__(SW_TX *SW_TY) void k_gs7_sweep(
   T mprev = first(kb - 1) ? fval(kb - 1, Pm1, xm2, P0.x, uCm2) : Pm1.x;
   T mcur = first(kb) ? fval(kb, P0, Pm1.x, P1.x, Pm1.uC) : P0.x;
   SM[kb % 3][ty][tx] = mcur;
+  // the first step rewrites xin slot (kb+1)&1 == (kb-1)&1, which the
+  // plane kb-1 values above read
+  __syncthreads();
   for (int k = kb; k <= ke; k++) {
     // plane ke+3 is never used: the last step re-reads plane ke+2 (an L2
     // hit) so that the load stays unconditional
@@ -429,8 +432,12 @@ void gs7(const Dim &d, const T *ac, const T *cc, const T *fc, T *x,
 // results.  iadjoint reverses the class order and the in-row order.
 // zmask: row classes (bit 2*(k%2) + (j%2)) whose values are still zero in
 // this sweep (zero start, Vazeros before the smoothing): read as 0.
-constexpr int ROWS_MAXN = 129;  // 64 lanes of point pairs
-template <class T, int ADJ>
+// Rows up to 128 * PPL + 1 points: lane l owns the point pairs
+// m = l + 64 q (q < PPL), i0 = 2 + 2m and i0 + 1.  Pair m's west neighbour
+// is pair m-1's second point (the previous lane, or lane 63 of the previous
+// segment for lane 0); its east neighbour is pair m+1's first point.
+constexpr int ROWS_MAXN = 128 * 4 + 1;
+template <class T, int ADJ, int PPL>
 __global__ __launch_bounds__(256) void k_gs27_rows(
     int nx, int ny, int nz, const T *__restrict__ ac, const T *__restrict__ cc,
     const T *__restrict__ fc, T *__restrict__ x, int pz, int py, int nj,
@@ -455,13 +462,22 @@ __global__ __launch_bounds__(256) void k_gs27_rows(
   };
   const bool zN = zr(1, 0), zU = zr(0, 1), zNU = zr(1, 1);  // classes
   const bool zown = (zmask >> (2 * pz + py)) & 1;
-  const int i0 = 2 + 2 * lane, i1 = i0 + 1;
   const size_t rb = ix(nx, ny, 1, j, k);  // row base (i = 1)
-  // current values of the own row, boundary included (i <= nx)
-  const T a0 = (i0 <= nx) ? (zown && i0 <= nx - 1 ? (T)0 : x[rb + i0 - 1])
-                          : (T)0;
-  const T a1 = (i1 <= nx) ? (zown && i1 <= nx - 1 ? (T)0 : x[rb + i1 - 1])
-                          : (T)0;
+  int i0[PPL], i1[PPL];
+  bool v0[PPL], v1[PPL];
+  T a0[PPL], a1[PPL], n0[PPL], n1[PPL];
+#pragma unroll
+  for (int q = 0; q < PPL; q++) {
+    i0[q] = 2 + 2 * (lane + 64 * q);
+    i1[q] = i0[q] + 1;
+    v0[q] = i0[q] <= nx - 1;
+    v1[q] = i1[q] <= nx - 1;
+    // current values of the own row, boundary included (i <= nx)
+    a0[q] = (i0[q] <= nx) ? (zown && v0[q] ? (T)0 : x[rb + i0[q] - 1]) : (T)0;
+    a1[q] = (i1[q] <= nx) ? (zown && v1[q] ? (T)0 : x[rb + i1[q] - 1]) : (T)0;
+    n0[q] = a0[q];
+    n1[q] = a1[q];
+  }
   const T xb1 = x[rb];  // i = 1 boundary
   auto X = [&](size_t q, bool z) -> T { return z ? (T)0 : x[q]; };
   // update of point i with in-row neighbours xw (i-1) and xe (i+1)
@@ -489,25 +505,48 @@ __global__ __launch_bounds__(256) void k_gs27_rows(
                  uNE[p - 1 - sy - sz] * X(p - 1 - sy - sz, zNU);
     return (fc[p] + ((o + u) + dd)) / (ac[p] + cc[p]);
   };
-  const bool v0 = i0 <= nx - 1, v1 = i1 <= nx - 1;
-  T n0 = a0, n1 = a1;
+  // value of the previous pair's second point / next pair's first point
+  // (every lane takes part in the shuffles)
+  auto west = [&](const T *s, int q) -> T {
+    T w = __shfl_up(s[q], 1, 64);
+    const T last = q > 0 ? __shfl(s[q > 0 ? q - 1 : 0], 63, 64) : xb1;
+    return lane == 0 ? last : w;
+  };
+  auto east = [&](const T *s, int q) -> T {
+    T e = __shfl_down(s[q], 1, 64);
+    const T first = q + 1 < PPL ? __shfl(s[q + 1 < PPL ? q + 1 : q], 0, 64)
+                                : e;  // beyond the last interior point
+    return lane == 63 ? first : e;
+  };
   if (ADJ == 0) {
-    // colour i%2 = 0 (points i0): neighbours i0-1 (lane-1's a1), i0+1 (a1)
-    T w = __shfl_up(a1, 1, 64);
-    if (lane == 0) w = xb1;
-    if (v0) n0 = upd(i0, w, a1);
-    // colour i%2 = 1 (points i1): neighbours i1-1 = n0, i1+1 = lane+1's n0
-    const T e = __shfl_down(n0, 1, 64);
-    if (v1) n1 = upd(i1, n0, e);
+    // colour i%2 = 0 (points i0) from the old odd values, then i%2 = 1
+#pragma unroll
+    for (int q = 0; q < PPL; q++) {
+      const T w = west(a1, q);
+      if (v0[q]) n0[q] = upd(i0[q], w, a1[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < PPL; q++) {
+      const T e = east(n0, q);
+      if (v1[q]) n1[q] = upd(i1[q], n0[q], e);
+    }
   } else {
-    const T e = __shfl_down(a0, 1, 64);
-    if (v1) n1 = upd(i1, a0, e);
-    T w = __shfl_up(n1, 1, 64);
-    if (lane == 0) w = xb1;
-    if (v0) n0 = upd(i0, w, n1);
+#pragma unroll
+    for (int q = 0; q < PPL; q++) {
+      const T e = east(a0, q);
+      if (v1[q]) n1[q] = upd(i1[q], a0[q], e);
+    }
+#pragma unroll
+    for (int q = 0; q < PPL; q++) {
+      const T w = west(n1, q);
+      if (v0[q]) n0[q] = upd(i0[q], w, n1[q]);
+    }
   }
-  if (v0) x[rb + i0 - 1] = n0;
-  if (v1) x[rb + i1 - 1] = n1;
+#pragma unroll
+  for (int q = 0; q < PPL; q++) {
+    if (v0[q]) x[rb + i0[q] - 1] = n0[q];
+    if (v1[q]) x[rb + i1[q] - 1] = n1[q];
+  }
 }
 
 static int count_parity(int n, int par) {  // #{v in [2, n-1] : v%2 == par}
@@ -529,12 +568,20 @@ void gs27(const Dim &d, const T *ac, const T *cc, const T *fc, T *x,
         const int nrows = nj * nk;
         if (nrows > 0) {
           const int nb = 8 * (((nrows + 3) / 4 + 7) / 8);
-          if (iadjoint)
-            k_gs27_rows<T, 1><<<nb, 256, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, fc,
-                                                 x, pz, py, nj, nrows, zmask);
-          else
-            k_gs27_rows<T, 0><<<nb, 256, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, fc,
-                                                 x, pz, py, nj, nrows, zmask);
+#define ROWS_LAUNCH(A, Q)                                                     \
+  k_gs27_rows<T, A, Q><<<nb, 256, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, fc, x, pz, \
+                                          py, nj, nrows, zmask)
+          const int ppl = d.nx <= 129 ? 1 : (d.nx <= 257 ? 2 : 4);
+          if (iadjoint) {
+            if (ppl == 1) ROWS_LAUNCH(1, 1);
+            else if (ppl == 2) ROWS_LAUNCH(1, 2);
+            else ROWS_LAUNCH(1, 4);
+          } else {
+            if (ppl == 1) ROWS_LAUNCH(0, 1);
+            else if (ppl == 2) ROWS_LAUNCH(0, 2);
+            else ROWS_LAUNCH(0, 4);
+          }
+#undef ROWS_LAUNCH
         }
         zmask &= ~(1 << cls);
       }

@@ -214,3 +214,24 @@ def test_full_size_c1_properties(mg, orc):
     assert s64["ierror"] == 0 and rel <= 1e-6 * 1.0001
     assert abs(rel - s64["relres"]) <= 1e-9
     assert _rel(umx, u64) < 1e-6
+
+
+def test_repeatable_bit_for_bit(mg, orc):
+    """Every kernel is deterministic: repeated solves on one context and on
+    a second context give identical bits (a race in LDS or in a reduction
+    shows up here)."""
+    print("repeatability 65^3 / 97^3", flush=True)
+    for n, nlev in ((65, 4), (97, 5)):
+        atoms = mg.pbe_molecule(150, 10.0, 1, 31 + n)
+        iparm, rparm = mg.make_params(n, nlev)
+        ref = {}
+        for rep in range(2):
+            ctx = mg.Context(iparm, rparm, pbe=(0.5, 0.15, atoms))
+            for _ in range(3):
+                for mode in (mg.PMG_FP64, mg.PMG_MIXED):
+                    ctx.solve(mode)
+                    u = ctx.u()
+                    if mode not in ref:
+                        ref[mode] = u
+                    assert np.array_equal(u, ref[mode]), (n, rep, mode)
+            ctx.close()
