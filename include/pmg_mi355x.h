@@ -174,6 +174,16 @@ int pmg_pbe_assemble(int n, double h, double ionic_strength, int natoms,
                      double *a2cf, double *a3cf, double *ccf, double *fcf,
                      double *tcf);
 
+/* pmg_pbe_assemble with the dielectric constants inside / outside the atom
+ * spheres and the ion-exclusion shell width (A) given (pmg_pbe_assemble:
+ * 2, 78.54, 2). */
+int pmg_pbe_assemble_ex(int n, double h, double ionic_strength, double eps_in,
+                        double eps_out, double ion_shell, int natoms,
+                        const double *atoms, double *xf, double *yf,
+                        double *zf, double *gxcf, double *gycf, double *gzcf,
+                        double *a1cf, double *a2cf, double *a3cf, double *ccf,
+                        double *fcf, double *tcf);
+
 /* pmg_pbe_molecule + pmg_pbe_assemble; atoms (5*natoms) may be NULL. */
 int pmg_pbe_synth(int n, double h, int natoms, double sphere_r,
                   double ionic_strength, int charge_mode, uint64_t seed,
@@ -227,6 +237,19 @@ int pmg_server_stats(pmg_server *srv, long long *completed, double *p50_ms,
                      int *max_concurrent);
 /* Finish every submitted request, then free the server. */
 void pmg_server_destroy(pmg_server *srv);
+
+/* --- OpenDX scalar fields (the reference writes them with
+ * Domain.h:136-185; APBS writes its potentials in the same format).  u is
+ * in the PMG layout (x fastest); files list values x slowest, z fastest.
+ * Host-only. --- */
+int pmg_dx_write(const char *path, int nx, int ny, int nz, double x0,
+                 double y0, double z0, double h, const double *u,
+                 const char *title);
+/* Reads counts[3], origin[3] and the grid spacings delta[3] (any may be
+ * NULL except counts); u (capacity cap values) may be NULL to read only the
+ * header. */
+int pmg_dx_read(const char *path, int *counts, double *origin, double *delta,
+                double *u, size_t cap);
 
 /* Version string of the library build. */
 const char *pmg_version(void);

@@ -86,6 +86,9 @@ def _declare(L):
         "pmg_pbe_synth": (i, [i, d, i, d, d, i, ctypes.c_uint64] + [dp] * 13),
         "pmg_pbe_molecule": (i, [i, d, i, ctypes.c_uint64, dp]),
         "pmg_pbe_assemble": (i, [i, d, d, i, dp] + [dp] * 12),
+        "pmg_pbe_assemble_ex": (i, [i, d, d, d, d, d, i, dp] + [dp] * 12),
+        "pmg_dx_write": (i, [ctypes.c_char_p, i, i, i, d, d, d, d, dp, ctypes.c_char_p]),
+        "pmg_dx_read": (i, [ctypes.c_char_p, ip, dp, dp, dp, ctypes.c_size_t]),
         "pmg_ctx_create_pbe": (i, [ctypes.POINTER(vp), ip, dp, d, d, i, dp]),
         "pmg_ctx_load_pbe": (i, [vp, d, d, i, dp]),
         "pmg_server_create": (i, [ctypes.POINTER(vp), ctypes.c_size_t, i]),
@@ -386,12 +389,12 @@ PBE_KEYS = ("xf", "yf", "zf", "gxcf", "gycf", "gzcf", "a1cf", "a2cf", "a3cf",
             "ccf", "fcf", "tcf")
 
 
-def pbe_assemble(n, h, ionic_strength, atoms):
+def pbe_assemble(n, h, ionic_strength, atoms, eps_in=2.0, eps_out=78.54, ion_shell=2.0):
     """Fine-level Vmgdriv inputs assembled on the GPU from an atom array."""
     at = np.ascontiguousarray(atoms, np.float64).reshape(-1)
     out = _pbe_arrays(n)
-    rc = lib().pmg_pbe_assemble(n, h, ionic_strength, at.size // 5, _dp(at),
-                                *(_dp(out[k]) for k in PBE_KEYS))
+    rc = lib().pmg_pbe_assemble_ex(n, h, ionic_strength, eps_in, eps_out, ion_shell,
+                                   at.size // 5, _dp(at), *(_dp(out[k]) for k in PBE_KEYS))
     _check(rc, "pbe_assemble")
     out["atoms"] = at.reshape(-1, 5)
     return out
@@ -400,6 +403,43 @@ def pbe_assemble(n, h, ionic_strength, atoms):
 def pbe_synth(n, h, natoms, sphere_r, ionic_strength=0.15, charge_mode=0, seed=20261202):
     """Fine-level inputs for a seeded synthetic molecule (GPU assembly)."""
     return pbe_assemble(n, h, ionic_strength, pbe_molecule(natoms, sphere_r, charge_mode, seed))
+
+
+# ---------------------------------------------------------------------------
+# Molecule and field files (PQR in, OpenDX in/out)
+# ---------------------------------------------------------------------------
+def read_pqr(path):
+    """Atoms of a PQR file as (natoms, 5) = x, y, z, radius, charge (the
+    reference reads the same fields, import_molecular_structure.cc)."""
+    rows = []
+    for line in open(path):
+        if line.startswith(("ATOM", "HETATM")):
+            f = line.split()
+            x, y, z, q, r = (float(v) for v in f[-5:])
+            rows.append((x, y, z, r, q))
+    return np.array(rows, np.float64).reshape(-1, 5)
+
+
+def dx_write(path, u, dims, origin, h, title="POTENTIAL (kT/e)"):
+    """Write an n^3 field (PMG layout, x fastest) as OpenDX."""
+    u = np.ascontiguousarray(u, np.float64).reshape(-1)
+    nx, ny, nz = dims
+    assert u.size == nx * ny * nz
+    _check(lib().pmg_dx_write(str(path).encode(), nx, ny, nz, origin[0], origin[1],
+                              origin[2], h, _dp(u), title.encode()), "dx_write")
+
+
+def dx_read(path):
+    """Read an OpenDX regular-grid field: (u in PMG layout, counts, origin,
+    delta)."""
+    c = np.zeros(3, np.int32)
+    o, dl = np.zeros(3), np.zeros(3)
+    _check(lib().pmg_dx_read(str(path).encode(), _ip(c), _dp(o), _dp(dl), None, 0),
+           "dx_read")
+    u = np.zeros(int(c[0]) * int(c[1]) * int(c[2]))
+    _check(lib().pmg_dx_read(str(path).encode(), _ip(c), _dp(o), _dp(dl), _dp(u), u.size),
+           "dx_read")
+    return u, tuple(int(v) for v in c), o, dl
 
 
 # Benchmark configurations (sizes and molecule parameters of the five

@@ -145,12 +145,16 @@ void buildG(const Dim &f, const Dim &c, int numdia, const double *pc,
 // scratch: pbe_scratch_bytes(n).  Writes the fine-level inputs of an n^3
 // grid of spacing h centred on the origin; gx/gy/gz get their two planes
 // (low, high).  Returns 0, or 2 for invalid arguments.
-void pbe_constants(double ionic, double *lambda_b, double *kappa2);
+// eps_in / eps_out: dielectric inside / outside the atom spheres; shell:
+// ion-exclusion shell width (A) beyond each radius.
+void pbe_constants(double ionic, double eps_out, double *lambda_b,
+                   double *kappa2);
 size_t pbe_scratch_bytes(int n);
 int pbe_assemble(int n, double h, double ionic, int natoms,
                  const double *atoms, int *next, void *scratch, double *xf,
                  double *yf, double *zf, double *gx, double *gy, double *gz,
                  double *a1, double *a2, double *a3, double *ccf, double *fcf,
-                 hipStream_t s);
+                 hipStream_t s, double eps_in = 2.0, double eps_out = 78.54,
+                 double shell = 2.0);
 
 }  // namespace pmg

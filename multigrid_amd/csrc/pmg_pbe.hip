@@ -46,9 +46,9 @@
 
 namespace pmg {
 
-namespace {
+constexpr double EPS_IN = 2.0, EPS_OUT = 78.54, ION_SHELL = 2.0;  // defaults
 
-constexpr double EPS_IN = 2.0, EPS_OUT = 78.54, ION_SHELL = 2.0;
+namespace {
 constexpr double PI_D = 3.14159265358979323846;
 
 inline int cdivp(long long a, int b) { return (int)((a + b - 1) / b); }
@@ -69,14 +69,15 @@ __global__ void k_pb_init(size_t N, int *head, uint8_t *inside,
 }
 
 __global__ __launch_bounds__(256) void k_pb_mark(
-    int n, double c0, double h, int natoms, const double *__restrict__ atoms,
-    int *head, int *next, uint8_t *inside, uint8_t *excl) {
+    int n, double c0, double h, double shell, int natoms,
+    const double *__restrict__ atoms, int *head, int *next, uint8_t *inside,
+    uint8_t *excl) {
 #pragma clang fp contract(off)
   const int a = blockIdx.x;
   if (a >= natoms) return;
   const double *A = atoms + 5 * (size_t)a;
   const double ax = A[0], ay = A[1], az = A[2], r = A[3];
-  const double re = r + ION_SHELL;
+  const double re = r + shell;
   int lo[3], hi[3];
   const double P[3] = {ax, ay, az};
 #pragma unroll
@@ -138,6 +139,7 @@ __device__ inline double corner_charge(const double *__restrict__ atoms, int a,
 
 __global__ __launch_bounds__(256) void k_pb_nodes(
     int n, double c0, double h, double q4, double h3, double kap2e,
+    double eps_in, double eps_out,
     const double *__restrict__ atoms, const int *__restrict__ head,
     const int *__restrict__ next, const uint8_t *__restrict__ inside,
     const uint8_t *__restrict__ excl, double *xf, double *yf, double *zf,
@@ -150,12 +152,12 @@ __global__ __launch_bounds__(256) void k_pb_nodes(
   const size_t nn = (size_t)n * n;
   const size_t p = (size_t)k * nn + (size_t)j * n + i;
   if (j == 0 && k == 0) xf[i] = yf[i] = zf[i] = c0 + i * h;
-  const double e0 = inside[p] ? EPS_IN : EPS_OUT;
-  a1[p] = i + 1 < n ? harmonic_average(e0, inside[p + 1] ? EPS_IN : EPS_OUT)
+  const double e0 = inside[p] ? eps_in : eps_out;
+  a1[p] = i + 1 < n ? harmonic_average(e0, inside[p + 1] ? eps_in : eps_out)
                     : e0;
-  a2[p] = j + 1 < n ? harmonic_average(e0, inside[p + n] ? EPS_IN : EPS_OUT)
+  a2[p] = j + 1 < n ? harmonic_average(e0, inside[p + n] ? eps_in : eps_out)
                     : e0;
-  a3[p] = k + 1 < n ? harmonic_average(e0, inside[p + nn] ? EPS_IN : EPS_OUT)
+  a3[p] = k + 1 < n ? harmonic_average(e0, inside[p + nn] ? eps_in : eps_out)
                     : e0;
   ccf[p] = excl[p] ? 0.0 : kap2e;
   // charge: atoms of the cells whose corner (di,dj,dk) is this node, in
@@ -192,7 +194,8 @@ constexpr int FACE_NT = 256, FACE_NODES = 64, FACE_SPLIT = FACE_NT / FACE_NODES;
 // one exp and one reciprocal square root per (node, atom).  The sum is
 // deterministic; its order differs from the serial one by rounding only.
 __global__ __launch_bounds__(FACE_NT) void k_pb_faces(
-    int n, double c0, double h, double lb, double kap, int natoms,
+    int n, double c0, double h, double lb, double kap, double eps_out,
+    int natoms,
     const double *__restrict__ atoms, double *gx, double *gy, double *gz) {
 #pragma clang fp contract(off)
   __shared__ double sx[FACE_NT], sy[FACE_NT], sz[FACE_NT], sc[FACE_NT];
@@ -219,7 +222,7 @@ __global__ __launch_bounds__(FACE_NT) void k_pb_faces(
       sx[threadIdx.x] = A[0];
       sy[threadIdx.x] = A[1];
       sz[threadIdx.x] = A[2];
-      sc[threadIdx.x] = lb * A[4] / (EPS_OUT * (1.0 + kap * A[3])) *
+      sc[threadIdx.x] = lb * A[4] / (eps_out * (1.0 + kap * A[3])) *
                         exp(kap * A[3]);
     }
     __syncthreads();
@@ -251,11 +254,11 @@ void check(const char *what) {
 
 }  // namespace
 
-void pbe_constants(double ionic, double *lb, double *kap2) {
+void pbe_constants(double ionic, double eps_out, double *lb, double *kap2) {
   const double kB = 1.380649e-23, qe = 1.602176634e-19,
                eps0 = 8.8541878128e-12, NA = 6.02214076e23, Tk = 298.15;
   *lb = qe * qe / (4.0 * PI_D * eps0 * kB * Tk) * 1e10;
-  *kap2 = 2.0 * ionic * 1000.0 * NA * qe * qe / (eps0 * EPS_OUT * kB * Tk) *
+  *kap2 = 2.0 * ionic * 1000.0 * NA * qe * qe / (eps0 * eps_out * kB * Tk) *
           1e-20;
 }
 
@@ -268,13 +271,14 @@ int pbe_assemble(int n, double h, double ionic, int natoms,
                  const double *atoms, int *next, void *scratch, double *xf,
                  double *yf, double *zf, double *gx, double *gy, double *gz,
                  double *a1, double *a2, double *a3, double *ccf, double *fcf,
-                 hipStream_t s) {
+                 hipStream_t s, double eps_in, double eps_out, double shell) {
   if (n < 3 || !(h > 0.0) || !(ionic >= 0.0) || natoms < 0 ||
+      !(eps_in > 0.0) || !(eps_out > 0.0) || !(shell >= 0.0) ||
       (long long)n * n * n > (1LL << 31) - 1)
     return 2;
   double lb, kap2;
-  pbe_constants(ionic, &lb, &kap2);
-  const double kap = sqrt(kap2), kap2e = EPS_OUT * kap2;
+  pbe_constants(ionic, eps_out, &lb, &kap2);
+  const double kap = sqrt(kap2), kap2e = eps_out * kap2;
   const double c0 = -(n - 1) * h / 2.0, h3 = h * h * h;
   const double q4 = 4.0 * PI_D * lb;
   const size_t N = (size_t)n * n * n;
@@ -285,17 +289,19 @@ int pbe_assemble(int n, double h, double ionic, int natoms,
       N, head, inside, excl);
   check("k_pb_init");
   if (natoms > 0) {
-    k_pb_mark<<<natoms, 256, 0, s>>>(n, c0, h, natoms, atoms, head, next,
-                                     inside, excl);
+    k_pb_mark<<<natoms, 256, 0, s>>>(n, c0, h, shell, natoms, atoms, head,
+                                     next, inside, excl);
     check("k_pb_mark");
   }
   dim3 gn(cdivp(n, 64), cdivp(n, 4), n);
-  k_pb_nodes<<<gn, dim3(64, 4), 0, s>>>(n, c0, h, q4, h3, kap2e, atoms, head,
+  k_pb_nodes<<<gn, dim3(64, 4), 0, s>>>(n, c0, h, q4, h3, kap2e, eps_in,
+                                        eps_out, atoms, head,
                                         next, inside, excl, xf, yf, zf, a1,
                                         a2, a3, ccf, fcf);
   check("k_pb_nodes");
   dim3 gf(cdivp((long long)n * n, FACE_NODES), 6);
-  k_pb_faces<<<gf, FACE_NT, 0, s>>>(n, c0, h, lb, kap, natoms, atoms, gx, gy,
+  k_pb_faces<<<gf, FACE_NT, 0, s>>>(n, c0, h, lb, kap, eps_out, natoms, atoms,
+                                    gx, gy,
                                     gz);
   check("k_pb_faces");
   return 0;
@@ -306,12 +312,11 @@ int pbe_assemble(int n, double h, double ionic, int natoms,
 // ---- C ABI on host arrays ----
 #include "../../include/pmg_mi355x.h"
 
-extern "C" int pmg_pbe_assemble(int n, double h, double ionic_strength,
-                                int natoms, const double *atoms, double *xf,
-                                double *yf, double *zf, double *gxcf,
-                                double *gycf, double *gzcf, double *a1cf,
-                                double *a2cf, double *a3cf, double *ccf,
-                                double *fcf, double *tcf) {
+extern "C" int pmg_pbe_assemble_ex(
+    int n, double h, double ionic_strength, double eps_in, double eps_out,
+    double ion_shell, int natoms, const double *atoms, double *xf, double *yf,
+    double *zf, double *gxcf, double *gycf, double *gzcf, double *a1cf,
+    double *a2cf, double *a3cf, double *ccf, double *fcf, double *tcf) {
   if (n < 3 || !(h > 0.0) || !(ionic_strength >= 0.0) || natoms < 0 ||
       (natoms > 0 && !atoms) || (long long)n * n * n > (1LL << 31) - 1)
     return PMG_EBADPARM;
@@ -341,7 +346,9 @@ extern "C" int pmg_pbe_assemble(int n, double h, double ionic_strength,
       hipMemcpyAsync(AT, atoms, sizeof(double) * 5 * natoms,
                      hipMemcpyHostToDevice, s);
     rc = pmg::pbe_assemble(n, h, ionic_strength, natoms, AT, nxt, scr, X, Y,
-                           Z, GX, GY, GZ, A1, A2, A3, CC, FC, s);
+                           Z, GX, GY, GZ, A1, A2, A3, CC, FC, s, eps_in,
+                           eps_out, ion_shell);
+    if (rc) rc = PMG_EBADPARM;
     auto get = [&](double *dst, const double *src, size_t cnt) {
       if (dst)
         hipMemcpyAsync(dst, src, cnt * sizeof(double), hipMemcpyDeviceToHost,
@@ -359,4 +366,15 @@ extern "C" int pmg_pbe_assemble(int n, double h, double ionic_strength,
   hipFree(scr);
   hipFree(nxt);
   return rc;
+}
+
+extern "C" int pmg_pbe_assemble(int n, double h, double ionic_strength,
+                                int natoms, const double *atoms, double *xf,
+                                double *yf, double *zf, double *gxcf,
+                                double *gycf, double *gzcf, double *a1cf,
+                                double *a2cf, double *a3cf, double *ccf,
+                                double *fcf, double *tcf) {
+  return pmg_pbe_assemble_ex(n, h, ionic_strength, pmg::EPS_IN, pmg::EPS_OUT,
+                             pmg::ION_SHELL, natoms, atoms, xf, yf, zf, gxcf,
+                             gycf, gzcf, a1cf, a2cf, a3cf, ccf, fcf, tcf);
 }

@@ -44,24 +44,30 @@
 #define ION_SHELL 2.0
 #define PI_D 3.14159265358979323846
 
-void pbo_constants(double ionic, double *lb, double *kap2) {
+void pbo_constants_eps(double ionic, double eps_out, double *lb,
+                       double *kap2) {
   *lb = QE_SI * QE_SI / (4.0 * PI_D * EPS0_SI * KB_SI * T_K) * 1e10;
   *kap2 = 2.0 * ionic * 1000.0 * NA_SI * QE_SI * QE_SI /
-          (EPS0_SI * EPS_OUT * KB_SI * T_K) * 1e-20;
+          (EPS0_SI * eps_out * KB_SI * T_K) * 1e-20;
+}
+
+void pbo_constants(double ionic, double *lb, double *kap2) {
+  pbo_constants_eps(ionic, EPS_OUT, lb, kap2);
 }
 
 static double harmonic_average(double x, double y) {
   return 2 * x * y / (x + y);
 }
 
-void pbo_assemble(int n, double h, double ionic, int natoms,
-                  const double *atoms, double *xf, double *yf, double *zf,
-                  double *gxcf, double *gycf, double *gzcf, double *a1cf,
-                  double *a2cf, double *a3cf, double *ccf, double *fcf,
-                  double *tcf) {
+void pbo_assemble_ex(int n, double h, double ionic, double eps_in,
+                     double eps_out, double shell, int natoms,
+                     const double *atoms, double *xf, double *yf, double *zf,
+                     double *gxcf, double *gycf, double *gzcf, double *a1cf,
+                     double *a2cf, double *a3cf, double *ccf, double *fcf,
+                     double *tcf) {
   double lb, kap2;
-  pbo_constants(ionic, &lb, &kap2);
-  const double kap = sqrt(kap2), kap2e = EPS_OUT * kap2;
+  pbo_constants_eps(ionic, eps_out, &lb, &kap2);
+  const double kap = sqrt(kap2), kap2e = eps_out * kap2;
   const double c0 = -(n - 1) * h / 2.0, h3 = h * h * h;
   const size_t nn = (size_t)n * n, N = nn * n;
   for (int i = 0; i < n; i++) xf[i] = yf[i] = zf[i] = c0 + i * h;
@@ -69,7 +75,7 @@ void pbo_assemble(int n, double h, double ionic, int natoms,
   unsigned char *inside = calloc(N, 1), *excl = calloc(N, 1);
   for (int a = 0; a < natoms; a++) {
     const double *A = atoms + 5 * (size_t)a;
-    const double r = A[3], re = r + ION_SHELL;
+    const double r = A[3], re = r + shell;
     int lo[3], hi[3];
     for (int d = 0; d < 3; d++) { /* a superset of the shell's nodes */
       double l = floor((A[d] - re - c0) / h) - 1.0;
@@ -94,15 +100,15 @@ void pbo_assemble(int n, double h, double ionic, int natoms,
     for (int j = 0; j < n; j++)
       for (int i = 0; i < n; i++) {
         size_t p = (size_t)k * nn + (size_t)j * n + i;
-        double e0 = inside[p] ? EPS_IN : EPS_OUT;
+        double e0 = inside[p] ? eps_in : eps_out;
         a1cf[p] = i + 1 < n
-                      ? harmonic_average(e0, inside[p + 1] ? EPS_IN : EPS_OUT)
+                      ? harmonic_average(e0, inside[p + 1] ? eps_in : eps_out)
                       : e0;
         a2cf[p] = j + 1 < n
-                      ? harmonic_average(e0, inside[p + n] ? EPS_IN : EPS_OUT)
+                      ? harmonic_average(e0, inside[p + n] ? eps_in : eps_out)
                       : e0;
         a3cf[p] = k + 1 < n
-                      ? harmonic_average(e0, inside[p + nn] ? EPS_IN : EPS_OUT)
+                      ? harmonic_average(e0, inside[p + nn] ? eps_in : eps_out)
                       : e0;
         ccf[p] = excl[p] ? 0.0 : kap2e;
         fcf[p] = 0.0;
@@ -158,9 +164,18 @@ void pbo_assemble(int n, double h, double ionic, int natoms,
           const double *A = atoms + 5 * (size_t)t;
           double dx = X - A[0], dy = Y - A[1], dz = Z - A[2];
           double d = sqrt(dx * dx + dy * dy + dz * dz);
-          s += lb * A[4] / (EPS_OUT * (1.0 + kap * A[3])) *
+          s += lb * A[4] / (eps_out * (1.0 + kap * A[3])) *
                exp(-kap * (d - A[3])) / d;
         }
         dst[(size_t)(f & 1) * nn + (size_t)b * n + a] = s;
       }
+}
+
+void pbo_assemble(int n, double h, double ionic, int natoms,
+                  const double *atoms, double *xf, double *yf, double *zf,
+                  double *gxcf, double *gycf, double *gzcf, double *a1cf,
+                  double *a2cf, double *a3cf, double *ccf, double *fcf,
+                  double *tcf) {
+  pbo_assemble_ex(n, h, ionic, EPS_IN, EPS_OUT, ION_SHELL, natoms, atoms, xf,
+                  yf, zf, gxcf, gycf, gzcf, a1cf, a2cf, a3cf, ccf, fcf, tcf);
 }

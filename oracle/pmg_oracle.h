@@ -139,6 +139,17 @@ int pmgo_last_stats(int *iters, int *ierror, double *relres, double *hist,
  * the exclusion-shell geometry and the Debye-Hueckel sum evaluated
  * independently in tests/test_pbinput.py. */
 void pbo_constants(double ionic, double *lambda_b, double *kappa2);
+void pbo_constants_eps(double ionic, double eps_out, double *lambda_b,
+                       double *kappa2);
+/* As pbo_assemble with the dielectric constants inside / outside the atoms
+ * and the ion-exclusion shell width as parameters (pbo_assemble: 2, 78.54,
+ * 2 A). */
+void pbo_assemble_ex(int n, double h, double ionic, double eps_in,
+                     double eps_out, double shell, int natoms,
+                     const double *atoms, double *xf, double *yf, double *zf,
+                     double *gxcf, double *gycf, double *gzcf, double *a1cf,
+                     double *a2cf, double *a3cf, double *ccf, double *fcf,
+                     double *tcf);
 void pbo_assemble(int n, double h, double ionic, int natoms,
                   const double *atoms, double *xf, double *yf, double *zf,
                   double *gxcf, double *gycf, double *gzcf, double *a1cf,

@@ -50,6 +50,7 @@ def olib():
             "pmgo_last_stats": (i, [ip, ip, dp, dp, i]),
             "pbo_constants": (None, [d, dp, dp]),
             "pbo_assemble": (None, [i, d, d, i, dp] + [dp] * 12),
+            "pbo_assemble_ex": (None, [i, d, d, d, d, d, i, dp] + [dp] * 12),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -208,7 +209,7 @@ def pbe_constants(ionic):
     return lb.value, k2.value
 
 
-def pbe_assemble(n, h, ionic, atoms):
+def pbe_assemble(n, h, ionic, atoms, eps_in=2.0, eps_out=78.54, ion_shell=2.0):
     """Oracle assembly of the fine-level inputs from an (natoms, 5) array."""
     at = np.ascontiguousarray(atoms, np.float64).reshape(-1)
     out = {k: np.zeros(n) for k in ("xf", "yf", "zf")}
@@ -216,7 +217,8 @@ def pbe_assemble(n, h, ionic, atoms):
         out[k] = np.zeros(4 * n * n)
     for k in ("a1cf", "a2cf", "a3cf", "ccf", "fcf", "tcf"):
         out[k] = np.zeros(n * n * n)
-    olib().pbo_assemble(n, h, ionic, at.size // 5, dp(at), *(dp(out[k]) for k in PBE_KEYS))
+    olib().pbo_assemble_ex(n, h, ionic, eps_in, eps_out, ion_shell, at.size // 5, dp(at),
+                           *(dp(out[k]) for k in PBE_KEYS))
     out["atoms"] = at.reshape(-1, 5)
     return out
 

@@ -89,3 +89,21 @@ def test_ctx_from_atoms_and_reload(mg, orc):
     fresh.close()
     print(f"  iters={s1['iters']} err vs host inputs={_rel(u1, u_ref):.2e}", flush=True)
     assert np.array_equal(u2, u2f)
+
+
+def test_apbs_single_atom(mg, orc):
+    """GPU assembly and solve of the APBS single-atom case: the oracle's
+    result to 1e-10 and APBS's within the physics bars of test_dx_apbs."""
+    from tests.test_dx_apbs import apbs_case, check_against_apbs
+    ua, n, h, a = apbs_case(mg)
+    print(f"apbs single atom n={n}", flush=True)
+    g = mg.pbe_assemble(n, h, 0.0, a, eps_in=4.0, eps_out=78.4, ion_shell=0.0)
+    iparm, rparm = mg.make_params(n, 3)
+    ctx = mg.Context(iparm, rparm, g)
+    ctx.solve(mg.PMG_FP64)
+    u = ctx.u()
+    ctx.close()
+    o = orc.pbe_assemble(n, h, 0.0, a, eps_in=4.0, eps_out=78.4, ion_shell=0.0)
+    _, u0, _, _ = orc.solve(o, n, 3)
+    assert _rel(u, u0) < 1e-10
+    check_against_apbs(u, ua, n)
