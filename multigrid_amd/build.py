@@ -10,7 +10,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpmg_mi355x.so")
-SOURCES = ["pmg_kernels.hip", "pmg_pbe.hip", "pmg_solver.hip", "pmg_serve.hip", "pbinput.cpp",
+SOURCES = ["pmg_kernels.hip", "pmg_pbe.hip", "pmg_band.hip", "pmg_solver.hip", "pmg_serve.hip", "pbinput.cpp",
            "dxio.cpp"]
 
 HIPCC_FLAGS = [

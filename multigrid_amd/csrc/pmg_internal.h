@@ -105,6 +105,28 @@ template <class T>
 void cg_single(const Dim &d, int numdia, const double *ac, const double *cc,
                const T *f, T *x, int itmax, double errtol, hipStream_t s);
 
+// ---- banded coarsest solve, mgsolv = 1 (pmgc/buildBd.cc:62-304,
+// pmgc/mlinpckd.cc:167-235), FP64 ----
+// n unknowns (interior points), half-bandwidth m, lda = m + 1, W = row
+// stride of the row-wise factor (a multiple of 64 above m); ok is false
+// for levels the band path does not take (one interior point in x or y
+// with 27 points, or a factor window m + 1 beyond the LDS of one CU).
+struct BandDims {
+  int n = 0, m = 0, lda = 0, W = 0;
+  bool ok = false;
+};
+BandDims band_dims(const Dim &d, int numdia);
+// Band of the operator ac of level d (abd: n * lda), its factor in place,
+// and the factor by rows and by columns for the substitution (rt, ct:
+// n * W each); info[0] = 0, or the first non-positive pivot.
+void band_build_factor(const Dim &d, int numdia, const double *ac,
+                       double *abd, double *rt, double *ct, double *info,
+                       hipStream_t s);
+// x = A_band^{-1} f on the interior, 0 on the boundary; y: n doubles.
+template <class T>
+void band_solve(const Dim &d, int numdia, const double *rt, const double *ct,
+                double *y, const T *f, T *x, hipStream_t s);
+
 // ---- BLAS-1 helpers ----
 template <class T>
 void fill(T *x, size_t n, T v, hipStream_t s);

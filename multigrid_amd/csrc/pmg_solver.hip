@@ -91,7 +91,8 @@ int decode(const int *iparm, const double *rparm, Params &p) {
   if (p.nlev < 1 || p.nlev > MAXLEV || p.nx < 3 || p.ny < 3 || p.nz < 3)
     return PMG_EBADPARM;
   if (p.nlev > 1 && p.nlev > maxlev(p.nx, p.ny, p.nz)) return PMG_EBADPARM;
-  if (p.mgcoar != 2 || p.mgdisc != 0 || p.mgsmoo != 1 || p.mgsolv != 0 ||
+  if (p.mgcoar != 2 || p.mgdisc != 0 || p.mgsmoo != 1 ||
+      (p.mgsolv != 0 && p.mgsolv != 1) ||
       (p.mgprol != 0 && p.mgprol != 1) || (p.istop != 0 && p.istop != 1) ||
       p.iperf != 0 || p.ipkey != 0 || p.nu1 < 0 || p.nu2 < 0 || p.itmax < 0 ||
       (p.mgkey != 0 && p.mgkey != 1))
@@ -100,7 +101,8 @@ int decode(const int *iparm, const double *rparm, Params &p) {
 }
 
 
-enum { S_RSDEN = 0, S_RSNRM = 1, S_XDAMP = 2, S_TMP = 4 };
+// device scalar slots; S_BINFO: Vdpbfa info of the band factor (mgsolv 1)
+enum { S_RSDEN = 0, S_RSNRM = 1, S_XDAMP = 2, S_BINFO = 3, S_TMP = 4 };
 
 thread_local int t_iters = 0, t_ierror = 0;
 thread_local double t_relres = 0.0;
@@ -163,6 +165,26 @@ void smooth(pmg_ctx *c, const LV<T> &l, const T *f, T *x, int nu, int iadj,
   }
 }
 
+// Coarsest-level solve: Vcghs with 100 iterations and errtol = epsiln = 0
+// (mgsolv 0, D5), or the factored band (mgsolv 1, mgcsd.cc:391-407, which
+// ignores cc as the reference's band does).  FP64 inside either way.
+template <class T>
+void coarse_solve(pmg_ctx *c, int l, const double *ac64, const double *cc64,
+                  const T *f, T *x) {
+  const Level &L = c->L[l];
+  if (c->p.mgsolv == 1)
+    band_solve<T>(L.d, L.numdia, c->band_rt, c->band_ct, c->band_y, f, x,
+                  c->s);
+  else
+    cg_single<T>(L.d, L.numdia, ac64, cc64, f, x, 100, 0.0, c->s);
+}
+
+// The coarsest level fits the coarse solver of p.
+bool coarse_fits(pmg_ctx *c) {
+  const Level &L = c->L[c->nlev - 1];
+  return c->p.mgsolv == 1 ? band_dims(L.d, L.numdia).ok : cg_fits(L.d);
+}
+
 // One V-cycle (Vmvcs body, mgcsd.cc:299-480).  L[0].x is the iterate on
 // the fine level; frhs its right-hand side; cc64c the FP64 Helmholtz term of
 // the coarsest level used by the FP64 coarse solve.
@@ -185,12 +207,7 @@ void vcycle(pmg_ctx *c, LV<T> *L, const T *frhs, const double *ac64c,
                   L[l].r, nullptr, s);
     }
   }
-  // coarsest: Vcghs, 100 iterations, errtol = epsiln = 0 (D5), FP64
-  {
-    const int l = nlev - 1;
-    cg_single<T>(L[l].d, L[l].numdia, ac64c, cc64c, L[l].w0, L[l].x, 100, 0.0,
-                 s);
-  }
+  coarse_solve<T>(c, nlev - 1, ac64c, cc64c, L[nlev - 1].w0, L[nlev - 1].x);
   for (int l = nlev - 2; l >= 0; l--) {  // up (mgcsd.cc:418-473)
     const int cl = l + 1;
     const unsigned nb =
@@ -219,8 +236,7 @@ int linear_fp64(pmg_ctx *c, LV<double> *L, const double *frhs,
   double rsnrm = rsden;
   const int nlev = c->nlev;
   if (nlev == 1) {  // direct coarse solve on the only level (mgcsd.cc:170-291)
-    cg_single<double>(F.d, F.numdia, F.ac, L[0].cc, frhs, L[0].x, 100, 0.0,
-                      c->s);
+    coarse_solve<double>(c, 0, F.ac, L[0].cc, frhs, L[0].x);
     resid<double, double>(F.d, F.numdia, F.ac, L[0].cc, frhs, L[0].x,
                           (double *)nullptr, &rnrm, c->s);
     rsnrm = host_scalar(c, S_RSNRM);
@@ -266,8 +282,7 @@ int linear_mixed(pmg_ctx *c, LV<double> *L64, LV<float> *L32,
   while (iters < itmax && rsnrm / rsden > errtol) {
     fill<float>(L32[0].x, F.d.n(), 0.0f, c->s);
     if (nlev == 1) {
-      cg_single<float>(F.d, F.numdia, F.ac, L64[0].cc, r32, L32[0].x, 100,
-                       0.0, c->s);
+      coarse_solve<float>(c, 0, F.ac, L64[0].cc, r32, L32[0].x);
     } else {
       vcycle<float>(c, L32, r32, c->L[nlev - 1].ac, cc64c);
     }
@@ -300,6 +315,15 @@ int alloc_hierarchy(pmg_ctx *c) {
     L.r = c->alloc<double>(n);
     L.t = c->alloc<double>(n);
     d = Dim{corsr(d.nx), corsr(d.ny), corsr(d.nz)};
+  }
+  if (p.mgsolv == 1) {  // Vbuildband storage (mgsubd.cc:249-262)
+    const Level &C = c->L[c->nlev - 1];
+    const BandDims b = band_dims(C.d, C.numdia);
+    if (!b.ok) return PMG_EBADPARM;
+    c->band = c->alloc<double>((size_t)b.n * b.lda);
+    c->band_rt = c->alloc<double>((size_t)b.n * b.W);
+    c->band_ct = c->alloc<double>((size_t)b.n * b.W);
+    c->band_y = c->alloc<double>((size_t)b.n);
   }
   size_t nf = c->L[0].d.n();
   c->u = c->alloc<double>(nf);
@@ -350,6 +374,11 @@ void build_ops(pmg_ctx *c) {
     buildG(Lf.d, Lc.d, Lf.numdia, Lc.pc, Lf.ac, Lc.ac, s);
     restrict_<double>(Lf.d, Lc.d, Lf.cc, Lc.cc, Lc.pc, s);
     restrict_<double>(Lf.d, Lc.d, Lf.fc, Lc.fc, Lc.pc, s);
+  }
+  if (c->p.mgsolv == 1) {  // Vbuildband + Vdpbfa (mgsubd.cc:249-262)
+    const Level &C = c->L[c->nlev - 1];
+    band_build_factor(C.d, C.numdia, C.ac, c->band, c->band_rt, c->band_ct,
+                      c->scal + S_BINFO, s);
   }
   c->ops32_ok = false;
   if (c->have32) {
@@ -424,6 +453,8 @@ int setup_wait(pmg_ctx *c) {
   float ms = 0.f;
   hipEventElapsedTime(&ms, c->su0_ev, c->su1_ev);
   c->setup_ms = ms;
+  // the reference stops when Vdpbfa finds a non-positive pivot
+  if (c->p.mgsolv == 1 && host_scalar(c, S_BINFO) != 0.0) return PMG_ENOTSPD;
   return PMG_OK;
 }
 
@@ -543,8 +574,7 @@ void step_body(pmg_ctx *c, int mode) {
     float *r32 = F.w032;
     fill<float>(L32[0].x, F.d.n(), 0.0f, c->s);
     if (nlev == 1)
-      cg_single<float>(F.d, F.numdia, F.ac, F.cc, r32, L32[0].x, 100, 0.0,
-                       c->s);
+      coarse_solve<float>(c, 0, F.ac, F.cc, r32, L32[0].x);
     else
       vcycle<float>(c, L32, r32, c->L[nlev - 1].ac, cc64c);
     add_correction<float>(F.d, L32[0].x, c->u, c->s);
@@ -552,7 +582,7 @@ void step_body(pmg_ctx *c, int mode) {
                          c->s);
   } else {
     if (nlev == 1)
-      cg_single<double>(F.d, F.numdia, F.ac, F.cc, F.fc, c->u, 100, 0.0, c->s);
+      coarse_solve<double>(c, 0, F.ac, F.cc, F.fc, c->u);
     else
       vcycle<double>(c, L64, F.fc, c->L[nlev - 1].ac, cc64c);
     resid<double, double>(F.d, F.numdia, F.ac, F.cc, F.fc, c->u,
@@ -593,7 +623,7 @@ int lin_begin(pmg_ctx *c, int mode) {
     if (!c->ops32_ok) convert_ops32(c);
     c->ops32_ok = true;
   }
-  if (!cg_fits(c->L[c->nlev - 1].d)) return PMG_EBADPARM;
+  if (!coarse_fits(c)) return PMG_EBADPARM;
   if (!c->cyc_ev) {
     hipEventCreateWithFlags(&c->cyc_ev, hipEventDisableTiming);
     hipEventCreateWithFlags(&c->fin_ev, hipEventDisableTiming);
@@ -624,7 +654,8 @@ int lin_begin(pmg_ctx *c, int mode) {
     resid<double, float>(F.d, F.numdia, F.ac, F.cc, F.fc, c->u, F.w032, &rnrm,
                          c->s);
   }
-  hipMemcpyAsync(c->hscal, c->scal, 2 * sizeof(double), hipMemcpyDeviceToHost,
+  // the two scalars and the band factor's info (slot S_BINFO)
+  hipMemcpyAsync(c->hscal, c->scal, 4 * sizeof(double), hipMemcpyDeviceToHost,
                  c->s);
   hipEventRecord(c->cyc_ev, c->s);
   S.pending = true;
@@ -658,6 +689,11 @@ int lin_poll(pmg_ctx *c, bool wait) {
   S.pending = false;
   if (S.phase == 0) {  // the start has landed
     S.phase = 1;
+    if (p.mgsolv == 1 && c->hscal[S_BINFO] != 0.0) {  // band not SPD
+      S.done = true;
+      S.rc = PMG_ENOTSPD;
+      return 1;
+    }
     if (p.istop == 1) S.rsden = c->hscal[S_RSDEN];
     if (S.rsden == 0.0) S.rsden = 1.0;  // mgcsd.cc:154-157
     S.rsnrm = S.mode == PMG_MIXED ? c->hscal[S_RSNRM] : S.rsden;
@@ -736,9 +772,10 @@ int ctx_solve(pmg_ctx *c, int mode) {
 // vectors; the resident fine inputs; padding of every allocation.
 size_t ctx_bytes_estimate(const Params &p, int mode) {
   size_t b64 = 0, b32 = 0, nalloc = 0;
-  Dim d{p.nx, p.ny, p.nz};
+  Dim d{p.nx, p.ny, p.nz}, dc = d;
   for (int l = 0; l < p.nlev; l++) {
     const size_t n = d.n();
+    dc = d;
     b64 += ((l == 0 ? 4 : 14) + 2 + (l ? 27 : 0) + 4) * n;
     b32 += ((l == 0 ? 4 : 14) + 1 + (l ? 27 : 0) + 4) * n;
     nalloc += 16;
@@ -749,6 +786,11 @@ size_t ctx_bytes_estimate(const Params &p, int mode) {
   b64 += (size_t)p.nx + p.ny + p.nz + 2 * ((size_t)p.ny * p.nz +
          (size_t)p.nx * p.nz + (size_t)p.nx * p.ny) + 5 * nf;  // inputs
   nalloc += 16;
+  if (p.mgsolv == 1) {  // band, its row and column copies, solve vector
+    const BandDims b = band_dims(dc, p.nlev == 1 ? 7 : 27);
+    b64 += (size_t)b.n * (b.lda + 2 * b.W + 1);
+    nalloc += 4;
+  }
   size_t bytes = 8 * b64 + (mode == PMG_MIXED ? 4 * b32 : 0) + 256 * nalloc;
   return bytes + 6 * sizeof(double) * 1024 + (1 << 20);  // atoms, slack
 }
@@ -787,7 +829,8 @@ int ctx_newton(pmg_ctx *c, int mode) {
       if (!c->L[l].cpr32) c->L[l].cpr32 = c->alloc<float>(c->L[l].d.n());
     if (!c->ok) return PMG_ENOMEM;
   }
-  if (!cg_fits(c->L[c->nlev - 1].d)) return PMG_EBADPARM;
+  if (!coarse_fits(c)) return PMG_EBADPARM;
+  if (p.mgsolv == 1 && host_scalar(c, S_BINFO) != 0.0) return PMG_ENOTSPD;
   Level &F = c->L[0];
   const Dim d = F.d;
   const size_t nf = d.n();
@@ -956,6 +999,15 @@ int export_work(pmg_ctx *c, int *iwork, double *rwork, bool newton) {
     if (lev > 1)
       hipMemcpyAsync(pcb + IZ(11, lev - 1) - 1, L.pc,
                      27 * nl * sizeof(double), hipMemcpyDeviceToHost, c->s);
+  }
+  if (p.mgsolv == 1) {  // factored band at ac(iz(7, nlev+1)) (mgsubd.cc:255)
+    const Level &C = c->L[p.nlev - 1];
+    const BandDims b = band_dims(C.d, C.numdia);
+    int *ipcb = ipc + IZ(5, p.nlev + 1) - 1;  // buildBd.cc:82-85, 295
+    ipcb[0] = b.n, ipcb[1] = b.m, ipcb[2] = b.lda, ipcb[3] = 1;
+    hipMemcpyAsync(acb + IZ(7, p.nlev + 1) - 1, c->band,
+                   (size_t)b.n * b.lda * sizeof(double),
+                   hipMemcpyDeviceToHost, c->s);
   }
   return hipStreamSynchronize(c->s) == hipSuccess ? PMG_OK : PMG_EDEVICE;
 }

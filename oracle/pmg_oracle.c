@@ -842,6 +842,152 @@ int pmgo_cghs(int nx, int ny, int nz, int numdia, const double *ac,
   return iters;
 }
 
+void pmgo_dpbfa(double *abd, int lda, int n, int m, int *info);
+void pmgo_dpbsl(const double *abd, int lda, int n, int m, double *b);
+
+/* ------------------------------------------------------------------------ */
+/* Banded coarsest solve, mgsolv = 1: Vbuildband (buildBd.cc:62-304) and the  */
+/* LINPACK routines Vddot / Vdpbfa / Vdpbsl (mlinpckd.cc:112-235).           */
+/* Band storage: column-major abd(lda, n), 1-based; A(i,j), i <= j, lives at  */
+/* abd(m + 1 + i - j, j).                                                    */
+/* ------------------------------------------------------------------------ */
+
+#define AB(abd, lda, r, c) (abd)[(size_t)((c)-1) * (lda) + ((r)-1)]
+
+/* Vddot (mlinpckd.cc:112-165), unit strides: the n % 5 leading products are
+ * summed one by one, then groups of five are summed and added. */
+static double ddot5(int n, const double *x, const double *y) {
+  double t = 0.0;
+  if (n <= 0) return 0.0;
+  int m = n % 5;
+  for (int i = 0; i < m; i++) t += x[i] * y[i];
+  if (n < 5) return t;
+  for (int i = m; i < n; i += 5) {
+    double g = x[i] * y[i];
+    g += x[i + 1] * y[i + 1];
+    g += x[i + 2] * y[i + 2];
+    g += x[i + 3] * y[i + 3];
+    g += x[i + 4] * y[i + 4];
+    t += g;
+  }
+  return t;
+}
+
+int pmgo_buildband(int nx, int ny, int nz, int numdia, const double *ac,
+                   double *abd, int *ipcb) {
+  /* Vbuildband1_7 (buildBd.cc:62-124) / Vbuildband1_27 (:126-249): the
+   * diagonal is oC alone (cc is not added); the couplings of point jj
+   * (lexicographic interior index) to its lower neighbours ii < jj are the
+   * negated upper planes of those neighbours. */
+  size_t N = (size_t)nx * ny * nz;
+  int sx = nx - 2, sxy = (nx - 2) * (ny - 2);
+  int n = sxy * (nz - 2);
+  int m = numdia == 7 ? sxy : sxy + sx + 1;
+  int lda = m + 1;
+  const double *oC = ac, *oE = ac + N, *oN = ac + 2 * N, *uC = ac + 3 * N;
+  ipcb[0] = n;
+  ipcb[1] = m;
+  ipcb[2] = lda;
+  ipcb[3] = 0;
+  for (int k = 2; k <= nz - 1; k++)
+    for (int j = 2; j <= ny - 1; j++)
+      for (int i = 2; i <= nx - 1; i++) {
+        int jj = (k - 2) * sxy + (j - 2) * sx + (i - 2) + 1;
+#define PUT(off, val) AB(abd, lda, m + 1 - (off), jj) = (val)
+        PUT(0, oC[IX(nx, ny, i, j, k)]);
+        PUT(1, -oE[IX(nx, ny, i - 1, j, k)]);
+        PUT(sx, -oN[IX(nx, ny, i, j - 1, k)]);
+        PUT(sxy, -uC[IX(nx, ny, i, j, k - 1)]);
+        if (numdia == 27) {
+          const double *oNE = ac + 4 * N, *oNW = ac + 5 * N, *uE = ac + 6 * N,
+                       *uW = ac + 7 * N, *uN = ac + 8 * N, *uS = ac + 9 * N,
+                       *uNE = ac + 10 * N, *uNW = ac + 11 * N,
+                       *uSE = ac + 12 * N, *uSW = ac + 13 * N;
+          PUT(sx - 1, -oNE[IX(nx, ny, i, j - 1, k)]);
+          PUT(sx + 1, -oNW[IX(nx, ny, i, j - 1, k)]);
+          PUT(sxy - 1, -uE[IX(nx, ny, i, j, k - 1)]);
+          PUT(sxy + 1, -uW[IX(nx, ny, i, j, k - 1)]);
+          PUT(sxy - sx, -uN[IX(nx, ny, i, j, k - 1)]);
+          PUT(sxy + sx, -uS[IX(nx, ny, i, j, k - 1)]);
+          PUT(sxy - sx - 1, -uNE[IX(nx, ny, i, j, k - 1)]);
+          PUT(sxy - sx + 1, -uNW[IX(nx, ny, i, j, k - 1)]);
+          PUT(sxy + sx - 1, -uSE[IX(nx, ny, i, j, k - 1)]);
+          PUT(sxy + sx + 1, -uSW[IX(nx, ny, i, j, k - 1)]);
+        }
+#undef PUT
+      }
+  int info = 0;
+  pmgo_dpbfa(abd, lda, n, m, &info); /* Vbuildband buildBd.cc:290-303 */
+  ipcb[3] = 1;
+  return info;
+}
+
+void pmgo_dpbfa(double *abd, int lda, int n, int m, int *info) {
+  /* Vdpbfa (mlinpckd.cc:167-206): left-looking band Cholesky A = R^T R,
+   * column j of R from the already factored columns j-m .. j-1. */
+  *info = 0;
+  for (int j = 1; j <= n; j++) {
+    double s = 0.0;
+    int ik = m + 1;
+    int jk = j - m > 1 ? j - m : 1;
+    int mu = m + 2 - j > 1 ? m + 2 - j : 1;
+    for (int k = mu; k <= m; k++) {
+      double t = AB(abd, lda, k, j) -
+                 ddot5(k - mu, &AB(abd, lda, ik, jk), &AB(abd, lda, mu, j));
+      t /= AB(abd, lda, m + 1, jk);
+      AB(abd, lda, k, j) = t;
+      s += t * t;
+      ik--;
+      jk++;
+    }
+    s = AB(abd, lda, m + 1, j) - s;
+    if (s <= 0.0) {
+      *info = j;
+      return;
+    }
+    AB(abd, lda, m + 1, j) = sqrt(s);
+  }
+}
+
+void pmgo_dpbsl(const double *abd, int lda, int n, int m, double *b) {
+  /* Vdpbsl (mlinpckd.cc:208-235): R^T y = b by inner products, then
+   * R x = y by column updates (Vdaxpy, element-wise). */
+  for (int k = 1; k <= n; k++) {
+    int lm = k - 1 < m ? k - 1 : m;
+    int la = m + 1 - lm, lb = k - lm;
+    double t = ddot5(lm, &AB(abd, lda, la, k), &b[lb - 1]);
+    b[k - 1] = (b[k - 1] - t) / AB(abd, lda, m + 1, k);
+  }
+  for (int kb = 1; kb <= n; kb++) {
+    int k = n + 1 - kb;
+    int lm = k - 1 < m ? k - 1 : m;
+    int la = m + 1 - lm, lb = k - lm;
+    b[k - 1] /= AB(abd, lda, m + 1, k);
+    double t = -b[k - 1];
+    if (t != 0.0) /* Vdaxpy returns early for da == 0 */
+      for (int i = 0; i < lm; i++)
+        b[lb - 1 + i] += t * AB(abd, lda, la + i, k);
+  }
+}
+
+/* Coarsest solve with the factored band (mgcsd.cc:391-407): the interior of
+ * rhs is packed (Vxcopy_small), solved, unpacked into x (Vxcopy_large) and
+ * the boundary of x zeroed (VfboundPMG00).  wk holds n doubles. */
+void pmgo_bandsolve(int nx, int ny, int nz, const double *abd, const int *ipcb,
+                    const double *rhs, double *x, double *wk) {
+  int sx = nx - 2, sxy = (nx - 2) * (ny - 2);
+  for (int k = 2; k <= nz - 1; k++)
+    for (int j = 2; j <= ny - 1; j++)
+      for (int i = 2; i <= nx - 1; i++)
+        wk[(k - 2) * sxy + (j - 2) * sx + (i - 2)] = rhs[IX(nx, ny, i, j, k)];
+  pmgo_dpbsl(abd, ipcb[2], ipcb[0], ipcb[1], wk);
+  for (int k = 2; k <= nz - 1; k++)
+    for (int j = 2; j <= ny - 1; j++)
+      for (int i = 2; i <= nx - 1; i++)
+        x[IX(nx, ny, i, j, k)] = wk[(k - 2) * sxy + (j - 2) * sx + (i - 2)];
+  pmgo_fbound00(nx, ny, nz, x);
+}
+
 /* ------------------------------------------------------------------------ */
 /* Hierarchy and V-cycle: Vmvcs (mgcsd.cc:60-530)                            */
 /* ------------------------------------------------------------------------ */
@@ -852,6 +998,9 @@ typedef struct {
   size_t n[32];
   int numdia[32];
   double *ac[32], *cc[32], *fc[32], *pc[32]; /* pc[l]: P from l+1 to l */
+  double *band;                              /* mgsolv 1: factored band */
+  const int *ipcb;                           /* {n, m, lda, factored} */
+  double *bwk;                               /* n doubles for the solve */
 } hier_t;
 
 static void hier_dims(hier_t *h, int nx, int ny, int nz, int nlev) {
@@ -913,8 +1062,11 @@ static int mvcs(const hier_t *h, const cyc_t *c, double **x, double **w0,
 
   if (nlev == 1) { /* mgcsd.cc:170-291 */
     memset(x[0], 0, h->n[0] * sizeof(double));
-    pmgo_cghs(nx0, ny0, nz0, h->numdia[0], h->ac[0], ccl[0], fc1, x[0], w1,
-              w2, w3, 100, c->epsiln);
+    if (c->mgsolv == 0)
+      pmgo_cghs(nx0, ny0, nz0, h->numdia[0], h->ac[0], ccl[0], fc1, x[0], w1,
+                w2, w3, 100, c->epsiln);
+    else
+      pmgo_bandsolve(nx0, ny0, nz0, h->band, h->ipcb, fc1, x[0], h->bwk);
     *iters_out = 1;
     rsnrm = stop_norm(h, c, x[0], fc1, ccl[0], w1, tru);
     if (record) {
@@ -943,12 +1095,16 @@ static int mvcs(const hier_t *h, const cyc_t *c, double **x, double **w0,
                     ccl[l], w0[l], x[l], w1);
       }
     }
-    { /* coarsest: Vcghs, 100 iterations, errtol = epsiln (D5) */
+    { /* coarsest: Vcghs, 100 iterations, errtol = epsiln (D5), or the
+       * factored band (mgcsd.cc:391-407) */
       int l = nlev - 1;
       memset(x[l], 0, h->n[l] * sizeof(double));
       if (c->mgsolv == 0)
         pmgo_cghs(h->nx[l], h->ny[l], h->nz[l], h->numdia[l], h->ac[l],
                   ccl[l], w0[l], x[l], w1, w2, w3, 100, c->epsiln);
+      else
+        pmgo_bandsolve(h->nx[l], h->ny[l], h->nz[l], h->band, h->ipcb, w0[l],
+                       x[l], h->bwk);
     }
     for (int l = nlev - 2; l >= 0; l--) { /* up: mgcsd.cc:418-473 */
       int cl = l + 1;
@@ -1006,7 +1162,8 @@ static int decode(const int *iparm, const double *rparm, params_t *p) {
   if (p->nlev < 1 || p->nx < 3 || p->ny < 3 || p->nz < 3 || p->nlev > 30)
     return 2;
   if (p->nlev > pmgo_maxlev(p->nx, p->ny, p->nz) && p->nlev > 1) return 2;
-  if (p->mgcoar != 2 || p->mgdisc != 0 || p->mgsmoo != 1 || p->mgsolv != 0 ||
+  if (p->mgcoar != 2 || p->mgdisc != 0 || p->mgsmoo != 1 ||
+      (p->mgsolv != 0 && p->mgsolv != 1) ||
       (p->mgprol != 0 && p->mgprol != 1) || p->istop < 0 || p->istop > 3 ||
       p->iperf != 0 || p->ipkey != 0 || p->nu1 < 0 || p->nu2 < 0)
     return 2;
@@ -1074,6 +1231,23 @@ static int setup(const params_t *p, int *iwork, double *rwork, int newton,
                 h->nz[l], h->fc[l - 1], h->fc[l], h->pc[l - 1]);
     IZ(iz, 7, lev + 1) = IZ(iz, 7, lev) + 14 * (int)h->n[l];
   }
+  h->band = 0;
+  h->ipcb = 0;
+  h->bwk = 0;
+  if (p->mgsolv == 1) { /* Vbuildops mgsubd.cc:249-262 */
+    int L = p->nlev;
+    /* With one interior point in x or y, offsets of buildBd.cc:192-245 fall
+     * on the diagonal or outside the band column (a write past the band in
+     * the reference): refused. */
+    if (h->numdia[L - 1] == 27 && (h->nx[L - 1] < 4 || h->ny[L - 1] < 4))
+      return 2;
+    int *ipcb = ipc + IZ(iz, 5, L + 1) - 1;
+    h->band = acb + (IZ(iz, 7, L + 1) - 1);
+    h->ipcb = ipcb;
+    if (pmgo_buildband(h->nx[L - 1], h->ny[L - 1], h->nz[L - 1],
+                       h->numdia[L - 1], h->ac[L - 1], h->band, ipcb))
+      return 3; /* not positive definite: the reference exits here */
+  }
   *cc = ccb;
   *fc = fcb;
   return 0;
@@ -1097,9 +1271,9 @@ int pmgo_mgdriv(int *iparm, double *rparm, int *iwork, double *rwork,
   if (p.mode != 0) return 2; /* mode 1/2 need FAS (out of scope) */
   hier_t h;
   double *ccb, *fcb;
-  if (setup(&p, iwork, rwork, 0, xf, yf, zf, gxcf, gycf, gzcf, a1cf, a2cf,
-            a3cf, ccf, fcf, &h, &ccb, &fcb, 0, 0))
-    return 2;
+  int rc = setup(&p, iwork, rwork, 0, xf, yf, zf, gxcf, gycf, gzcf, a1cf,
+                 a2cf, a3cf, ccf, fcf, &h, &ccb, &fcb, 0, 0);
+  if (rc) return rc;
   size_t narr = 0;
   for (int l = 0; l < h.nlev; l++) narr += h.n[l];
   size_t nf = h.n[0];
@@ -1113,6 +1287,8 @@ int pmgo_mgdriv(int *iparm, double *rparm, int *iwork, double *rwork,
   /* Only the fine level of u is the caller's; coarse-level iterates live in
    * a private buffer (the reference's u is narr long, mgdrvd.cc:445). */
   double *xcb = (double *)calloc(narr, sizeof(double));
+  double *bwk = (double *)calloc(h.n[h.nlev - 1], sizeof(double));
+  h.bwk = bwk;
   double *x[32], *w0[32], *ccl[32];
   level_ptrs(&h, xcb, x);
   x[0] = u;
@@ -1132,6 +1308,7 @@ int pmgo_mgdriv(int *iparm, double *rparm, int *iwork, double *rwork,
   free(w3);
   free(tru);
   free(xcb);
+  free(bwk);
   (void)a1cf, (void)a2cf, (void)a3cf, (void)ccf, (void)fcf;
   return ierr;
 }
@@ -1147,9 +1324,9 @@ int pmgo_newdriv(int *iparm, double *rparm, int *iwork, double *rwork,
   if (p.mgkey != 0) return 2; /* Vfnewton (nested) not provided */
   hier_t h;
   double *ccb, *fcb, *w1n, *w2n;
-  if (setup(&p, iwork, rwork, 1, xf, yf, zf, gxcf, gycf, gzcf, a1cf, a2cf,
-            a3cf, ccf, fcf, &h, &ccb, &fcb, &w1n, &w2n))
-    return 2;
+  int rc = setup(&p, iwork, rwork, 1, xf, yf, zf, gxcf, gycf, gzcf, a1cf,
+                 a2cf, a3cf, ccf, fcf, &h, &ccb, &fcb, &w1n, &w2n);
+  if (rc) return rc;
   (void)w1n, (void)w2n;
   int nx = h.nx[0], ny = h.ny[0], nz = h.nz[0];
   size_t nf = h.n[0], narr = 0;
@@ -1164,6 +1341,8 @@ int pmgo_newdriv(int *iparm, double *rparm, int *iwork, double *rwork,
   double *xtmpb = (double *)calloc(narr, sizeof(double));
   double *t1 = (double *)calloc(nf, sizeof(double));
   double *t2 = (double *)calloc(nf, sizeof(double));
+  double *bwk = (double *)calloc(h.n[h.nlev - 1], sizeof(double));
+  h.bwk = bwk;
   double *x[32], *w0[32], *cpr[32], *rhs[32], *xtmp[32];
   x[0] = u;
   level_ptrs(&h, w0b, w0);
@@ -1252,7 +1431,7 @@ int pmgo_newdriv(int *iparm, double *rparm, int *iwork, double *rwork,
   g_last_ierror = rsnrm / rsden <= p.errtol ? 0 : 1;
   g_last_relres = rsnrm / rsden;
   free(w0b), free(w1), free(w2), free(w3), free(tru), free(cprb), free(rhsb),
-      free(xtmpb), free(t1), free(t2);
+      free(xtmpb), free(t1), free(t2), free(bwk);
   (void)a1cf, (void)a2cf, (void)a3cf, (void)ccf, (void)fcf;
   return g_last_ierror;
 }

@@ -105,6 +105,23 @@ int pmgo_cghs(int nx, int ny, int nz, int numdia, const double *ac,
               const double *cc, const double *fc, double *x, double *p,
               double *ap, double *r, int itmax, double errtol);
 
+/* Banded coarsest solve, mgsolv = 1.  pmgo_buildband: Vbuildband
+ * (pmgc/buildBd.cc:251-304) with Vbuildband1_7 / Vbuildband1_27 (:62-249):
+ * LINPACK band storage abd(lda, n) of the interior operator, diagonal oC
+ * only, then factored in place; ipcb = {n, m, lda, 1}; returns the Vdpbfa
+ * info (0, or the first non-positive pivot).  pmgo_dpbfa / pmgo_dpbsl:
+ * Vdpbfa / Vdpbsl (pmgc/mlinpckd.cc:167-235).  pmgo_bandsolve: the
+ * coarsest-level use (pmgc/mgcsd.cc:391-407), wk of n doubles.
+ * Pinned: band entries against the dense interior matrix with cc removed,
+ * R^T R == A, and solves against numpy's dense solver
+ * (tests/test_oracle_band.py). */
+int pmgo_buildband(int nx, int ny, int nz, int numdia, const double *ac,
+                   double *abd, int *ipcb);
+void pmgo_dpbfa(double *abd, int lda, int n, int m, int *info);
+void pmgo_dpbsl(const double *abd, int lda, int n, int m, double *b);
+void pmgo_bandsolve(int nx, int ny, int nz, const double *abd, const int *ipcb,
+                    const double *rhs, double *x, double *wk);
+
 /* Vmypdefinitlpbe / npbe (pmgc/mypdec.cc:72-106): ion table. */
 void pmgo_mypdefinit(int nion, const double *charge, const double *sconc);
 /* Vc_vec / Vdc_vec (pmgc/mypdec.cc:136-212, 348-414), ipkey 0. */

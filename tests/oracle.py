@@ -42,6 +42,10 @@ def olib():
             "pmgo_xnrm2": (d, [i, i, i, dp]),
             "pmgo_xdot": (d, [i, i, i, dp, dp]),
             "pmgo_cghs": (i, [i, i, i, i, dp, dp, dp, dp, dp, dp, dp, i, d]),
+            "pmgo_buildband": (i, [i, i, i, i, dp, dp, ip]),
+            "pmgo_dpbfa": (None, [dp, i, i, i, ip]),
+            "pmgo_dpbsl": (None, [dp, i, i, i, dp]),
+            "pmgo_bandsolve": (None, [i, i, i, dp, ip, dp, dp, dp]),
             "pmgo_mypdefinit": (None, [i, dp, dp]),
             "pmgo_c_vec": (None, [dp, dp, dp, i, i, i]),
             "pmgo_dc_vec": (None, [dp, dp, dp, i, i, i]),
@@ -141,6 +145,37 @@ def cghs(n, numdia, ac, cc, fc, itmax=100, errtol=0.0):
     it = olib().pmgo_cghs(n, n, n, numdia, dp(ac), dp(cc), dp(fc), dp(x), dp(p),
                           dp(ap), dp(r), itmax, errtol)
     return x, it
+
+
+def band_dims(n, numdia):
+    """(n_unknowns, m, lda) of the band of an n^3 level (buildBd.cc:264-285)."""
+    s = n - 2
+    m = s * s if numdia == 7 else s * s + s + 1
+    return s ** 3, m, m + 1
+
+
+def buildband(n, numdia, ac):
+    """Factored band of an n^3 level; returns (info, abd (lda, nu) view, ipcb)."""
+    nu, m, lda = band_dims(n, numdia)
+    abd = np.zeros(lda * nu)
+    ipcb = np.zeros(4, np.int32)
+    info = olib().pmgo_buildband(n, n, n, numdia, dp(np.ascontiguousarray(ac, np.float64)),
+                                 dp(abd), ip(ipcb))
+    return info, abd, ipcb
+
+
+def dpbsl(abd, lda, nu, m, b):
+    x = np.ascontiguousarray(b, np.float64).copy()
+    olib().pmgo_dpbsl(dp(abd), lda, nu, m, dp(x))
+    return x
+
+
+def bandsolve(n, abd, ipcb, rhs):
+    x = np.full(n ** 3, 7.0)
+    wk = np.zeros(int(ipcb[0]))
+    olib().pmgo_bandsolve(n, n, n, dp(abd), ip(ipcb), dp(np.ascontiguousarray(rhs)),
+                          dp(x), dp(wk))
+    return x
 
 
 def mypdefinit(charge, sconc):
