@@ -876,15 +876,17 @@ static double ddot5(int n, const double *x, const double *y) {
 int pmgo_buildband(int nx, int ny, int nz, int numdia, const double *ac,
                    double *abd, int *ipcb) {
   /* Vbuildband1_7 (buildBd.cc:62-124) / Vbuildband1_27 (:126-249): the
-   * diagonal is oC alone (cc is not added); the couplings of point jj
-   * (lexicographic interior index) to its lower neighbours ii < jj are the
-   * negated upper planes of those neighbours. */
+   * diagonal is oC alone (cc is not added); column jj (lexicographic
+   * interior index) holds at offset jj - ii the negated plane values the
+   * reference reads at (i-1,j,k), (i,j-1,k) or (i,j,k-1).  For the
+   * diagonal directions these are the mirrored planes: e.g. point
+   * (i+1,j-1,k) gets oNE(i,j-1,k), where Vmatvec27 (matvecd.cc:137-175)
+   * couples the two through oNW(i+1,j-1,k). */
   size_t N = (size_t)nx * ny * nz;
   int sx = nx - 2, sxy = (nx - 2) * (ny - 2);
   int n = sxy * (nz - 2);
   int m = numdia == 7 ? sxy : sxy + sx + 1;
   int lda = m + 1;
-  const double *oC = ac, *oE = ac + N, *oN = ac + 2 * N, *uC = ac + 3 * N;
   ipcb[0] = n;
   ipcb[1] = m;
   ipcb[2] = lda;
@@ -894,26 +896,28 @@ int pmgo_buildband(int nx, int ny, int nz, int numdia, const double *ac,
       for (int i = 2; i <= nx - 1; i++) {
         int jj = (k - 2) * sxy + (j - 2) * sx + (i - 2) + 1;
 #define PUT(off, val) AB(abd, lda, m + 1 - (off), jj) = (val)
-        PUT(0, oC[IX(nx, ny, i, j, k)]);
-        PUT(1, -oE[IX(nx, ny, i - 1, j, k)]);
-        PUT(sx, -oN[IX(nx, ny, i, j - 1, k)]);
-        PUT(sxy, -uC[IX(nx, ny, i, j, k - 1)]);
+        /* in the reference's order: later writes win where offsets
+         * coincide (nx - 2 == 2); planes as pmgc/matvecd.cc:185-189 */
+#define PL(pl, a, b, c) ac[(size_t)(pl) * N + IX(nx, ny, a, b, c)]
+        PUT(0, PL(0, i, j, k));               /* oC */
+        PUT(1, -PL(1, i - 1, j, k));          /* oE */
+        PUT(sx, -PL(2, i, j - 1, k));         /* oN */
         if (numdia == 27) {
-          const double *oNE = ac + 4 * N, *oNW = ac + 5 * N, *uE = ac + 6 * N,
-                       *uW = ac + 7 * N, *uN = ac + 8 * N, *uS = ac + 9 * N,
-                       *uNE = ac + 10 * N, *uNW = ac + 11 * N,
-                       *uSE = ac + 12 * N, *uSW = ac + 13 * N;
-          PUT(sx - 1, -oNE[IX(nx, ny, i, j - 1, k)]);
-          PUT(sx + 1, -oNW[IX(nx, ny, i, j - 1, k)]);
-          PUT(sxy - 1, -uE[IX(nx, ny, i, j, k - 1)]);
-          PUT(sxy + 1, -uW[IX(nx, ny, i, j, k - 1)]);
-          PUT(sxy - sx, -uN[IX(nx, ny, i, j, k - 1)]);
-          PUT(sxy + sx, -uS[IX(nx, ny, i, j, k - 1)]);
-          PUT(sxy - sx - 1, -uNE[IX(nx, ny, i, j, k - 1)]);
-          PUT(sxy - sx + 1, -uNW[IX(nx, ny, i, j, k - 1)]);
-          PUT(sxy + sx - 1, -uSE[IX(nx, ny, i, j, k - 1)]);
-          PUT(sxy + sx + 1, -uSW[IX(nx, ny, i, j, k - 1)]);
+          PUT(sx - 1, -PL(4, i, j - 1, k));   /* oNE */
+          PUT(sx + 1, -PL(5, i, j - 1, k));   /* oNW */
         }
+        PUT(sxy, -PL(3, i, j, k - 1));        /* uC */
+        if (numdia == 27) {
+          PUT(sxy - 1, -PL(6, i, j, k - 1));       /* uE */
+          PUT(sxy + 1, -PL(7, i, j, k - 1));       /* uW */
+          PUT(sxy - sx, -PL(8, i, j, k - 1));      /* uN */
+          PUT(sxy + sx, -PL(9, i, j, k - 1));      /* uS */
+          PUT(sxy - sx - 1, -PL(10, i, j, k - 1)); /* uNE */
+          PUT(sxy - sx + 1, -PL(11, i, j, k - 1)); /* uNW */
+          PUT(sxy + sx - 1, -PL(12, i, j, k - 1)); /* uSE */
+          PUT(sxy + sx + 1, -PL(13, i, j, k - 1)); /* uSW */
+        }
+#undef PL
 #undef PUT
       }
   int info = 0;
