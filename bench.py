@@ -38,6 +38,9 @@ def parse():
                     help="solves in flight at once in the serving loop")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the 289^3 mixed-precision and 129^3 Newton legs")
+    ap.add_argument("--legs", default="large,newton,band",
+                    help="extra legs to run: large (289^3 mixed), newton (129^3), "
+                         "band (161^3 with the banded coarsest solve)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (a reference "
                          "kernel of known traffic for hardware-counter runs)")
@@ -252,6 +255,40 @@ def newton_leg(mg, cpu):
     return out
 
 
+def band_leg(mg, cpu, steps=5):
+    """The 161^3 workload with the banded coarsest solve (iparm(21) = 1:
+    Vbuildband + Vdpbfa at setup, Vdpbsl per V-cycle) instead of 100 CG
+    iterations: FP64 and mixed solves (assembly + build + V-cycles) and,
+    with the CPU leg, the error against the oracle's mgsolv = 1 solve."""
+    cfg = mg.CONFIGS["c1"]
+    n, nlev = cfg["n"], cfg["nlev"]
+    atoms = mg.pbe_molecule(cfg["natoms"], cfg["sphere_r"], cfg["charge_mode"], cfg["seed"])
+    mol = (cfg["h"], cfg["ionic"], atoms)
+    iparm, rparm = mg.make_params(n, nlev, mgsolv=1)
+    ctx = mg.Context(iparm, rparm, pbe=mol)
+    out = {"workload": f"linearised PBE {n}^3 h={cfg['h']} nlev={nlev} {cfg['natoms']} atoms, "
+                       "banded coarsest solve (mgsolv=1)"}
+    us = {}
+    for name, mode in (("fp64", mg.PMG_FP64), ("mixed", mg.PMG_MIXED)):
+        wall, lat, its = timed_steps(ctx, mol, mode, steps, 1)
+        st = ctx.stats()
+        out[name] = {"solves_per_s": steps / wall, "p50_ms": float(np.percentile(lat, 50)),
+                     "vcycles": its[-1], "relres": st["relres"], "setup_ms": st["setup_ms"],
+                     "solve_ms": st["solve_ms"]}
+        us[name] = ctx.u()
+    ctx.close()
+    out["mixed_vs_fp64_speedup"] = out["mixed"]["solves_per_s"] / out["fp64"]["solves_per_s"]
+    out["mixed_rel_err_vs_fp64_gpu"] = rel_err(us["mixed"], us["fp64"])
+    if cpu:
+        from tests import oracle as orc
+        inp = orc.pbe_assemble(n, cfg["h"], cfg["ionic"], atoms)
+        _, u0, _, _ = orc.solve(inp, n, nlev, mgsolv=1)
+        out["oracle_vcycles"] = orc.last_stats()[0]
+        out["fp64_rel_err_vs_oracle"] = rel_err(us["fp64"], u0)
+        out["mixed_rel_err_vs_oracle"] = rel_err(us["mixed"], u0)
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -280,6 +317,7 @@ def main():
     # FP64 GPU path (the drop-in arithmetic)
     wall64, lat64, it64 = timed_steps(ctx, mol, mg.PMG_FP64, args.steps, args.warmup)
     u64 = ctx.u()
+    st64 = ctx.stats()
     # mixed precision: FP32 V-cycles inside FP64 defect correction
     wallmx, latmx, itmx = timed_steps(ctx, mol, mg.PMG_MIXED, args.steps, args.warmup)
     umx = ctx.u()
@@ -316,11 +354,12 @@ def main():
     serving = serving_leg(mg, args.serve_requests, args.serve_active) \
         if args.serve_requests > 0 else None
     extra = {}
-    if not args.no_extra:
-        extra["large_289"] = large_leg(mg, not args.no_cpu)
-        print(f"large: {extra['large_289']}", file=sys.stderr, flush=True)
-        extra["newton_129"] = newton_leg(mg, not args.no_cpu)
-        print(f"newton: {extra['newton_129']}", file=sys.stderr, flush=True)
+    legs = set() if args.no_extra else set(args.legs.split(","))
+    for key, name, fn in (("large", "large_289", large_leg), ("newton", "newton_129", newton_leg),
+                          ("band", "band_161", band_leg)):
+        if key in legs:
+            extra[name] = fn(mg, not args.no_cpu)
+            print(f"{key}: {extra[name]}", file=sys.stderr, flush=True)
 
     value = args.steps / wallmx
     out = {
@@ -342,7 +381,9 @@ def main():
         "p50_ms": float(np.percentile(latmx, 50)),
         "p99_ms": float(np.percentile(latmx, 99)),
         "fp64": {"value": args.steps / wall64, "p50_ms": float(np.percentile(lat64, 50)),
-                 "p99_ms": float(np.percentile(lat64, 99)), "vcycles": it64[-1]},
+                 "p99_ms": float(np.percentile(lat64, 99)), "vcycles": it64[-1],
+                 "solve_ms": st64["solve_ms"], "setup_ms": st64["setup_ms"]},
+        "mixed_solve_ms": st["solve_ms"],
         "mixed_vs_fp64_speedup": (args.steps / wallmx) / (args.steps / wall64),
         "mixed_outer_iters": itmx[-1],
         "mixed_potential_rel_err_vs_fp64": err,

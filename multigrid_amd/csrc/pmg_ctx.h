@@ -72,10 +72,9 @@ struct pmg_ctx {
   double *u = nullptr;       // fine solution (FP64)
   double *work[4] = {};      // fine FP64 scratch (Newton)
   double *part = nullptr;    // reduction partials
-  // mgsolv = 1: factored band of the coarsest operator (reference layout),
-  // the factor by rows and by columns, and the substitution vector
-  double *band = nullptr, *band_rt = nullptr, *band_ct = nullptr,
-         *band_y = nullptr;
+  // mgsolv = 1: factored band of the coarsest operator (reference layout)
+  // and the factor by rows and by columns for the substitution
+  double *band = nullptr, *band_rt = nullptr, *band_ct = nullptr;
   double *scal = nullptr;    // device scalars
   double *hscal = nullptr;   // pinned host mirror
   bool have32 = false, haveN = false;

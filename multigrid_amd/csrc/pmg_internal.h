@@ -113,19 +113,20 @@ void cg_single(const Dim &d, int numdia, const double *ac, const double *cc,
 // with 27 points, or a factor window m + 1 beyond the LDS of one CU).
 struct BandDims {
   int n = 0, m = 0, lda = 0, W = 0;
+  int npad = 0, nrow = 0;  // n rounded up to a multiple of W; rows of rt/ct
   bool ok = false;
 };
 BandDims band_dims(const Dim &d, int numdia);
 // Band of the operator ac of level d (abd: n * lda), its factor in place,
 // and the factor by rows and by columns for the substitution (rt, ct:
-// n * W each); info[0] = 0, or the first non-positive pivot.
+// nrow * W each); info[0] = 0, or the first non-positive pivot.
 void band_build_factor(const Dim &d, int numdia, const double *ac,
                        double *abd, double *rt, double *ct, double *info,
                        hipStream_t s);
-// x = A_band^{-1} f on the interior, 0 on the boundary; y: n doubles.
+// x = A_band^{-1} f on the interior, 0 on the boundary.
 template <class T>
 void band_solve(const Dim &d, int numdia, const double *rt, const double *ct,
-                double *y, const T *f, T *x, hipStream_t s);
+                const T *f, T *x, hipStream_t s);
 
 // ---- BLAS-1 helpers ----
 template <class T>

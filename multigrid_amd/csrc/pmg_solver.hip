@@ -173,8 +173,7 @@ void coarse_solve(pmg_ctx *c, int l, const double *ac64, const double *cc64,
                   const T *f, T *x) {
   const Level &L = c->L[l];
   if (c->p.mgsolv == 1)
-    band_solve<T>(L.d, L.numdia, c->band_rt, c->band_ct, c->band_y, f, x,
-                  c->s);
+    band_solve<T>(L.d, L.numdia, c->band_rt, c->band_ct, f, x, c->s);
   else
     cg_single<T>(L.d, L.numdia, ac64, cc64, f, x, 100, 0.0, c->s);
 }
@@ -321,9 +320,8 @@ int alloc_hierarchy(pmg_ctx *c) {
     const BandDims b = band_dims(C.d, C.numdia);
     if (!b.ok) return PMG_EBADPARM;
     c->band = c->alloc<double>((size_t)b.n * b.lda);
-    c->band_rt = c->alloc<double>((size_t)b.n * b.W);
-    c->band_ct = c->alloc<double>((size_t)b.n * b.W);
-    c->band_y = c->alloc<double>((size_t)b.n);
+    c->band_rt = c->alloc<double>((size_t)b.nrow * b.W);
+    c->band_ct = c->alloc<double>((size_t)b.nrow * b.W);
   }
   size_t nf = c->L[0].d.n();
   c->u = c->alloc<double>(nf);
@@ -786,10 +784,10 @@ size_t ctx_bytes_estimate(const Params &p, int mode) {
   b64 += (size_t)p.nx + p.ny + p.nz + 2 * ((size_t)p.ny * p.nz +
          (size_t)p.nx * p.nz + (size_t)p.nx * p.ny) + 5 * nf;  // inputs
   nalloc += 16;
-  if (p.mgsolv == 1) {  // band, its row and column copies, solve vector
+  if (p.mgsolv == 1) {  // band and its row and column copies
     const BandDims b = band_dims(dc, p.nlev == 1 ? 7 : 27);
-    b64 += (size_t)b.n * (b.lda + 2 * b.W + 1);
-    nalloc += 4;
+    b64 += (size_t)b.n * b.lda + 2 * (size_t)b.nrow * b.W;
+    nalloc += 3;
   }
   size_t bytes = 8 * b64 + (mode == PMG_MIXED ? 4 * b32 : 0) + 256 * nalloc;
   return bytes + 6 * sizeof(double) * 1024 + (1 << 20);  // atoms, slack

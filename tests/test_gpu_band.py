@@ -48,9 +48,9 @@ def _band(mg, n, nlev, iw, rw, newton=False):
 
 
 # (n, nlev, natoms, R, eps_in): coarsest 9^3 (one-slot window), 5^3, 4^3
-# (colliding band offsets), 11^3 and 13^3 (two- and three-slot windows)
+# (colliding band offsets), 11^3 and 12^3 (two-slot windows)
 CASES = [(33, 3, 40, 6.0, 78.54), (33, 4, 40, 6.0, 2.0), (25, 4, 20, 4.0, 2.0),
-         (41, 3, 60, 8.0, 78.54), (49, 3, 80, 9.0, 78.54)]
+         (41, 3, 60, 8.0, 78.54), (45, 3, 80, 9.0, 78.54)]
 
 
 @pytest.mark.parametrize("n,nlev,natoms,R,eps_in", CASES)
@@ -141,3 +141,14 @@ def test_newdriv_band_parity(mg, orc):
     assert e1 == e0 == 0 and it1 == it0
     assert _rel(b1, b0) < 1e-12
     assert _rel(u1, u0) < 1e-9
+
+
+def test_band_width_limit(mg, orc):
+    """A 13^3 coarsest level (half-bandwidth 133) is beyond the band
+    factor's window of 128: refused with PMG_EBADPARM before any work."""
+    print("mgsolv=1 13^3 coarsest refused", flush=True)
+    n = 49
+    inp = _inputs(orc, mg, n, 80, 9.0, 78.54)
+    with pytest.raises(mg.PMGError) as ex:
+        _drive(mg, mg.Vmgdriv, inp, n, 3)
+    assert ex.value.code == mg.PMG_EBADPARM
