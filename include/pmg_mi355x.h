@@ -49,7 +49,10 @@ int pmg_maxlev(int nx, int ny, int nz);
  * On return u holds the solution with Dirichlet boundary values restored
  * (mgdrvd.cc:500-502); iwork holds iz/ipc and rwork holds rpc/cc/fc/pc/ac
  * as the reference leaves them (mgdrvd.cc:145-166).  Unlike the reference,
- * a1cf..ccf, fcf and tcf are not overwritten. */
+ * a1cf..ccf, fcf and tcf are not overwritten.  iparm(21) = mgsolv 0 solves
+ * the coarsest level by CG (cgd.cc:60-124), 1 by the factored band
+ * (buildBd.cc:251-304, mlinpckd.cc:167-235), left in rwork at
+ * ac(iz(7,nlev+1)); an indefinite band returns PMG_ENOTSPD. */
 int pmg_mgdriv(int *iparm, double *rparm, int *iwork, double *rwork,
                double *u, const double *xf, const double *yf, const double *zf,
                const double *gxcf, const double *gycf, const double *gzcf,
@@ -64,6 +67,22 @@ int pmg_newdriv(int *iparm, double *rparm, int *iwork, double *rwork,
                 const double *zf, const double *gxcf, const double *gycf,
                 const double *gzcf, double *a1cf, double *a2cf, double *a3cf,
                 double *ccf, double *fcf, double *tcf);
+
+/* The reference's class-style solver (PBE_pmgc_class.h:1070-1133 with
+ * CG_Solver.h:45-242, run by PBE_pmgc_class_with_non_zero_IS.cc:129-136):
+ * CG preconditioned by the class V-cycle (two zero-start red-black sweeps,
+ * restriction, recursion, coarsest CG to an absolute 1e-8 from its previous
+ * solution, Hackbusch-Reusken-damped prolongation, two sweeps in the same
+ * colour order), here on the hierarchy pmg_mgdriv builds; same arguments,
+ * iwork and rwork as pmg_mgdriv.  num_iters > 0: that many CG iterations
+ * (the reference's counting criterion); 0: until the relative L1 residual
+ * of the CG residual is <= rparm(1) (errtol) or iparm(10) iterations.
+ * Needs nlev >= 2 and a 27-point coarsest level of at most 13^3. */
+int pmg_pcgdriv(int *iparm, double *rparm, int *iwork, double *rwork,
+                double *u, const double *xf, const double *yf,
+                const double *zf, const double *gxcf, const double *gycf,
+                const double *gzcf, double *a1cf, double *a2cf, double *a3cf,
+                double *ccf, double *fcf, double *tcf, int num_iters);
 
 /* Replaces pmgc::Vmypdefinitlpbe / Vmypdefinitnpbe
  * (pmgc/mypdec.cc:72-106): the ion table used by the nonlinear term. */
@@ -95,6 +114,10 @@ int pmg_ctx_create(pmg_ctx **out, const int *iparm, const double *rparm,
 /* Solve on the resident hierarchy (mode PMG_FP64 or PMG_MIXED).  The
  * solution stays in HBM until pmg_ctx_get_u. */
 int pmg_ctx_solve(pmg_ctx *ctx, int mode);
+/* The class-style PCG of pmg_pcgdriv on a resident context; PMG_MIXED runs
+ * the V-cycle preconditioner in FP32 under the FP64 CG.  Statistics as
+ * pmg_ctx_solve (iterations are CG iterations). */
+int pmg_ctx_pcg(pmg_ctx *ctx, int mode, int num_iters);
 /* Copy the solution (with Dirichlet boundary values) to host u (nx*ny*nz). */
 int pmg_ctx_get_u(pmg_ctx *ctx, double *u);
 /* Stats of the last solve on ctx: iterations, ierror, relative residual,

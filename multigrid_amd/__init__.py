@@ -63,6 +63,8 @@ def _declare(L):
         "pmg_maxlev": (i, [i, i, i]),
         "pmg_mgdriv": (i, [ip, dp, ip, dp, dp] + [dp] * 12),
         "pmg_newdriv": (i, [ip, dp, ip, dp, dp] + [dp] * 12),
+        "pmg_pcgdriv": (i, [ip, dp, ip, dp, dp] + [dp] * 12 + [i]),
+        "pmg_ctx_pcg": (i, [vp, i, i]),
         "pmg_mypdefinit": (i, [i, dp, dp]),
         "pmg_last_stats": (i, [ip, ip, dp]),
         "pmg_ctx_create": (i, [ctypes.POINTER(vp), ip, dp] + [dp] * 11),
@@ -190,6 +192,19 @@ def Vnewdriv(iparm, rparm, iwork, rwork, u, xf, yf, zf, gxcf, gycf, gzcf,
                    u, xf, yf, zf, gxcf, gycf, gzcf, a1cf, a2cf, a3cf, ccf, fcf, tcf)
 
 
+def Vpcgdriv(iparm, rparm, iwork, rwork, u, xf, yf, zf, gxcf, gycf, gzcf,
+             a1cf, a2cf, a3cf, ccf, fcf, tcf, num_iters=0):
+    """The reference's class-style solver (PBE_pmgc_class.h:1070-1133,
+    CG_Solver.h:45-242): CG preconditioned by the class V-cycle, on the
+    Vmgdriv hierarchy.  num_iters > 0 runs that many CG iterations; 0 runs
+    to rparm[0] (relative L1 residual) or iparm[9] iterations."""
+    rc = lib().pmg_pcgdriv(_ip(iparm), _dp(rparm), _ip(iwork), _dp(rwork), _dp(u),
+                           _dp(xf), _dp(yf), _dp(zf), _dp(gxcf), _dp(gycf), _dp(gzcf),
+                           _dp(a1cf), _dp(a2cf), _dp(a3cf), _dp(ccf), _dp(fcf), _dp(tcf),
+                           int(num_iters))
+    return _check(rc, "Vpcgdriv")
+
+
 def Vmypdefinitlpbe(nion, charge, sconc):
     """Ion table (pmgc/mypdec.cc:72-88)."""
     charge = np.ascontiguousarray(charge, np.float64)
@@ -243,6 +258,10 @@ class Context:
 
     def solve(self, mode=PMG_FP64):
         return _check(lib().pmg_ctx_solve(self._h, mode), "pmg_ctx_solve")
+
+    def pcg(self, mode=PMG_FP64, num_iters=0):
+        """The class-style PCG (pmg_ctx_pcg) on the resident hierarchy."""
+        return _check(lib().pmg_ctx_pcg(self._h, mode, int(num_iters)), "pmg_ctx_pcg")
 
     def u(self):
         out = np.empty(self.n[0] * self.n[1] * self.n[2], np.float64)

@@ -71,6 +71,7 @@ struct pmg_ctx {
          *a3 = nullptr, *ccf = nullptr, *fcf = nullptr;
   double *u = nullptr;       // fine solution (FP64)
   double *work[4] = {};      // fine FP64 scratch (Newton)
+  double *pcg[4] = {};       // class-style PCG: r, p, A p, s (FP64)
   double *part = nullptr;    // reduction partials
   // mgsolv = 1: factored band of the coarsest operator (reference layout)
   // and the factor by rows and by columns for the substitution

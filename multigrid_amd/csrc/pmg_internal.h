@@ -105,6 +105,26 @@ template <class T>
 void cg_single(const Dim &d, int numdia, const double *ac, const double *cc,
                const T *f, T *x, int itmax, double errtol, hipStream_t s);
 
+// ---- the class-style solver (PBE_pmgc_class.h:1070-1133,
+// CG_Solver.h:45-242) ----
+// Coarsest CG of the class V-cycle: starts from x as it is and stops when
+// sqrt(r.r) <= thresh (first test on sqrt(sqrt(r0.r0))) or after itmax
+// iterations; 27-point levels that fit one workgroup's row-segment kernel.
+bool cg_class_fits(const Dim &d, int numdia);
+template <class T>
+void cg_class(const Dim &d, int numdia, const double *ac, const double *cc,
+              const T *f, T *x, int itmax, double thresh, hipStream_t s);
+// red.out[0] = x.y over the interior (deterministic order)
+void dot_interior(const Dim &d, const double *x, const double *y,
+                  const Red &red, hipStream_t s);
+// x += (rs/pap) p, r -= (rs/pap) q; p = s + (rsn/rs) p (interior; device
+// scalars, a zero denominator gives a zero coefficient)
+void pcg_xr(const Dim &d, double *x, double *r, const double *p,
+            const double *q, const double *rs, const double *pap,
+            hipStream_t s);
+void pcg_p(const Dim &d, double *p, const double *sv, const double *rsn,
+           const double *rs, hipStream_t s);
+
 // ---- banded coarsest solve, mgsolv = 1 (pmgc/buildBd.cc:62-304,
 // pmgc/mlinpckd.cc:167-235), FP64 ----
 // n unknowns (interior points), half-bandwidth m, lda = m + 1, W = row

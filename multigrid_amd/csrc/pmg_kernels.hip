@@ -1231,7 +1231,7 @@ template <class T>
 __global__ __launch_bounds__(CG_RNT) void k_cg_rows(
     int nx, int ny, int nz, const double *__restrict__ ac,
     const double *__restrict__ cc, const T *__restrict__ f,
-    T *__restrict__ xout, int itmax, double errtol) {
+    T *__restrict__ xout, int itmax, double errtol, int warm, int absolute) {
   constexpr int R = CG_R;
   extern __shared__ double P[];
   __shared__ double rbuf[2][CG_NT / 64];  // cg_allsum reads CG_NT / 64
@@ -1256,8 +1256,8 @@ __global__ __launch_bounds__(CG_RNT) void k_cg_rows(
     val[r] = act && i0 + r <= nx - 1;
     const size_t p = ix(nx, ny, val[r] ? i0 + r : 2, j, k);
     pidx[r] = (int)p;
-    xr[r] = 0.0;
-    rr[r] = val[r] ? (double)f[p] : 0.0;  // r = f - A*0
+    xr[r] = warm && val[r] ? (double)xout[p] : 0.0;
+    rr[r] = val[r] ? (double)f[p] : 0.0;  // r = f - A x (x = 0 unless warm)
     apv[r] = 0.0;
     dg[r] = val[r] ? ac[p] + cc[p] : 0.0;
     cg_load_row<27>(ac, n, sy, sz, p, cf[r], off);
@@ -1271,7 +1271,56 @@ __global__ __launch_bounds__(CG_RNT) void k_cg_rows(
 #pragma unroll
   for (int a = 0; a < 9; a++)
     rb[a] = (int)ix(nx, ny, i0 - 1, j + (a % 3) - 1, k + (a / 3) - 1);
+  // offset q of cg_load_row -> (di, dj, dk)
+  constexpr signed char DI[26] = {0, 0, 1, -1, 1, -1, 1, -1, 0, 0, 0, 1, -1,
+                                  1, -1, 1, -1, 0, 0, 0, 1, -1, 1, -1, 1, -1};
+  constexpr signed char DJ[26] = {1, -1, 0, 0, 1, 1, -1, -1, 0, 1, -1, 0, 0,
+                                  1, 1, -1, -1, 0, 1, -1, 0, 0, 1, 1, -1, -1};
+  constexpr signed char DK[26] = {0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1,
+                                  1, 1, 1, 1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
+  // v = A P on this thread's points (P in LDS, complete); pp = P there
+  auto apply = [&](double (&v)[R], double (&pp)[R]) {
+    double o[R], u[R], dd[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) o[r] = u[r] = dd[r] = 0.0;
+#pragma unroll
+    for (int a = 0; a < 9; a++) {
+      const int dj = (a % 3) - 1, dk = (a / 3) - 1;
+      double w[R + 2];
+#pragma unroll
+      for (int c = 0; c < R + 2; c++) w[c] = P[rb[a] + c];
+      if (dj == 0 && dk == 0) {
+#pragma unroll
+        for (int r = 0; r < R; r++) pp[r] = w[r + 1];
+      }
+#pragma unroll
+      for (int q = 0; q < 26; q++) {
+        if (DJ[q] != dj || DK[q] != dk) continue;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const double t = cf[r][q] * w[r + 1 + DI[q]];
+          if (DK[q] == 0) o[r] += t;
+          else if (DK[q] > 0) u[r] += t;
+          else dd[r] += t;
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) v[r] = -o[r] + -u[r] + -dd[r] + dg[r] * pp[r];
+  };
   __syncthreads();
+  if (warm) {  // r = f - A x0
+#pragma unroll
+    for (int r = 0; r < R; r++)
+      if (val[r]) P[pidx[r]] = xr[r];
+    __syncthreads();
+    double v[R], pp[R];
+    apply(v, pp);
+#pragma unroll
+    for (int r = 0; r < R; r++)
+      if (val[r]) rr[r] -= v[r];
+    __syncthreads();
+  }
   double s = 0.0;
 #pragma unroll
   for (int r = 0; r < R; r++) s += rr[r] * rr[r];
@@ -1280,16 +1329,13 @@ __global__ __launch_bounds__(CG_RNT) void k_cg_rows(
   const double denom = sqrt(rhok2);
   int iters = 0;
   double rhok1 = 0.0;
-  // offset q of cg_load_row -> (di, dj, dk)
-  constexpr signed char DI[26] = {0, 0, 1, -1, 1, -1, 1, -1, 0, 0, 0, 1, -1,
-                                  1, -1, 1, -1, 0, 0, 0, 1, -1, 1, -1, 1, -1};
-  constexpr signed char DJ[26] = {1, -1, 0, 0, 1, 1, -1, -1, 0, 1, -1, 0, 0,
-                                  1, 1, -1, -1, 0, 1, -1, 0, 0, 1, 1, -1, -1};
-  constexpr signed char DK[26] = {0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1,
-                                  1, 1, 1, 1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
   if (denom != 0.0) {
     for (;;) {
-      if (errtol == 0.0 ? rhok2 == 0.0 : sqrt(rhok2) / denom <= errtol)
+      // absolute: the class solver's criterion, sqrt(r.r) > errtol, its
+      // first test made on sqrt(sqrt(r0.r0)) (CG_Solver.h:125-133)
+      if (absolute ? (iters == 0 ? sqrt(sqrt(rhok2)) : sqrt(rhok2)) <= errtol
+                   : (errtol == 0.0 ? rhok2 == 0.0
+                                    : sqrt(rhok2) / denom <= errtol))
         break;
       if (iters >= itmax) break;
       if (iters == 0) {
@@ -1304,37 +1350,13 @@ __global__ __launch_bounds__(CG_RNT) void k_cg_rows(
           if (val[r]) P[pidx[r]] = (P[pidx[r]] + ib * rr[r]) * beta;
       }
       __syncthreads();
-      double o[R], u[R], dd[R], pp[R];
-#pragma unroll
-      for (int r = 0; r < R; r++) o[r] = u[r] = dd[r] = 0.0;
-#pragma unroll
-      for (int a = 0; a < 9; a++) {
-        const int dj = (a % 3) - 1, dk = (a / 3) - 1;
-        double w[R + 2];
-#pragma unroll
-        for (int c = 0; c < R + 2; c++) w[c] = P[rb[a] + c];
-        if (dj == 0 && dk == 0) {
-#pragma unroll
-          for (int r = 0; r < R; r++) pp[r] = w[r + 1];
-        }
-#pragma unroll
-        for (int q = 0; q < 26; q++) {
-          if (DJ[q] != dj || DK[q] != dk) continue;
-#pragma unroll
-          for (int r = 0; r < R; r++) {
-            const double t = cf[r][q] * w[r + 1 + DI[q]];
-            if (DK[q] == 0) o[r] += t;
-            else if (DK[q] > 0) u[r] += t;
-            else dd[r] += t;
-          }
-        }
-      }
+      double v[R], pp[R];
+      apply(v, pp);
       s = 0.0;
 #pragma unroll
       for (int r = 0; r < R; r++) {
-        const double v = -o[r] + -u[r] + -dd[r] + dg[r] * pp[r];
-        apv[r] = val[r] ? v : 0.0;
-        if (val[r]) s += pp[r] * v;
+        apv[r] = val[r] ? v[r] : 0.0;
+        if (val[r]) s += pp[r] * v[r];
       }
       const double pAp = cg_allsum(s, rbuf[b], nw);
       b ^= 1;
@@ -1374,7 +1396,7 @@ static void cg_launch(const Dim &d, const double *ac, const double *cc,
   if (ND == 27 && nrow <= CG_RNT) {
     const int nt = (nrow + 63) / 64 * 64;
     k_cg_rows<T><<<1, nt, lds + (CG_R + 2) * sizeof(double), s>>>(
-        d.nx, d.ny, d.nz, ac, cc, f, x, itmax, errtol);
+        d.nx, d.ny, d.nz, ac, cc, f, x, itmax, errtol, 0, 0);
     return;
   }
   // one interior point per thread when they fit: as few wavefronts as the
@@ -1401,6 +1423,103 @@ void cg_single(const Dim &d, int numdia, const double *ac, const double *cc,
   else
     cg_launch<T, 27>(d, ac, cc, f, x, itmax, errtol, s);
   check_launch("cg_single");
+}
+
+bool cg_class_fits(const Dim &d, int numdia) {
+  const int nrow = (int)((d.nx - 2 + CG_R - 1) / CG_R) * (d.ny - 2) * (d.nz - 2);
+  return numdia == 27 && d.n() * sizeof(double) <= CG_LDS_MAX &&
+         nrow <= CG_RNT;
+}
+
+template <class T>
+void cg_class(const Dim &d, int numdia, const double *ac, const double *cc,
+              const T *f, T *x, int itmax, double thresh, hipStream_t s) {
+  if (!cg_class_fits(d, numdia)) {
+    fprintf(stderr, "pmg: coarsest level too large for cg_class\n");
+    return;
+  }
+  const int nrow = (int)((d.nx - 2 + CG_R - 1) / CG_R) * (d.ny - 2) * (d.nz - 2);
+  const int nt = (nrow + 63) / 64 * 64;
+  k_cg_rows<T><<<1, nt, d.n() * sizeof(double) + (CG_R + 2) * sizeof(double),
+                 s>>>(d.nx, d.ny, d.nz, ac, cc, f, x, itmax, thresh, 1, 1);
+  check_launch("cg_class");
+}
+
+// ---------------------------------------------------------------------------
+// Class-style PCG (CG_Solver.h:152-237): dot product and the two vector
+// updates, their coefficients read from device scalars so the loop needs
+// no host round trip.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_dot(int nx, int ny, int nz,
+                                             const double *__restrict__ x,
+                                             const double *__restrict__ y,
+                                             Red rd) {
+  __shared__ double red[4];
+  const int i = 2 + blockIdx.x * BX + threadIdx.x;
+  const int j = 2 + blockIdx.y * BY + threadIdx.y;
+  const int k0 = 2 + blockIdx.z * KCH;
+  double acc = 0.0;
+  if (i <= nx - 1 && j <= ny - 1)
+    for (int kk = 0; kk < KCH; kk++) {
+      const int k = k0 + kk;
+      if (k > nz - 1) break;
+      const size_t p = ix(nx, ny, i, j, k);
+      acc += x[p] * y[p];
+    }
+  const double t = block_sum<256>(acc, red);
+  red_store<1>(rd, t, 0.0);
+}
+
+void dot_interior(const Dim &d, const double *x, const double *y,
+                  const Red &red, hipStream_t s) {
+  dim3 g(cdiv(d.nx - 2, BX), cdiv(d.ny - 2, BY), cdiv(d.nz - 2, KCH));
+  k_dot<<<g, dim3(BX, BY), 0, s>>>(d.nx, d.ny, d.nz, x, y, red);
+  red_final<1, 0>(red, g, s);
+  check_launch("dot_interior");
+}
+
+// x += a p, r -= a q with a = rs / pap (0 when pap == 0), interior
+__global__ __launch_bounds__(256) void k_pcg_xr(
+    int nx, int ny, int nz, double *__restrict__ x, double *__restrict__ r,
+    const double *__restrict__ p, const double *__restrict__ q,
+    const double *__restrict__ rs, const double *__restrict__ pap) {
+  const int i = 2 + blockIdx.x * BX + threadIdx.x;
+  const int j = 2 + blockIdx.y * BY + threadIdx.y;
+  const int k = 2 + blockIdx.z;
+  if (i > nx - 1 || j > ny - 1) return;
+  const double a = pap[0] == 0.0 ? 0.0 : rs[0] / pap[0];
+  const size_t o = ix(nx, ny, i, j, k);
+  x[o] += a * p[o];
+  r[o] -= a * q[o];
+}
+
+// p = s + b p with b = rsn / rs (0 when rs == 0), interior
+__global__ __launch_bounds__(256) void k_pcg_p(
+    int nx, int ny, int nz, double *__restrict__ p,
+    const double *__restrict__ sv, const double *__restrict__ rsn,
+    const double *__restrict__ rs) {
+  const int i = 2 + blockIdx.x * BX + threadIdx.x;
+  const int j = 2 + blockIdx.y * BY + threadIdx.y;
+  const int k = 2 + blockIdx.z;
+  if (i > nx - 1 || j > ny - 1) return;
+  const double b = rs[0] == 0.0 ? 0.0 : rsn[0] / rs[0];
+  const size_t o = ix(nx, ny, i, j, k);
+  p[o] = sv[o] + b * p[o];
+}
+
+void pcg_xr(const Dim &d, double *x, double *r, const double *p,
+            const double *q, const double *rs, const double *pap,
+            hipStream_t s) {
+  dim3 g(cdiv(d.nx - 2, BX), cdiv(d.ny - 2, BY), d.nz - 2);
+  k_pcg_xr<<<g, dim3(BX, BY), 0, s>>>(d.nx, d.ny, d.nz, x, r, p, q, rs, pap);
+  check_launch("pcg_xr");
+}
+
+void pcg_p(const Dim &d, double *p, const double *sv, const double *rsn,
+           const double *rs, hipStream_t s) {
+  dim3 g(cdiv(d.nx - 2, BX), cdiv(d.ny - 2, BY), d.nz - 2);
+  k_pcg_p<<<g, dim3(BX, BY), 0, s>>>(d.nx, d.ny, d.nz, p, sv, rsn, rs);
+  check_launch("pcg_p");
 }
 
 // ---------------------------------------------------------------------------
@@ -1897,6 +2016,8 @@ void buildG(const Dim &f, const Dim &c, int numdia, const double *pc,
   template void cg_single<T>(const Dim &, int, const double *,                \
                              const double *, const T *, T *, int, double,     \
                              hipStream_t);                                    \
+  template void cg_class<T>(const Dim &, int, const double *, const double *, \
+                            const T *, T *, int, double, hipStream_t);        \
   template void fill<T>(T *, size_t, T, hipStream_t);                         \
   template void axpy_interior<T>(const Dim &, double, const T *, T *,         \
                                  hipStream_t);                                \

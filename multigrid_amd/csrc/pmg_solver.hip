@@ -931,6 +931,165 @@ int ctx_newton(pmg_ctx *c, int mode) {
   return c->ierror ? PMG_ITMAX : PMG_OK;
 }
 
+// ---- class-style solver (PBE_pmgc_class.h:1070-1133, CG_Solver.h:45-242)
+// CG preconditioned by the class V-cycle, on this context's hierarchy.
+
+enum { S_PCG_RS = 6, S_PCG_PAP = 8, S_PCG_RN1 = 9 };  // RS: 6 and 7 alternate
+
+// x_fine = M rhs: per level two red-black sweeps from zero (iadjoint 0),
+// the defect restricted, the recursion, the Hackbusch-Reusken factor of the
+// coarse solution folded into the prolongation (the same partial sums and
+// k_interp as the PMG cycle), two sweeps again with iadjoint 0.  The
+// coarsest level runs the class CG from its previous solution.
+template <class T>
+void class_vcycle(pmg_ctx *c, LV<T> *L, const T *frhs) {
+  const int nlev = c->nlev;
+  hipStream_t s = c->s;
+  for (int l = 0; l < nlev - 1; l++) {
+    const T *rhs = l == 0 ? frhs : L[l].w0;
+    smooth<T>(c, L[l], rhs, L[l].x, 2, 0, false, true);
+    resid<T, T>(L[l].d, L[l].numdia, L[l].ac, L[l].cc, rhs, L[l].x, L[l].r,
+                nullptr, s);
+    restrict_<T>(L[l].d, L[l + 1].d, L[l].r, L[l + 1].w0, L[l + 1].pc, s);
+  }
+  {
+    const Level &C = c->L[nlev - 1];
+    const int ni = (int)C.d.interior();
+    cg_class<T>(C.d, C.numdia, C.ac, C.cc, L[nlev - 1].w0, L[nlev - 1].x,
+                std::max(1000, 4 * ni), 1e-8, s);
+  }
+  for (int l = nlev - 2; l >= 0; l--) {
+    const int cl = l + 1;
+    const unsigned nb =
+        matvec_dots_part<T>(L[cl].d, L[cl].numdia, L[cl].ac, L[cl].cc,
+                            L[cl].x, L[cl].w0, L[cl].t, c->part, s);
+    interp_damped<T>(L[cl].d, L[l].d, L[cl].x, L[cl].pc, L[l].x, c->part, nb,
+                     s);
+    smooth<T>(c, L[l], l == 0 ? frhs : L[l].w0, L[l].x, 2, 0, false);
+  }
+}
+
+// Preconditioner application s = M r (FP64 r and s; PMG_MIXED runs the
+// cycle in FP32).
+void pcg_precond(pmg_ctx *c, int mode, LV<double> *L64, LV<float> *L32,
+                 const double *r, double *sv) {
+  Level &F = c->L[0];
+  const size_t nf = F.d.n();
+  if (mode == PMG_MIXED) {
+    convert<double, float>(r, F.w032, nf, c->s);
+    class_vcycle<float>(c, L32, F.w032);
+    convert<float, double>(F.x32, sv, nf, c->s);
+  } else {
+    class_vcycle<double>(c, L64, r);  // result in F.x == sv
+  }
+}
+
+// num_iters > 0: exactly that many CG iterations (counting_criterion,
+// CG_Solver.h:38-43); 0: until ||r||_1 / ||f||_1 <= errtol, or itmax.
+int ctx_pcg(pmg_ctx *c, int mode, int num_iters) {
+  const Params &p = c->p;
+  if (c->nlev < 2 || num_iters < 0) return PMG_EBADPARM;
+  const Level &C = c->L[c->nlev - 1];
+  if (!cg_class_fits(C.d, C.numdia)) return PMG_EBADPARM;
+  if (mode == PMG_MIXED) {
+    int rc = alloc_fp32(c);
+    if (rc) return rc;
+    if (!c->ops32_ok) convert_ops32(c);
+    c->ops32_ok = true;
+  }
+  Level &F = c->L[0];
+  const Dim d = F.d;
+  const size_t nf = d.n();
+  if (!c->pcg[0]) {
+    for (int q = 0; q < 4; q++) c->pcg[q] = c->alloc<double>(nf);
+    if (!c->ok) return PMG_ENOMEM;
+  }
+  hipStream_t s = c->s;
+  double *r = c->pcg[0], *pv = c->pcg[1], *q = c->pcg[2];
+  double *sv = mode == PMG_MIXED ? c->pcg[3] : F.x;
+  LV<double> L64[MAXLEV];
+  LV<float> L32[MAXLEV];
+  for (int l = 0; l < c->nlev; l++) {
+    L64[l] = view<double>(c->L[l]);
+    if (mode == PMG_MIXED) L32[l] = view<float>(c->L[l]);
+  }
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  hipEventRecord(e0, s);
+  // x0 = 0; the coarsest solution, which the class CG starts from, too
+  fill<double>(c->u, nf, 0.0, s);
+  if (mode == PMG_MIXED)
+    fill<float>(c->L[c->nlev - 1].x32, C.d.n(), 0.0f, s);
+  else
+    fill<double>(c->L[c->nlev - 1].x, C.d.n(), 0.0, s);
+  const Red rden = red_to(c, S_RSDEN), rn1 = red_to(c, S_PCG_RN1),
+            rpap = red_to(c, S_PCG_PAP);
+  nrm1<double>(d, F.fc, rden, s);
+  double rsden = host_scalar(c, S_RSDEN);
+  if (rsden == 0.0) rsden = 1.0;
+  // r = f - A x0, s = M r, p = s, rs = r.s, pap = p.A p
+  resid<double, double>(d, F.numdia, F.ac, F.cc, F.fc, c->u, r, nullptr, s);
+  pcg_precond(c, mode, L64, L32, r, sv);
+  copy_interior(d, sv, pv, s);
+  dot_interior(d, r, sv, red_to(c, S_PCG_RS), s);
+  matvec_dots<double>(d, F.numdia, F.ac, F.cc, pv, pv, q, nullptr, s);
+  dot_interior(d, pv, q, rpap, s);
+  nrm1<double>(d, r, rn1, s);
+  hipMemcpyAsync(c->hscal + S_PCG_PAP, c->scal + S_PCG_PAP,
+                 2 * sizeof(double), hipMemcpyDeviceToHost, s);
+  hipStreamSynchronize(s);
+  const double pap0 = c->hscal[S_PCG_PAP];
+  double rel = c->hscal[S_PCG_RN1] / rsden;
+  int iters = 0, ierr = 0;
+  if (pap0 != 0.0) {  // CG_Solver.h:124-125
+    for (;;) {
+      if (num_iters > 0) {
+        if (iters >= num_iters) break;
+      } else {
+        if (rel <= p.errtol) break;
+        if (iters >= p.itmax) {
+          ierr = 1;
+          break;
+        }
+      }
+      const double *rs_old = c->scal + S_PCG_RS + (iters & 1);
+      double *rs_new = c->scal + S_PCG_RS + ((iters + 1) & 1);
+      iters++;
+      pcg_xr(d, c->u, r, pv, q, rs_old, c->scal + S_PCG_PAP, s);
+      pcg_precond(c, mode, L64, L32, r, sv);
+      Red rrs = red_to(c, S_PCG_RS);
+      rrs.out = rs_new;
+      dot_interior(d, r, sv, rrs, s);
+      pcg_p(d, pv, sv, rs_new, rs_old, s);
+      matvec_dots<double>(d, F.numdia, F.ac, F.cc, pv, pv, q, nullptr, s);
+      dot_interior(d, pv, q, rpap, s);
+      if (num_iters == 0) {
+        nrm1<double>(d, r, rn1, s);
+        rel = host_scalar(c, S_PCG_RN1) / rsden;
+      }
+    }
+  }
+  // the reported residual: that of the final u
+  const Red rnrm = red_to(c, S_RSNRM);
+  resid<double, double>(d, F.numdia, F.ac, F.cc, F.fc, c->u, (double *)nullptr,
+                        &rnrm, s);
+  const double rsnrm = host_scalar(c, S_RSNRM);
+  fbound_dirichlet(d, c->u, c->gx, c->gy, c->gz, s);
+  hipEventRecord(e1, s);
+  hipError_t se = hipStreamSynchronize(s);
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, e0, e1);
+  c->solve_ms = ms;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (se != hipSuccess || hipGetLastError() != hipSuccess) return PMG_EDEVICE;
+  c->iters = iters;
+  c->relres = rsnrm / rsden;
+  c->ierror = ierr;
+  return ierr ? PMG_ITMAX : PMG_OK;
+}
+
 // Fill iwork/rwork as the reference leaves them (mgdrvd.cc:145-166,
 // mgsubd.cc:266-373): iz, ipc per level; rpc | cc | fc | pc | ac.
 int export_work(pmg_ctx *c, int *iwork, double *rwork, bool newton) {
@@ -1019,11 +1178,14 @@ int check_sizes(const Params &p, bool newton) {
   return PMG_OK;
 }
 
+// kind 0: Vmgdriv, 1: Vnewdriv, 2: the class-style PCG (num_iters as
+// ctx_pcg)
 int driver(int *iparm, double *rparm, int *iwork, double *rwork, double *u,
            const double *xf, const double *yf, const double *zf,
            const double *gxcf, const double *gycf, const double *gzcf,
            const double *a1cf, const double *a2cf, const double *a3cf,
-           const double *ccf, const double *fcf, bool newton) {
+           const double *ccf, const double *fcf, int kind, int num_iters = 0) {
+  const bool newton = kind == 1;
   t_iters = 0, t_ierror = 0, t_relres = 0.0;
   if (!iparm || !rparm || !iwork || !rwork || !u || !xf || !yf || !zf ||
       !gxcf || !gycf || !gzcf || !a1cf || !a2cf || !a3cf || !ccf || !fcf)
@@ -1032,6 +1194,7 @@ int driver(int *iparm, double *rparm, int *iwork, double *rwork, double *u,
   int rc = decode(iparm, rparm, p);
   if (rc) return rc;
   if (!newton && p.mode != 0) return PMG_EBADPARM;  // FAS not provided
+  if (kind == 2 && (p.nlev < 2 || num_iters < 0)) return PMG_EBADPARM;
   if (newton && p.mgkey != 0) return PMG_EBADPARM;  // Vfnewton not provided
   rc = check_sizes(p, newton);
   if (rc) return rc;
@@ -1039,7 +1202,9 @@ int driver(int *iparm, double *rparm, int *iwork, double *rwork, double *u,
   rc = ctx_create(&c, p, xf, yf, zf, gxcf, gycf, gzcf, a1cf, a2cf, a3cf, ccf,
                   fcf);
   if (rc) return rc;
-  int src = newton ? ctx_newton(c, PMG_FP64) : ctx_solve(c, PMG_FP64);
+  int src = kind == 1   ? ctx_newton(c, PMG_FP64)
+            : kind == 2 ? ctx_pcg(c, PMG_FP64, num_iters)
+                        : ctx_solve(c, PMG_FP64);
   if (src == PMG_OK || src == PMG_ITMAX) {
     hipMemcpyAsync(u, c->u, c->L[0].d.n() * sizeof(double),
                    hipMemcpyDeviceToHost, c->s);
@@ -1113,7 +1278,23 @@ int pmg_mgdriv(int *iparm, double *rparm, int *iwork, double *rwork,
                double *fcf, double *tcf) {
   (void)tcf;
   return driver(iparm, rparm, iwork, rwork, u, xf, yf, zf, gxcf, gycf, gzcf,
-                a1cf, a2cf, a3cf, ccf, fcf, false);
+                a1cf, a2cf, a3cf, ccf, fcf, 0);
+}
+
+int pmg_pcgdriv(int *iparm, double *rparm, int *iwork, double *rwork,
+                double *u, const double *xf, const double *yf,
+                const double *zf, const double *gxcf, const double *gycf,
+                const double *gzcf, double *a1cf, double *a2cf, double *a3cf,
+                double *ccf, double *fcf, double *tcf, int num_iters) {
+  (void)tcf;
+  return driver(iparm, rparm, iwork, rwork, u, xf, yf, zf, gxcf, gycf, gzcf,
+                a1cf, a2cf, a3cf, ccf, fcf, 2, num_iters);
+}
+
+int pmg_ctx_pcg(pmg_ctx *ctx, int mode, int num_iters) {
+  if (!ctx || (mode != PMG_FP64 && mode != PMG_MIXED) || ctx->p.mode != 0)
+    return PMG_EBADPARM;
+  return ctx_pcg(ctx, mode, num_iters);
 }
 
 int pmg_newdriv(int *iparm, double *rparm, int *iwork, double *rwork,
@@ -1123,7 +1304,7 @@ int pmg_newdriv(int *iparm, double *rparm, int *iwork, double *rwork,
                 double *ccf, double *fcf, double *tcf) {
   (void)tcf;
   return driver(iparm, rparm, iwork, rwork, u, xf, yf, zf, gxcf, gycf, gzcf,
-                a1cf, a2cf, a3cf, ccf, fcf, true);
+                a1cf, a2cf, a3cf, ccf, fcf, 1);
 }
 
 int pmg_mypdefinit(int nion, const double *charge, const double *sconc) {

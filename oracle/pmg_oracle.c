@@ -1317,6 +1317,173 @@ int pmgo_mgdriv(int *iparm, double *rparm, int *iwork, double *rwork,
   return ierr;
 }
 
+/* ------------------------------------------------------------------------ */
+/* Class-style solver: CG_solver (CG_Solver.h:45-242) preconditioned by the   */
+/* class V-cycle v_cycle<false, nlev, true> (PBE_pmgc_class.h:1070-1133),     */
+/* run here on the PMG hierarchy built by setup().                            */
+/* ------------------------------------------------------------------------ */
+
+/* Coarsest solve of the class V-cycle: Solver_CG with thresh_criterion(1e-8)
+ * (PBE_pmgc_class.h:852-860, CG_Solver.h:258-276), started from x as it is
+ * (the previous call's coarse solution).  The criterion is absolute on
+ * sqrt(r.r); its first test is made on sqrt(sqrt(r0.r0)) (CG_Solver.h:
+ * 125-133).  itmax caps the loop (the reference has no cap). */
+int pmgo_cg_class(int nx, int ny, int nz, int numdia, const double *ac,
+                  const double *cc, const double *f, double *x, double *p,
+                  double *ap, double *r, double thresh, int itmax) {
+  pmgo_mresid(nx, ny, nz, numdia, ac, cc, f, x, r);
+  pmgo_xcopy(nx, ny, nz, r, p);
+  double rs = pmgo_xdot(nx, ny, nz, r, r);
+  pmgo_matvec(nx, ny, nz, numdia, ac, cc, p, ap);
+  double pap = pmgo_xdot(nx, ny, nz, p, ap);
+  if (pap == 0.0) return 0;
+  double alpha = rs / pap;
+  double residual = sqrt(sqrt(rs));
+  int it = 0;
+  while (residual > thresh && it < itmax) {
+    it++;
+    pmgo_xaxpy(nx, ny, nz, alpha, p, x);
+    pmgo_xaxpy(nx, ny, nz, -alpha, ap, r);
+    double rsn = pmgo_xdot(nx, ny, nz, r, r);
+    double beta = rs == 0.0 ? 0.0 : rsn / rs;
+    rs = rsn;
+    xscal(nx, ny, nz, beta, p); /* p = r + beta p */
+    pmgo_xaxpy(nx, ny, nz, 1.0, r, p);
+    pmgo_matvec(nx, ny, nz, numdia, ac, cc, p, ap);
+    pap = pmgo_xdot(nx, ny, nz, p, ap);
+    alpha = pap == 0.0 ? 0.0 : rs / pap;
+    residual = sqrt(rs);
+  }
+  return it;
+}
+
+/* One application of the class V-cycle as a preconditioner: x[0] = M rhs[0].
+ * Per level (PBE_pmgc_class.h:1077-1113): two red-black sweeps from zero
+ * (iadjoint 0), the defect restricted to the next level's right-hand side,
+ * the recursion, the Hackbusch-Reusken factor (r.x)/(x.Ax) of the coarse
+ * solution (MultigridDomain.h:329-353; 0 for a zero denominator), the
+ * prolongation added with that factor, two sweeps again with iadjoint 0.
+ * The coarsest level is pmgo_cg_class on its warm x. */
+static void class_vcycle(const hier_t *h, int l, double **x, double **rhs,
+                         double **ccl, double *w1, double *w2, double *w3) {
+  int nx = h->nx[l], ny = h->ny[l], nz = h->nz[l];
+  if (l == h->nlev - 1) {
+    int ni = (nx - 2) * (ny - 2) * (nz - 2);
+    pmgo_cg_class(nx, ny, nz, h->numdia[l], h->ac[l], ccl[l], rhs[l], x[l],
+                  w1, w2, w3, 1e-8, ni * 4 > 1000 ? ni * 4 : 1000);
+    return;
+  }
+  memset(x[l], 0, h->n[l] * sizeof(double));
+  pmgo_gsrb(nx, ny, nz, h->numdia[l], h->ac[l], ccl[l], rhs[l], x[l], 2, 0);
+  pmgo_mresid(nx, ny, nz, h->numdia[l], h->ac[l], ccl[l], rhs[l], x[l], w1);
+  pmgo_restrc(nx, ny, nz, h->nx[l + 1], h->ny[l + 1], h->nz[l + 1], w1,
+              rhs[l + 1], h->pc[l]);
+  class_vcycle(h, l + 1, x, rhs, ccl, w1, w2, w3);
+  int cl = l + 1;
+  pmgo_matvec(h->nx[cl], h->ny[cl], h->nz[cl], h->numdia[cl], h->ac[cl],
+              ccl[cl], x[cl], w2);
+  double up = pmgo_xdot(h->nx[cl], h->ny[cl], h->nz[cl], rhs[cl], x[cl]);
+  double lo = pmgo_xdot(h->nx[cl], h->ny[cl], h->nz[cl], x[cl], w2);
+  double lambda = lo != 0.0 ? up / lo : 0.0;
+  pmgo_interp(h->nx[cl], h->ny[cl], h->nz[cl], nx, ny, nz, x[cl], w1,
+              h->pc[l]);
+  pmgo_xaxpy(nx, ny, nz, lambda, w1, x[l]);
+  pmgo_gsrb(nx, ny, nz, h->numdia[l], h->ac[l], ccl[l], rhs[l], x[l], 2, 0);
+}
+
+int pmgo_pcgdriv(int *iparm, double *rparm, int *iwork, double *rwork,
+                 double *u, const double *xf, const double *yf,
+                 const double *zf, const double *gxcf, const double *gycf,
+                 const double *gzcf, double *a1cf, double *a2cf, double *a3cf,
+                 double *ccf, double *fcf, double *tcf, int num_iters) {
+  /* CG_solver (CG_Solver.h:45-242) with the class V-cycle as the
+   * preconditioner (PBE_pmgc_class.h:1116-1133), on the Vmgdriv hierarchy:
+   * x0 = 0, r = f - A x, s = M r, p = s, then per iteration
+   * x += a p, r -= a A p, s = M r, b = (r.s)_new / (r.s), p = s + b p,
+   * a = (r.s) / (p.A p).  num_iters > 0: counting_criterion (exactly that
+   * many iterations, as PBE_pmgc_class_with_non_zero_IS.cc:129-136);
+   * num_iters == 0: iterate until ||r||_1 / ||f||_1 <= errtol or itmax
+   * iterations.  Returns 0, 1 (itmax, tolerance mode) or 2 (bad iparm). */
+  params_t p;
+  if (decode(iparm, rparm, &p)) return 2;
+  if (p.mode != 0 || p.nlev < 2) return 2;
+  hier_t h;
+  double *ccb, *fcb;
+  int rc = setup(&p, iwork, rwork, 0, xf, yf, zf, gxcf, gycf, gzcf, a1cf,
+                 a2cf, a3cf, ccf, fcf, &h, &ccb, &fcb, 0, 0);
+  if (rc) return rc;
+  size_t narr = 0;
+  for (int l = 0; l < h.nlev; l++) narr += h.n[l];
+  size_t nf = h.n[0];
+  int nx = h.nx[0], ny = h.ny[0], nz = h.nz[0];
+  double *xb = (double *)calloc(narr, sizeof(double));
+  double *rhsb = (double *)calloc(narr, sizeof(double));
+  double *w1 = (double *)calloc(nf, sizeof(double));
+  double *w2 = (double *)calloc(nf, sizeof(double));
+  double *w3 = (double *)calloc(nf, sizeof(double));
+  double *r = (double *)calloc(nf, sizeof(double));
+  double *pv = (double *)calloc(nf, sizeof(double));
+  double *q = (double *)calloc(nf, sizeof(double));
+  double *x[32], *rhs[32], *ccl[32];
+  level_ptrs(&h, xb, x);
+  level_ptrs(&h, rhsb, rhs);
+  for (int l = 0; l < h.nlev; l++) ccl[l] = h.cc[l];
+  const double *fc1 = h.fc[0];
+  memset(u, 0, nf * sizeof(double));
+  double rsden = pmgo_xnrm1(nx, ny, nz, fc1);
+  if (rsden == 0.0) rsden = 1.0;
+  /* r = f - A x0; s = M r (the preconditioner's rhs is r, its result x[0]) */
+  pmgo_mresid(nx, ny, nz, 7, h.ac[0], h.cc[0], fc1, u, r);
+  pmgo_xcopy(nx, ny, nz, r, rhs[0]);
+  class_vcycle(&h, 0, x, rhs, ccl, w1, w2, w3);
+  pmgo_xcopy(nx, ny, nz, x[0], pv);
+  double rs = pmgo_xdot(nx, ny, nz, r, x[0]);
+  pmgo_matvec(nx, ny, nz, 7, h.ac[0], h.cc[0], pv, q);
+  double pap = pmgo_xdot(nx, ny, nz, pv, q);
+  int iters = 0, ierr = 0;
+  g_nhist = 0;
+  double rel = pmgo_xnrm1(nx, ny, nz, r) / rsden;
+  if (pap != 0.0) {
+    double alpha = rs / pap;
+    for (;;) {
+      if (num_iters > 0) {
+        if (iters >= num_iters) break;
+      } else {
+        if (rel <= p.errtol) break;
+        if (iters >= p.itmax) {
+          ierr = 1;
+          break;
+        }
+      }
+      iters++;
+      pmgo_xaxpy(nx, ny, nz, alpha, pv, u);
+      pmgo_xaxpy(nx, ny, nz, -alpha, q, r);
+      pmgo_xcopy(nx, ny, nz, r, rhs[0]);
+      class_vcycle(&h, 0, x, rhs, ccl, w1, w2, w3);
+      double rsn = pmgo_xdot(nx, ny, nz, r, x[0]);
+      double beta = rs == 0.0 ? 0.0 : rsn / rs;
+      rs = rsn;
+      xscal(nx, ny, nz, beta, pv); /* p = s + beta p */
+      pmgo_xaxpy(nx, ny, nz, 1.0, x[0], pv);
+      pmgo_matvec(nx, ny, nz, 7, h.ac[0], h.cc[0], pv, q);
+      pap = pmgo_xdot(nx, ny, nz, pv, q);
+      alpha = pap == 0.0 ? 0.0 : rs / pap;
+      rel = pmgo_xnrm1(nx, ny, nz, r) / rsden;
+      if (g_nhist < HIST_CAP) g_hist[g_nhist++] = rel;
+    }
+  }
+  /* the stopping measure reported: the true residual of the final u */
+  pmgo_mresid(nx, ny, nz, 7, h.ac[0], h.cc[0], fc1, u, w1);
+  g_last_relres = pmgo_xnrm1(nx, ny, nz, w1) / rsden;
+  pmgo_fbound(1, nx, ny, nz, u, gxcf, gycf, gzcf);
+  g_last_iters = iters;
+  g_last_ierror = ierr;
+  free(xb), free(rhsb), free(w1), free(w2), free(w3), free(r), free(pv),
+      free(q);
+  (void)a1cf, (void)a2cf, (void)a3cf, (void)ccf, (void)fcf, (void)tcf;
+  return ierr;
+}
+
 /* Vnewton (newtond.cc:140-486) with Vgetjac (:488-531). */
 int pmgo_newdriv(int *iparm, double *rparm, int *iwork, double *rwork,
                  double *u, const double *xf, const double *yf,

@@ -143,6 +143,22 @@ int pmgo_newdriv(int *iparm, double *rparm, int *iwork, double *rwork,
                  const double *zf, const double *gxcf, const double *gycf,
                  const double *gzcf, double *a1cf, double *a2cf, double *a3cf,
                  double *ccf, double *fcf, double *tcf);
+/* Class-style solver (PBE_pmgc_class.h:1070-1133, CG_Solver.h:45-242): CG
+ * preconditioned by the class V-cycle, on the Vmgdriv hierarchy; see
+ * pmg_oracle.c.  num_iters > 0 runs that many CG iterations, 0 runs to
+ * errtol (L1 residual, relative) or itmax.  pmgo_cg_class: the coarsest
+ * solve (absolute 1e-8 criterion, warm start).  Pinned: identity-free
+ * checks in tests/test_oracle_pcg.py (CG on a dense SPD system, the
+ * V-cycle preconditioner is linear, PCG converges to the dense solution). */
+int pmgo_pcgdriv(int *iparm, double *rparm, int *iwork, double *rwork,
+                 double *u, const double *xf, const double *yf,
+                 const double *zf, const double *gxcf, const double *gycf,
+                 const double *gzcf, double *a1cf, double *a2cf, double *a3cf,
+                 double *ccf, double *fcf, double *tcf, int num_iters);
+int pmgo_cg_class(int nx, int ny, int nz, int numdia, const double *ac,
+                  const double *cc, const double *f, double *x, double *p,
+                  double *ap, double *r, double thresh, int itmax);
+
 /* iters / ierror / relative residual of the last driver call, plus the
  * per-cycle relative residual history (up to hist_cap entries). */
 int pmgo_last_stats(int *iters, int *ierror, double *relres, double *hist,

@@ -46,6 +46,8 @@ def olib():
             "pmgo_dpbfa": (None, [dp, i, i, i, ip]),
             "pmgo_dpbsl": (None, [dp, i, i, i, dp]),
             "pmgo_bandsolve": (None, [i, i, i, dp, ip, dp, dp, dp]),
+            "pmgo_pcgdriv": (i, [ip, dp, ip, dp, dp] + [dp] * 12 + [i]),
+            "pmgo_cg_class": (i, [i] * 4 + [dp] * 7 + [d, i]),
             "pmgo_mypdefinit": (None, [i, dp, dp]),
             "pmgo_c_vec": (None, [dp, dp, dp, i, i, i]),
             "pmgo_dc_vec": (None, [dp, dp, dp, i, i, i]),
@@ -236,6 +238,29 @@ def solve(inp, n, nlev, driver=None, nonlin=False, **kw):
 
 PBE_KEYS = ("xf", "yf", "zf", "gxcf", "gycf", "gzcf", "a1cf", "a2cf", "a3cf",
             "ccf", "fcf", "tcf")
+
+
+def pcg(inp, n, nlev, num_iters=0, **kw):
+    """Class-style PCG (oracle) on inputs; returns (ierror, u, iwork, rwork)."""
+    import multigrid_amd as mg
+
+    iparm, rparm = mg.make_params(n, nlev, **kw)
+    sz = mgsz(2, 0, int(iparm[20]), n, n, n, nlev)[1]
+    iw = np.zeros(int(sz[11]), np.int32)
+    rw = np.zeros(int(iparm[0]))
+    u = np.zeros(n ** 3)
+    a = {k: np.ascontiguousarray(inp[k], np.float64).copy() for k in PBE_KEYS}
+    ierr = olib().pmgo_pcgdriv(ip(iparm), dp(rparm), ip(iw), dp(rw), dp(u),
+                               *(dp(a[k]) for k in PBE_KEYS), int(num_iters))
+    return ierr, u, iw, rw
+
+
+def cg_class(n, numdia, ac, cc, f, x0, thresh=1e-8, itmax=100000):
+    x = np.ascontiguousarray(x0, np.float64).copy()
+    wk = [np.zeros(n ** 3) for _ in range(3)]
+    it = olib().pmgo_cg_class(n, n, n, numdia, dp(ac), dp(cc), dp(np.ascontiguousarray(f)),
+                              dp(x), dp(wk[0]), dp(wk[1]), dp(wk[2]), thresh, itmax)
+    return it, x
 
 
 def pbe_constants(ionic):

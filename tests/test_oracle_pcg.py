@@ -1,0 +1,73 @@
+"""Pins of the oracle's class-style solver: CG_solver (CG_Solver.h:45-242)
+preconditioned by the class V-cycle (PBE_pmgc_class.h:1070-1133), run on
+the PMG hierarchy.  Checks: the coarsest CG (absolute 1e-8 criterion, warm
+start, first test on sqrt(sqrt(r.r)), CG_Solver.h:125-133) against a dense
+solve; the whole PCG against a dense solve of the fine system on a tiny
+grid; agreement with the PMG driver at the same tolerance."""
+import numpy as np
+import pytest
+
+from tests import helpers as H
+
+
+def test_cg_class_reaches_absolute_threshold(orc):
+    n = 7
+    rng = np.random.default_rng(2)
+    ac, cc = H.random_op(n, 27, rng)
+    f = H.from_int(n, rng.standard_normal((n - 2) ** 3))
+    A, _ = H.dense_A(n, 27, ac, cc)
+    ref = np.linalg.solve(A, H.to_int(n, f))
+    it, x = orc.cg_class(n, 27, ac, cc, f, np.zeros(n ** 3))
+    r = H.to_int(n, f) - A @ H.to_int(n, x)
+    assert it > 0 and np.sqrt(r @ r) <= 1e-8
+    assert np.abs(H.to_int(n, x) - ref).max() <= 1e-8 * np.abs(ref).max()
+    # warm start from a nearly exact guess: ||r0|| ~ 1e-14 is below the
+    # threshold, but the first test is made on its square root
+    x0 = H.from_int(n, ref)
+    it, x = orc.cg_class(n, 27, ac, cc, f, x0)
+    assert it == 1
+    # an exactly zero residual runs no iteration (p.Ap = 0 returns at once)
+    it, x = orc.cg_class(n, 27, ac, cc, np.zeros(n ** 3), np.zeros(n ** 3))
+    assert it == 0 and np.all(x == 0)
+
+
+def test_pcg_converges_to_dense_solution(orc):
+    """9^3, 2 levels: enough PCG iterations give the dense solution of the
+    fine 7-point system (boundary values folded into fc by VbuildA_fv)."""
+    n = 9
+    inp = orc.pbe_assemble(n, 1.0, 0.15, np_atoms())
+    ierr, u, iw, rw = orc.pcg(inp, n, 2, num_iters=40)
+    sz = orc.mgsz(2, 0, 0, n, n, n, 2)[1]
+    nrpc, narr = int(sz[7]), int(sz[5])
+    N = n ** 3
+    cc = rw[nrpc: nrpc + N]
+    fc = rw[nrpc + narr: nrpc + narr + N]
+    ac = rw[nrpc + 2 * narr + 27 * int(sz[6]):][:4 * N]
+    A, _ = H.dense_A(n, 7, ac, cc)
+    ref = np.linalg.solve(A, H.to_int(n, fc))
+    assert ierr == 0
+    assert np.abs(H.to_int(n, u) - ref).max() <= 1e-10 * np.abs(ref).max()
+
+
+def np_atoms():
+    import multigrid_amd as mg
+    return mg.pbe_molecule(6, 2.5, 0, 3)
+
+
+@pytest.mark.parametrize("n,nlev", [(33, 3), (65, 4)])
+def test_pcg_agrees_with_pmg(orc, n, nlev):
+    """To a relative L1 residual of 1e-6 the class-style PCG and the PMG
+    V-cycle driver give the same potential to the tolerance, the PCG in
+    fewer outer iterations than PMG has V-cycles (each costs one V-cycle)."""
+    c, inp = orc.config_inputs("c0", n=n, natoms=40 if n == 33 else 200,
+                               sphere_r=6.0 if n == 33 else 14.0)
+    e0, u0, _, _ = orc.solve(inp, n, nlev)
+    it0 = orc.last_stats()[0]
+    e1, u1, _, _ = orc.pcg(inp, n, nlev, num_iters=0)
+    it1, ie1, rr1, hist = orc.last_stats()
+    assert e0 == e1 == 0 and rr1 <= 1e-6
+    assert it1 <= it0
+    assert np.abs(u1 - u0).max() <= 1e-5 * np.abs(u0).max()
+    # counting mode runs exactly the number of iterations asked for
+    e2, u2, _, _ = orc.pcg(inp, n, nlev, num_iters=3)
+    assert orc.last_stats()[0] == 3
