@@ -38,9 +38,10 @@ def parse():
                     help="solves in flight at once in the serving loop")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the 289^3 mixed-precision and 129^3 Newton legs")
-    ap.add_argument("--legs", default="large,newton,band",
+    ap.add_argument("--legs", default="large,newton,band,pcg",
                     help="extra legs to run: large (289^3 mixed), newton (129^3), "
-                         "band (161^3 with the banded coarsest solve)")
+                         "band (161^3 with the banded coarsest solve), pcg (161^3 "
+                         "with the class-style PCG)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (a reference "
                          "kernel of known traffic for hardware-counter runs)")
@@ -289,6 +290,50 @@ def band_leg(mg, cpu, steps=5):
     return out
 
 
+def pcg_leg(mg, cpu, steps=5):
+    """The 161^3 workload solved by the class-style solver (CG preconditioned
+    by the class V-cycle, PBE_pmgc_class.h:1070-1133 + CG_Solver.h:45-242) on
+    the same hierarchy, to the same 1e-6 relative L1 residual: FP64 and with
+    the FP32 V-cycle preconditioner; with the CPU leg, the error against the
+    oracle's class-style solve."""
+    cfg = mg.CONFIGS["c1"]
+    n, nlev = cfg["n"], cfg["nlev"]
+    atoms = mg.pbe_molecule(cfg["natoms"], cfg["sphere_r"], cfg["charge_mode"], cfg["seed"])
+    mol = (cfg["h"], cfg["ionic"], atoms)
+    iparm, rparm = mg.make_params(n, nlev)
+    ctx = mg.Context(iparm, rparm, pbe=mol)
+    out = {"workload": f"linearised PBE {n}^3 h={cfg['h']} nlev={nlev} {cfg['natoms']} atoms, "
+                       "class-style PCG to 1e-6"}
+    us = {}
+    for name, mode in (("fp64", mg.PMG_FP64), ("mixed", mg.PMG_MIXED)):
+        ctx.load_pbe(*mol)
+        ctx.pcg(mode)  # warm
+        lat = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            s = time.perf_counter()
+            ctx.load_pbe(*mol)
+            ctx.pcg(mode)
+            lat.append((time.perf_counter() - s) * 1e3)
+        wall = time.perf_counter() - t0
+        st = ctx.stats()
+        out[name] = {"solves_per_s": steps / wall, "p50_ms": float(np.percentile(lat, 50)),
+                     "cg_iters": st["iters"], "relres": st["relres"],
+                     "setup_ms": st["setup_ms"], "solve_ms": st["solve_ms"]}
+        us[name] = ctx.u()
+    ctx.close()
+    out["mixed_vs_fp64_speedup"] = out["mixed"]["solves_per_s"] / out["fp64"]["solves_per_s"]
+    out["mixed_rel_err_vs_fp64_gpu"] = rel_err(us["mixed"], us["fp64"])
+    if cpu:
+        from tests import oracle as orc
+        inp = orc.pbe_assemble(n, cfg["h"], cfg["ionic"], atoms)
+        _, u0, _, _ = orc.pcg(inp, n, nlev)
+        out["oracle_cg_iters"] = orc.last_stats()[0]
+        out["fp64_rel_err_vs_oracle"] = rel_err(us["fp64"], u0)
+        out["mixed_rel_err_vs_oracle"] = rel_err(us["mixed"], u0)
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -356,7 +401,7 @@ def main():
     extra = {}
     legs = set() if args.no_extra else set(args.legs.split(","))
     for key, name, fn in (("large", "large_289", large_leg), ("newton", "newton_129", newton_leg),
-                          ("band", "band_161", band_leg)):
+                          ("band", "band_161", band_leg), ("pcg", "pcg_161", pcg_leg)):
         if key in legs:
             extra[name] = fn(mg, not args.no_cpu)
             print(f"{key}: {extra[name]}", file=sys.stderr, flush=True)

@@ -1227,11 +1227,13 @@ __global__ __launch_bounds__(CG_NT) void k_cg_single(
 // are summed in the reference's three groups (in-plane, up, down), each
 // group row by row.
 constexpr int CG_R = 3, CG_RNT = 512;
-template <class T>
+// WARM: start from xout (else 0); ABS: the class solver's absolute
+// criterion (else Vcghs's relative one).  The PMG cycle uses <T, 0, 0>.
+template <class T, bool WARM, bool ABS>
 __global__ __launch_bounds__(CG_RNT) void k_cg_rows(
     int nx, int ny, int nz, const double *__restrict__ ac,
     const double *__restrict__ cc, const T *__restrict__ f,
-    T *__restrict__ xout, int itmax, double errtol, int warm, int absolute) {
+    T *__restrict__ xout, int itmax, double errtol) {
   constexpr int R = CG_R;
   extern __shared__ double P[];
   __shared__ double rbuf[2][CG_NT / 64];  // cg_allsum reads CG_NT / 64
@@ -1256,7 +1258,7 @@ __global__ __launch_bounds__(CG_RNT) void k_cg_rows(
     val[r] = act && i0 + r <= nx - 1;
     const size_t p = ix(nx, ny, val[r] ? i0 + r : 2, j, k);
     pidx[r] = (int)p;
-    xr[r] = warm && val[r] ? (double)xout[p] : 0.0;
+    xr[r] = WARM && val[r] ? (double)xout[p] : 0.0;
     rr[r] = val[r] ? (double)f[p] : 0.0;  // r = f - A x (x = 0 unless warm)
     apv[r] = 0.0;
     dg[r] = val[r] ? ac[p] + cc[p] : 0.0;
@@ -1278,85 +1280,96 @@ __global__ __launch_bounds__(CG_RNT) void k_cg_rows(
                                   1, 1, -1, -1, 0, 1, -1, 0, 0, 1, 1, -1, -1};
   constexpr signed char DK[26] = {0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1,
                                   1, 1, 1, 1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
-  // v = A P on this thread's points (P in LDS, complete); pp = P there
-  auto apply = [&](double (&v)[R], double (&pp)[R]) {
-    double o[R], u[R], dd[R];
-#pragma unroll
-    for (int r = 0; r < R; r++) o[r] = u[r] = dd[r] = 0.0;
-#pragma unroll
-    for (int a = 0; a < 9; a++) {
-      const int dj = (a % 3) - 1, dk = (a / 3) - 1;
-      double w[R + 2];
-#pragma unroll
-      for (int c = 0; c < R + 2; c++) w[c] = P[rb[a] + c];
-      if (dj == 0 && dk == 0) {
-#pragma unroll
-        for (int r = 0; r < R; r++) pp[r] = w[r + 1];
-      }
-#pragma unroll
-      for (int q = 0; q < 26; q++) {
-        if (DJ[q] != dj || DK[q] != dk) continue;
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-          const double t = cf[r][q] * w[r + 1 + DI[q]];
-          if (DK[q] == 0) o[r] += t;
-          else if (DK[q] > 0) u[r] += t;
-          else dd[r] += t;
-        }
-      }
-    }
-#pragma unroll
-    for (int r = 0; r < R; r++) v[r] = -o[r] + -u[r] + -dd[r] + dg[r] * pp[r];
-  };
   __syncthreads();
-  if (warm) {  // r = f - A x0
-#pragma unroll
-    for (int r = 0; r < R; r++)
-      if (val[r]) P[pidx[r]] = xr[r];
-    __syncthreads();
-    double v[R], pp[R];
-    apply(v, pp);
-#pragma unroll
-    for (int r = 0; r < R; r++)
-      if (val[r]) rr[r] -= v[r];
-    __syncthreads();
-  }
+  // With WARM the first pass through the loop computes r = f - A x0 with
+  // the loop's own stencil code (P = x0), then the CG proper starts.
+  bool warm_pass = WARM;
+  int b = 0;
+  double rhok2 = 0.0, denom = 0.0;
   double s = 0.0;
+  if constexpr (!WARM) {
 #pragma unroll
-  for (int r = 0; r < R; r++) s += rr[r] * rr[r];
-  double rhok2 = cg_allsum(s, rbuf[0], nw);
-  int b = 1;
-  const double denom = sqrt(rhok2);
+    for (int r = 0; r < R; r++) s += rr[r] * rr[r];
+    rhok2 = cg_allsum(s, rbuf[0], nw);
+    b = 1;
+    denom = sqrt(rhok2);
+  }
   int iters = 0;
   double rhok1 = 0.0;
-  if (denom != 0.0) {
+  {
     for (;;) {
-      // absolute: the class solver's criterion, sqrt(r.r) > errtol, its
-      // first test made on sqrt(sqrt(r0.r0)) (CG_Solver.h:125-133)
-      if (absolute ? (iters == 0 ? sqrt(sqrt(rhok2)) : sqrt(rhok2)) <= errtol
-                   : (errtol == 0.0 ? rhok2 == 0.0
-                                    : sqrt(rhok2) / denom <= errtol))
-        break;
-      if (iters >= itmax) break;
-      if (iters == 0) {
+      if (!warm_pass) {
+        if (denom == 0.0) break;
+        // ABS: the class solver's criterion, sqrt(r.r) > errtol, its first
+        // test made on sqrt(sqrt(r0.r0)) (CG_Solver.h:125-133)
+        if (ABS ? (iters == 0 ? sqrt(sqrt(rhok2)) : sqrt(rhok2)) <= errtol
+                : (errtol == 0.0 ? rhok2 == 0.0
+                                 : sqrt(rhok2) / denom <= errtol))
+          break;
+        if (iters >= itmax) break;
+        if (iters == 0) {
 #pragma unroll
-        for (int r = 0; r < R; r++)
-          if (val[r]) P[pidx[r]] = rr[r];
+          for (int r = 0; r < R; r++)
+            if (val[r]) P[pidx[r]] = rr[r];
+        } else {
+          const double beta = rhok2 / rhok1;
+          const double ib = 1.0 / beta;
+#pragma unroll
+          for (int r = 0; r < R; r++)
+            if (val[r]) P[pidx[r]] = (P[pidx[r]] + ib * rr[r]) * beta;
+        }
       } else {
-        const double beta = rhok2 / rhok1;
-        const double ib = 1.0 / beta;
 #pragma unroll
         for (int r = 0; r < R; r++)
-          if (val[r]) P[pidx[r]] = (P[pidx[r]] + ib * rr[r]) * beta;
+          if (val[r]) P[pidx[r]] = xr[r];
       }
       __syncthreads();
-      double v[R], pp[R];
-      apply(v, pp);
+      double o[R], u[R], dd[R], pp[R];
+#pragma unroll
+      for (int r = 0; r < R; r++) o[r] = u[r] = dd[r] = 0.0;
+#pragma unroll
+      for (int a = 0; a < 9; a++) {
+        const int dj = (a % 3) - 1, dk = (a / 3) - 1;
+        double w[R + 2];
+#pragma unroll
+        for (int c = 0; c < R + 2; c++) w[c] = P[rb[a] + c];
+        if (dj == 0 && dk == 0) {
+#pragma unroll
+          for (int r = 0; r < R; r++) pp[r] = w[r + 1];
+        }
+#pragma unroll
+        for (int q = 0; q < 26; q++) {
+          if (DJ[q] != dj || DK[q] != dk) continue;
+#pragma unroll
+          for (int r = 0; r < R; r++) {
+            const double t = cf[r][q] * w[r + 1 + DI[q]];
+            if (DK[q] == 0) o[r] += t;
+            else if (DK[q] > 0) u[r] += t;
+            else dd[r] += t;
+          }
+        }
+      }
       s = 0.0;
+      if (warm_pass) {  // r = f - A x0
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const double v = -o[r] + -u[r] + -dd[r] + dg[r] * pp[r];
+          if (val[r]) {
+            rr[r] -= v;
+            s += rr[r] * rr[r];
+          }
+        }
+        rhok2 = cg_allsum(s, rbuf[b], nw);
+        b ^= 1;
+        denom = sqrt(rhok2);
+        warm_pass = false;
+        continue;
+      }
 #pragma unroll
       for (int r = 0; r < R; r++) {
-        apv[r] = val[r] ? v[r] : 0.0;
-        if (val[r]) s += pp[r] * v[r];
+        const double v = -o[r] + -u[r] + -dd[r] + dg[r] * pp[r];
+        apv[r] = val[r] ? v : 0.0;
+        if (val[r]) s += pp[r] * v;
       }
       const double pAp = cg_allsum(s, rbuf[b], nw);
       b ^= 1;
@@ -1395,8 +1408,9 @@ static void cg_launch(const Dim &d, const double *ac, const double *cc,
   const int nrow = (int)((d.nx - 2 + CG_R - 1) / CG_R) * (d.ny - 2) * (d.nz - 2);
   if (ND == 27 && nrow <= CG_RNT) {
     const int nt = (nrow + 63) / 64 * 64;
-    k_cg_rows<T><<<1, nt, lds + (CG_R + 2) * sizeof(double), s>>>(
-        d.nx, d.ny, d.nz, ac, cc, f, x, itmax, errtol, 0, 0);
+    k_cg_rows<T, false, false>
+        <<<1, nt, lds + (CG_R + 2) * sizeof(double), s>>>(
+            d.nx, d.ny, d.nz, ac, cc, f, x, itmax, errtol);
     return;
   }
   // one interior point per thread when they fit: as few wavefronts as the
@@ -1440,8 +1454,9 @@ void cg_class(const Dim &d, int numdia, const double *ac, const double *cc,
   }
   const int nrow = (int)((d.nx - 2 + CG_R - 1) / CG_R) * (d.ny - 2) * (d.nz - 2);
   const int nt = (nrow + 63) / 64 * 64;
-  k_cg_rows<T><<<1, nt, d.n() * sizeof(double) + (CG_R + 2) * sizeof(double),
-                 s>>>(d.nx, d.ny, d.nz, ac, cc, f, x, itmax, thresh, 1, 1);
+  k_cg_rows<T, true, true>
+      <<<1, nt, d.n() * sizeof(double) + (CG_R + 2) * sizeof(double), s>>>(
+          d.nx, d.ny, d.nz, ac, cc, f, x, itmax, thresh);
   check_launch("cg_class");
 }
 
