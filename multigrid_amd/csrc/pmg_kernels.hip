@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 #include <cstdio>
 
@@ -261,8 +262,8 @@ __global__ __launch_bounds__(256) void k_gs27(int nx, int ny, int nz,
 // reader has passed).
 constexpr int SW_TX = 64, SW_TY = 16;
 template <class T>
-struct SwPl {  // own-point operands of one plane
-  T oC, cc, fc, oE, oN, uC, x;
+struct SwCo {  // own-point operator values of one plane
+  T oC, cc, fc, oE, oN, uC;
 };
 
 template <class T>
@@ -293,53 +294,79 @@ __global__ __launch_bounds__(SW_TX *SW_TY) void k_gs7_sweep(
                      ty <= SW_TY - 3;
   // column base, clamped into the grid: loads stay unconditional
   const size_t p0 = ix(nx, ny, min(max(i, 1), nx), min(max(j, 1), ny), 1);
-  auto load = [&](int q, SwPl<T> &P) {
-    const int qc = min(max(q, 1), nz);
-    const size_t p = p0 + (size_t)(qc - 1) * sz;
+  auto at = [&](int q) { return p0 + (size_t)(min(max(q, 1), nz) - 1) * sz; };
+  auto load = [&](int q, SwCo<T> &P) {
+    const size_t p = at(q);
     P.oC = ac[p], P.cc = cc[p], P.fc = fc[p], P.oE = oE[p], P.oN = oN[p],
-    P.uC = uC[p], P.x = xin[p];
+    P.uC = uC[p];
   };
   auto first = [&](int q) {
     return midr && q >= 2 && q <= nz - 1 && ((i + j + q) & 1) == c1;
   };
   // first-colour value at plane q (reference term order, gsd.cc:98-105)
-  auto fval = [&](int q, const SwPl<T> &P, T xm, T xp, T uCm) -> T {
+  auto fval = [&](int q, const SwCo<T> &P, T xm, T xp, T uCm) -> T {
     const int s = q % 3, sx = q & 1;
     const T sm = P.oN * SX[sx][ty + 1][tx] + SN[s][ty - 1][tx] * SX[sx][ty - 1][tx] +
                  P.oE * SX[sx][ty][tx + 1] + SE[s][ty][tx - 1] * SX[sx][ty][tx - 1] +
                  uCm * xm + P.uC * xp;
     return (P.fc + sm) / (P.oC + P.cc);
   };
-  SwPl<T> Pm1, P0, P1, P2, P3;  // planes k-1 .. k+3
-  load(kb - 2, P3);
-  const T uCm2 = P3.uC, xm2 = P3.x;
-  load(kb - 1, Pm1);
-  load(kb, P0);
-  load(kb + 1, P1);
-  load(kb + 2, P2);
-  SX[(kb - 1) & 1][ty][tx] = Pm1.x;
-  SE[(kb - 1) % 3][ty][tx] = Pm1.oE;
-  SN[(kb - 1) % 3][ty][tx] = Pm1.oN;
-  SX[kb & 1][ty][tx] = P0.x;
-  SE[kb % 3][ty][tx] = P0.oE;
-  SN[kb % 3][ty][tx] = P0.oN;
-  __syncthreads();
-  T mprev = first(kb - 1) ? fval(kb - 1, Pm1, xm2, P0.x, uCm2) : Pm1.x;
-  T mcur = first(kb) ? fval(kb, P0, Pm1.x, P1.x, Pm1.uC) : P0.x;
+  // Register rings of five planes, the operator values three planes ahead
+  // and xin four planes ahead: step k reads the operator of planes k-1
+  // (uC), k and k+1 and xin of planes k, k+1 and k+2, all loaded at least
+  // two steps earlier.  Plane q sits in slot (q - kb) mod 5 of each ring and
+  // the loop is unrolled by 5, so every slot is one fixed set of registers
+  // (rotating values between registers instead would copy registers whose
+  // loads are still in flight and make each step wait for the step before).
+  SwCo<T> C[5];
+  T X[5];
+  T mprev, mcur;
+  {
+    // Load order as in the loop's steady state: the last loads issued here
+    // (C[2], X[3]) are the ones the first step does not read, so that step
+    // waits for the same loads whether it is entered from here or from the
+    // loop's back edge.
+    SwCo<T> Q;  // plane kb-2: its uC and x enter plane kb-1's first colour
+    load(kb - 2, Q);
+    const T xq = xin[at(kb - 2)];
+    load(kb - 1, C[4]);
+    X[4] = xin[at(kb - 1)];
+    load(kb, C[0]);
+    X[0] = xin[at(kb)];
+    X[1] = xin[at(kb + 1)];
+    load(kb + 1, C[1]);
+    X[2] = xin[at(kb + 2)];
+    load(kb + 2, C[2]);
+    X[3] = xin[at(kb + 3)];
+    SX[(kb - 1) & 1][ty][tx] = X[4];
+    SE[(kb - 1) % 3][ty][tx] = C[4].oE;
+    SN[(kb - 1) % 3][ty][tx] = C[4].oN;
+    SX[kb & 1][ty][tx] = X[0];
+    SE[kb % 3][ty][tx] = C[0].oE;
+    SN[kb % 3][ty][tx] = C[0].oN;
+    __syncthreads();
+    mprev = first(kb - 1) ? fval(kb - 1, C[4], xq, X[0], Q.uC) : X[4];
+  }
+  mcur = first(kb) ? fval(kb, C[0], X[4], X[1], C[4].uC) : X[0];
   SM[kb % 3][ty][tx] = mcur;
   // the first step rewrites xin slot (kb+1)&1 == (kb-1)&1, which the
   // plane kb-1 values above read
   __syncthreads();
-  for (int k = kb; k <= ke; k++) {
-    // plane ke+3 is never used: the last step re-reads plane ke+2 (an L2
-    // hit) so that the load stays unconditional
-    load(min(k + 3, ke + 2), P3);
+  auto step = [&](int k, auto slot) __attribute__((always_inline)) {
+    constexpr int r = decltype(slot)::value;
+    const SwCo<T> &P0 = C[r], &P1 = C[(r + 1) % 5];
+    const T uCm = C[(r + 4) % 5].uC;  // plane k-1
+    // planes past ke+2 are never used: the last steps re-read plane ke+2
+    // (an L2 hit) so that the loads stay unconditional
+    load(min(k + 3, ke + 2), C[(r + 3) % 5]);
+    X[(r + 4) % 5] = xin[at(min(k + 4, ke + 2))];
+    const T x0 = X[r], x1 = X[(r + 1) % 5], x2 = X[(r + 2) % 5];
     const int s1 = (k + 1) % 3, s0 = k % 3;
-    SX[(k + 1) & 1][ty][tx] = P1.x;
+    SX[(k + 1) & 1][ty][tx] = x1;
     SE[s1][ty][tx] = P1.oE;
     SN[s1][ty][tx] = P1.oN;
     __syncthreads();
-    const T mnext = first(k + 1) ? fval(k + 1, P1, P0.x, P2.x, P0.uC) : P1.x;
+    const T mnext = first(k + 1) ? fval(k + 1, P1, x0, x2, P0.uC) : x1;
     SM[s1][ty][tx] = mnext;
     if (owned) {
       T v = mcur;
@@ -348,18 +375,31 @@ __global__ __launch_bounds__(SW_TX *SW_TY) void k_gs7_sweep(
                      SN[s0][ty - 1][tx] * SM[s0][ty - 1][tx] +
                      P0.oE * SM[s0][ty][tx + 1] +
                      SE[s0][ty][tx - 1] * SM[s0][ty][tx - 1] +
-                     Pm1.uC * mprev + P0.uC * mnext;
+                     uCm * mprev + P0.uC * mnext;
         v = (P0.fc + sm) / (P0.oC + P0.cc);
       }
       xout[p0 + (size_t)(k - 1) * sz] = v;
     }
     mprev = mcur;
     mcur = mnext;
-    Pm1 = P0;
-    P0 = P1;
-    P1 = P2;
-    P2 = P3;
+  };
+  using I0 = std::integral_constant<int, 0>;
+  using I1 = std::integral_constant<int, 1>;
+  using I2 = std::integral_constant<int, 2>;
+  using I3 = std::integral_constant<int, 3>;
+  using I4 = std::integral_constant<int, 4>;
+  int k = kb;
+  for (; k + 4 <= ke; k += 5) {
+    step(k, I0{});
+    step(k + 1, I1{});
+    step(k + 2, I2{});
+    step(k + 3, I3{});
+    step(k + 4, I4{});
   }
+  if (k <= ke) step(k, I0{});
+  if (k + 1 <= ke) step(k + 1, I1{});
+  if (k + 2 <= ke) step(k + 2, I2{});
+  if (k + 3 <= ke) step(k + 3, I3{});
 }
 
 // Compute units of the current device (cached per device).
@@ -891,6 +931,126 @@ void restrict_(const Dim &f, const Dim &c, const T *xin, T *xout, const T *pc,
   k_restrict<T><<<xcd_blocks(g), dim3(BX, BY), 0, s>>>(f.nx, f.ny, c.nx, c.ny, c.nz, xin,
                                            xout, pc, xg_of(g));
   check_launch("restrict");
+}
+
+// ---------------------------------------------------------------------------
+// Residual and restriction in one pass: Vmresid then Vrestrc2, the down leg
+// of Vmvcs (mgcsd.cc:325-360, matvecd.cc:381-528 and 755-795).  The fine
+// residual never goes to memory.
+// ---------------------------------------------------------------------------
+// A 1024-thread workgroup owns RR_CX x RR_CY coarse columns (ii, jj) and
+// marches a chunk of coarse planes kk.  Thread (tx, ty) holds the fine
+// column (2 ii0 - 2 + tx, 2 jj0 - 2 + ty): together the threads with
+// tx <= 2 RR_CX and ty <= 2 RR_CY cover the fine points 2ii-1 +- 1 of every
+// owned coarse column.  Per coarse plane they compute the residual of fine
+// planes 2kk-1 and 2kk (k_resid's expression, so the same values) into an
+// LDS ring of five fine planes; after one barrier the owners of the coarse
+// columns combine planes 2kk-2 .. 2kk with P in k_restrict's order.  Step kk
+// writes the slots of planes 2kk-6 and 2kk-5, last read in step kk-2, which
+// every thread has left before it passed the barrier of step kk-1.
+constexpr int RR_TX = 64, RR_TY = 16, RR_CX = 31, RR_CY = 7, RR_RING = 5;
+template <class T, int ND>
+__global__ __launch_bounds__(RR_TX *RR_TY) void k_resid_restrict(
+    int nxf, int nyf, int nzf, const T *__restrict__ ac,
+    const T *__restrict__ cc, const T *__restrict__ fc,
+    const T *__restrict__ x, int nxc, int nyc, int nzc,
+    const T *__restrict__ pc, T *__restrict__ xout, int kc, int gx, int gy,
+    int gz) {
+  __shared__ T R[RR_RING][RR_TY][RR_TX];
+  // XCD-aware tile order, as k_gs7_sweep
+  const int ntile = gx * gy * gz, per = (ntile + 7) / 8;
+  const int t = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+  if (t >= ntile) return;  // whole workgroup, before any barrier
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int ii0 = 2 + (t % gx) * RR_CX, jj0 = 2 + ((t / gx) % gy) * RR_CY;
+  const int kb = 2 + (t / (gx * gy)) * kc;
+  const int ke = min(kb + kc - 1, nzc - 1);
+  const size_t n = (size_t)nxf * nyf * nzf, sy = nxf, sz = (size_t)nxf * nyf;
+  // fine column; i, j >= 2 as ii0, jj0 >= 2
+  const int i = 2 * ii0 - 2 + tx, j = 2 * jj0 - 2 + ty;
+  const bool act = tx <= 2 * RR_CX && ty <= 2 * RR_CY && i <= nxf - 1 &&
+                   j <= nyf - 1;
+  const size_t col = (size_t)(j - 1) * sy + (size_t)(i - 1);
+  auto rval = [&](int k) -> T {  // residual at fine (i, j, k), k interior
+    const size_t p = col + (size_t)(k - 1) * sz;
+    if (ND == 7) {
+      const T *oE = ac + n, *oN = ac + 2 * n, *uC = ac + 3 * n;
+      return fc[p] + oN[p] * x[p + sy] + oN[p - sy] * x[p - sy] +
+             oE[p] * x[p + 1] + oE[p - 1] * x[p - 1] +
+             uC[p - sz] * x[p - sz] + uC[p] * x[p + sz] -
+             (ac[p] + cc[p]) * x[p];
+    } else {
+      double sO, sU, sD;
+      sum27<T, T>(ac, n, sy, sz, p, x, sO, sU, sD);
+      return (T)((T)((T)(fc[p] + (T)sO) + (T)sU) + (T)sD) -
+             (ac[p] + cc[p]) * x[p];
+    }
+  };
+  // coarse column of the restricting threads
+  const int tid = tx + RR_TX * ty;
+  const int ii = ii0 + tid % RR_CX, jj = jj0 + tid / RR_CX;
+  const bool own = tid < RR_CX * RR_CY && ii <= nxc - 1 && jj <= nyc - 1;
+  const int lx = 2 * (tid % RR_CX) + 1, ly = 2 * (tid / RR_CX) + 1;
+  const size_t nc = (size_t)nxc * nyc * nzc;
+  if (act) R[(2 * kb - 2) % RR_RING][ty][tx] = rval(2 * kb - 2);
+  for (int kk = kb; kk <= ke; kk++) {
+    if (act) {
+      const T ra = rval(2 * kk - 1), rb = rval(2 * kk);
+      R[(2 * kk - 1) % RR_RING][ty][tx] = ra;
+      R[(2 * kk) % RR_RING][ty][tx] = rb;
+    }
+    __syncthreads();
+    if (own) {
+      const size_t pC = ix(nxc, nyc, ii, jj, kk);
+      // P plane blocks: same fine plane, the one above, the one below
+      const int sl[3] = {(2 * kk - 1) % RR_RING, (2 * kk) % RR_RING,
+                         (2 * kk - 2) % RR_RING};
+      T tb[3];
+#pragma unroll
+      for (int b = 0; b < 3; b++) {
+        const T(*Q)[RR_TX] = R[sl[b]];
+        const T *P = pc + (size_t)(9 * b) * nc + pC;
+        // in-plane order C, N, S, E, W, NE, NW, SE, SW
+        T acc = P[0] * Q[ly][lx];
+        acc += P[nc] * Q[ly + 1][lx];
+        acc += P[2 * nc] * Q[ly - 1][lx];
+        acc += P[3 * nc] * Q[ly][lx + 1];
+        acc += P[4 * nc] * Q[ly][lx - 1];
+        acc += P[5 * nc] * Q[ly + 1][lx + 1];
+        acc += P[6 * nc] * Q[ly + 1][lx - 1];
+        acc += P[7 * nc] * Q[ly - 1][lx + 1];
+        acc += P[8 * nc] * Q[ly - 1][lx - 1];
+        tb[b] = acc;
+      }
+      xout[pC] = tb[0] + tb[1] + tb[2];
+    }
+  }
+}
+
+template <class T>
+void resid_restrict(const Dim &f, const Dim &c, int numdia, const T *ac,
+                    const T *cc, const T *fc, const T *x, const T *pc,
+                    T *xout, hipStream_t s) {
+  if (c.nx < 3 || c.ny < 3 || c.nz < 3) return;
+  if (f.nx != 2 * c.nx - 1 || f.ny != 2 * c.ny - 1 || f.nz != 2 * c.nz - 1) {
+    fprintf(stderr, "pmg: resid_restrict shape mismatch\n");
+    return;
+  }
+  const int gx = cdiv(c.nx - 2, RR_CX), gy = cdiv(c.ny - 2, RR_CY);
+  const int zb = std::max(1, device_cus() / (gx * gy));
+  const int kc = cdiv(c.nz - 2, std::min(zb, c.nz - 2));
+  const int gz = cdiv(c.nz - 2, kc);
+  const int nblk = 8 * cdiv(gx * gy * gz, 8);
+  dim3 b(RR_TX, RR_TY);
+  if (numdia == 7)
+    k_resid_restrict<T, 7><<<nblk, b, 0, s>>>(f.nx, f.ny, f.nz, ac, cc, fc, x,
+                                              c.nx, c.ny, c.nz, pc, xout, kc,
+                                              gx, gy, gz);
+  else
+    k_resid_restrict<T, 27><<<nblk, b, 0, s>>>(f.nx, f.ny, f.nz, ac, cc, fc,
+                                               x, c.nx, c.ny, c.nz, pc, xout,
+                                               kc, gx, gy, gz);
+  check_launch("resid_restrict");
 }
 
 enum : int {
@@ -2019,6 +2179,9 @@ void buildG(const Dim &f, const Dim &c, int numdia, const double *pc,
                                hipStream_t);                                  \
   template void restrict_<T>(const Dim &, const Dim &, const T *, T *,        \
                              const T *, hipStream_t);                         \
+  template void resid_restrict<T>(const Dim &, const Dim &, int, const T *,  \
+                                  const T *, const T *, const T *, const T *, \
+                                  T *, hipStream_t);                          \
   template void interp<T>(const Dim &, const Dim &, const T *, const T *,     \
                           T *, const double *, hipStream_t);                  \
   template void interp_damped<T>(const Dim &, const Dim &, const T *,         \
