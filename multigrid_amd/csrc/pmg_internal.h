@@ -80,6 +80,12 @@ unsigned matvec_dots_part(const Dim &d, int numdia, const T *ac, const T *cc,
 template <class T>
 void restrict_(const Dim &f, const Dim &c, const T *xin, T *xout, const T *pc,
                hipStream_t s);
+// xout = restriction (Vrestrc2) of the residual fc - A x on the fine level
+// f, in one pass without storing the residual.
+template <class T>
+void resid_restrict(const Dim &f, const Dim &c, int numdia, const T *ac,
+                    const T *cc, const T *fc, const T *x, const T *pc,
+                    T *xout, hipStream_t s);
 // xf(interior) += (*xdamp) * (P xc); xdamp == nullptr means plain
 // interpolation into xf with the fine boundary set to zero.
 template <class T>

@@ -477,6 +477,28 @@ void gs7(const Dim &d, const T *ac, const T *cc, const T *fc, T *x,
 // is pair m-1's second point (the previous lane, or lane 63 of the previous
 // segment for lane 0); its east neighbour is pair m+1's first point.
 constexpr int ROWS_MAXN = 128 * 4 + 1;
+// A point's operands for its update (gsd.cc:160-211), as loaded: the
+// couplings and neighbour values of its own plane (all but the two in-row
+// neighbours), of the plane above (u) and of the plane below (d).
+template <class T>
+struct G27Raw {
+  T cN[2], xN[2];  // oN(p) x(p+sy), oN(p-sy) x(p-sy)
+  T cE, cW;        // oE(p), oE(p-1): the in-row couplings
+  T c[4], xv[4];   // the other four in-plane terms
+  T cu[9], xu[9], cd[9], xd[9];
+  T f, oC, cc;
+};
+
+// The terms that do not involve the two in-row neighbours, summed in the
+// reference's order as far as they go.
+template <class T>
+struct G27Part {
+  T a;             // oN(p) x(p+sy) + oN(p-sy) x(p-sy): the first two terms
+  T cE, cW;
+  T c[4], xv[4];   // the four in-plane terms after the in-row ones
+  T u, dd, f, dg;  // upper and lower plane sums, fc, oC + cc
+};
+
 template <class T, int ADJ, int PPL>
 __global__ __launch_bounds__(256) void k_gs27_rows(
     int nx, int ny, int nz, const T *__restrict__ ac, const T *__restrict__ cc,
@@ -503,6 +525,73 @@ __global__ __launch_bounds__(256) void k_gs27_rows(
   const bool zN = zr(1, 0), zU = zr(0, 1), zNU = zr(1, 1);  // classes
   const bool zown = (zmask >> (2 * pz + py)) & 1;
   const size_t rb = ix(nx, ny, 1, j, k);  // row base (i = 1)
+  // The loads are unconditional: rows that are still zero in a zero-start
+  // sweep are loaded and replaced by 0 afterwards, and points past the row
+  // end read the last interior point's operands and are not stored.
+  auto load = [&](int i) -> G27Raw<T> {
+    const size_t p = rb + min(i, nx - 1) - 1;
+    G27Raw<T> r;
+    r.cN[0] = oN[p], r.xN[0] = x[p + sy];
+    r.cN[1] = oN[p - sy], r.xN[1] = x[p - sy];
+    r.cE = oE[p];
+    r.cW = oE[p - 1];
+    r.c[0] = oNE[p], r.xv[0] = x[p + 1 + sy];
+    r.c[1] = oNW[p], r.xv[1] = x[p - 1 + sy];
+    r.c[2] = oNW[p + 1 - sy], r.xv[2] = x[p + 1 - sy];
+    r.c[3] = oNE[p - 1 - sy], r.xv[3] = x[p - 1 - sy];
+    const T *cu[9] = {uC, uN, uS, uE, uW, uNE, uNW, uSE, uSW};
+    const long long du[9] = {0,
+                             (long long)sy,
+                             -(long long)sy,
+                             1,
+                             -1,
+                             1 + (long long)sy,
+                             -1 + (long long)sy,
+                             1 - (long long)sy,
+                             -1 - (long long)sy};
+    // the plane below couples through the mirrored plane of the neighbour
+    const T *cd[9] = {uC, uS, uN, uW, uE, uSW, uSE, uNW, uNE};
+#pragma unroll
+    for (int m = 0; m < 9; m++) {
+      r.cu[m] = cu[m][p];
+      r.xu[m] = x[p + du[m] + sz];
+      r.cd[m] = cd[m][p + du[m] - sz];
+      r.xd[m] = x[p + du[m] - sz];
+    }
+    r.f = fc[p];
+    r.oC = ac[p];
+    r.cc = cc[p];
+    return r;
+  };
+  auto part = [&](const G27Raw<T> &r) -> G27Part<T> {
+    auto Z = [](T v, bool z) -> T { return z ? (T)0 : v; };
+    G27Part<T> g;
+    g.a = r.cN[0] * Z(r.xN[0], zN) + r.cN[1] * Z(r.xN[1], zN);
+    g.cE = r.cE;
+    g.cW = r.cW;
+#pragma unroll
+    for (int m = 0; m < 4; m++) g.c[m] = r.c[m], g.xv[m] = Z(r.xv[m], zN);
+    // u: uC with the plane above's own column, the others diagonal (zNU)
+    // except uE, uW (same row, zU)
+    const bool zu[9] = {zU, zNU, zNU, zU, zU, zNU, zNU, zNU, zNU};
+    T u = r.cu[0] * Z(r.xu[0], zu[0]);
+#pragma unroll
+    for (int m = 1; m < 9; m++) u += r.cu[m] * Z(r.xu[m], zu[m]);
+    T dd = r.cd[0] * Z(r.xd[0], zu[0]);
+#pragma unroll
+    for (int m = 1; m < 9; m++) dd += r.cd[m] * Z(r.xd[m], zu[m]);
+    g.u = u;
+    g.dd = dd;
+    g.f = r.f;
+    g.dg = r.oC + r.cc;
+    return g;
+  };
+  // the update from the part and the in-row neighbours xw (i-1), xe (i+1)
+  auto fin = [&](const G27Part<T> &g, T xw, T xe) -> T {
+    const T o = g.a + g.cE * xe + g.cW * xw + g.c[0] * g.xv[0] +
+                g.c[1] * g.xv[1] + g.c[2] * g.xv[2] + g.c[3] * g.xv[3];
+    return (g.f + ((o + g.u) + g.dd)) / g.dg;
+  };
   int i0[PPL], i1[PPL];
   bool v0[PPL], v1[PPL];
   T a0[PPL], a1[PPL], n0[PPL], n1[PPL];
@@ -513,37 +602,29 @@ __global__ __launch_bounds__(256) void k_gs27_rows(
     v0[q] = i0[q] <= nx - 1;
     v1[q] = i1[q] <= nx - 1;
     // current values of the own row, boundary included (i <= nx)
-    a0[q] = (i0[q] <= nx) ? (zown && v0[q] ? (T)0 : x[rb + i0[q] - 1]) : (T)0;
-    a1[q] = (i1[q] <= nx) ? (zown && v1[q] ? (T)0 : x[rb + i1[q] - 1]) : (T)0;
+    const T r0 = x[rb + min(i0[q], nx) - 1], r1 = x[rb + min(i1[q], nx) - 1];
+    a0[q] = (i0[q] <= nx) ? (zown && v0[q] ? (T)0 : r0) : (T)0;
+    a1[q] = (i1[q] <= nx) ? (zown && v1[q] ? (T)0 : r1) : (T)0;
     n0[q] = a0[q];
     n1[q] = a1[q];
   }
   const T xb1 = x[rb];  // i = 1 boundary
-  auto X = [&](size_t q, bool z) -> T { return z ? (T)0 : x[q]; };
-  // update of point i with in-row neighbours xw (i-1) and xe (i+1)
-  auto upd = [&](int i, T xw, T xe) -> T {
-    const size_t p = rb + i - 1;
-    const T o = oN[p] * X(p + sy, zN) + oN[p - sy] * X(p - sy, zN) +
-                oE[p] * xe + oE[p - 1] * xw + oNE[p] * X(p + 1 + sy, zN) +
-                oNW[p] * X(p - 1 + sy, zN) +
-                oNW[p + 1 - sy] * X(p + 1 - sy, zN) +
-                oNE[p - 1 - sy] * X(p - 1 - sy, zN);
-    const T u = uC[p] * X(p + sz, zU) + uN[p] * X(p + sy + sz, zNU) +
-                uS[p] * X(p - sy + sz, zNU) + uE[p] * X(p + 1 + sz, zU) +
-                uW[p] * X(p - 1 + sz, zU) + uNE[p] * X(p + 1 + sy + sz, zNU) +
-                uNW[p] * X(p - 1 + sy + sz, zNU) +
-                uSE[p] * X(p + 1 - sy + sz, zNU) +
-                uSW[p] * X(p - 1 - sy + sz, zNU);
-    const T dd = uC[p - sz] * X(p - sz, zU) +
-                 uS[p + sy - sz] * X(p + sy - sz, zNU) +
-                 uN[p - sy - sz] * X(p - sy - sz, zNU) +
-                 uW[p + 1 - sz] * X(p + 1 - sz, zU) +
-                 uE[p - 1 - sz] * X(p - 1 - sz, zU) +
-                 uSW[p + 1 + sy - sz] * X(p + 1 + sy - sz, zNU) +
-                 uSE[p - 1 + sy - sz] * X(p - 1 + sy - sz, zNU) +
-                 uNW[p + 1 - sy - sz] * X(p + 1 - sy - sz, zNU) +
-                 uNE[p - 1 - sy - sz] * X(p - 1 - sy - sz, zNU);
-    return (fc[p] + ((o + u) + dd)) / (ac[p] + cc[p]);
+  // One lane pair per lane (rows up to 129 points): every load of both
+  // points is issued before any arithmetic, so the launch waits for memory
+  // once; only the in-row terms wait for the first colour.  Longer rows
+  // load each point's operands just before its update.
+  G27Part<T> g0[PPL], g1[PPL];
+  if (PPL == 1) {
+    const G27Raw<T> r0 = load(i0[0]), r1 = load(i1[0]);
+    __builtin_amdgcn_sched_barrier(0);
+    g0[0] = part(r0);
+    g1[0] = part(r1);
+  }
+  auto part0 = [&](int q) -> G27Part<T> {
+    return PPL == 1 ? g0[0] : part(load(i0[q]));
+  };
+  auto part1 = [&](int q) -> G27Part<T> {
+    return PPL == 1 ? g1[0] : part(load(i1[q]));
   };
   // value of the previous pair's second point / next pair's first point
   // (every lane takes part in the shuffles)
@@ -563,23 +644,27 @@ __global__ __launch_bounds__(256) void k_gs27_rows(
 #pragma unroll
     for (int q = 0; q < PPL; q++) {
       const T w = west(a1, q);
-      if (v0[q]) n0[q] = upd(i0[q], w, a1[q]);
+      const G27Part<T> g = part0(q);
+      if (v0[q]) n0[q] = fin(g, w, a1[q]);
     }
 #pragma unroll
     for (int q = 0; q < PPL; q++) {
       const T e = east(n0, q);
-      if (v1[q]) n1[q] = upd(i1[q], n0[q], e);
+      const G27Part<T> g = part1(q);
+      if (v1[q]) n1[q] = fin(g, n0[q], e);
     }
   } else {
 #pragma unroll
     for (int q = 0; q < PPL; q++) {
       const T e = east(a0, q);
-      if (v1[q]) n1[q] = upd(i1[q], a0[q], e);
+      const G27Part<T> g = part1(q);
+      if (v1[q]) n1[q] = fin(g, a0[q], e);
     }
 #pragma unroll
     for (int q = 0; q < PPL; q++) {
       const T w = west(n1, q);
-      if (v0[q]) n0[q] = upd(i0[q], w, n1[q]);
+      const G27Part<T> g = part0(q);
+      if (v0[q]) n0[q] = fin(g, w, n1[q]);
     }
   }
 #pragma unroll

@@ -1491,6 +1491,27 @@ int pmg_restrc(int nxf, int nyf, int nzf, int nxc, int nyc, int nzc,
   return finish(S, xout, dxo, c.n());
 }
 
+int pmg_mresid_restrc(int nxf, int nyf, int nzf, int nxc, int nyc, int nzc,
+                      int numdia, const double *ac, const double *cc,
+                      const double *fc, const double *x, const double *pc,
+                      double *xout) {
+  if (!dims_ok(nxc, nyc, nzc) || !coarse_of(nxf, nxc) ||
+      !coarse_of(nyf, nyc) || !coarse_of(nzf, nzc) ||
+      (numdia != 7 && numdia != 27) || !ac || !cc || !fc || !x || !pc ||
+      !xout)
+    return PMG_EBADPARM;
+  if (!device_ok()) return PMG_EDEVICE;
+  Dim f{nxf, nyf, nzf}, c{nxc, nyc, nzc};
+  const size_t n = f.n();
+  Scratch S;
+  double *dac = S.put(ac, (numdia == 7 ? 4 : 14) * n), *dcc = S.put(cc, n),
+         *dfc = S.put(fc, n), *dx = S.put(x, n), *dpc = S.put(pc, 27 * c.n()),
+         *dxo = S.put(xout, c.n());
+  if (!S.ok) return PMG_ENOMEM;
+  resid_restrict<double>(f, c, numdia, dac, dcc, dfc, dx, dpc, dxo, 0);
+  return finish(S, xout, dxo, c.n());
+}
+
 int pmg_interp(int nxc, int nyc, int nzc, int nxf, int nyf, int nzf,
                const double *xin, double *xout, const double *pc) {
   if (!dims_ok(nxc, nyc, nzc) || !coarse_of(nxf, nxc) ||
