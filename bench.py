@@ -65,6 +65,16 @@ def timed_steps(ctx, mol, mode, steps, warmup):
     return wall, np.array(lat), iters
 
 
+def sweep_bytes(pts, sweeps, nu=(2, 2)):
+    """Algorithmic bytes of `sweeps` timed FP32 fine sweeps of a mixed solve
+    (V(nu1, nu2) cycles).  A sweep reads oC, cc, fc, oE, oN, uC and x and
+    writes x: 8 arrays x 4 B per interior point.  The first pre-smoothing
+    sweep of each correction cycle starts from x = 0 and does not read x:
+    7 arrays."""
+    zero_start = sweeps // (nu[0] + nu[1])
+    return 4 * pts * (8 * sweeps - zero_start)
+
+
 def measured_traffic(n):
     """HBM bytes per launch of the fine sweep for an n^3 level, as measured
     with rocprofv3 FETCH_SIZE / WRITE_SIZE counters and recorded in
@@ -209,7 +219,7 @@ def large_leg(mg, cpu):
     ctx.solve(mg.PMG_MIXED)
     kms, sweeps = ctx.ktime()
     ctx.close()
-    out["fp32_sweep_gbs"] = 8 * 4 * (n - 2) ** 3 / (kms / 1e3 / max(sweeps, 1)) / 1e9
+    out["fp32_sweep_gbs"] = sweep_bytes((n - 2) ** 3, sweeps) / (kms / 1e3) / 1e9
     u64, umx = out.pop("fp64_u"), out.pop("mixed_u")
     out["mixed_rel_err_vs_fp64_gpu"] = rel_err(umx, u64)
     if cpu:
@@ -378,13 +388,11 @@ def main():
     err = float(np.max(np.abs(umx - u64)) / np.max(np.abs(u64)))
 
     # roofline of the dominant kernel: the fine-level fused red-black sweep
-    # (FP32 in the mixed path), one launch per full sweep.  Algorithmic bytes
-    # per fine interior point of a two-colour fused sweep: oC, cc, fc, oE,
-    # oN, uC, x read, x write = 8 arrays.
+    # (FP32 in the mixed path), one launch per full sweep.
     pts = (n - 2) ** 3
-    bytes_launch = 8 * 4 * pts
     launches = ksweeps
     avg_launch_s = (kms / 1e3) / max(launches, 1)
+    bytes_launch = sweep_bytes(pts, launches) / max(launches, 1)
     achieved = bytes_launch / avg_launch_s / 1e9 if launches else 0.0
 
     ctx.close()

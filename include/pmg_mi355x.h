@@ -166,13 +166,6 @@ int pmg_restrc(int nxf, int nyf, int nzf, int nxc, int nyc, int nzc,
                const double *xin, double *xout, const double *pc);
 int pmg_interp(int nxc, int nyc, int nzc, int nxf, int nyf, int nzf,
                const double *xin, double *xout, const double *pc);
-/* Vmresid followed by Vrestrc, as in the down leg of Vmvcs
- * (pmgc/mgcsd.cc:325-360): xout = restriction of fc - A x, computed in one
- * pass without the fine residual array (numdia 7 or 27). */
-int pmg_mresid_restrc(int nxf, int nyf, int nzf, int nxc, int nyc, int nzc,
-                      int numdia, const double *ac, const double *cc,
-                      const double *fc, const double *x, const double *pc,
-                      double *xout);
 /* VbuildP / VbuildG (pmgc/buildPd.cc:1118-1147, buildGd.cc:3276-3368). */
 int pmg_buildP(int nxf, int nyf, int nzf, int nxc, int nyc, int nzc,
                int mgprol, int numdia, const double *acf, double *pc);

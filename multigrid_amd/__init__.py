@@ -83,7 +83,6 @@ def _declare(L):
         "pmg_gsrb": (i, [i, i, i, i, dp, dp, dp, dp, i, i]),
         "pmg_restrc": (i, [i, i, i, i, i, i, dp, dp, dp]),
         "pmg_interp": (i, [i, i, i, i, i, i, dp, dp, dp]),
-        "pmg_mresid_restrc": (i, [i, i, i, i, i, i, i, dp, dp, dp, dp, dp, dp]),
         "pmg_buildP": (i, [i, i, i, i, i, i, i, i, dp, dp]),
         "pmg_buildG": (i, [i, i, i, i, i, i, i, dp, dp, dp]),
         "pmg_pbe_synth": (i, [i, d, i, d, d, i, ctypes.c_uint64] + [dp] * 13),

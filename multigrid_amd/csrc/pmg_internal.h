@@ -42,9 +42,12 @@ void gs7(const Dim &d, const T *ac, const T *cc, const T *fc, T *x,
          int nsweeps, int iadjoint, hipStream_t s);
 // nsweeps full red-black sweeps, each one pass over memory (x and tmp
 // ping-pong; tmp must share x's boundary values).  Same result as gs7.
+// zero_init: the sweeps start from x = 0 (Vazeros); x's interior is not
+// read, its boundary entries must already be zero.
 template <class T>
 void gs7_fused(const Dim &d, const T *ac, const T *cc, const T *fc, T *x,
-               T *tmp, int nsweeps, int iadjoint, hipStream_t s);
+               T *tmp, int nsweeps, int iadjoint, hipStream_t s,
+               bool zero_init = false);
 // zero_init: x starts from 0 (the caller need not clear it).
 template <class T>
 void gs27(const Dim &d, const T *ac, const T *cc, const T *fc, T *x,
@@ -80,12 +83,6 @@ unsigned matvec_dots_part(const Dim &d, int numdia, const T *ac, const T *cc,
 template <class T>
 void restrict_(const Dim &f, const Dim &c, const T *xin, T *xout, const T *pc,
                hipStream_t s);
-// xout = restriction (Vrestrc2) of the residual fc - A x on the fine level
-// f, in one pass without storing the residual.
-template <class T>
-void resid_restrict(const Dim &f, const Dim &c, int numdia, const T *ac,
-                    const T *cc, const T *fc, const T *x, const T *pc,
-                    T *xout, hipStream_t s);
 // xf(interior) += (*xdamp) * (P xc); xdamp == nullptr means plain
 // interpolation into xf with the fine boundary set to zero.
 template <class T>

@@ -247,8 +247,8 @@ __global__ __launch_bounds__(256) void k_gs27(int nx, int ny, int nz,
 // A workgroup owns a (TX-4) x (TY-4) tile of (i, j) columns and marches a
 // chunk of KC planes in k; each thread holds one column of a TX x TY tile
 // that includes two rings of halo.  Per plane each thread loads only its own
-// point of the six operator arrays and of xin (prefetched three planes
-// ahead into registers) and shares the in-plane operands through LDS:
+// point of the six operator arrays and of xin (prefetched into registers,
+// see the rings below) and shares the in-plane operands through LDS:
 //   - first-colour values ("mid") are computed on the tile minus the outer
 //     ring, reading xin and the west/south couplings oE(i-1), oN(j-1) of
 //     their neighbours from LDS;
@@ -266,7 +266,8 @@ struct SwCo {  // own-point operator values of one plane
   T oC, cc, fc, oE, oN, uC;
 };
 
-template <class T>
+// ZX: xin is zero everywhere (the sweep after Vazeros) and is not read.
+template <class T, bool ZX>
 __global__ __launch_bounds__(SW_TX *SW_TY) void k_gs7_sweep(
     int nx, int ny, int nz, const T *__restrict__ ac, const T *__restrict__ cc,
     const T *__restrict__ fc, const T *__restrict__ xin, T *__restrict__ xout,
@@ -300,6 +301,7 @@ __global__ __launch_bounds__(SW_TX *SW_TY) void k_gs7_sweep(
     P.oC = ac[p], P.cc = cc[p], P.fc = fc[p], P.oE = oE[p], P.oN = oN[p],
     P.uC = uC[p];
   };
+  auto xat = [&](int q) -> T { return ZX ? (T)0 : xin[at(q)]; };
   auto first = [&](int q) {
     return midr && q >= 2 && q <= nz - 1 && ((i + j + q) & 1) == c1;
   };
@@ -328,16 +330,16 @@ __global__ __launch_bounds__(SW_TX *SW_TY) void k_gs7_sweep(
     // loop's back edge.
     SwCo<T> Q;  // plane kb-2: its uC and x enter plane kb-1's first colour
     load(kb - 2, Q);
-    const T xq = xin[at(kb - 2)];
+    const T xq = xat(kb - 2);
     load(kb - 1, C[4]);
-    X[4] = xin[at(kb - 1)];
+    X[4] = xat(kb - 1);
     load(kb, C[0]);
-    X[0] = xin[at(kb)];
-    X[1] = xin[at(kb + 1)];
+    X[0] = xat(kb);
+    X[1] = xat(kb + 1);
     load(kb + 1, C[1]);
-    X[2] = xin[at(kb + 2)];
+    X[2] = xat(kb + 2);
     load(kb + 2, C[2]);
-    X[3] = xin[at(kb + 3)];
+    X[3] = xat(kb + 3);
     SX[(kb - 1) & 1][ty][tx] = X[4];
     SE[(kb - 1) % 3][ty][tx] = C[4].oE;
     SN[(kb - 1) % 3][ty][tx] = C[4].oN;
@@ -359,7 +361,7 @@ __global__ __launch_bounds__(SW_TX *SW_TY) void k_gs7_sweep(
     // planes past ke+2 are never used: the last steps re-read plane ke+2
     // (an L2 hit) so that the loads stay unconditional
     load(min(k + 3, ke + 2), C[(r + 3) % 5]);
-    X[(r + 4) % 5] = xin[at(min(k + 4, ke + 2))];
+    X[(r + 4) % 5] = xat(min(k + 4, ke + 2));
     const T x0 = X[r], x1 = X[(r + 1) % 5], x2 = X[(r + 2) % 5];
     const int s1 = (k + 1) % 3, s0 = k % 3;
     SX[(k + 1) & 1][ty][tx] = x1;
@@ -420,10 +422,12 @@ static int device_cus() {
 // nsweeps full sweeps, ping-ponging between x and tmp (same boundary
 // values); the result is left in x.  The k chunk is sized so that one round
 // of workgroups (one per compute unit: the LDS and registers of a
-// 1024-thread workgroup fill a CU) covers the grid.
+// 1024-thread workgroup fill a CU) covers the grid.  zero_init: x starts
+// at zero (Vazeros), boundary included, and its old contents are not read.
 template <class T>
 void gs7_fused(const Dim &d, const T *ac, const T *cc, const T *fc, T *x,
-               T *tmp, int nsweeps, int iadjoint, hipStream_t s) {
+               T *tmp, int nsweeps, int iadjoint, hipStream_t s,
+               bool zero_init) {
   if (d.nx < 3 || d.ny < 3 || d.nz < 3 || nsweeps <= 0) return;
   const int gx = cdiv(d.nx - 2, SW_TX - 4), gy = cdiv(d.ny - 2, SW_TY - 4);
   const int zb = std::max(1, device_cus() / (gx * gy));
@@ -434,8 +438,12 @@ void gs7_fused(const Dim &d, const T *ac, const T *cc, const T *fc, T *x,
   const int c1 = iadjoint ? 1 : 0;
   T *src = x, *dst = tmp;
   for (int it = 0; it < nsweeps; it++) {
-    k_gs7_sweep<T><<<nblk, b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, fc, src, dst,
-                                      c1, kc, gx, gy, gz);
+    if (zero_init && it == 0)
+      k_gs7_sweep<T, true><<<nblk, b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, fc,
+                                              src, dst, c1, kc, gx, gy, gz);
+    else
+      k_gs7_sweep<T, false><<<nblk, b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, fc,
+                                               src, dst, c1, kc, gx, gy, gz);
     T *t = src;
     src = dst;
     dst = t;
@@ -1016,126 +1024,6 @@ void restrict_(const Dim &f, const Dim &c, const T *xin, T *xout, const T *pc,
   k_restrict<T><<<xcd_blocks(g), dim3(BX, BY), 0, s>>>(f.nx, f.ny, c.nx, c.ny, c.nz, xin,
                                            xout, pc, xg_of(g));
   check_launch("restrict");
-}
-
-// ---------------------------------------------------------------------------
-// Residual and restriction in one pass: Vmresid then Vrestrc2, the down leg
-// of Vmvcs (mgcsd.cc:325-360, matvecd.cc:381-528 and 755-795).  The fine
-// residual never goes to memory.
-// ---------------------------------------------------------------------------
-// A 1024-thread workgroup owns RR_CX x RR_CY coarse columns (ii, jj) and
-// marches a chunk of coarse planes kk.  Thread (tx, ty) holds the fine
-// column (2 ii0 - 2 + tx, 2 jj0 - 2 + ty): together the threads with
-// tx <= 2 RR_CX and ty <= 2 RR_CY cover the fine points 2ii-1 +- 1 of every
-// owned coarse column.  Per coarse plane they compute the residual of fine
-// planes 2kk-1 and 2kk (k_resid's expression, so the same values) into an
-// LDS ring of five fine planes; after one barrier the owners of the coarse
-// columns combine planes 2kk-2 .. 2kk with P in k_restrict's order.  Step kk
-// writes the slots of planes 2kk-6 and 2kk-5, last read in step kk-2, which
-// every thread has left before it passed the barrier of step kk-1.
-constexpr int RR_TX = 64, RR_TY = 16, RR_CX = 31, RR_CY = 7, RR_RING = 5;
-template <class T, int ND>
-__global__ __launch_bounds__(RR_TX *RR_TY) void k_resid_restrict(
-    int nxf, int nyf, int nzf, const T *__restrict__ ac,
-    const T *__restrict__ cc, const T *__restrict__ fc,
-    const T *__restrict__ x, int nxc, int nyc, int nzc,
-    const T *__restrict__ pc, T *__restrict__ xout, int kc, int gx, int gy,
-    int gz) {
-  __shared__ T R[RR_RING][RR_TY][RR_TX];
-  // XCD-aware tile order, as k_gs7_sweep
-  const int ntile = gx * gy * gz, per = (ntile + 7) / 8;
-  const int t = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
-  if (t >= ntile) return;  // whole workgroup, before any barrier
-  const int tx = threadIdx.x, ty = threadIdx.y;
-  const int ii0 = 2 + (t % gx) * RR_CX, jj0 = 2 + ((t / gx) % gy) * RR_CY;
-  const int kb = 2 + (t / (gx * gy)) * kc;
-  const int ke = min(kb + kc - 1, nzc - 1);
-  const size_t n = (size_t)nxf * nyf * nzf, sy = nxf, sz = (size_t)nxf * nyf;
-  // fine column; i, j >= 2 as ii0, jj0 >= 2
-  const int i = 2 * ii0 - 2 + tx, j = 2 * jj0 - 2 + ty;
-  const bool act = tx <= 2 * RR_CX && ty <= 2 * RR_CY && i <= nxf - 1 &&
-                   j <= nyf - 1;
-  const size_t col = (size_t)(j - 1) * sy + (size_t)(i - 1);
-  auto rval = [&](int k) -> T {  // residual at fine (i, j, k), k interior
-    const size_t p = col + (size_t)(k - 1) * sz;
-    if (ND == 7) {
-      const T *oE = ac + n, *oN = ac + 2 * n, *uC = ac + 3 * n;
-      return fc[p] + oN[p] * x[p + sy] + oN[p - sy] * x[p - sy] +
-             oE[p] * x[p + 1] + oE[p - 1] * x[p - 1] +
-             uC[p - sz] * x[p - sz] + uC[p] * x[p + sz] -
-             (ac[p] + cc[p]) * x[p];
-    } else {
-      double sO, sU, sD;
-      sum27<T, T>(ac, n, sy, sz, p, x, sO, sU, sD);
-      return (T)((T)((T)(fc[p] + (T)sO) + (T)sU) + (T)sD) -
-             (ac[p] + cc[p]) * x[p];
-    }
-  };
-  // coarse column of the restricting threads
-  const int tid = tx + RR_TX * ty;
-  const int ii = ii0 + tid % RR_CX, jj = jj0 + tid / RR_CX;
-  const bool own = tid < RR_CX * RR_CY && ii <= nxc - 1 && jj <= nyc - 1;
-  const int lx = 2 * (tid % RR_CX) + 1, ly = 2 * (tid / RR_CX) + 1;
-  const size_t nc = (size_t)nxc * nyc * nzc;
-  if (act) R[(2 * kb - 2) % RR_RING][ty][tx] = rval(2 * kb - 2);
-  for (int kk = kb; kk <= ke; kk++) {
-    if (act) {
-      const T ra = rval(2 * kk - 1), rb = rval(2 * kk);
-      R[(2 * kk - 1) % RR_RING][ty][tx] = ra;
-      R[(2 * kk) % RR_RING][ty][tx] = rb;
-    }
-    __syncthreads();
-    if (own) {
-      const size_t pC = ix(nxc, nyc, ii, jj, kk);
-      // P plane blocks: same fine plane, the one above, the one below
-      const int sl[3] = {(2 * kk - 1) % RR_RING, (2 * kk) % RR_RING,
-                         (2 * kk - 2) % RR_RING};
-      T tb[3];
-#pragma unroll
-      for (int b = 0; b < 3; b++) {
-        const T(*Q)[RR_TX] = R[sl[b]];
-        const T *P = pc + (size_t)(9 * b) * nc + pC;
-        // in-plane order C, N, S, E, W, NE, NW, SE, SW
-        T acc = P[0] * Q[ly][lx];
-        acc += P[nc] * Q[ly + 1][lx];
-        acc += P[2 * nc] * Q[ly - 1][lx];
-        acc += P[3 * nc] * Q[ly][lx + 1];
-        acc += P[4 * nc] * Q[ly][lx - 1];
-        acc += P[5 * nc] * Q[ly + 1][lx + 1];
-        acc += P[6 * nc] * Q[ly + 1][lx - 1];
-        acc += P[7 * nc] * Q[ly - 1][lx + 1];
-        acc += P[8 * nc] * Q[ly - 1][lx - 1];
-        tb[b] = acc;
-      }
-      xout[pC] = tb[0] + tb[1] + tb[2];
-    }
-  }
-}
-
-template <class T>
-void resid_restrict(const Dim &f, const Dim &c, int numdia, const T *ac,
-                    const T *cc, const T *fc, const T *x, const T *pc,
-                    T *xout, hipStream_t s) {
-  if (c.nx < 3 || c.ny < 3 || c.nz < 3) return;
-  if (f.nx != 2 * c.nx - 1 || f.ny != 2 * c.ny - 1 || f.nz != 2 * c.nz - 1) {
-    fprintf(stderr, "pmg: resid_restrict shape mismatch\n");
-    return;
-  }
-  const int gx = cdiv(c.nx - 2, RR_CX), gy = cdiv(c.ny - 2, RR_CY);
-  const int zb = std::max(1, device_cus() / (gx * gy));
-  const int kc = cdiv(c.nz - 2, std::min(zb, c.nz - 2));
-  const int gz = cdiv(c.nz - 2, kc);
-  const int nblk = 8 * cdiv(gx * gy * gz, 8);
-  dim3 b(RR_TX, RR_TY);
-  if (numdia == 7)
-    k_resid_restrict<T, 7><<<nblk, b, 0, s>>>(f.nx, f.ny, f.nz, ac, cc, fc, x,
-                                              c.nx, c.ny, c.nz, pc, xout, kc,
-                                              gx, gy, gz);
-  else
-    k_resid_restrict<T, 27><<<nblk, b, 0, s>>>(f.nx, f.ny, f.nz, ac, cc, fc,
-                                               x, c.nx, c.ny, c.nz, pc, xout,
-                                               kc, gx, gy, gz);
-  check_launch("resid_restrict");
 }
 
 enum : int {
@@ -2256,7 +2144,7 @@ void buildG(const Dim &f, const Dim &c, int numdia, const double *pc,
   template void gs7<T>(const Dim &, const T *, const T *, const T *, T *,     \
                        int, int, hipStream_t);                                \
   template void gs7_fused<T>(const Dim &, const T *, const T *, const T *,    \
-                             T *, T *, int, int, hipStream_t);                \
+                             T *, T *, int, int, hipStream_t, bool);          \
   template void gs27<T>(const Dim &, const T *, const T *, const T *, T *,    \
                         int, int, hipStream_t, bool);                         \
   template void matvec_dots<T>(const Dim &, int, const T *, const T *,        \
@@ -2264,9 +2152,6 @@ void buildG(const Dim &f, const Dim &c, int numdia, const double *pc,
                                hipStream_t);                                  \
   template void restrict_<T>(const Dim &, const Dim &, const T *, T *,        \
                              const T *, hipStream_t);                         \
-  template void resid_restrict<T>(const Dim &, const Dim &, int, const T *,  \
-                                  const T *, const T *, const T *, const T *, \
-                                  T *, hipStream_t);                          \
   template void interp<T>(const Dim &, const Dim &, const T *, const T *,     \
                           T *, const double *, hipStream_t);                  \
   template void interp_damped<T>(const Dim &, const Dim &, const T *,         \

@@ -139,9 +139,12 @@ LV<float> view<float>(Level &l) {
   return {l.d, l.numdia, l.ac32, l.cc32, l.pc32, l.x32, l.w032, l.r32, l.t32};
 }
 
+// zero_init: x starts at zero (Vazeros).  bnd_zero: x's boundary entries
+// are zero already, so a 7-point level skips clearing x and the first sweep
+// does not read it.
 template <class T>
 void smooth(pmg_ctx *c, const LV<T> &l, const T *f, T *x, int nu, int iadj,
-            bool fine, bool zero_init = false) {
+            bool fine, bool zero_init = false, bool bnd_zero = false) {
   if (nu <= 0) {
     if (zero_init) fill<T>(x, l.d.n(), (T)0, c->s);
     return;
@@ -150,11 +153,14 @@ void smooth(pmg_ctx *c, const LV<T> &l, const T *f, T *x, int nu, int iadj,
                c->nev + 2 <= (int)c->ev.size();
   if (timed) hipEventRecord(c->ev[c->nev], c->s);
   if (l.numdia == 7) {
-    if (zero_init) fill<T>(x, l.d.n(), (T)0, c->s);
-    if (l.t && l.t != x)
-      gs7_fused<T>(l.d, l.ac, l.cc, f, x, l.t, nu, iadj, c->s);
-    else
+    if (l.t && l.t != x) {
+      if (zero_init && !bnd_zero) fill<T>(x, l.d.n(), (T)0, c->s);
+      gs7_fused<T>(l.d, l.ac, l.cc, f, x, l.t, nu, iadj, c->s,
+                   zero_init && bnd_zero);
+    } else {
+      if (zero_init) fill<T>(x, l.d.n(), (T)0, c->s);
       gs7<T>(l.d, l.ac, l.cc, f, x, nu, iadj, c->s);
+    }
   } else {
     gs27<T>(l.d, l.ac, l.cc, f, x, nu, iadj, c->s, zero_init);
   }
@@ -187,14 +193,16 @@ bool coarse_fits(pmg_ctx *c) {
 // One V-cycle (Vmvcs body, mgcsd.cc:299-480).  L[0].x is the iterate on
 // the fine level; frhs its right-hand side; cc64c the FP64 Helmholtz term of
 // the coarsest level used by the FP64 coarse solve.
+// fine_zero: the fine iterate starts at zero (the correction cycle of the
+// mixed solve); its boundary entries are zero already.
 template <class T>
 void vcycle(pmg_ctx *c, LV<T> *L, const T *frhs, const double *ac64c,
-            const double *cc64c) {
+            const double *cc64c, bool fine_zero = false) {
   const int nlev = c->nlev;
   const Params &p = c->p;
   hipStream_t s = c->s;
   // nu1 pre-smoothings on the fine grid, then the residual (D1)
-  smooth<T>(c, L[0], frhs, L[0].x, p.nu1, 0, true);
+  smooth<T>(c, L[0], frhs, L[0].x, p.nu1, 0, true, fine_zero, fine_zero);
   resid<T, T>(L[0].d, L[0].numdia, L[0].ac, L[0].cc, frhs, L[0].x, L[0].r,
               nullptr, s);
   for (int l = 1; l < nlev; l++) {  // down (mgcsd.cc:325-360)
@@ -279,11 +287,11 @@ int linear_mixed(pmg_ctx *c, LV<double> *L64, LV<float> *L32,
   int iters = 0;
   const int nlev = c->nlev;
   while (iters < itmax && rsnrm / rsden > errtol) {
-    fill<float>(L32[0].x, F.d.n(), 0.0f, c->s);
     if (nlev == 1) {
+      fill<float>(L32[0].x, F.d.n(), 0.0f, c->s);
       coarse_solve<float>(c, 0, F.ac, L64[0].cc, r32, L32[0].x);
     } else {
-      vcycle<float>(c, L32, r32, c->L[nlev - 1].ac, cc64c);
+      vcycle<float>(c, L32, r32, c->L[nlev - 1].ac, cc64c, true);
     }
     add_correction<float>(F.d, L32[0].x, u, c->s);
     iters++;
@@ -570,11 +578,12 @@ void step_body(pmg_ctx *c, int mode) {
   const Red rnrm = red_to(c, S_RSNRM);
   if (mode == PMG_MIXED) {
     float *r32 = F.w032;
-    fill<float>(L32[0].x, F.d.n(), 0.0f, c->s);
-    if (nlev == 1)
+    if (nlev == 1) {
+      fill<float>(L32[0].x, F.d.n(), 0.0f, c->s);
       coarse_solve<float>(c, 0, F.ac, F.cc, r32, L32[0].x);
-    else
-      vcycle<float>(c, L32, r32, c->L[nlev - 1].ac, cc64c);
+    } else {
+      vcycle<float>(c, L32, r32, c->L[nlev - 1].ac, cc64c, true);
+    }
     add_correction<float>(F.d, L32[0].x, c->u, c->s);
     resid<double, float>(F.d, F.numdia, F.ac, F.cc, F.fc, c->u, r32, &rnrm,
                          c->s);
@@ -1488,27 +1497,6 @@ int pmg_restrc(int nxf, int nyf, int nzf, int nxc, int nyc, int nzc,
          *dpc = S.put(pc, 27 * c.n());
   if (!S.ok) return PMG_ENOMEM;
   restrict_<double>(f, c, dxi, dxo, dpc, 0);
-  return finish(S, xout, dxo, c.n());
-}
-
-int pmg_mresid_restrc(int nxf, int nyf, int nzf, int nxc, int nyc, int nzc,
-                      int numdia, const double *ac, const double *cc,
-                      const double *fc, const double *x, const double *pc,
-                      double *xout) {
-  if (!dims_ok(nxc, nyc, nzc) || !coarse_of(nxf, nxc) ||
-      !coarse_of(nyf, nyc) || !coarse_of(nzf, nzc) ||
-      (numdia != 7 && numdia != 27) || !ac || !cc || !fc || !x || !pc ||
-      !xout)
-    return PMG_EBADPARM;
-  if (!device_ok()) return PMG_EDEVICE;
-  Dim f{nxf, nyf, nzf}, c{nxc, nyc, nzc};
-  const size_t n = f.n();
-  Scratch S;
-  double *dac = S.put(ac, (numdia == 7 ? 4 : 14) * n), *dcc = S.put(cc, n),
-         *dfc = S.put(fc, n), *dx = S.put(x, n), *dpc = S.put(pc, 27 * c.n()),
-         *dxo = S.put(xout, c.n());
-  if (!S.ok) return PMG_ENOMEM;
-  resid_restrict<double>(f, c, numdia, dac, dcc, dfc, dx, dpc, dxo, 0);
   return finish(S, xout, dxo, c.n());
 }
 

@@ -99,40 +99,6 @@ def test_buildP_buildG_transfer(mg, orc, numdia, mgprol):
     assert _rel(xf, orc.interp(nc, nf, e, pco)) < 1e-13
 
 
-@pytest.mark.parametrize("numdia,nf", [(7, 17), (27, 17), (7, 65), (27, 65), (7, 129),
-                                       (27, 129)])
-def test_mresid_restrc_fused(mg, orc, numdia, nf):
-    """The fused residual + restriction of the V-cycle's down leg against the
-    oracle's Vmresid then Vrestrc, and bit for bit against the two separate
-    GPU kernels.  65^3 runs one coarse plane per workgroup, 129^3 chunks of
-    seven coarse planes over 3 x 9 coarse column tiles."""
-    print(f"mresid+restrc fused numdia={numdia} nf={nf}", flush=True)
-    nc = (nf - 1) // 2 + 1
-    rng = np.random.default_rng(7 * nf + numdia)
-    if nf <= 17:
-        ac, cc = H.random_op(nf, numdia, rng)
-        f = H.from_int(nf, rng.standard_normal((nf - 2) ** 3))
-        x = H.from_int(nf, rng.standard_normal((nf - 2) ** 3))
-    else:
-        ac, cc = H.random_op_vec(nf, numdia, rng)
-        mask = np.zeros((nf, nf, nf)); mask[1:-1, 1:-1, 1:-1] = 1
-        f = (rng.standard_normal((nf, nf, nf)) * mask).ravel()
-        x = (rng.standard_normal((nf, nf, nf)) * mask).ravel()
-    pc = orc.buildP(nf, nc, 1, numdia, ac)
-    xo = np.zeros(nc ** 3)
-    _call(mg, "pmg_mresid_restrc", nf, nf, nf, nc, nc, nc, numdia, dp(ac), dp(cc), dp(f),
-          dp(x), dp(pc), dp(xo))
-    ref = orc.restrc(nf, nc, orc.mresid(nf, numdia, ac, cc, f, x), pc)
-    r = np.zeros(nf ** 3)
-    _call(mg, "pmg_mresid", nf, nf, nf, numdia, dp(ac), dp(cc), dp(f), dp(x), dp(r))
-    xs = np.zeros(nc ** 3)
-    _call(mg, "pmg_restrc", nf, nf, nf, nc, nc, nc, dp(r), dp(xs), dp(pc))
-    print(f"  vs oracle {_rel(xo, ref):.2e}, vs separate kernels "
-          f"{np.max(np.abs(xo - xs)):.2e}", flush=True)
-    assert _rel(xo, ref) < 1e-13
-    assert np.array_equal(xo, xs)
-
-
 def _run_driver(mg, fn, inp, n, nlev, nonlin=False, **kw):
     iparm, rparm = mg.make_params(n, nlev, nonlin=1 if nonlin else 0, **kw)
     sz = mg.Vmgsz(2, 0, 0, n, n, n, nlev)
