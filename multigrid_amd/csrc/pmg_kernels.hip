@@ -1186,23 +1186,32 @@ bool cg_fits(const Dim &d) {
          d.interior() <= (size_t)CG_NT * 2;
 }
 
-// Sum over the workgroup with a single barrier: every thread reads the 16
-// per-wavefront partials in the same order, so all get the same value.
-// Consecutive calls must alternate buf (see the barrier argument in the
-// CG loop below).
-__device__ inline double cg_allsum(double v, double *buf, int nw) {
+// Sum over the workgroup with a single barrier: every thread reads the NWM
+// per-wavefront partial slots and adds them in the same pairwise tree, so
+// all get the same value.  Slots of wavefronts the workgroup does not have
+// hold zero (cg_zero_slots at the start of the kernel).  Consecutive calls
+// must alternate buf (see the barrier argument in the CG loop below).
+template <int NWM>
+__device__ inline double cg_allsum(double v, double *buf) {
   v = wave_sum(v);
   if ((threadIdx.x & 63) == 0) buf[threadIdx.x >> 6] = v;
   __syncthreads();
-  // all per-wavefront values are requested before any is added (a loop of
-  // dependent LDS reads would pay the LDS latency once per wavefront)
-  double p[CG_NT / 64];
+  // all per-wavefront values are requested before any is added
+  double p[NWM];
 #pragma unroll
-  for (int w = 0; w < CG_NT / 64; w++) p[w] = w < nw ? buf[w] : 0.0;
-  double t = 0.0;
+  for (int w = 0; w < NWM; w++) p[w] = buf[w];
 #pragma unroll
-  for (int w = 0; w < CG_NT / 64; w++) t += p[w];
-  return t;
+  for (int h = 1; h < NWM; h *= 2) {
+#pragma unroll
+    for (int w = 0; w + h < NWM; w += 2 * h) p[w] += p[w + h];
+  }
+  return p[0];
+}
+
+// Zero both partial buffers (before the kernel's first barrier).
+template <int NWM>
+__device__ inline void cg_zero_slots(double (*rbuf)[NWM]) {
+  if (threadIdx.x < 2 * NWM) rbuf[threadIdx.x / NWM][threadIdx.x % NWM] = 0.0;
 }
 
 // 26 neighbour offsets in the reference's term order (in-plane 8, up 9,
@@ -1243,7 +1252,8 @@ __global__ __launch_bounds__(CG_NT) void k_cg_single(
     T *__restrict__ xout, int itmax, double errtol) {
   extern __shared__ double P[];
   __shared__ double rbuf[2][CG_NT / 64];
-  const int tid = threadIdx.x, NTB = blockDim.x, nw = NTB / 64;
+  cg_zero_slots<CG_NT / 64>(rbuf);
+  const int tid = threadIdx.x, NTB = blockDim.x;
   const size_t n = (size_t)nx * ny * nz;
   const long long sy = nx, sz = (long long)nx * ny;
   const int ni = nx - 2, nj = ny - 2;
@@ -1278,7 +1288,7 @@ __global__ __launch_bounds__(CG_NT) void k_cg_single(
   double s = 0.0;
 #pragma unroll
   for (int m = 0; m < NPT; m++) s += rr[m] * rr[m];
-  double rhok2 = cg_allsum(s, rbuf[0], nw);
+  double rhok2 = cg_allsum<CG_NT / 64>(s, rbuf[0]);
   int b = 1;
   const double denom = sqrt(rhok2);
   int iters = 0;
@@ -1323,7 +1333,7 @@ __global__ __launch_bounds__(CG_NT) void k_cg_single(
         apv[m] = val[m] ? v : 0.0;
         if (val[m]) s += pp * v;
       }
-      const double pAp = cg_allsum(s, rbuf[b], nw);
+      const double pAp = cg_allsum<CG_NT / 64>(s, rbuf[b]);
       b ^= 1;
       const double alpha = rhok2 / pAp;
       rhok1 = rhok2;
@@ -1336,7 +1346,7 @@ __global__ __launch_bounds__(CG_NT) void k_cg_single(
         s += rr[m] * rr[m];
       }
       iters++;
-      rhok2 = cg_allsum(s, rbuf[b], nw);
+      rhok2 = cg_allsum<CG_NT / 64>(s, rbuf[b]);
       b ^= 1;
     }
   }
@@ -1361,16 +1371,19 @@ __global__ __launch_bounds__(CG_NT) void k_cg_single(
 // group row by row.
 constexpr int CG_R = 3, CG_RNT = 512;
 // WARM: start from xout (else 0); ABS: the class solver's absolute
-// criterion (else Vcghs's relative one).  The PMG cycle uses <T, 0, 0>.
-template <class T, bool WARM, bool ABS>
+// criterion (else Vcghs's relative one); ZT: errtol is 0, so Vcghs's test
+// sqrt(r.r)/sqrt(r0.r0) <= errtol is r.r == 0 (no square root or divide in
+// the loop).  The PMG cycle uses <T, 0, 0, 1> (errtol = epsiln = 0).
+template <class T, bool WARM, bool ABS, bool ZT>
 __global__ __launch_bounds__(CG_RNT) void k_cg_rows(
     int nx, int ny, int nz, const double *__restrict__ ac,
     const double *__restrict__ cc, const T *__restrict__ f,
     T *__restrict__ xout, int itmax, double errtol) {
   constexpr int R = CG_R;
   extern __shared__ double P[];
-  __shared__ double rbuf[2][CG_NT / 64];  // cg_allsum reads CG_NT / 64
-  const int tid = threadIdx.x, NTB = blockDim.x, nw = NTB / 64;
+  __shared__ double rbuf[2][CG_RNT / 64];
+  cg_zero_slots<CG_RNT / 64>(rbuf);
+  const int tid = threadIdx.x, NTB = blockDim.x;
   const size_t n = (size_t)nx * ny * nz;
   const long long sy = nx, sz = (long long)nx * ny;
   const int ni = nx - 2, nj = ny - 2, nk = nz - 2;
@@ -1423,7 +1436,7 @@ __global__ __launch_bounds__(CG_RNT) void k_cg_rows(
   if constexpr (!WARM) {
 #pragma unroll
     for (int r = 0; r < R; r++) s += rr[r] * rr[r];
-    rhok2 = cg_allsum(s, rbuf[0], nw);
+    rhok2 = cg_allsum<CG_RNT / 64>(s, rbuf[0]);
     b = 1;
     denom = sqrt(rhok2);
   }
@@ -1436,8 +1449,7 @@ __global__ __launch_bounds__(CG_RNT) void k_cg_rows(
         // ABS: the class solver's criterion, sqrt(r.r) > errtol, its first
         // test made on sqrt(sqrt(r0.r0)) (CG_Solver.h:125-133)
         if (ABS ? (iters == 0 ? sqrt(sqrt(rhok2)) : sqrt(rhok2)) <= errtol
-                : (errtol == 0.0 ? rhok2 == 0.0
-                                 : sqrt(rhok2) / denom <= errtol))
+                : (ZT ? rhok2 == 0.0 : sqrt(rhok2) / denom <= errtol))
           break;
         if (iters >= itmax) break;
         if (iters == 0) {
@@ -1492,7 +1504,7 @@ __global__ __launch_bounds__(CG_RNT) void k_cg_rows(
             s += rr[r] * rr[r];
           }
         }
-        rhok2 = cg_allsum(s, rbuf[b], nw);
+        rhok2 = cg_allsum<CG_RNT / 64>(s, rbuf[b]);
         b ^= 1;
         denom = sqrt(rhok2);
         warm_pass = false;
@@ -1504,7 +1516,7 @@ __global__ __launch_bounds__(CG_RNT) void k_cg_rows(
         apv[r] = val[r] ? v : 0.0;
         if (val[r]) s += pp[r] * v;
       }
-      const double pAp = cg_allsum(s, rbuf[b], nw);
+      const double pAp = cg_allsum<CG_RNT / 64>(s, rbuf[b]);
       b ^= 1;
       const double alpha = rhok2 / pAp;
       rhok1 = rhok2;
@@ -1517,7 +1529,7 @@ __global__ __launch_bounds__(CG_RNT) void k_cg_rows(
         s += rr[r] * rr[r];
       }
       iters++;
-      rhok2 = cg_allsum(s, rbuf[b], nw);
+      rhok2 = cg_allsum<CG_RNT / 64>(s, rbuf[b]);
       b ^= 1;
     }
   }
@@ -1541,9 +1553,14 @@ static void cg_launch(const Dim &d, const double *ac, const double *cc,
   const int nrow = (int)((d.nx - 2 + CG_R - 1) / CG_R) * (d.ny - 2) * (d.nz - 2);
   if (ND == 27 && nrow <= CG_RNT) {
     const int nt = (nrow + 63) / 64 * 64;
-    k_cg_rows<T, false, false>
-        <<<1, nt, lds + (CG_R + 2) * sizeof(double), s>>>(
-            d.nx, d.ny, d.nz, ac, cc, f, x, itmax, errtol);
+    if (errtol == 0.0)
+      k_cg_rows<T, false, false, true>
+          <<<1, nt, lds + (CG_R + 2) * sizeof(double), s>>>(
+              d.nx, d.ny, d.nz, ac, cc, f, x, itmax, errtol);
+    else
+      k_cg_rows<T, false, false, false>
+          <<<1, nt, lds + (CG_R + 2) * sizeof(double), s>>>(
+              d.nx, d.ny, d.nz, ac, cc, f, x, itmax, errtol);
     return;
   }
   // one interior point per thread when they fit: as few wavefronts as the
@@ -1587,7 +1604,7 @@ void cg_class(const Dim &d, int numdia, const double *ac, const double *cc,
   }
   const int nrow = (int)((d.nx - 2 + CG_R - 1) / CG_R) * (d.ny - 2) * (d.nz - 2);
   const int nt = (nrow + 63) / 64 * 64;
-  k_cg_rows<T, true, true>
+  k_cg_rows<T, true, true, false>
       <<<1, nt, d.n() * sizeof(double) + (CG_R + 2) * sizeof(double), s>>>(
           d.nx, d.ny, d.nz, ac, cc, f, x, itmax, thresh);
   check_launch("cg_class");
