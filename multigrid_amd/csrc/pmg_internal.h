@@ -48,10 +48,13 @@ template <class T>
 void gs7_fused(const Dim &d, const T *ac, const T *cc, const T *fc, T *x,
                T *tmp, int nsweeps, int iadjoint, hipStream_t s,
                bool zero_init = false);
-// zero_init: x starts from 0 (the caller need not clear it).
+// zero_init: x starts from 0 (the caller need not clear it).  tmp (may be
+// null): an array of the level's size with the same boundary values as x,
+// the second buffer of the two-launch sweeps; its interior is overwritten.
 template <class T>
 void gs27(const Dim &d, const T *ac, const T *cc, const T *fc, T *x,
-          int nsweeps, int iadjoint, hipStream_t s, bool zero_init = false);
+          int nsweeps, int iadjoint, hipStream_t s, bool zero_init = false,
+          T *tmp = nullptr);
 
 // ---- residual r = f - A x (pmgc/matvecd.cc:381-528) ----
 // TO is the output type (FP64 operator may write an FP32 residual for the

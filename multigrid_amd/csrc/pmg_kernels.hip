@@ -507,19 +507,17 @@ struct G27Part {
   T u, dd, f, dg;  // upper and lower plane sums, fc, oC + cc
 };
 
+// One row (j, k) of class (pz, py) of a 27-point sweep, by one wavefront.
+// The neighbour rows come from four arrays by their offset (dy, dz) from
+// this row: xO (dy, dz even: the row itself), xN (dy odd, dz = 0), xU
+// (dy = 0, dz odd), xNU (dy and dz odd); the new values go to xW.  A
+// sweep kept in one array passes it for all five.
 template <class T, int ADJ, int PPL>
-__global__ __launch_bounds__(256) void k_gs27_rows(
+__device__ __forceinline__ void gs27_row(
     int nx, int ny, int nz, const T *__restrict__ ac, const T *__restrict__ cc,
-    const T *__restrict__ fc, T *__restrict__ x, int pz, int py, int nj,
-    int nrows, int zmask) {
+    const T *__restrict__ fc, int pz, int py, int j, int k, int zmask,
+    const T *xO, const T *xN, const T *xU, const T *xNU, T *xW) {
   const int lane = threadIdx.x & 63;
-  // XCD-contiguous runs of rows (see xcd_tile): the row blocks of one XCD
-  // are neighbours in (j, k), in every class launch of the sweep
-  const int nrb = (nrows + 3) / 4, per = (nrb + 7) / 8;
-  const int rblk = (int)(blockIdx.x % 8u) * per + (int)(blockIdx.x / 8u);
-  const int row = rblk * 4 + (threadIdx.x >> 6);
-  if (rblk >= nrb || row >= nrows) return;  // whole wavefronts; no barrier
-  const int j = 2 + py + 2 * (row % nj), k = 2 + pz + 2 * (row / nj);
   const size_t n = (size_t)nx * ny * nz, sy = nx, sz = (size_t)nx * ny;
   const T *oE = ac + n, *oN = ac + 2 * n, *uC = ac + 3 * n, *oNE = ac + 4 * n,
           *oNW = ac + 5 * n, *uE = ac + 6 * n, *uW = ac + 7 * n,
@@ -539,14 +537,14 @@ __global__ __launch_bounds__(256) void k_gs27_rows(
   auto load = [&](int i) -> G27Raw<T> {
     const size_t p = rb + min(i, nx - 1) - 1;
     G27Raw<T> r;
-    r.cN[0] = oN[p], r.xN[0] = x[p + sy];
-    r.cN[1] = oN[p - sy], r.xN[1] = x[p - sy];
+    r.cN[0] = oN[p], r.xN[0] = xN[p + sy];
+    r.cN[1] = oN[p - sy], r.xN[1] = xN[p - sy];
     r.cE = oE[p];
     r.cW = oE[p - 1];
-    r.c[0] = oNE[p], r.xv[0] = x[p + 1 + sy];
-    r.c[1] = oNW[p], r.xv[1] = x[p - 1 + sy];
-    r.c[2] = oNW[p + 1 - sy], r.xv[2] = x[p + 1 - sy];
-    r.c[3] = oNE[p - 1 - sy], r.xv[3] = x[p - 1 - sy];
+    r.c[0] = oNE[p], r.xv[0] = xN[p + 1 + sy];
+    r.c[1] = oNW[p], r.xv[1] = xN[p - 1 + sy];
+    r.c[2] = oNW[p + 1 - sy], r.xv[2] = xN[p + 1 - sy];
+    r.c[3] = oNE[p - 1 - sy], r.xv[3] = xN[p - 1 - sy];
     const T *cu[9] = {uC, uN, uS, uE, uW, uNE, uNW, uSE, uSW};
     const long long du[9] = {0,
                              (long long)sy,
@@ -559,12 +557,16 @@ __global__ __launch_bounds__(256) void k_gs27_rows(
                              -1 - (long long)sy};
     // the plane below couples through the mirrored plane of the neighbour
     const T *cd[9] = {uC, uS, uN, uW, uE, uSW, uSE, uNW, uNE};
+    // terms 0, 3, 4 stay in the row's column of planes (dy = 0)
+    const bool dy0[9] = {true, false, false, true, true,
+                         false, false, false, false};
 #pragma unroll
     for (int m = 0; m < 9; m++) {
+      const T *xz = dy0[m] ? xU : xNU;
       r.cu[m] = cu[m][p];
-      r.xu[m] = x[p + du[m] + sz];
+      r.xu[m] = xz[p + du[m] + sz];
       r.cd[m] = cd[m][p + du[m] - sz];
-      r.xd[m] = x[p + du[m] - sz];
+      r.xd[m] = xz[p + du[m] - sz];
     }
     r.f = fc[p];
     r.oC = ac[p];
@@ -610,15 +612,15 @@ __global__ __launch_bounds__(256) void k_gs27_rows(
     v0[q] = i0[q] <= nx - 1;
     v1[q] = i1[q] <= nx - 1;
     // current values of the own row, boundary included (i <= nx)
-    const T r0 = x[rb + min(i0[q], nx) - 1], r1 = x[rb + min(i1[q], nx) - 1];
+    const T r0 = xO[rb + min(i0[q], nx) - 1], r1 = xO[rb + min(i1[q], nx) - 1];
     a0[q] = (i0[q] <= nx) ? (zown && v0[q] ? (T)0 : r0) : (T)0;
     a1[q] = (i1[q] <= nx) ? (zown && v1[q] ? (T)0 : r1) : (T)0;
     n0[q] = a0[q];
     n1[q] = a1[q];
   }
-  const T xb1 = x[rb];  // i = 1 boundary
+  const T xb1 = xO[rb];  // i = 1 boundary
   // One lane pair per lane (rows up to 129 points): every load of both
-  // points is issued before any arithmetic, so the launch waits for memory
+  // points is issued before any arithmetic, so the row waits for memory
   // once; only the in-row terms wait for the first colour.  Longer rows
   // load each point's operands just before its update.
   G27Part<T> g0[PPL], g1[PPL];
@@ -677,9 +679,66 @@ __global__ __launch_bounds__(256) void k_gs27_rows(
   }
 #pragma unroll
   for (int q = 0; q < PPL; q++) {
-    if (v0[q]) x[rb + i0[q] - 1] = n0[q];
-    if (v1[q]) x[rb + i1[q] - 1] = n1[q];
+    if (v0[q]) xW[rb + i0[q] - 1] = n0[q];
+    if (v1[q]) xW[rb + i1[q] - 1] = n1[q];
   }
+}
+
+template <class T, int ADJ, int PPL>
+__global__ __launch_bounds__(256) void k_gs27_rows(
+    int nx, int ny, int nz, const T *__restrict__ ac, const T *__restrict__ cc,
+    const T *__restrict__ fc, T *__restrict__ x, int pz, int py, int nj,
+    int nrows, int zmask) {
+  // XCD-contiguous runs of rows (see xcd_tile): the row blocks of one XCD
+  // are neighbours in (j, k), in every class launch of the sweep
+  const int nrb = (nrows + 3) / 4, per = (nrb + 7) / 8;
+  const int rblk = (int)(blockIdx.x % 8u) * per + (int)(blockIdx.x / 8u);
+  const int row = rblk * 4 + (threadIdx.x >> 6);
+  if (rblk >= nrb || row >= nrows) return;  // whole wavefronts; no barrier
+  const int j = 2 + py + 2 * (row % nj), k = 2 + pz + 2 * (row / nj);
+  gs27_row<T, ADJ, PPL>(nx, ny, nz, ac, cc, fc, pz, py, j, k, zmask, x, x, x,
+                        x, x);
+}
+
+// Two row classes of one k parity in one launch: the classes (pz, 1-bp)
+// and then (pz, bp) of the sweep order, bp being the y parity of the class
+// that comes second.  A workgroup of RS2 + 1 wavefronts owns a strip of
+// RS2 rows of class
+// (pz, bp) in one plane k and first updates the rows of class (pz, 1-bp)
+// next to them (one row more than its own; a strip's edge row is also
+// computed by the neighbouring strip, from the same values), then, after a
+// barrier, its own rows.  The sweep reads cur and writes every new row to
+// nxt: a row's old values, and the rows of classes the sweep has not
+// reached, come from cur, which no launch of the sweep writes; the classes
+// already updated come from nxt (the first class of this launch, read by
+// the second after the barrier) and from po, the other k parity's array (cur
+// in the sweep's first launch, nxt in its second).  So no workgroup reads a
+// value another workgroup writes in the same launch, except the duplicated
+// edge rows, which both write with the same bits.
+template <class T, int ADJ, int PPL, int RS2>
+__global__ __launch_bounds__(64 * (RS2 + 1)) void k_gs27_pair(
+    int nx, int ny, int nz, const T *__restrict__ ac, const T *__restrict__ cc,
+    const T *__restrict__ fc, const T *cur, T *nxt, const T *po, int pz,
+    int bp, int nplanes, int nstrips, int zmask) {
+  // XCD-contiguous runs of strips, planes outer
+  const int nt = nplanes * nstrips, per = (nt + 7) / 8;
+  const int t = (int)(blockIdx.x % 8u) * per + (int)(blockIdx.x / 8u);
+  if (t >= nt) return;  // whole workgroup, before the barrier
+  const int k = 2 + pz + 2 * (t / nstrips), st = t % nstrips;
+  const int w = threadIdx.x >> 6;
+  const int jf = 2 + bp + 2 * RS2 * st;            // first own row
+  const int jl = min(jf + 2 * (RS2 - 1), ny - 1);  // own rows are <= jl
+  // first class: rows jf-1, jf+1, ..., up to jl+1, inside 2..ny-1
+  const int j1 = jf - 1 + 2 * w;
+  if (j1 >= 2 && j1 <= ny - 1 && j1 <= jl + 1)
+    gs27_row<T, ADJ, PPL>(nx, ny, nz, ac, cc, fc, pz, 1 - bp, j1, k, zmask,
+                          cur, cur, po, po, nxt);
+  __syncthreads();
+  const int j2 = jf + 2 * w;
+  if (w < RS2 && j2 <= jl)
+    gs27_row<T, ADJ, PPL>(nx, ny, nz, ac, cc, fc, pz, bp, j2, k,
+                          zmask & ~(1 << (2 * pz + 1 - bp)), cur, nxt, po, po,
+                          nxt);
 }
 
 static int count_parity(int n, int par) {  // #{v in [2, n-1] : v%2 == par}
@@ -689,10 +748,64 @@ static int count_parity(int n, int par) {  // #{v in [2, n-1] : v%2 == par}
 
 template <class T>
 void gs27(const Dim &d, const T *ac, const T *cc, const T *fc, T *x,
-          int nsweeps, int iadjoint, hipStream_t s, bool zero_init) {
+          int nsweeps, int iadjoint, hipStream_t s, bool zero_init, T *tmp) {
   if (d.nx < 3 || d.ny < 3 || d.nz < 3) return;
+  const int ppl = d.nx <= 129 ? 1 : (d.nx <= 257 ? 2 : 4);
+  // Two launches per sweep (k_gs27_pair) for pairs of sweeps, ping-ponging
+  // between x and tmp, which must hold the same boundary values.  An odd
+  // last sweep uses the four class launches in x.
+  int done = 0;
+  if (tmp && tmp != x && d.nx <= ROWS_MAXN && d.ny >= 4) {
+    const int bp = iadjoint ? 0 : 1;
+    constexpr int RSA = 3, RSB = 7;
+    // strips of 7 rows (8 wavefronts) on levels of 65^3 and more, of 3 rows
+    // (4 wavefronts, more workgroups) below, where the launch is latency-
+    // rather than bandwidth-bound (measured at 81^3, 41^3, 21^3)
+    const bool big = d.nx >= 65;
+    const int nstrips = cdiv(count_parity(d.ny, bp), big ? RSB : RSA);
+    T *cur = x, *nxt = tmp;
+    for (; done < (nsweeps & ~1); done++) {
+      int zmask = (zero_init && done == 0) ? 0xF : 0;
+      for (int h = 0; h < 2; h++) {
+        const int pz = iadjoint ? 1 - h : h;
+        const int npl = count_parity(d.nz, pz);
+        if (npl > 0) {
+          const int nb = 8 * cdiv(npl * nstrips, 8);
+          const T *po = h == 0 ? cur : nxt;
+#define PAIR_LAUNCH(A, Q)                                                     \
+  do {                                                                        \
+    if (big)                                                                  \
+      k_gs27_pair<T, A, Q, RSB><<<nb, 64 * (RSB + 1), 0, s>>>(                \
+          d.nx, d.ny, d.nz, ac, cc, fc, cur, nxt, po, pz, bp, npl, nstrips,   \
+          zmask);                                                             \
+    else                                                                      \
+      k_gs27_pair<T, A, Q, RSA><<<nb, 64 * (RSA + 1), 0, s>>>(                \
+          d.nx, d.ny, d.nz, ac, cc, fc, cur, nxt, po, pz, bp, npl, nstrips,   \
+          zmask);                                                             \
+  } while (0)
+          if (iadjoint) {
+            if (ppl == 1) PAIR_LAUNCH(1, 1);
+            else if (ppl == 2) PAIR_LAUNCH(1, 2);
+            else PAIR_LAUNCH(1, 4);
+          } else {
+            if (ppl == 1) PAIR_LAUNCH(0, 1);
+            else if (ppl == 2) PAIR_LAUNCH(0, 2);
+            else PAIR_LAUNCH(0, 4);
+          }
+#undef PAIR_LAUNCH
+        }
+        zmask &= ~(0x3 << (2 * pz));
+      }
+      T *t = cur;  // the sweep's result is in nxt
+      cur = nxt;
+      nxt = t;
+    }
+    // an even number of sweeps leaves the result in x
+    check_launch("gs27_pair");
+  }
+  if (done == nsweeps) return;
   if (d.nx <= ROWS_MAXN) {
-    for (int it = 0; it < nsweeps; it++) {
+    for (int it = done; it < nsweeps; it++) {
       int zmask = (zero_init && it == 0) ? 0xF : 0;
       for (int ph = 0; ph < 4; ph++) {
         const int cls = iadjoint ? 3 - ph : ph;
@@ -704,7 +817,6 @@ void gs27(const Dim &d, const T *ac, const T *cc, const T *fc, T *x,
 #define ROWS_LAUNCH(A, Q)                                                     \
   k_gs27_rows<T, A, Q><<<nb, 256, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, fc, x, pz, \
                                           py, nj, nrows, zmask)
-          const int ppl = d.nx <= 129 ? 1 : (d.nx <= 257 ? 2 : 4);
           if (iadjoint) {
             if (ppl == 1) ROWS_LAUNCH(1, 1);
             else if (ppl == 2) ROWS_LAUNCH(1, 2);
@@ -2163,7 +2275,7 @@ void buildG(const Dim &f, const Dim &c, int numdia, const double *pc,
   template void gs7_fused<T>(const Dim &, const T *, const T *, const T *,    \
                              T *, T *, int, int, hipStream_t, bool);          \
   template void gs27<T>(const Dim &, const T *, const T *, const T *, T *,    \
-                        int, int, hipStream_t, bool);                         \
+                        int, int, hipStream_t, bool, T *);                    \
   template void matvec_dots<T>(const Dim &, int, const T *, const T *,        \
                                const T *, const T *, T *, const Red *,        \
                                hipStream_t);                                  \

@@ -162,7 +162,9 @@ void smooth(pmg_ctx *c, const LV<T> &l, const T *f, T *x, int nu, int iadj,
       gs7<T>(l.d, l.ac, l.cc, f, x, nu, iadj, c->s);
     }
   } else {
-    gs27<T>(l.d, l.ac, l.cc, f, x, nu, iadj, c->s, zero_init);
+    // coarse x and t both have zero boundary entries (nothing writes them)
+    gs27<T>(l.d, l.ac, l.cc, f, x, nu, iadj, c->s, zero_init,
+            l.t != x ? l.t : nullptr);
   }
   if (timed) {
     hipEventRecord(c->ev[c->nev + 1], c->s);
@@ -1480,8 +1482,12 @@ int pmg_gsrb(int nx, int ny, int nz, int numdia, const double *ac,
     double *tmp = S.put(x, n);  // same boundary values as x
     if (!S.ok) return PMG_ENOMEM;
     gs7_fused<double>(d, dac, dcc, dfc, dx, tmp, itmax, iadjoint, 0);
-  } else
-    gs27<double>(d, dac, dcc, dfc, dx, itmax, iadjoint, 0);
+  } else {
+    // x's boundary values: the second buffer of the two-launch sweeps
+    double *tmp = S.put(x, n);
+    if (!S.ok) return PMG_ENOMEM;
+    gs27<double>(d, dac, dcc, dfc, dx, itmax, iadjoint, 0, false, tmp);
+  }
   return finish(S, x, dx, n);
 }
 

@@ -55,12 +55,15 @@ def test_gsrb(mg, orc, numdia, iadj):
     assert _rel(x, orc.gsrb(n, numdia, ac, cc, f, x0, 3, iadj)) < 1e-12
 
 
-@pytest.mark.parametrize("numdia,n", [(7, 97), (27, 97), (27, 131), (7, 19), (7, 22),
-                                      (7, 23)])
+@pytest.mark.parametrize("numdia,n", [(7, 97), (27, 97), (27, 131), (27, 41), (7, 19),
+                                      (7, 22), (7, 23)])
 def test_gsrb_large(mg, orc, numdia, n):
     """Several workgroups and k-chunks for the fused 7-point sweep (97^3);
-    the row-parallel 27-point sweep at 97^3 and its per-colour fallback for
-    rows longer than 129 points (131^3).  The fused sweep marches its k
+    the row-parallel 27-point sweep at 97^3 and with two point pairs per
+    lane for rows longer than 129 points (131^3); for 27 points a pair of sweeps
+    runs as two paired-class launches per sweep (strips of 7 rows at 97^3
+    and 131^3, of 3 rows at 41^3), an odd third sweep as the four class
+    launches.  The fused 7-point sweep marches its k
     chunk five planes per loop trip: 8-plane chunks leave 3 planes over, and
     19^3, 22^3, 23^3 end on a chunk of 1, 4 and 5 planes."""
     print(f"gsrb large n={n} numdia={numdia}", flush=True)
@@ -69,7 +72,7 @@ def test_gsrb_large(mg, orc, numdia, n):
     mask = np.zeros((n, n, n)); mask[1:-1, 1:-1, 1:-1] = 1
     f = (rng.standard_normal((n, n, n)) * mask).ravel()
     x0 = (rng.standard_normal((n, n, n)) * mask).ravel()
-    for iadj, sweeps in ((0, 2), (1, 1)):
+    for iadj, sweeps in ((0, 2), (1, 3)):
         x = x0.copy()
         _call(mg, "pmg_gsrb", n, n, n, numdia, dp(ac), dp(cc), dp(f), dp(x), sweeps, iadj)
         assert _rel(x, orc.gsrb(n, numdia, ac, cc, f, x0, sweeps, iadj)) < 1e-12
