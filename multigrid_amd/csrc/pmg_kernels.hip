@@ -192,8 +192,14 @@ __device__ inline T sum7(const T *__restrict__ ac, size_t n, size_t sy,
 }
 
 size_t max_partials(const Dim &d) {
-  return (size_t)cdiv(d.nx, BX) * cdiv(d.ny, BY) * cdiv(d.nz, KCH) + 64;
+  // bound for any k chunk (kchunk() may pick one plane per block)
+  return (size_t)cdiv(d.nx, BX) * cdiv(d.ny, BY) * d.nz + 64;
 }
+
+// Planes per block for the residual and matvec kernels: KCH on large
+// levels; one on levels of at most 40 interior planes, whose few blocks
+// would otherwise wait for KCH dependent memory round trips each.
+static int kchunk(const Dim &d) { return d.nz - 2 <= 40 ? 1 : KCH; }
 
 // ---------------------------------------------------------------------------
 // Smoothers: Vgsrb7x (gsd.cc:60-111) and Vgsrb27x (gsd.cc:113-212)
@@ -857,17 +863,18 @@ template <class T, class TO, int ND>
 __global__ __launch_bounds__(256) void k_resid(
     int nx, int ny, int nz, const T *__restrict__ ac,
     const T *__restrict__ cc, const T *__restrict__ fc,
-    const T *__restrict__ x, TO *__restrict__ r, Red rd, int do_red, XG xg) {
+    const T *__restrict__ x, TO *__restrict__ r, Red rd, int do_red, int kc,
+    XG xg) {
   __shared__ double red[4];
   const size_t n = (size_t)nx * ny * nz, sy = nx, sz = (size_t)nx * ny;
   int bx, by, bz;
   const bool tok = xcd_tile(xg, bx, by, bz);
   const int i = 2 + bx * BX + threadIdx.x;
   const int j = 2 + by * BY + threadIdx.y;
-  const int k0 = 2 + bz * KCH;
+  const int k0 = 2 + bz * kc;
   double acc = 0.0;
   if (tok && i <= nx - 1 && j <= ny - 1) {
-    for (int kk = 0; kk < KCH; kk++) {
+    for (int kk = 0; kk < kc; kk++) {
       const int k = k0 + kk;
       if (k > nz - 1) break;
       const size_t p = ix(nx, ny, i, j, k);
@@ -897,15 +904,16 @@ __global__ __launch_bounds__(256) void k_resid(
 template <class T, class TO>
 void resid(const Dim &d, int numdia, const T *ac, const T *cc, const T *fc,
            const T *x, TO *r, const Red *red, hipStream_t s) {
-  dim3 g(cdiv(d.nx - 2, BX), cdiv(d.ny - 2, BY), cdiv(d.nz - 2, KCH));
+  const int kc = kchunk(d);
+  dim3 g(cdiv(d.nx - 2, BX), cdiv(d.ny - 2, BY), cdiv(d.nz - 2, kc));
   dim3 b(BX, BY);
   const Red rd = red ? *red : Red{};
   if (numdia == 7)
     k_resid<T, TO, 7><<<xcd_blocks(g), b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, fc, x, r, rd,
-                                      red ? 1 : 0, xg_of(g));
+                                      red ? 1 : 0, kc, xg_of(g));
   else
     k_resid<T, TO, 27><<<xcd_blocks(g), b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, fc, x, r, rd,
-                                       red ? 1 : 0, xg_of(g));
+                                       red ? 1 : 0, kc, xg_of(g));
   if (red) red_final<1, 0>(rd, dim3(xcd_blocks(g)), s);
   check_launch("resid");
 }
@@ -1011,17 +1019,18 @@ template <class T, int ND>
 __global__ __launch_bounds__(256) void k_matvec_dots(
     int nx, int ny, int nz, const T *__restrict__ ac,
     const T *__restrict__ cc, const T *__restrict__ x,
-    const T *__restrict__ w0, T *__restrict__ y, Red rd, int do_red, XG xg) {
+    const T *__restrict__ w0, T *__restrict__ y, Red rd, int do_red, int kc,
+    XG xg) {
   __shared__ double red[4];
   const size_t n = (size_t)nx * ny * nz, sy = nx, sz = (size_t)nx * ny;
   int bx, by, bz;
   const bool tok = xcd_tile(xg, bx, by, bz);
   const int i = 2 + bx * BX + threadIdx.x;
   const int j = 2 + by * BY + threadIdx.y;
-  const int k0 = 2 + bz * KCH;
+  const int k0 = 2 + bz * kc;
   double a0 = 0.0, a1 = 0.0;
   if (tok && i <= nx - 1 && j <= ny - 1) {
-    for (int kk = 0; kk < KCH; kk++) {
+    for (int kk = 0; kk < kc; kk++) {
       const int k = k0 + kk;
       if (k > nz - 1) break;
       const size_t p = ix(nx, ny, i, j, k);
@@ -1052,15 +1061,16 @@ template <class T>
 void matvec_dots(const Dim &d, int numdia, const T *ac, const T *cc,
                  const T *x, const T *w0, T *y, const Red *red,
                  hipStream_t s) {
-  dim3 g(cdiv(d.nx - 2, BX), cdiv(d.ny - 2, BY), cdiv(d.nz - 2, KCH));
+  const int kc = kchunk(d);
+  dim3 g(cdiv(d.nx - 2, BX), cdiv(d.ny - 2, BY), cdiv(d.nz - 2, kc));
   dim3 b(BX, BY);
   const Red rd = red ? *red : Red{};
   if (numdia == 7)
     k_matvec_dots<T, 7><<<xcd_blocks(g), b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, x, w0, y,
-                                        rd, red ? 1 : 0, xg_of(g));
+                                        rd, red ? 1 : 0, kc, xg_of(g));
   else
     k_matvec_dots<T, 27><<<xcd_blocks(g), b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, x, w0, y,
-                                         rd, red ? 1 : 0, xg_of(g));
+                                         rd, red ? 1 : 0, kc, xg_of(g));
   if (red) red_final<2, 1>(rd, dim3(xcd_blocks(g)), s);
   check_launch("matvec_dots");
 }
@@ -1069,16 +1079,17 @@ template <class T>
 unsigned matvec_dots_part(const Dim &d, int numdia, const T *ac, const T *cc,
                           const T *x, const T *w0, T *y, double *part,
                           hipStream_t s) {
-  dim3 g(cdiv(d.nx - 2, BX), cdiv(d.ny - 2, BY), cdiv(d.nz - 2, KCH));
+  const int kc = kchunk(d);
+  dim3 g(cdiv(d.nx - 2, BX), cdiv(d.ny - 2, BY), cdiv(d.nz - 2, kc));
   dim3 b(BX, BY);
   Red rd;
   rd.part = part;
   if (numdia == 7)
     k_matvec_dots<T, 7><<<xcd_blocks(g), b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, x, w0, y,
-                                        rd, 1, xg_of(g));
+                                        rd, 1, kc, xg_of(g));
   else
     k_matvec_dots<T, 27><<<xcd_blocks(g), b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, x, w0, y,
-                                         rd, 1, xg_of(g));
+                                         rd, 1, kc, xg_of(g));
   check_launch("matvec_dots");
   return xcd_blocks(g);
 }
