@@ -197,9 +197,14 @@ size_t max_partials(const Dim &d) {
 }
 
 // Planes per block for the residual and matvec kernels: KCH on large
-// levels; one on levels of at most 40 interior planes, whose few blocks
-// would otherwise wait for KCH dependent memory round trips each.
-static int kchunk(const Dim &d) { return d.nz - 2 <= 40 ? 1 : KCH; }
+// levels; one on levels of at most `small` interior planes, whose few
+// blocks would otherwise wait for KCH dependent memory round trips each.
+// The matvec's xdamp partials (one pair per block) are summed again by
+// every workgroup of the following prolongation, so it keeps KCH above 40
+// planes; the residual inside the cycle has no partials.
+static int kchunk(const Dim &d, int small = 40) {
+  return d.nz - 2 <= small ? 1 : KCH;
+}
 
 // ---------------------------------------------------------------------------
 // Smoothers: Vgsrb7x (gsd.cc:60-111) and Vgsrb27x (gsd.cc:113-212)
@@ -904,7 +909,7 @@ __global__ __launch_bounds__(256) void k_resid(
 template <class T, class TO>
 void resid(const Dim &d, int numdia, const T *ac, const T *cc, const T *fc,
            const T *x, TO *r, const Red *red, hipStream_t s) {
-  const int kc = kchunk(d);
+  const int kc = kchunk(d, red ? 40 : 80);
   dim3 g(cdiv(d.nx - 2, BX), cdiv(d.ny - 2, BY), cdiv(d.nz - 2, kc));
   dim3 b(BX, BY);
   const Red rd = red ? *red : Red{};
