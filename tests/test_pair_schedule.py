@@ -123,3 +123,4 @@ def test_paired_schedule_matches_oracle(orc, rs, adj, zero, order):
     ref = orc.gsrb(n, 27, ac, cc, f, np.zeros_like(x0) if zero else x0, 2, adj)
     got = paired_sweeps(n, ac, cc, f, x0, 2, adj, zero, rs, order)
     assert np.array_equal(got, ref)
+
