@@ -339,8 +339,8 @@ __global__ __launch_bounds__(SW_TX *SW_TY) void k_gs7_sweep(
     // (C[2], X[3]) are the ones the first step does not read, so that step
     // waits for the same loads whether it is entered from here or from the
     // loop's back edge.
-    SwCo<T> Q;  // plane kb-2: its uC and x enter plane kb-1's first colour
-    load(kb - 2, Q);
+    // plane kb-2: its uC and x enter plane kb-1's first colour
+    const T uCq = uC[at(kb - 2)];
     const T xq = xat(kb - 2);
     load(kb - 1, C[4]);
     X[4] = xat(kb - 1);
@@ -358,7 +358,7 @@ __global__ __launch_bounds__(SW_TX *SW_TY) void k_gs7_sweep(
     SE[kb % 3][ty][tx] = C[0].oE;
     SN[kb % 3][ty][tx] = C[0].oN;
     __syncthreads();
-    mprev = first(kb - 1) ? fval(kb - 1, C[4], xq, X[0], Q.uC) : X[4];
+    mprev = first(kb - 1) ? fval(kb - 1, C[4], xq, X[0], uCq) : X[4];
   }
   mcur = first(kb) ? fval(kb, C[0], X[4], X[1], C[4].uC) : X[0];
   SM[kb % 3][ty][tx] = mcur;
@@ -369,9 +369,10 @@ __global__ __launch_bounds__(SW_TX *SW_TY) void k_gs7_sweep(
     constexpr int r = decltype(slot)::value;
     const SwCo<T> &P0 = C[r], &P1 = C[(r + 1) % 5];
     const T uCm = C[(r + 4) % 5].uC;  // plane k-1
-    // planes past ke+2 are never used: the last steps re-read plane ke+2
-    // (an L2 hit) so that the loads stay unconditional
-    load(min(k + 3, ke + 2), C[(r + 3) % 5]);
+    // the operator is read up to plane ke+1 and xin up to ke+2; the last
+    // steps re-read those planes (L2 hits) so that the loads stay
+    // unconditional
+    load(min(k + 3, ke + 1), C[(r + 3) % 5]);
     X[(r + 4) % 5] = xat(min(k + 4, ke + 2));
     const T x0 = X[r], x1 = X[(r + 1) % 5], x2 = X[(r + 2) % 5];
     const int s1 = (k + 1) % 3, s0 = k % 3;
