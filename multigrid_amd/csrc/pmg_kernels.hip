@@ -3,9 +3,11 @@
 // Every kernel works on the reference's 1-based Fortran-order arrays with i
 // fastest (pmgc/vmatrix.h:79-80).  Work is mapped with i on the 64-lane
 // wavefront dimension so every stencil load is a coalesced row segment;
-// blocks are 64 x 4 threads over (i, j) and march KCH planes in k.
-// Reductions are deterministic: wavefront shuffles, then LDS in a fixed
-// order, one partial per block, and a single-block final pass.
+// the plain stencil kernels use blocks of 64 x 4 threads over (i, j) that
+// march KCH planes in k (one plane on small levels), the smoothers their own
+// tilings described at each kernel.  Reductions are deterministic:
+// wavefront sums, then LDS in a fixed order, one partial per block, and a
+// single-block final pass.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
