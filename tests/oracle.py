@@ -123,9 +123,11 @@ def nmresid(n, numdia, ac, cc, fc, x):
     return r
 
 
-def gsrb(n, numdia, ac, cc, fc, x, itmax, iadjoint):
+def gsrb(n, numdia, ac, cc, fc, x, itmax, iadjoint, dims=None):
+    """dims = (nx, ny, nz) for a non-cubic grid (default n^3)."""
+    nx, ny, nz = dims if dims else (n, n, n)
     x = x.copy()
-    olib().pmgo_gsrb(n, n, n, numdia, dp(ac), dp(cc), dp(fc), dp(x), itmax, iadjoint)
+    olib().pmgo_gsrb(nx, ny, nz, numdia, dp(ac), dp(cc), dp(fc), dp(x), itmax, iadjoint)
     return x
 
 

@@ -78,6 +78,32 @@ def test_gsrb_large(mg, orc, numdia, n):
         assert _rel(x, orc.gsrb(n, numdia, ac, cc, f, x0, sweeps, iadj)) < 1e-12
 
 
+@pytest.mark.parametrize("numdia", [7, 27])
+@pytest.mark.parametrize("dims", [(33, 25, 19), (23, 41, 17)])
+def test_gsrb_noncubic(mg, orc, numdia, dims):
+    """Non-cubic grids: tile, strip and plane counts differ per axis for the
+    fused 7-point sweep and the paired 27-point launches (strips of 3 rows
+    at these sizes), two sweeps forward and three backward.  Positive random
+    couplings everywhere (the sweep formula, not the discretisation, is
+    under test)."""
+    print(f"gsrb non-cubic dims={dims} numdia={numdia}", flush=True)
+    nx, ny, nz = dims
+    N = nx * ny * nz
+    rng = np.random.default_rng(nx + 3 * ny + 7 * nz + numdia)
+    nplanes = 4 if numdia == 7 else 14
+    ac = rng.uniform(0.1, 1.0, nplanes * N)
+    ac[:N] += 2.0 * (nplanes - 1)  # oC dominates the couplings
+    cc = rng.uniform(0.0, 1.0, N)
+    mask = np.zeros((nz, ny, nx)); mask[1:-1, 1:-1, 1:-1] = 1
+    f = (rng.standard_normal((nz, ny, nx)) * mask).ravel()
+    x0 = (rng.standard_normal((nz, ny, nx)) * mask).ravel()
+    for iadj, sweeps in ((0, 2), (1, 3)):
+        x = x0.copy()
+        _call(mg, "pmg_gsrb", nx, ny, nz, numdia, dp(ac), dp(cc), dp(f), dp(x), sweeps, iadj)
+        ref = orc.gsrb(0, numdia, ac, cc, f, x0, sweeps, iadj, dims=dims)
+        assert _rel(x, ref) < 1e-12, (iadj, sweeps)
+
+
 @pytest.mark.parametrize("mgprol", [0, 1])
 @pytest.mark.parametrize("numdia", [7, 27])
 def test_buildP_buildG_transfer(mg, orc, numdia, mgprol):
