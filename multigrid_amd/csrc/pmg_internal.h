@@ -84,8 +84,9 @@ unsigned matvec_dots_part(const Dim &d, int numdia, const T *ac, const T *cc,
 
 // ---- grid transfer (pmgc/matvecd.cc:695-983) ----
 template <class T>
+// out32, when non-null, also receives the coarse interior rounded to FP32.
 void restrict_(const Dim &f, const Dim &c, const T *xin, T *xout, const T *pc,
-               hipStream_t s);
+               hipStream_t s, float *out32 = nullptr);
 // xf(interior) += (*xdamp) * (P xc); xdamp == nullptr means plain
 // interpolation into xf with the fine boundary set to zero.
 template <class T>
@@ -179,15 +180,20 @@ void xpay_interior(const Dim &d, const double *x, double damp,
                    const double *dx, double *out, hipStream_t s);
 
 // ---- setup (pmgc/buildAd.cc, buildPd.cc, buildGd.cc), FP64 ----
+// A non-null *32 output also receives the interior values rounded to FP32
+// (the boundary entries are left as they are: zero from allocation).
 void buildA(const Dim &d, double *ac, double *cc, double *fc,
             const double *xf, const double *yf, const double *zf,
             const double *gx, const double *gy, const double *gz,
             const double *a1, const double *a2, const double *a3,
-            const double *ccf, const double *fcf, hipStream_t s);
+            const double *ccf, const double *fcf, hipStream_t s,
+            float *ac32 = nullptr, float *cc32 = nullptr);
 void buildP(const Dim &f, const Dim &c, int mgprol, int numdia,
-            const double *acf, double *pc, hipStream_t s);
+            const double *acf, double *pc, hipStream_t s,
+            float *pc32 = nullptr);
 void buildG(const Dim &f, const Dim &c, int numdia, const double *pc,
-            const double *acf, double *acc, hipStream_t s);
+            const double *acf, double *acc, hipStream_t s,
+            float *acc32 = nullptr);
 
 // ---- Poisson-Boltzmann input assembly from atoms (PBE_pmgc_class.h:191-344)
 // atoms: device array natoms x {x, y, z, radius, charge}; next: natoms ints;

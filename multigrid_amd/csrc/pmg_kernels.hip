@@ -1110,7 +1110,8 @@ __global__ __launch_bounds__(256) void k_restrict(int nxf, int nyf, int nxc,
                                                   int nyc, int nzc,
                                                   const T *__restrict__ xin,
                                                   T *__restrict__ xout,
-                                                  const T *__restrict__ pc, XG xg) {
+                                                  const T *__restrict__ pc, XG xg,
+                                                  float *__restrict__ out32) {
   int bx, by, bz;
   if (!xcd_tile(xg, bx, by, bz)) return;
   const int ii = 2 + bx * BX + threadIdx.x;
@@ -1140,12 +1141,14 @@ __global__ __launch_bounds__(256) void k_restrict(int nxf, int nyf, int nxc,
       acc += pc[(size_t)(9 * b + q) * nc + pC] * xin[f + zo[b] + off[q]];
     t[b] = acc;
   }
-  xout[pC] = t[0] + t[1] + t[2];
+  const T v = t[0] + t[1] + t[2];
+  xout[pC] = v;
+  if (out32) out32[pC] = (float)v;
 }
 
 template <class T>
 void restrict_(const Dim &f, const Dim &c, const T *xin, T *xout, const T *pc,
-               hipStream_t s) {
+               hipStream_t s, float *out32) {
   if (c.nx < 3 || c.ny < 3 || c.nz < 3) return;
   if (f.nx != 2 * c.nx - 1 || f.ny != 2 * c.ny - 1 || f.nz != 2 * c.nz - 1) {
     fprintf(stderr, "pmg: restrict shape mismatch\n");
@@ -1153,7 +1156,7 @@ void restrict_(const Dim &f, const Dim &c, const T *xin, T *xout, const T *pc,
   }
   dim3 g(cdiv(c.nx - 2, BX), cdiv(c.ny - 2, BY), c.nz - 2);
   k_restrict<T><<<xcd_blocks(g), dim3(BX, BY), 0, s>>>(f.nx, f.ny, c.nx, c.ny, c.nz, xin,
-                                           xout, pc, xg_of(g));
+                                           xout, pc, xg_of(g), out32);
   check_launch("restrict");
 }
 
@@ -1986,7 +1989,8 @@ __global__ __launch_bounds__(256) void k_buildA(
     const double *__restrict__ gx, const double *__restrict__ gy,
     const double *__restrict__ gz, const double *__restrict__ a1,
     const double *__restrict__ a2, const double *__restrict__ a3,
-    const double *__restrict__ ccf, const double *__restrict__ fcf) {
+    const double *__restrict__ ccf, const double *__restrict__ fcf,
+    float *__restrict__ ac32, float *__restrict__ cc32) {
   const int i = 2 + blockIdx.x * BX + threadIdx.x;
   const int j = 2 + blockIdx.y * BY + threadIdx.y;
   const int k = 2 + blockIdx.z;
@@ -2006,19 +2010,24 @@ __global__ __launch_bounds__(256) void k_buildA(
   const double cF = (hxm1 + hx) * (hym1 + hy) * (hzm1 + hz) / 8.0;
   const size_t pxm = p - 1, pym = p - nx, pzm = p - (size_t)nx * ny;
   double f = cF * fcf[p];
-  cc[p] = cF * ccf[p];
-  oC[p] = cE * a1[p] + cEm * a1[pxm] + cN * a2[p] + cNm * a2[pym] +
-          cU * a3[p] + cUm * a3[pzm];
+  const double vcc = cF * ccf[p];
+  const double vC = cE * a1[p] + cEm * a1[pxm] + cN * a2[p] + cNm * a2[pym] +
+                    cU * a3[p] + cUm * a3[pzm];
+  cc[p] = vcc;
+  oC[p] = vC;
   const int ike = (i == nx - 1) ? 0 : 1;
-  oE[p] = ike * cE * a1[p];
+  const double vE = ike * cE * a1[p];
+  oE[p] = vE;
   f += (1 - ike) * cE * a1[p] * gx[(j - 1) + (size_t)(k - 1) * ny +
                                    (size_t)ny * nz];
   const int jke = (j == ny - 1) ? 0 : 1;
-  oN[p] = jke * cN * a2[p];
+  const double vN = jke * cN * a2[p];
+  oN[p] = vN;
   f += (1 - jke) * cN * a2[p] * gy[(i - 1) + (size_t)(k - 1) * nx +
                                    (size_t)nx * nz];
   const int kke = (k == nz - 1) ? 0 : 1;
-  uC[p] = kke * cU * a3[p];
+  const double vU = kke * cU * a3[p];
+  uC[p] = vU;
   f += (1 - kke) * cU * a3[p] * gz[(i - 1) + (size_t)(j - 1) * nx +
                                    (size_t)nx * ny];
   const int ikw = (i == 2) ? 0 : 1;
@@ -2028,16 +2037,25 @@ __global__ __launch_bounds__(256) void k_buildA(
   const int kkd = (k == 2) ? 0 : 1;
   f += (1 - kkd) * cUm * a3[pzm] * gz[(i - 1) + (size_t)(j - 1) * nx];
   fc[p] = f;
+  if (ac32) {  // the FP32 copies, rounded as a separate conversion would
+    cc32[p] = (float)vcc;
+    ac32[p] = (float)vC;
+    ac32[n + p] = (float)vE;
+    ac32[2 * n + p] = (float)vN;
+    ac32[3 * n + p] = (float)vU;
+  }
 }
 
 void buildA(const Dim &d, double *ac, double *cc, double *fc,
             const double *xf, const double *yf, const double *zf,
             const double *gx, const double *gy, const double *gz,
             const double *a1, const double *a2, const double *a3,
-            const double *ccf, const double *fcf, hipStream_t s) {
+            const double *ccf, const double *fcf, hipStream_t s,
+            float *ac32, float *cc32) {
   dim3 g(cdiv(d.nx - 2, BX), cdiv(d.ny - 2, BY), d.nz - 2);
   k_buildA<<<g, dim3(BX, BY), 0, s>>>(d.nx, d.ny, d.nz, ac, cc, fc, xf, yf,
-                                      zf, gx, gy, gz, a1, a2, a3, ccf, fcf);
+                                      zf, gx, gy, gz, a1, a2, a3, ccf, fcf,
+                                      ac32, cc32);
   check_launch("buildA");
 }
 
@@ -2085,6 +2103,7 @@ __global__ __launch_bounds__(128) void k_buildP(int nxf, int nyf, int nzf,
                                                 int nxc, int nyc, int nzc,
                                                 const double *__restrict__ ac,
                                                 double *__restrict__ pc,
+                                                float *__restrict__ pc32,
                                                 int mgprol, XG xg) {
   int bx, by, bz;
   if (!xcd_tile(xg, bx, by, bz)) return;
@@ -2151,19 +2170,20 @@ __global__ __launch_bounds__(128) void k_buildP(int nxf, int nyf, int nzf,
   for (int a = 0; a < 27; a++) {
     const int dx = a % 3 - 1, dy = (a / 3) % 3 - 1, dz = a / 9 - 1;
     pc[(size_t)pplane(dx, dy, dz) * nc + pC] = w[a];
+    if (pc32) pc32[(size_t)pplane(dx, dy, dz) * nc + pC] = (float)w[a];
   }
 }
 
 void buildP(const Dim &f, const Dim &c, int mgprol, int numdia,
-            const double *acf, double *pc, hipStream_t s) {
+            const double *acf, double *pc, hipStream_t s, float *pc32) {
   if (c.nx < 3 || c.ny < 3 || c.nz < 3) return;
   dim3 g(cdiv(c.nx - 2, 64), cdiv(c.ny - 2, 2), c.nz - 2);
   if (numdia == 7)
     k_buildP<7><<<xcd_blocks(g), dim3(64, 2), 0, s>>>(f.nx, f.ny, f.nz, c.nx, c.ny, c.nz,
-                                          acf, pc, mgprol, xg_of(g));
+                                          acf, pc, pc32, mgprol, xg_of(g));
   else
     k_buildP<27><<<xcd_blocks(g), dim3(64, 2), 0, s>>>(f.nx, f.ny, f.nz, c.nx, c.ny, c.nz,
-                                           acf, pc, mgprol, xg_of(g));
+                                           acf, pc, pc32, mgprol, xg_of(g));
   check_launch("buildP");
 }
 
@@ -2244,7 +2264,8 @@ __global__ __launch_bounds__(64) void k_buildG(int nxf, int nyf, int nzf,
                                                int nxc, int nyc, int nzc,
                                                const double *__restrict__ pc,
                                                const double *__restrict__ ac,
-                                               double *__restrict__ acc) {
+                                               double *__restrict__ acc,
+                                               float *__restrict__ acc32) {
   const int ii = 2 + blockIdx.x * 64 + threadIdx.x;
   const int jj = 2 + blockIdx.y;
   const int kk = 2 + blockIdx.z / 14;
@@ -2269,19 +2290,21 @@ __global__ __launch_bounds__(64) void k_buildG(int nxf, int nyf, int nzf,
     RAP_CASE(11) RAP_CASE(12) RAP_CASE(13)
   }
 #undef RAP_CASE
-  acc[(size_t)sidx * nc + pC] = sidx == 0 ? v : -v;
+  const double out = sidx == 0 ? v : -v;
+  acc[(size_t)sidx * nc + pC] = out;
+  if (acc32) acc32[(size_t)sidx * nc + pC] = (float)out;
 }
 
 void buildG(const Dim &f, const Dim &c, int numdia, const double *pc,
-            const double *acf, double *acc, hipStream_t s) {
+            const double *acf, double *acc, hipStream_t s, float *acc32) {
   if (c.nx < 3 || c.ny < 3 || c.nz < 3) return;
   dim3 g(cdiv(c.nx - 2, 64), c.ny - 2, (c.nz - 2) * 14);
   if (numdia == 7)
     k_buildG<7><<<g, 64, 0, s>>>(f.nx, f.ny, f.nz, c.nx, c.ny, c.nz, pc, acf,
-                                 acc);
+                                 acc, acc32);
   else
     k_buildG<27><<<g, 64, 0, s>>>(f.nx, f.ny, f.nz, c.nx, c.ny, c.nz, pc, acf,
-                                  acc);
+                                  acc, acc32);
   check_launch("buildG");
 }
 
@@ -2299,7 +2322,7 @@ void buildG(const Dim &f, const Dim &c, int numdia, const double *pc,
                                const T *, const T *, T *, const Red *,        \
                                hipStream_t);                                  \
   template void restrict_<T>(const Dim &, const Dim &, const T *, T *,        \
-                             const T *, hipStream_t);                         \
+                             const T *, hipStream_t, float *);                \
   template void interp<T>(const Dim &, const Dim &, const T *, const T *,     \
                           T *, const double *, hipStream_t);                  \
   template void interp_damped<T>(const Dim &, const Dim &, const T *,         \

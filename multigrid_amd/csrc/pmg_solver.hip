@@ -374,13 +374,21 @@ void convert_ops32(pmg_ctx *c) {
 void build_ops(pmg_ctx *c) {
   hipStream_t s = c->s;
   Level &F = c->L[0];
+  // With the FP32 hierarchy allocated, the build kernels write its copies
+  // as they go (the same values convert_ops32 would give: every entry the
+  // kernels do not write is zero in both precisions).
+  const bool w32 = c->have32;
   buildA(F.d, F.ac, F.cc, F.fc, c->xf, c->yf, c->zf, c->gx, c->gy, c->gz,
-         c->a1, c->a2, c->a3, c->ccf, c->fcf, s);
+         c->a1, c->a2, c->a3, c->ccf, c->fcf, s, w32 ? F.ac32 : nullptr,
+         w32 ? F.cc32 : nullptr);
   for (int l = 1; l < c->nlev; l++) {
     Level &Lf = c->L[l - 1], &Lc = c->L[l];
-    buildP(Lf.d, Lc.d, c->p.mgprol, Lf.numdia, Lf.ac, Lc.pc, s);
-    buildG(Lf.d, Lc.d, Lf.numdia, Lc.pc, Lf.ac, Lc.ac, s);
-    restrict_<double>(Lf.d, Lc.d, Lf.cc, Lc.cc, Lc.pc, s);
+    buildP(Lf.d, Lc.d, c->p.mgprol, Lf.numdia, Lf.ac, Lc.pc, s,
+           w32 ? Lc.pc32 : nullptr);
+    buildG(Lf.d, Lc.d, Lf.numdia, Lc.pc, Lf.ac, Lc.ac, s,
+           w32 ? Lc.ac32 : nullptr);
+    restrict_<double>(Lf.d, Lc.d, Lf.cc, Lc.cc, Lc.pc, s,
+                      w32 ? Lc.cc32 : nullptr);
     restrict_<double>(Lf.d, Lc.d, Lf.fc, Lc.fc, Lc.pc, s);
   }
   if (c->p.mgsolv == 1) {  // Vbuildband + Vdpbfa (mgsubd.cc:249-262)
@@ -388,11 +396,7 @@ void build_ops(pmg_ctx *c) {
     band_build_factor(C.d, C.numdia, C.ac, c->band, c->band_rt, c->band_ct,
                       c->scal + S_BINFO, s);
   }
-  c->ops32_ok = false;
-  if (c->have32) {
-    convert_ops32(c);
-    c->ops32_ok = true;
-  }
+  c->ops32_ok = w32;
 }
 
 template <class T>

@@ -171,9 +171,13 @@ def test_ctx_fp64_and_mixed(mg, orc):
     ctx = mg.Context(iparm, rparm, inp)
     ctx.solve(mg.PMG_FP64)
     u64, s64 = ctx.u(), ctx.stats()
-    ctx.solve(mg.PMG_MIXED)
+    ctx.solve(mg.PMG_MIXED)  # first FP32 use: operators converted after setup
     umx, smx = ctx.u(), ctx.stats()
+    ctx.rebuild()  # FP32 copies now written by the setup kernels themselves
+    ctx.solve(mg.PMG_MIXED)
+    umx2 = ctx.u()
     ctx.close()
+    assert np.array_equal(umx2, umx), "FP32 operators from setup differ"
     _, u0, _, _ = orc.solve(inp, n, nlev)
     it0 = orc.last_stats()[0]
     print(f"  fp64 iters={s64['iters']} mixed iters={smx['iters']} oracle={it0} "
