@@ -2256,9 +2256,11 @@ __device__ inline double rap_entry(const double *__restrict__ pc,
   return acc;
 }
 
-// Grid: (coarse i tiles, coarse j, coarse k x 14 entries); one thread per
-// (coarse point, stored entry), so each coarse point's 14 entries run in 14
-// different wavefronts (enough of them in flight to hide the gather latency).
+// Grid: (64-point runs of a coarse plane's interior, taken in i-fastest
+// order across row ends so that every lane has a point, coarse k x 14
+// entries); one thread per (coarse point, stored entry), so each coarse
+// point's 14 entries run in 14 different wavefronts (enough of them in
+// flight to hide the gather latency).
 template <int ND>
 __global__ __launch_bounds__(64) void k_buildG(int nxf, int nyf, int nzf,
                                                int nxc, int nyc, int nzc,
@@ -2266,11 +2268,12 @@ __global__ __launch_bounds__(64) void k_buildG(int nxf, int nyf, int nzf,
                                                const double *__restrict__ ac,
                                                double *__restrict__ acc,
                                                float *__restrict__ acc32) {
-  const int ii = 2 + blockIdx.x * 64 + threadIdx.x;
-  const int jj = 2 + blockIdx.y;
+  const int q = blockIdx.x * 64 + threadIdx.x;
+  if (q >= (nxc - 2) * (nyc - 2)) return;
+  const int ii = 2 + q % (nxc - 2);
+  const int jj = 2 + q / (nxc - 2);
   const int kk = 2 + blockIdx.z / 14;
   const int sidx = blockIdx.z % 14;
-  if (ii > nxc - 1) return;
   const size_t nf = (size_t)nxf * nyf * nzf, nc = (size_t)nxc * nyc * nzc;
   const size_t pC = ix(nxc, nyc, ii, jj, kk);
   const size_t pCf = ix(nxf, nyf, 2 * ii - 1, 2 * jj - 1, 2 * kk - 1);
@@ -2298,7 +2301,7 @@ __global__ __launch_bounds__(64) void k_buildG(int nxf, int nyf, int nzf,
 void buildG(const Dim &f, const Dim &c, int numdia, const double *pc,
             const double *acf, double *acc, hipStream_t s, float *acc32) {
   if (c.nx < 3 || c.ny < 3 || c.nz < 3) return;
-  dim3 g(cdiv(c.nx - 2, 64), c.ny - 2, (c.nz - 2) * 14);
+  dim3 g(cdiv((c.nx - 2) * (c.ny - 2), 64), 1, (c.nz - 2) * 14);
   if (numdia == 7)
     k_buildG<7><<<g, 64, 0, s>>>(f.nx, f.ny, f.nz, c.nx, c.ny, c.nz, pc, acf,
                                  acc, acc32);

@@ -228,6 +228,10 @@ __global__ __launch_bounds__(FACE_NT) void k_pb_faces(
     __syncthreads();
     const int per = (m + FACE_SPLIT - 1) / FACE_SPLIT;
     const int t0 = split * per, t1 = min(m, t0 + per);
+    // two atoms per trip: the pair of exp chains interleave, and the
+    // polynomial's coefficients stay operands of three-address FMAs
+    // instead of being copied into the accumulator each time
+#pragma unroll 2
     for (int t = t0; t < t1; t++) {
       const double dx = X - sx[t], dy = Y - sy[t], dz = Z - sz[t];
       const double d2 = dx * dx + dy * dy + dz * dz;
