@@ -2107,10 +2107,13 @@ __global__ __launch_bounds__(128) void k_buildP(int nxf, int nyf, int nzf,
                                                 int mgprol, XG xg) {
   int bx, by, bz;
   if (!xcd_tile(xg, bx, by, bz)) return;
-  const int ii = 2 + bx * 64 + threadIdx.x;
-  const int jj = 2 + by * 2 + threadIdx.y;
+  // 128 consecutive interior points of coarse plane kk, i fastest across
+  // row ends, so that short coarse rows leave no lanes idle
+  const int q = bx * 128 + threadIdx.y * 64 + threadIdx.x;
+  if (q >= (nxc - 2) * (nyc - 2)) return;
+  const int ii = 2 + q % (nxc - 2);
+  const int jj = 2 + q / (nxc - 2);
   const int kk = 2 + bz;
-  if (ii > nxc - 1 || jj > nyc - 1) return;
   const size_t nf = (size_t)nxf * nyf * nzf, nc = (size_t)nxc * nyc * nzc;
   const size_t pC = ix(nxc, nyc, ii, jj, kk);
   const size_t pCf = ix(nxf, nyf, 2 * ii - 1, 2 * jj - 1, 2 * kk - 1);
@@ -2177,7 +2180,7 @@ __global__ __launch_bounds__(128) void k_buildP(int nxf, int nyf, int nzf,
 void buildP(const Dim &f, const Dim &c, int mgprol, int numdia,
             const double *acf, double *pc, hipStream_t s, float *pc32) {
   if (c.nx < 3 || c.ny < 3 || c.nz < 3) return;
-  dim3 g(cdiv(c.nx - 2, 64), cdiv(c.ny - 2, 2), c.nz - 2);
+  dim3 g(cdiv((c.nx - 2) * (c.ny - 2), 128), 1, c.nz - 2);
   if (numdia == 7)
     k_buildP<7><<<xcd_blocks(g), dim3(64, 2), 0, s>>>(f.nx, f.ny, f.nz, c.nx, c.ny, c.nz,
                                           acf, pc, pc32, mgprol, xg_of(g));
