@@ -106,6 +106,31 @@ def test_gsrb_noncubic(mg, orc, numdia, dims):
 
 @pytest.mark.parametrize("mgprol", [0, 1])
 @pytest.mark.parametrize("numdia", [7, 27])
+def test_buildP_buildG_noncubic(mg, orc, numdia, mgprol):
+    """One non-cubic coarsening step, 33x25x19 -> 17x13x10: the prolongation
+    and Galerkin kernels give each lane one of the consecutive interior
+    points of a coarse plane, across row ends, with 165 points per plane
+    (several workgroups per plane, a partial last one).  Positive random
+    couplings everywhere, diagonal dominant."""
+    print(f"buildP/buildG non-cubic numdia={numdia} mgprol={mgprol}", flush=True)
+    df, dc = (33, 25, 19), (17, 13, 10)
+    Nf, Nc = df[0] * df[1] * df[2], dc[0] * dc[1] * dc[2]
+    rng = np.random.default_rng(11 * numdia + mgprol)
+    nplanes = 4 if numdia == 7 else 14
+    ac = rng.uniform(0.1, 1.0, nplanes * Nf)
+    ac[:Nf] += 2.0 * (nplanes - 1)
+    pc, pco = np.zeros(27 * Nc), np.zeros(27 * Nc)
+    _call(mg, "pmg_buildP", *df, *dc, mgprol, numdia, dp(ac), dp(pc))
+    orc.olib().pmgo_buildP(*df, *dc, mgprol, numdia, dp(ac), dp(pco))
+    assert np.abs(pco).max() > 0 and _rel(pc, pco) < 1e-13
+    acc, acco = np.zeros(14 * Nc), np.zeros(14 * Nc)
+    _call(mg, "pmg_buildG", *df, *dc, numdia, dp(pco), dp(ac), dp(acc))
+    orc.olib().pmgo_buildG(*df, *dc, numdia, dp(pco), dp(ac), dp(acco))
+    assert np.abs(acco).max() > 0 and _rel(acc, acco) < 1e-12
+
+
+@pytest.mark.parametrize("mgprol", [0, 1])
+@pytest.mark.parametrize("numdia", [7, 27])
 def test_buildP_buildG_transfer(mg, orc, numdia, mgprol):
     print(f"buildP/buildG numdia={numdia} mgprol={mgprol}", flush=True)
     nf, nc = 17, 9
