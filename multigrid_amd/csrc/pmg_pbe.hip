@@ -186,21 +186,48 @@ __global__ __launch_bounds__(256) void k_pb_nodes(
 
 constexpr int FACE_NT = 256, FACE_NODES = 64, FACE_SPLIT = FACE_NT / FACE_NODES;
 
-// Debye-Hueckel faces.  A workgroup takes 64 boundary nodes of one face;
-// its four wavefronts each sum a quarter of every staged chunk of atoms
-// (all lanes of a wavefront read the same atom: an LDS broadcast), and the
-// four partial sums are added in a fixed order.  Per term the atom factor
-// lambda_B q exp(kappa r) / (eps_w (1 + kappa r)) is staged once, leaving
-// one exp and one reciprocal square root per (node, atom).  The sum is
-// deterministic; its order differs from the serial one by rounding only.
+// Debye-Hueckel faces.  A workgroup takes 64 boundary node pairs of one
+// axis: each lane owns the node (a, b) on the low face and on the high face
+// of that axis, which share the two in-plane coordinate differences to
+// every atom.  Its four wavefronts each sum a quarter of every staged chunk
+// of atoms (all lanes of a wavefront read the same atom: an LDS broadcast),
+// and the four partial sums are added in a fixed order.  Per term the atom
+// factor lambda_B q exp(kappa r) / (eps_w (1 + kappa r)) is staged once,
+// leaving one exp and one reciprocal square root per (node, atom).  The sum
+// is deterministic; its order differs from the serial one by rounding only.
+// Atoms [t0, t1) of the staged chunk for the node pair at in-plane
+// coordinates (va, vb) of axis AX: su/sw hold the atoms' in-plane
+// coordinates, sn the normal one.  The squared distance is summed in x, y, z
+// order, as the serial restatement does.
+template <int AX>
+__device__ __forceinline__ void face_terms(int t0, int t1, double va, double vb,
+                                           double lo, double hi, double kap,
+                                           const double *su, const double *sw,
+                                           const double *sn, const double *sc,
+                                           double &s0, double &s1) {
+#pragma clang fp contract(off)
+  for (int t = t0; t < t1; t++) {
+    const double du = va - su[t], dw = vb - sw[t];
+    const double d0 = lo - sn[t], d1 = hi - sn[t];
+    const double d0s = d0 * d0, d1s = d1 * d1, dus = du * du, dws = dw * dw;
+    const double r0 = AX == 0 ? d0s + dus + dws
+                              : (AX == 1 ? dus + d0s + dws : dus + dws + d0s);
+    const double r1 = AX == 0 ? d1s + dus + dws
+                              : (AX == 1 ? dus + d1s + dws : dus + dws + d1s);
+    const double i0 = rsqrt(r0), i1 = rsqrt(r1);
+    s0 += sc[t] * exp(-kap * (r0 * i0)) * i0;
+    s1 += sc[t] * exp(-kap * (r1 * i1)) * i1;
+  }
+}
+
 __global__ __launch_bounds__(FACE_NT) void k_pb_faces(
     int n, double c0, double h, double lb, double kap, double eps_out,
     int natoms,
     const double *__restrict__ atoms, double *gx, double *gy, double *gz) {
 #pragma clang fp contract(off)
   __shared__ double sx[FACE_NT], sy[FACE_NT], sz[FACE_NT], sc[FACE_NT];
-  __shared__ double part[FACE_SPLIT][FACE_NODES];
-  const int f = blockIdx.y;
+  __shared__ double part[2][FACE_SPLIT][FACE_NODES];
+  const int axis = blockIdx.y;
   const int lane = threadIdx.x % FACE_NODES, split = threadIdx.x / FACE_NODES;
   const long long nn = (long long)n * n;
   const long long idx = (long long)blockIdx.x * FACE_NODES + lane;
@@ -208,12 +235,7 @@ __global__ __launch_bounds__(FACE_NT) void k_pb_faces(
   const int a = valid ? (int)(idx % n) : 0, b = valid ? (int)(idx / n) : 0;
   const double lo = c0, hi = c0 + (n - 1) * h;
   const double va = c0 + a * h, vb = c0 + b * h;
-  const double side = (f & 1) ? hi : lo;
-  double X, Y, Z;
-  if ((f >> 1) == 0) X = side, Y = va, Z = vb;
-  else if ((f >> 1) == 1) X = va, Y = side, Z = vb;
-  else X = va, Y = vb, Z = side;
-  double s = 0.0;
+  double s0 = 0.0, s1 = 0.0;
   for (int base = 0; base < natoms; base += FACE_NT) {
     const int m = min(FACE_NT, natoms - base);
     __syncthreads();
@@ -228,25 +250,25 @@ __global__ __launch_bounds__(FACE_NT) void k_pb_faces(
     __syncthreads();
     const int per = (m + FACE_SPLIT - 1) / FACE_SPLIT;
     const int t0 = split * per, t1 = min(m, t0 + per);
-    // two atoms per trip: the pair of exp chains interleave, and the
-    // polynomial's coefficients stay operands of three-address FMAs
-    // instead of being copied into the accumulator each time
-#pragma unroll 2
-    for (int t = t0; t < t1; t++) {
-      const double dx = X - sx[t], dy = Y - sy[t], dz = Z - sz[t];
-      const double d2 = dx * dx + dy * dy + dz * dz;
-      const double ri = rsqrt(d2);
-      s += sc[t] * exp(-kap * (d2 * ri)) * ri;
-    }
+    if (axis == 0)
+      face_terms<0>(t0, t1, va, vb, lo, hi, kap, sy, sz, sx, sc, s0, s1);
+    else if (axis == 1)
+      face_terms<1>(t0, t1, va, vb, lo, hi, kap, sx, sz, sy, sc, s0, s1);
+    else
+      face_terms<2>(t0, t1, va, vb, lo, hi, kap, sx, sy, sz, sc, s0, s1);
   }
-  part[split][lane] = s;
+  part[0][split][lane] = s0;
+  part[1][split][lane] = s1;
   __syncthreads();
   if (split != 0 || !valid) return;
-  double tot = part[0][lane];
+  double *dst = axis == 0 ? gx : (axis == 1 ? gy : gz);
 #pragma unroll
-  for (int q = 1; q < FACE_SPLIT; q++) tot += part[q][lane];
-  double *dst = (f >> 1) == 0 ? gx : ((f >> 1) == 1 ? gy : gz);
-  dst[(size_t)(f & 1) * nn + (size_t)b * n + a] = tot;
+  for (int side = 0; side < 2; side++) {
+    double tot = part[side][0][lane];
+#pragma unroll
+    for (int q = 1; q < FACE_SPLIT; q++) tot += part[side][q][lane];
+    dst[(size_t)side * nn + (size_t)b * n + a] = tot;
+  }
 }
 
 void check(const char *what) {
@@ -303,7 +325,7 @@ int pbe_assemble(int n, double h, double ionic, int natoms,
                                         next, inside, excl, xf, yf, zf, a1,
                                         a2, a3, ccf, fcf);
   check("k_pb_nodes");
-  dim3 gf(cdivp((long long)n * n, FACE_NODES), 6);
+  dim3 gf(cdivp((long long)n * n, FACE_NODES), 3);
   k_pb_faces<<<gf, FACE_NT, 0, s>>>(n, c0, h, lb, kap, eps_out, natoms, atoms,
                                     gx, gy,
                                     gz);
