@@ -28,8 +28,8 @@
 //               the charge of a node is gathered from the lists of its (up to)
 //               8 cells, taken in increasing atom index, so the sum has the
 //               reference's serial order and the result is deterministic;
-//   k_pb_faces  four wavefronts per 64 boundary nodes split the atoms,
-//               staged through LDS in chunks.
+//   k_pb_faces  four wavefronts per 64 low/high boundary node pairs split
+//               the atoms, staged through LDS in chunks.
 // Floating-point contraction is off in these kernels so that every
 // coordinate, weight and mark is the same double as the serial restatement
 // computes; only the boundary values (exp, reordered sums) differ from it
@@ -186,15 +186,6 @@ __global__ __launch_bounds__(256) void k_pb_nodes(
 
 constexpr int FACE_NT = 256, FACE_NODES = 64, FACE_SPLIT = FACE_NT / FACE_NODES;
 
-// Debye-Hueckel faces.  A workgroup takes 64 boundary node pairs of one
-// axis: each lane owns the node (a, b) on the low face and on the high face
-// of that axis, which share the two in-plane coordinate differences to
-// every atom.  Its four wavefronts each sum a quarter of every staged chunk
-// of atoms (all lanes of a wavefront read the same atom: an LDS broadcast),
-// and the four partial sums are added in a fixed order.  Per term the atom
-// factor lambda_B q exp(kappa r) / (eps_w (1 + kappa r)) is staged once,
-// leaving one exp and one reciprocal square root per (node, atom).  The sum
-// is deterministic; its order differs from the serial one by rounding only.
 // Atoms [t0, t1) of the staged chunk for the node pair at in-plane
 // coordinates (va, vb) of axis AX: su/sw hold the atoms' in-plane
 // coordinates, sn the normal one.  The squared distance is summed in x, y, z
@@ -220,6 +211,15 @@ __device__ __forceinline__ void face_terms(int t0, int t1, double va, double vb,
   }
 }
 
+// Debye-Hueckel faces.  A workgroup takes 64 boundary node pairs of one
+// axis: each lane owns the node (a, b) on the low face and on the high face
+// of that axis, which share the two in-plane coordinate differences to
+// every atom.  Its four wavefronts each sum a quarter of every staged chunk
+// of atoms (all lanes of a wavefront read the same atom: an LDS broadcast),
+// and the four partial sums are added in a fixed order.  Per term the atom
+// factor lambda_B q exp(kappa r) / (eps_w (1 + kappa r)) is staged once,
+// leaving one exp and one reciprocal square root per (node, atom).  The sum
+// is deterministic; its order differs from the serial one by rounding only.
 __global__ __launch_bounds__(FACE_NT) void k_pb_faces(
     int n, double c0, double h, double lb, double kap, double eps_out,
     int natoms,
