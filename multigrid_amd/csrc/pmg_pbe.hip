@@ -145,10 +145,11 @@ __global__ __launch_bounds__(256) void k_pb_nodes(
     const uint8_t *__restrict__ excl, double *xf, double *yf, double *zf,
     double *a1, double *a2, double *a3, double *ccf, double *fcf) {
 #pragma clang fp contract(off)
-  const int i = blockIdx.x * 64 + threadIdx.x;
-  const int j = blockIdx.y * 4 + threadIdx.y;
+  // 256 consecutive nodes of plane k, i fastest across row ends
+  const int q = blockIdx.x * 256 + threadIdx.y * 64 + threadIdx.x;
+  if (q >= n * n) return;
+  const int i = q % n, j = q / n;
   const int k = blockIdx.z;
-  if (i >= n || j >= n) return;
   const size_t nn = (size_t)n * n;
   const size_t p = (size_t)k * nn + (size_t)j * n + i;
   if (j == 0 && k == 0) xf[i] = yf[i] = zf[i] = c0 + i * h;
@@ -319,7 +320,7 @@ int pbe_assemble(int n, double h, double ionic, int natoms,
                                      next, inside, excl);
     check("k_pb_mark");
   }
-  dim3 gn(cdivp(n, 64), cdivp(n, 4), n);
+  dim3 gn(cdivp((long long)n * n, 256), 1, n);
   k_pb_nodes<<<gn, dim3(64, 4), 0, s>>>(n, c0, h, q4, h3, kap2e, eps_in,
                                         eps_out, atoms, head,
                                         next, inside, excl, xf, yf, zf, a1,
