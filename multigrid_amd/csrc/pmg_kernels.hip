@@ -1991,10 +1991,12 @@ __global__ __launch_bounds__(256) void k_buildA(
     const double *__restrict__ a2, const double *__restrict__ a3,
     const double *__restrict__ ccf, const double *__restrict__ fcf,
     float *__restrict__ ac32, float *__restrict__ cc32) {
-  const int i = 2 + blockIdx.x * BX + threadIdx.x;
-  const int j = 2 + blockIdx.y * BY + threadIdx.y;
+  // BX*BY consecutive interior points of plane k, i fastest across row ends
+  const int q = blockIdx.x * (BX * BY) + threadIdx.y * BX + threadIdx.x;
+  if (q >= (nx - 2) * (ny - 2)) return;
+  const int i = 2 + q % (nx - 2);
+  const int j = 2 + q / (nx - 2);
   const int k = 2 + blockIdx.z;
-  if (i > nx - 1 || j > ny - 1) return;
   const size_t n = (size_t)nx * ny * nz;
   double *oC = ac, *oE = ac + n, *oN = ac + 2 * n, *uC = ac + 3 * n;
   const size_t p = ix(nx, ny, i, j, k);
@@ -2052,7 +2054,7 @@ void buildA(const Dim &d, double *ac, double *cc, double *fc,
             const double *a1, const double *a2, const double *a3,
             const double *ccf, const double *fcf, hipStream_t s,
             float *ac32, float *cc32) {
-  dim3 g(cdiv(d.nx - 2, BX), cdiv(d.ny - 2, BY), d.nz - 2);
+  dim3 g(cdiv((d.nx - 2) * (d.ny - 2), BX * BY), 1, d.nz - 2);
   k_buildA<<<g, dim3(BX, BY), 0, s>>>(d.nx, d.ny, d.nz, ac, cc, fc, xf, yf,
                                       zf, gx, gy, gz, a1, a2, a3, ccf, fcf,
                                       ac32, cc32);
