@@ -41,6 +41,23 @@ extern "C" {
 int pmg_mgsz(int mgcoar, int mgdisc, int mgsolv, int nx, int ny, int nz,
              int nlev, int *out);
 
+/* pmgc::Vmgsz with its own pointer list (pmgc/mgdrvd.h:236-255): reads
+ * mgcoar..nlev, writes nxc..iintot.  PMG_EBADPARM where the reference
+ * prints "invalid ... parameter" (mgdrvd.cc:561, 570, 585).  Host-only. */
+int pmg_Vmgsz(int *mgcoar, int *mgdisc, int *mgsolv, int *nx, int *ny,
+              int *nz, int *nlev, int *nxc, int *nyc, int *nzc, int *nf,
+              int *nc, int *narr, int *narrc, int *n_rpc, int *n_iz,
+              int *n_ipc, int *iretot, int *iintot);
+
+/* pmgc::Vpackmg (pmgc/mgsubd.h:273, mgsubd.cc:538-576): encodes iparm(1..22)
+ * and rparm(1), (9), (10) from the reference's pointer list.  Host-only. */
+int pmg_Vpackmg(int *iparm, double *rparm, size_t *nrwk, int *niwk, int *nx,
+                int *ny, int *nz, int *nlev, int *nu1, int *nu2, int *mgkey,
+                int *itmax, int *istop, int *ipcon, int *nonlin, int *mgsmoo,
+                int *mgprol, int *mgcoar, int *mgsolv, int *mgdisc, int *iinfo,
+                double *errtol, int *ipkey, double *omegal, double *omegan,
+                int *irite, int *iperf);
+
 /* Replaces pmgc::Vmaxlev (pmgc/mgsubd.cc:456-484).  Host-only. */
 int pmg_maxlev(int nx, int ny, int nz);
 
@@ -85,8 +102,15 @@ int pmg_pcgdriv(int *iparm, double *rparm, int *iwork, double *rwork,
                 double *ccf, double *fcf, double *tcf, int num_iters);
 
 /* Replaces pmgc::Vmypdefinitlpbe / Vmypdefinitnpbe
- * (pmgc/mypdec.cc:72-106): the ion table used by the nonlinear term. */
+ * (pmgc/mypdec.cc:72-106): the ion table used by the nonlinear term (at
+ * most 50 species, MAXIONS of pmgc/mypdec.h:63; further ones are ignored
+ * as the reference ignores them). */
 int pmg_mypdefinit(int nion, const double *charge, const double *sconc);
+/* The same with the reference's pointer lists: pmgc::Vmypdefinitlpbe
+ * (pmgc/mypdec.h:89, mypdec.cc:72-88) and pmgc::Vmypdefinitnpbe
+ * (mypdec.h:106, mypdec.cc:90-106). */
+int pmg_Vmypdefinitlpbe(int *nion, double *charge, double *sconc);
+int pmg_Vmypdefinitnpbe(int *nion, double *charge, double *sconc);
 
 /* Iterations, error flag and final relative residual of the last
  * pmg_mgdriv / pmg_newdriv call on this thread.  The reference computes
@@ -166,6 +190,48 @@ int pmg_restrc(int nxf, int nyf, int nzf, int nxc, int nyc, int nzc,
                const double *xin, double *xout, const double *pc);
 int pmg_interp(int nxc, int nyc, int nzc, int nxf, int nyf, int nzf,
                const double *xin, double *xout, const double *pc);
+/* The coarse-grid correction of Vmvcs's way up (pmgc/mgcsd.cc:429-458):
+ * xf(interior) += xdamp * VinterpPMG(xc) with the Hackbusch-Reusken factor
+ * xdamp = (xc . w0c) / (xc . A_c xc) of Vmatvec + 2 x Vxdot on the coarse
+ * level (acc/ccc its operator, w0c its right-hand side; xdamp = 0 when the
+ * denominator is 0). */
+int pmg_correct(int nxc, int nyc, int nzc, int nxf, int nyf, int nzf,
+                int numdia, const double *acc, const double *ccc,
+                const double *xc, const double *w0c, const double *pc,
+                double *xf);
+/* pmg_gsrb starting from x = 0 (Vazeros before Vsmooth, mgcsd.cc:341-346):
+ * x's interior is not read; its boundary entries must be 0. */
+int pmg_gsrb_zero(int nx, int ny, int nz, int numdia, const double *ac,
+                  const double *cc, const double *fc, double *x, int itmax,
+                  int iadjoint);
+/* FP32 instances of the same kernels, as the inner V-cycle of the
+ * mixed-precision solve runs them (operators rounded to FP32 by the
+ * caller).  zero_start as pmg_gsrb_zero. */
+int pmg_mresid_f32(int nx, int ny, int nz, int numdia, const float *ac,
+                   const float *cc, const float *fc, const float *x,
+                   float *r);
+int pmg_gsrb_f32(int nx, int ny, int nz, int numdia, const float *ac,
+                 const float *cc, const float *fc, float *x, int itmax,
+                 int iadjoint, int zero_start);
+int pmg_restrc_f32(int nxf, int nyf, int nzf, int nxc, int nyc, int nzc,
+                   const float *xin, float *xout, const float *pc);
+int pmg_interp_f32(int nxc, int nyc, int nzc, int nxf, int nyf, int nzf,
+                   const float *xin, float *xout, const float *pc);
+int pmg_correct_f32(int nxc, int nyc, int nzc, int nxf, int nyf, int nzf,
+                    int numdia, const float *acc, const float *ccc,
+                    const float *xc, const float *w0c, const float *pc,
+                    float *xf);
+/* The outer residual of the mixed-precision solve: r = f - A x of the
+ * 7-point fine operator in FP64 (Vmresid7_1s, pmgc/matvecd.cc:381-420),
+ * stored rounded to FP32 (the next correction cycle's right-hand side);
+ * *l1 (may be NULL) = its FP64 L1 norm (Vxnrm1, mikpckd.cc:133-159). */
+int pmg_mresid_mixed(int nx, int ny, int nz, const double *ac,
+                     const double *cc, const double *fc, const double *x,
+                     float *r, double *l1);
+/* The update of the mixed-precision solve: u(interior) += e (FP64 += FP32,
+ * Vxaxpy with alpha 1, mikpckd.cc:115-131). */
+int pmg_addcorr_f32(int nx, int ny, int nz, const float *e, double *u);
+
 /* VbuildP / VbuildG (pmgc/buildPd.cc:1118-1147, buildGd.cc:3276-3368). */
 int pmg_buildP(int nxf, int nyf, int nzf, int nxc, int nyc, int nzc,
                int mgprol, int numdia, const double *acf, double *pc);

@@ -57,9 +57,15 @@ def lib():
 
 def _declare(L):
     dp, ip, i, d = _D, _I, ctypes.c_int, ctypes.c_double
+    fp = ctypes.POINTER(ctypes.c_float)
     vp = ctypes.c_void_p
     sig = {
         "pmg_mgsz": (i, [i, i, i, i, i, i, i, ip]),
+        "pmg_Vmgsz": (i, [ip] * 19),
+        "pmg_Vpackmg": (i, [ip, dp, ctypes.POINTER(ctypes.c_size_t)] + [ip] * 18
+                        + [dp, ip, dp, dp, ip, ip]),
+        "pmg_Vmypdefinitlpbe": (i, [ip, dp, dp]),
+        "pmg_Vmypdefinitnpbe": (i, [ip, dp, dp]),
         "pmg_maxlev": (i, [i, i, i]),
         "pmg_mgdriv": (i, [ip, dp, ip, dp, dp] + [dp] * 12),
         "pmg_newdriv": (i, [ip, dp, ip, dp, dp] + [dp] * 12),
@@ -83,6 +89,15 @@ def _declare(L):
         "pmg_gsrb": (i, [i, i, i, i, dp, dp, dp, dp, i, i]),
         "pmg_restrc": (i, [i, i, i, i, i, i, dp, dp, dp]),
         "pmg_interp": (i, [i, i, i, i, i, i, dp, dp, dp]),
+        "pmg_correct": (i, [i] * 7 + [dp] * 6),
+        "pmg_gsrb_zero": (i, [i, i, i, i, dp, dp, dp, dp, i, i]),
+        "pmg_mresid_f32": (i, [i, i, i, i, fp, fp, fp, fp, fp]),
+        "pmg_gsrb_f32": (i, [i, i, i, i, fp, fp, fp, fp, i, i, i]),
+        "pmg_restrc_f32": (i, [i] * 6 + [fp, fp, fp]),
+        "pmg_interp_f32": (i, [i] * 6 + [fp, fp, fp]),
+        "pmg_correct_f32": (i, [i] * 7 + [fp] * 6),
+        "pmg_mresid_mixed": (i, [i, i, i, dp, dp, dp, dp, fp, dp]),
+        "pmg_addcorr_f32": (i, [i, i, i, fp, dp]),
         "pmg_buildP": (i, [i, i, i, i, i, i, i, i, dp, dp]),
         "pmg_buildG": (i, [i, i, i, i, i, i, i, dp, dp, dp]),
         "pmg_pbe_synth": (i, [i, d, i, d, d, i, ctypes.c_uint64] + [dp] * 13),

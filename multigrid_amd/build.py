@@ -6,6 +6,7 @@ sits beside this file so that it travels with the source tree.
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -18,7 +19,6 @@ HIPCC_FLAGS = [
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-shared",
     "-pthread",
     "-Wno-unused-result",
     "-Wno-unused-value",
@@ -34,19 +34,30 @@ def hipcc_path():
     return "hipcc"
 
 
-def build(verbose=False):
-    srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
-    tmp = LIB + ".tmp"
-    cmd = [hipcc_path()] + HIPCC_FLAGS + ["-o", tmp] + srcs
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     out = "\n".join(l for l in res.stdout.splitlines() if "packed-fp32-ops" not in l)
     if res.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + out)
-    os.replace(tmp, LIB)
+        raise RuntimeError("hipcc failed:\n" + " ".join(cmd) + "\n" + out)
     if verbose and out.strip():
         print(out)
+
+
+def build(verbose=False):
+    """Compile every source to an object (in parallel), then link."""
+    srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    objs = [os.path.join(objdir, os.path.basename(s) + ".o") for s in srcs]
+    hipcc = hipcc_path()
+    with ThreadPoolExecutor(max_workers=min(len(srcs), 4)) as ex:
+        list(ex.map(lambda so: _run([hipcc] + HIPCC_FLAGS + ["-c", "-o", so[1], so[0]], verbose),
+                    zip(srcs, objs)))
+    tmp = LIB + ".tmp"
+    _run([hipcc, "--offload-arch=gfx950", "-shared", "-pthread", "-o", tmp] + objs, verbose)
+    os.replace(tmp, LIB)
     return LIB
 
 

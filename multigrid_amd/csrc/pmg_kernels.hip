@@ -1112,6 +1112,11 @@ __global__ __launch_bounds__(256) void k_restrict(int nxf, int nyf, int nxc,
                                                   T *__restrict__ xout,
                                                   const T *__restrict__ pc, XG xg,
                                                   float *__restrict__ out32) {
+  // no multiply-add contraction: the setup restricts cc and fc with this
+  // kernel, and a source of both signs cancels in those sums, so the
+  // coarse cc and fc are the oracle's to the bit only with the same
+  // roundings (same term order, separate multiply and add)
+#pragma clang fp contract(off)
   int bx, by, bz;
   if (!xcd_tile(xg, bx, by, bz)) return;
   const int ii = 2 + bx * BX + threadIdx.x;
@@ -1991,6 +1996,8 @@ __global__ __launch_bounds__(256) void k_buildA(
     const double *__restrict__ a2, const double *__restrict__ a3,
     const double *__restrict__ ccf, const double *__restrict__ fcf,
     float *__restrict__ ac32, float *__restrict__ cc32) {
+  // no multiply-add contraction: the operator is the oracle's to the bit
+#pragma clang fp contract(off)
   // BX*BY consecutive interior points of plane k, i fastest across row ends
   const int q = blockIdx.x * (BX * BY) + threadIdx.y * BX + threadIdx.x;
   if (q >= (nx - 2) * (ny - 2)) return;
@@ -2107,6 +2114,10 @@ __global__ __launch_bounds__(128) void k_buildP(int nxf, int nyf, int nzf,
                                                 double *__restrict__ pc,
                                                 float *__restrict__ pc32,
                                                 int mgprol, XG xg) {
+  // no multiply-add contraction: the weights divide differences of
+  // couplings that cancel where the operator's row sums vanish (no ions),
+  // so they match the oracle only with the same roundings
+#pragma clang fp contract(off)
   int bx, by, bz;
   if (!xcd_tile(xg, bx, by, bz)) return;
   // 128 consecutive interior points of coarse plane kk, i fastest across
@@ -2223,6 +2234,8 @@ __device__ inline double rap_entry(const double *__restrict__ pc,
                                    const double *__restrict__ ac, size_t nf,
                                    size_t nc, size_t pC, size_t pC2,
                                    size_t pCf, long long sy, long long sz) {
+  // no multiply-add contraction (the sums cancel; see k_buildP)
+#pragma clang fp contract(off)
   double acc = 0.0;
 #pragma unroll
   for (int a = 0; a < 27; a++) {

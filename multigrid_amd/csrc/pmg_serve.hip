@@ -6,7 +6,11 @@
 // queue up as they arrive.  One worker thread owns all device work:
 //   - admission: a queued request is started as soon as a slot is free
 //     (at most max_active solves in flight) and its context fits the
-//     device-memory budget, first fit in arrival order.  Contexts are
+//     device-memory budget, first fit in arrival order; a request that has
+//     been passed over for max_active later ones holds back everything
+//     behind it until it fits, so a large request cannot starve behind a
+//     stream of small ones (idle pooled contexts are evicted for it as the
+//     running solves finish).  Contexts are
 //     pooled by solve shape and reused, so a request whose shape was seen
 //     before needs no allocation, no graph capture and no budget;
 //   - every admitted solve runs on its context's own HIP stream: GPU input
@@ -56,6 +60,7 @@ struct Request {
   bool done = false;
   pmg_ctx *ctx = nullptr;
   size_t bytes = 0;  // budget charged for a newly created context
+  int passed = 0;    // later requests admitted while this one did not fit
 };
 
 // Contexts are interchangeable when everything their buffers and captured
@@ -170,6 +175,7 @@ void worker_main(pmg_server *S) {
     // admission: first fit in arrival order
     std::vector<Request *> admit;
     std::vector<pmg_ctx *> dead;
+    Request *waiting = nullptr;  // first request of this pass that did not fit
     for (auto it = S->queue.begin();
          it != S->queue.end() &&
          (int)(active.size() + admit.size()) < S->max_active;) {
@@ -179,12 +185,15 @@ void worker_main(pmg_server *S) {
         const size_t need = ctx_bytes_estimate(r->p, r->mode);
         if (S->used + need > S->budget) evict_for(S, need, dead);
         if (S->used + need > S->budget) {
+          if (r->passed >= S->max_active) break;  // aged: hold the rest
+          if (!waiting) waiting = r;
           ++it;
           continue;
         }
         S->used += need;
         r->bytes = need;
       }
+      if (waiting) waiting->passed++;
       r->ctx = c;
       admit.push_back(r);
       it = S->queue.erase(it);

@@ -1256,10 +1256,11 @@ struct Scratch {
   }
 };
 
-int finish(Scratch &S, double *host, const double *dev, size_t n) {
+template <class T>
+int finish(Scratch &S, T *host, const T *dev, size_t n) {
   if (!S.ok) return PMG_ENOMEM;
   if (hipDeviceSynchronize() != hipSuccess) return PMG_EDEVICE;
-  if (hipMemcpy(host, dev, n * sizeof(double), hipMemcpyDeviceToHost) !=
+  if (hipMemcpy(host, dev, n * sizeof(T), hipMemcpyDeviceToHost) !=
       hipSuccess)
     return PMG_EDEVICE;
   return PMG_OK;
@@ -1267,6 +1268,119 @@ int finish(Scratch &S, double *host, const double *dev, size_t n) {
 
 bool dims_ok(int nx, int ny, int nz) { return nx >= 3 && ny >= 3 && nz >= 3; }
 bool coarse_of(int f, int c) { return f == 2 * c - 1; }
+bool coarse_dims(int nxf, int nyf, int nzf, int nxc, int nyc, int nzc) {
+  return dims_ok(nxc, nyc, nzc) && coarse_of(nxf, nxc) &&
+         coarse_of(nyf, nyc) && coarse_of(nzf, nzc);
+}
+
+// The single-kernel entry points in either precision: each runs exactly the
+// launcher the V-cycle of that precision runs.
+template <class T>
+int mresid_t(int nx, int ny, int nz, int numdia, const T *ac, const T *cc,
+             const T *fc, const T *x, T *r) {
+  if (!dims_ok(nx, ny, nz) || (numdia != 7 && numdia != 27) || !ac || !cc ||
+      !fc || !x || !r)
+    return PMG_EBADPARM;
+  if (!device_ok()) return PMG_EDEVICE;
+  Dim d{nx, ny, nz};
+  size_t n = d.n();
+  Scratch S;
+  T *dac = S.put(ac, (numdia == 7 ? 4 : 14) * n), *dcc = S.put(cc, n),
+    *dfc = S.put(fc, n), *dx = S.put(x, n), *dr = S.put(r, n);
+  if (!S.ok) return PMG_ENOMEM;
+  resid<T, T>(d, numdia, dac, dcc, dfc, dx, dr, nullptr, 0);
+  return finish(S, r, dr, n);
+}
+
+// zero_start: the sweeps start from x = 0 (Vazeros) and x's interior is
+// not read, as in the first pre-smoothing of a correction cycle; x's
+// boundary entries must be zero then (as on every level of that cycle).
+template <class T>
+int gsrb_t(int nx, int ny, int nz, int numdia, const T *ac, const T *cc,
+           const T *fc, T *x, int itmax, int iadjoint, int zero_start) {
+  if (!dims_ok(nx, ny, nz) || (numdia != 7 && numdia != 27) || !ac || !cc ||
+      !fc || !x || itmax < 0 || (iadjoint != 0 && iadjoint != 1) ||
+      (zero_start != 0 && zero_start != 1))
+    return PMG_EBADPARM;
+  Dim d{nx, ny, nz};
+  size_t n = d.n();
+  if (zero_start)
+    for (int k = 1; k <= nz; k++)
+      for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++)
+          if ((i == 1 || i == nx || j == 1 || j == ny || k == 1 || k == nz) &&
+              x[((size_t)(k - 1) * ny + (j - 1)) * nx + (i - 1)] != (T)0)
+            return PMG_EBADPARM;
+  if (!device_ok()) return PMG_EDEVICE;
+  Scratch S;
+  T *dac = S.put(ac, (numdia == 7 ? 4 : 14) * n), *dcc = S.put(cc, n),
+    *dfc = S.put(fc, n), *dx = S.put(x, n);
+  // second buffer of the sweeps: the same boundary values as x
+  T *tmp = S.put(x, n);
+  if (!S.ok) return PMG_ENOMEM;
+  if (numdia == 7)
+    gs7_fused<T>(d, dac, dcc, dfc, dx, tmp, itmax, iadjoint, 0,
+                 zero_start != 0);
+  else
+    gs27<T>(d, dac, dcc, dfc, dx, itmax, iadjoint, 0, zero_start != 0, tmp);
+  return finish(S, x, dx, n);
+}
+
+template <class T>
+int restrc_t(int nxf, int nyf, int nzf, int nxc, int nyc, int nzc,
+             const T *xin, T *xout, const T *pc) {
+  if (!coarse_dims(nxf, nyf, nzf, nxc, nyc, nzc) || !xin || !xout || !pc)
+    return PMG_EBADPARM;
+  if (!device_ok()) return PMG_EDEVICE;
+  Dim f{nxf, nyf, nzf}, c{nxc, nyc, nzc};
+  Scratch S;
+  T *dxi = S.put(xin, f.n()), *dxo = S.put(xout, c.n()),
+    *dpc = S.put(pc, 27 * c.n());
+  if (!S.ok) return PMG_ENOMEM;
+  restrict_<T>(f, c, dxi, dxo, dpc, 0);
+  return finish(S, xout, dxo, c.n());
+}
+
+template <class T>
+int interp_t(int nxc, int nyc, int nzc, int nxf, int nyf, int nzf,
+             const T *xin, T *xout, const T *pc) {
+  if (!coarse_dims(nxf, nyf, nzf, nxc, nyc, nzc) || !xin || !xout || !pc)
+    return PMG_EBADPARM;
+  if (!device_ok()) return PMG_EDEVICE;
+  Dim f{nxf, nyf, nzf}, c{nxc, nyc, nzc};
+  Scratch S;
+  T *dxi = S.put(xin, c.n()), *dxo = S.put(xout, f.n()),
+    *dpc = S.put(pc, 27 * c.n());
+  if (!S.ok) return PMG_ENOMEM;
+  interp<T>(c, f, dxi, dpc, dxo, nullptr, 0);
+  return finish(S, xout, dxo, f.n());
+}
+
+// xf(interior) += xdamp * (P xc), xdamp = (xc . w0c) / (xc . A_c xc): the
+// coarse-grid correction of the V-cycle's way up (mgcsd.cc:429-458), as
+// the cycle runs it: k_matvec_dots partials, then k_interp<T, true>.
+template <class T>
+int correct_t(int nxc, int nyc, int nzc, int nxf, int nyf, int nzf,
+              int numdia, const T *acc, const T *ccc, const T *xc,
+              const T *w0c, const T *pc, T *xf) {
+  if (!coarse_dims(nxf, nyf, nzf, nxc, nyc, nzc) ||
+      (numdia != 7 && numdia != 27) || !acc || !ccc || !xc || !w0c || !pc ||
+      !xf)
+    return PMG_EBADPARM;
+  if (!device_ok()) return PMG_EDEVICE;
+  Dim f{nxf, nyf, nzf}, c{nxc, nyc, nzc};
+  Scratch S;
+  T *dac = S.put(acc, (numdia == 7 ? 4 : 14) * c.n()), *dcc = S.put(ccc, c.n()),
+    *dxc = S.put(xc, c.n()), *dw0 = S.put(w0c, c.n()),
+    *dpc = S.put(pc, 27 * c.n()), *dxf = S.put(xf, f.n()),
+    *dy = S.put((const T *)nullptr, c.n());
+  double *part = S.put((const double *)nullptr, max_partials(c));
+  if (!S.ok) return PMG_ENOMEM;
+  const unsigned nb =
+      matvec_dots_part<T>(c, numdia, dac, dcc, dxc, dw0, dy, part, 0);
+  interp_damped<T>(c, f, dxc, dpc, dxf, part, nb, 0);
+  return finish(S, xf, dxf, f.n());
+}
 
 }  // namespace drv
 }  // namespace pmg
@@ -1285,6 +1399,47 @@ int pmg_mgsz(int mgcoar, int mgdisc, int mgsolv, int nx, int ny, int nz,
 }
 
 int pmg_maxlev(int nx, int ny, int nz) { return maxlev(nx, ny, nz); }
+
+int pmg_Vmgsz(int *mgcoar, int *mgdisc, int *mgsolv, int *nx, int *ny,
+              int *nz, int *nlev, int *nxc, int *nyc, int *nzc, int *nf,
+              int *nc, int *narr, int *narrc, int *n_rpc, int *n_iz,
+              int *n_ipc, int *iretot, int *iintot) {
+  if (!mgcoar || !mgdisc || !mgsolv || !nx || !ny || !nz || !nlev || !nxc ||
+      !nyc || !nzc || !nf || !nc || !narr || !narrc || !n_rpc || !n_iz ||
+      !n_ipc || !iretot || !iintot)
+    return PMG_EBADPARM;
+  int out[12];
+  const int rc =
+      mgsz(*mgcoar, *mgdisc, *mgsolv, *nx, *ny, *nz, *nlev, out);
+  if (rc) return rc;
+  int *dst[12] = {nxc, nyc, nzc, nf, nc, narr, narrc, n_rpc, n_iz, n_ipc,
+                  iretot, iintot};
+  for (int q = 0; q < 12; q++) *dst[q] = out[q];
+  return PMG_OK;
+}
+
+int pmg_Vpackmg(int *iparm, double *rparm, size_t *nrwk, int *niwk, int *nx,
+                int *ny, int *nz, int *nlev, int *nu1, int *nu2, int *mgkey,
+                int *itmax, int *istop, int *ipcon, int *nonlin, int *mgsmoo,
+                int *mgprol, int *mgcoar, int *mgsolv, int *mgdisc, int *iinfo,
+                double *errtol, int *ipkey, double *omegal, double *omegan,
+                int *irite, int *iperf) {
+  if (!iparm || !rparm || !nrwk || !niwk || !nx || !ny || !nz || !nlev ||
+      !nu1 || !nu2 || !mgkey || !itmax || !istop || !ipcon || !nonlin ||
+      !mgsmoo || !mgprol || !mgcoar || !mgsolv || !mgdisc || !iinfo ||
+      !errtol || !ipkey || !omegal || !omegan || !irite || !iperf)
+    return PMG_EBADPARM;
+  if (*nrwk > 0x7fffffffULL) return PMG_EBADPARM;  // iparm(1) is an int
+  const int v[22] = {(int)*nrwk, *niwk,  *nx,     *ny,     *nz,     *nlev,
+                     *nu1,       *nu2,   *mgkey,  *itmax,  *istop,  *iinfo,
+                     *irite,     *ipkey, *ipcon,  *nonlin, *mgprol, *mgcoar,
+                     *mgdisc,    *mgsmoo, *mgsolv, *iperf};
+  for (int q = 0; q < 22; q++) iparm[q] = v[q];
+  rparm[0] = *errtol;
+  rparm[8] = *omegal;
+  rparm[9] = *omegan;
+  return PMG_OK;
+}
 
 int pmg_mgdriv(int *iparm, double *rparm, int *iwork, double *rwork,
                double *u, const double *xf, const double *yf, const double *zf,
@@ -1327,6 +1482,16 @@ int pmg_mypdefinit(int nion, const double *charge, const double *sconc) {
   if (!device_ok()) return PMG_EDEVICE;
   set_ions(nion, charge, sconc);
   return hipDeviceSynchronize() == hipSuccess ? PMG_OK : PMG_EDEVICE;
+}
+
+int pmg_Vmypdefinitlpbe(int *nion, double *charge, double *sconc) {
+  if (!nion) return PMG_EBADPARM;
+  return pmg_mypdefinit(*nion, charge, sconc);
+}
+
+int pmg_Vmypdefinitnpbe(int *nion, double *charge, double *sconc) {
+  if (!nion) return PMG_EBADPARM;
+  return pmg_mypdefinit(*nion, charge, sconc);
 }
 
 int pmg_last_stats(int *iters, int *ierror, double *relres) {
@@ -1523,6 +1688,91 @@ int pmg_interp(int nxc, int nyc, int nzc, int nxf, int nyf, int nzf,
   if (!S.ok) return PMG_ENOMEM;
   interp<double>(c, f, dxi, dpc, dxo, nullptr, 0);
   return finish(S, xout, dxo, f.n());
+}
+
+int pmg_correct(int nxc, int nyc, int nzc, int nxf, int nyf, int nzf,
+                int numdia, const double *acc, const double *ccc,
+                const double *xc, const double *w0c, const double *pc,
+                double *xf) {
+  return correct_t<double>(nxc, nyc, nzc, nxf, nyf, nzf, numdia, acc, ccc, xc,
+                           w0c, pc, xf);
+}
+
+int pmg_mresid_f32(int nx, int ny, int nz, int numdia, const float *ac,
+                   const float *cc, const float *fc, const float *x,
+                   float *r) {
+  return mresid_t<float>(nx, ny, nz, numdia, ac, cc, fc, x, r);
+}
+
+int pmg_gsrb_f32(int nx, int ny, int nz, int numdia, const float *ac,
+                 const float *cc, const float *fc, float *x, int itmax,
+                 int iadjoint, int zero_start) {
+  return gsrb_t<float>(nx, ny, nz, numdia, ac, cc, fc, x, itmax, iadjoint,
+                       zero_start);
+}
+
+int pmg_gsrb_zero(int nx, int ny, int nz, int numdia, const double *ac,
+                  const double *cc, const double *fc, double *x, int itmax,
+                  int iadjoint) {
+  return gsrb_t<double>(nx, ny, nz, numdia, ac, cc, fc, x, itmax, iadjoint,
+                        1);
+}
+
+int pmg_restrc_f32(int nxf, int nyf, int nzf, int nxc, int nyc, int nzc,
+                   const float *xin, float *xout, const float *pc) {
+  return restrc_t<float>(nxf, nyf, nzf, nxc, nyc, nzc, xin, xout, pc);
+}
+
+int pmg_interp_f32(int nxc, int nyc, int nzc, int nxf, int nyf, int nzf,
+                   const float *xin, float *xout, const float *pc) {
+  return interp_t<float>(nxc, nyc, nzc, nxf, nyf, nzf, xin, xout, pc);
+}
+
+int pmg_correct_f32(int nxc, int nyc, int nzc, int nxf, int nyf, int nzf,
+                    int numdia, const float *acc, const float *ccc,
+                    const float *xc, const float *w0c, const float *pc,
+                    float *xf) {
+  return correct_t<float>(nxc, nyc, nzc, nxf, nyf, nzf, numdia, acc, ccc, xc,
+                          w0c, pc, xf);
+}
+
+int pmg_mresid_mixed(int nx, int ny, int nz, const double *ac,
+                     const double *cc, const double *fc, const double *x,
+                     float *r, double *l1) {
+  if (!dims_ok(nx, ny, nz) || !ac || !cc || !fc || !x || !r)
+    return PMG_EBADPARM;
+  if (!device_ok()) return PMG_EDEVICE;
+  Dim d{nx, ny, nz};
+  size_t n = d.n();
+  Scratch S;
+  double *dac = S.put(ac, 4 * n), *dcc = S.put(cc, n), *dfc = S.put(fc, n),
+         *dx = S.put(x, n);
+  float *dr = S.put(r, n);
+  double *part = S.put((const double *)nullptr, max_partials(d) + 8);
+  if (!S.ok) return PMG_ENOMEM;
+  Red red;
+  red.part = part;
+  red.out = part + max_partials(d);
+  resid<double, float>(d, 7, dac, dcc, dfc, dx, dr, &red, 0);
+  int rc = finish(S, r, dr, n);
+  if (!rc && l1 &&
+      hipMemcpy(l1, red.out, sizeof(double), hipMemcpyDeviceToHost) !=
+          hipSuccess)
+    rc = PMG_EDEVICE;
+  return rc;
+}
+
+int pmg_addcorr_f32(int nx, int ny, int nz, const float *e, double *u) {
+  if (!dims_ok(nx, ny, nz) || !e || !u) return PMG_EBADPARM;
+  if (!device_ok()) return PMG_EDEVICE;
+  Dim d{nx, ny, nz};
+  size_t n = d.n();
+  Scratch S;
+  float *de = S.put(e, n);
+  double *du = S.put(u, n);
+  if (!S.ok) return PMG_ENOMEM;
+  add_correction<float>(d, de, du, 0);
+  return finish(S, u, du, n);
 }
 
 int pmg_buildP(int nxf, int nyf, int nzf, int nxc, int nyc, int nzc,

@@ -352,48 +352,22 @@ void pmgo_buildP(int nxf, int nyf, int nzf, int nxc, int nyc, int nzc,
 /* The reference's VbuildG_7 / VbuildG_27 are expanded (Maple-generated)
  * forms of XA(C, C+delta) = sum_p P_C(p) (A P_{C+delta})(p) over the 27-point
  * supports of P; stored off-diagonals are -XA (buildGd.cc:527-1360,
- * 1449-3273).  Here the same sum is enumerated: for each stored direction
- * delta, the (d, e) pairs with |d| <= 1, |e| <= 1 and |d + e - 2 delta| <= 1
- * per axis are tabulated once. */
-#define G_MAXPAIRS 343
-static int g_npairs[14];
-static signed char g_pairs[14][G_MAXPAIRS][6];
-static const int g_delta[14][3] = {
-    {0, 0, 0},  {1, 0, 0},  {0, 1, 0},  {0, 0, 1},   {1, 1, 0},
-    {-1, 1, 0}, {1, 0, 1},  {-1, 0, 1}, {0, 1, 1},   {0, -1, 1},
-    {1, 1, 1},  {-1, 1, 1}, {1, -1, 1}, {-1, -1, 1}};
-static int g_init = 0;
-static void g_tables(void) {
-  if (g_init) return;
-  for (int s = 0; s < 14; s++) {
-    int m = 0;
-    for (int dz = -1; dz <= 1; dz++)
-      for (int dy = -1; dy <= 1; dy++)
-        for (int dx = -1; dx <= 1; dx++)
-          for (int ez = -1; ez <= 1; ez++)
-            for (int ey = -1; ey <= 1; ey++)
-              for (int ex = -1; ex <= 1; ex++) {
-                int qx = dx + ex - 2 * g_delta[s][0];
-                int qy = dy + ey - 2 * g_delta[s][1];
-                int qz = dz + ez - 2 * g_delta[s][2];
-                if (qx < -1 || qx > 1 || qy < -1 || qy > 1 || qz < -1 ||
-                    qz > 1)
-                  continue;
-                signed char *r = g_pairs[s][m++];
-                r[0] = dx, r[1] = dy, r[2] = dz, r[3] = ex, r[4] = ey,
-                r[5] = ez;
-              }
-    g_npairs[s] = m;
-  }
-  g_init = 1;
-}
-
+ * 1449-3273).  Here the same sum is enumerated, grouped by the fine point
+ * F = C_f + d of P_C's support: XA = sum_d P_C(d) * (sum_e A(F, F+e)
+ * P_{C+delta}(d + e - 2 delta)) over the e with |d + e - 2 delta| <= 1 per
+ * axis (d and e each in dz, dy, dx order, dx fastest).  The Maple forms fix
+ * no summation order of their own that a restatement could follow term for
+ * term; this grouping is the one the GPU kernel uses, so the two agree to
+ * the last bit where neither contracts multiply-adds. */
 void pmgo_buildG(int nxf, int nyf, int nzf, int nxc, int nyc, int nzc,
                  int numdia, const double *pc, const double *acf,
                  double *acc) {
+  static const int g_delta[14][3] = {
+      {0, 0, 0},  {1, 0, 0},  {0, 1, 0},  {0, 0, 1},   {1, 1, 0},
+      {-1, 1, 0}, {1, 0, 1},  {-1, 0, 1}, {0, 1, 1},   {0, -1, 1},
+      {1, 1, 1},  {-1, 1, 1}, {1, -1, 1}, {-1, -1, 1}};
   size_t nf = (size_t)nxf * nyf * nzf, nc = (size_t)nxc * nyc * nzc;
   (void)nzf;
-  g_tables();
   long long sx = 1, sy = nxf, sz = (long long)nxf * nyf;
   long long cx = 1, cy = nxc, cz = (long long)nxc * nyc;
 #pragma omp parallel for schedule(static)
@@ -403,29 +377,38 @@ void pmgo_buildG(int nxf, int nyf, int nzf, int nxc, int nyc, int nzc,
         size_t pC = IX(nxc, nyc, ii, jj, kk);
         size_t pCf = IX(nxf, nyf, 2 * ii - 1, 2 * jj - 1, 2 * kk - 1);
         for (int s = 0; s < 14; s++) {
-          size_t pC2 = (size_t)((long long)pC + g_delta[s][0] * cx +
-                                g_delta[s][1] * cy + g_delta[s][2] * cz);
+          const int *dl = g_delta[s];
+          size_t pC2 = (size_t)((long long)pC + dl[0] * cx + dl[1] * cy +
+                                dl[2] * cz);
           double acc_v = 0.0;
-          for (int m = 0; m < g_npairs[s]; m++) {
-            const signed char *r = g_pairs[s][m];
-            int dx = r[0], dy = r[1], dz = r[2], ex = r[3], ey = r[4],
-                ez = r[5];
-            size_t pF = (size_t)((long long)pCf + dx * sx + dy * sy + dz * sz);
-            double A;
-            if (!ex && !ey && !ez) {
-              A = acf[pF];
-            } else {
-              A = -coupling(acf, nf, numdia, pF, ex * sx + ey * sy + ez * sz,
-                            ex, ey, ez);
-              if (A == 0.0) continue;
-            }
-            int qx = dx + ex - 2 * g_delta[s][0];
-            int qy = dy + ey - 2 * g_delta[s][1];
-            int qz = dz + ez - 2 * g_delta[s][2];
-            double p1 = pc[(size_t)pplane(dx, dy, dz) * nc + pC];
-            double p2 = pc[(size_t)pplane(qx, qy, qz) * nc + pC2];
-            acc_v += p1 * A * p2;
-          }
+          for (int dz = -1; dz <= 1; dz++)
+            for (int dy = -1; dy <= 1; dy++)
+              for (int dx = -1; dx <= 1; dx++) {
+                size_t pF =
+                    (size_t)((long long)pCf + dx * sx + dy * sy + dz * sz);
+                double part = 0.0;
+                int any = 0;
+                for (int ez = -1; ez <= 1; ez++)
+                  for (int ey = -1; ey <= 1; ey++)
+                    for (int ex = -1; ex <= 1; ex++) {
+                      int qx = dx + ex - 2 * dl[0], qy = dy + ey - 2 * dl[1],
+                          qz = dz + ez - 2 * dl[2];
+                      if (qx < -1 || qx > 1 || qy < -1 || qy > 1 || qz < -1 ||
+                          qz > 1)
+                        continue;
+                      int ne = (ex != 0) + (ey != 0) + (ez != 0);
+                      if (numdia == 7 && ne > 1) continue;
+                      any = 1;
+                      double A = ne == 0 ? acf[pF]
+                                         : -coupling(acf, nf, numdia, pF,
+                                                     ex * sx + ey * sy +
+                                                         ez * sz,
+                                                     ex, ey, ez);
+                      part += A * pc[(size_t)pplane(qx, qy, qz) * nc + pC2];
+                    }
+                if (!any) continue;
+                acc_v += pc[(size_t)pplane(dx, dy, dz) * nc + pC] * part;
+              }
           acc[(size_t)s * nc + pC] = (s == 0) ? acc_v : -acc_v;
         }
       }

@@ -1,0 +1,305 @@
+"""Pins the oracle's V-cycle (pmgo_mgdriv, restating Vmvcs
+pmgc/mgcsd.cc:299-527) with an independent restatement in sparse-matrix
+form, written from the reference text:
+
+* the operators as matrices over the interior unknowns: the fine A from
+  the VbuildA_fv planes, P from the VbuildP planes (both pinned
+  elsewhere), coarse operators as the Galerkin products P^T A P of the
+  previous level's planes with cc restricted as a vector
+  (Vbuildgaler0, mgsubd.cc:349-373);
+* Gauss-Seidel as colour-ordered sequential row updates
+  (x_S = (f_S - A_{S,rest} x) / A_SS for each colour S in turn): red-black
+  colour (i+j+k) % 2 on the 7-point fine level, 8 colours
+  (i%2)+2(j%2)+4(k%2) on the 27-point levels, order reversed with iadjoint
+  (gsd.cc:85-93, 161-169);
+* the cycle of mgcsd.cc:299-480: nu1 sweeps with iadjoint 0, the
+  residual restricted (D1), zero start and nu1 sweeps on every level above
+  the coarsest, 100 CG iterations from zero with errtol 0 on the coarsest
+  (Vcghs, cgd.cc:60-124, D5), and on the way up the prolongated coarse
+  solution added with the Hackbusch-Reusken factor
+  (x_c . w0_c) / (x_c . A_c x_c) (mgcsd.cc:429-458), then nu2 sweeps with
+  iadjoint 1; the L1 relative residual after every cycle.
+
+The oracle's iterate after each of the first three cycles and its
+residual history must equal the restatement's to 1e-12.  Negative
+controls show the comparison is sensitive to the plausible mistakes: the
+same restatement with the post-smoothing colour order not reversed, with
+xdamp = 1, or with 50 coarse CG iterations, differs by far more.  The
+Hackbusch-Reusken factor is checked against its meaning: for a Galerkin
+hierarchy without a Helmholtz term it is the step along P x_c that
+minimises the fine-level energy."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from tests import helpers as H
+
+PPLANE_OF = {d: pl for pl, d in H.PPLANE.items()}
+
+
+def _coords(n):
+    """i, j, k (1-based) of every flat index of an n^3 grid, i fastest."""
+    k, j, i = np.meshgrid(np.arange(1, n + 1), np.arange(1, n + 1), np.arange(1, n + 1),
+                          indexing="ij")
+    return i.ravel(), j.ravel(), k.ravel()
+
+
+def _interior_ids(n):
+    i, j, k = _coords(n)
+    inside = (i > 1) & (i < n) & (j > 1) & (j < n) & (k > 1) & (k < n)
+    ids = np.full(n ** 3, -1)
+    ids[inside] = np.arange(int(inside.sum()))
+    return ids, inside
+
+
+def A_from_planes(n, numdia, ac):
+    """Off-diagonal couplings and diagonal oC (no cc) of the planes as a
+    sparse matrix over the interior unknowns (matvecd.cc:137-175: stored
+    upper directions, mirrored below)."""
+    N = n ** 3
+    ids, inside = _interior_ids(n)
+    i, j, k = _coords(n)
+    p = np.nonzero(inside)[0]
+    rows, cols, vals = [ids[p]], [ids[p]], [ac[p]]
+    for pl in ([1, 2, 3] if numdia == 7 else range(1, 14)):
+        ex, ey, ez = H.UPPER27[pl]
+        ii, jj, kk = i[p] + ex, j[p] + ey, k[p] + ez
+        ok = (ii > 1) & (ii < n) & (jj > 1) & (jj < n) & (kk > 1) & (kk < n)
+        q = p[ok] + ex + ey * n + ez * n * n
+        a = -ac[pl * N + p[ok]]
+        rows += [ids[p[ok]], ids[q]]
+        cols += [ids[q], ids[p[ok]]]
+        vals += [a, a]
+    m = int(inside.sum())
+    return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                         shape=(m, m))
+
+
+def P_from_planes(nf, nc, pc):
+    """Prolongation from coarse to fine interior unknowns: the fine point
+    C_f + d takes P_C(d) x(C) (VinterpPMG2, matvecd.cc:878-960)."""
+    Nc = nc ** 3
+    idf, _ = _interior_ids(nf)
+    idc, insc = _interior_ids(nc)
+    i, j, k = _coords(nc)
+    pcs = np.nonzero(insc)[0]
+    rows, cols, vals = [], [], []
+    for (dx, dy, dz), pl in PPLANE_OF.items():
+        fi, fj, fk = 2 * i[pcs] - 1 + dx, 2 * j[pcs] - 1 + dy, 2 * k[pcs] - 1 + dz
+        ok = (fi > 1) & (fi < nf) & (fj > 1) & (fj < nf) & (fk > 1) & (fk < nf)
+        F = (fk[ok] - 1) * nf * nf + (fj[ok] - 1) * nf + (fi[ok] - 1)
+        rows.append(idf[F])
+        cols.append(idc[pcs[ok]])
+        vals.append(pc[pl * Nc + pcs[ok]])
+    return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                         shape=(int((idf >= 0).sum()), int(insc.sum())))
+
+
+def colours(n, numdia):
+    _, inside = _interior_ids(n)
+    i, j, k = (c[inside] for c in _coords(n))
+    return (i + j + k) % 2 if numdia == 7 else (i % 2) + 2 * (j % 2) + 4 * (k % 2)
+
+
+class Hier:
+    """The hierarchy as matrices, built from the oracle's fine operator and
+    its prolongations (both pinned by other tests)."""
+
+    def __init__(self, orc, inp, n, nlev, rw):
+        import multigrid_amd as mg
+        self.n = [n]
+        for _ in range(1, nlev):
+            self.n.append((self.n[-1] - 1) // 2 + 1)
+        ac, cc, fc, _ = orc.buildA_fv(n, inp)
+        _, inside = _interior_ids(n)
+        self.A = [A_from_planes(n, 7, ac)]          # ac part of each level
+        self.cc = [cc[inside]]
+        self.f = fc[inside]
+        self.P = []
+        # pc of every coarse level from the oracle's rwork (rpc|cc|fc|pc|ac)
+        sz = mg.Vmgsz(2, 0, 0, n, n, n, nlev)
+        off = sz["n_rpc"] + 2 * sz["narr"]
+        for lev in range(1, nlev):
+            nc = self.n[lev]
+            pc = rw[off:off + 27 * nc ** 3]
+            off += 27 * nc ** 3
+            P = P_from_planes(self.n[lev - 1], nc, pc)
+            self.P.append(P)
+            self.A.append((P.T @ self.A[-1] @ P).tocsr())
+            self.cc.append(P.T @ self.cc[-1])
+        self.full = [(A + sp.diags(c)).tocsr() for A, c in zip(self.A, self.cc)]
+        self.numdia = [7] + [27] * (nlev - 1)
+        self.col = [colours(m, nd) for m, nd in zip(self.n, self.numdia)]
+
+
+def gs(H_, l, b, x, sweeps, iadjoint):
+    A = H_.full[l]
+    d = A.diagonal()
+    off = (A - sp.diags(d)).tocsr()
+    ncol = 2 if H_.numdia[l] == 7 else 8
+    order = list(range(ncol))
+    if iadjoint:
+        order = order[::-1]
+    rows = [np.nonzero(H_.col[l] == c)[0] for c in order]
+    for _ in range(sweeps):
+        for S in rows:
+            x[S] = (b[S] - off[S] @ x) / d[S]
+    return x
+
+
+def cg(A, b, itmax):
+    """Vcghs with errtol 0 (cgd.cc:60-124): from x = 0, stops early only on a
+    zero residual."""
+    x = np.zeros_like(b)
+    r = b.copy()
+    if np.sqrt(r @ r) == 0.0:
+        return x
+    p = None
+    rho_old = None
+    for it in range(itmax):
+        rho = r @ r
+        if rho == 0.0:
+            break
+        p = r.copy() if it == 0 else r + (rho / rho_old) * p
+        Ap = A @ p
+        alpha = rho / (p @ Ap)
+        rho_old = rho
+        x += alpha * p
+        r -= alpha * Ap
+    return x
+
+
+def vcycle(H_, x, nu1=2, nu2=2, post_iadjoint=1, damp=True, cg_its=100, trace=None):
+    nlev = len(H_.n)
+    rhs = [H_.f] + [None] * (nlev - 1)
+    xs = [x] + [None] * (nlev - 1)
+    gs(H_, 0, H_.f, x, nu1, 0)
+    r = H_.f - H_.full[0] @ x
+    for l in range(1, nlev):
+        rhs[l] = H_.P[l - 1].T @ r
+        if l != nlev - 1:
+            xs[l] = gs(H_, l, rhs[l], np.zeros_like(rhs[l]), nu1, 0)
+            r = rhs[l] - H_.full[l] @ xs[l]
+    xs[-1] = cg(H_.full[-1], rhs[-1], cg_its)
+    for l in range(nlev - 2, -1, -1):
+        xc = xs[l + 1]
+        e = H_.P[l] @ xc
+        den = xc @ (H_.full[l + 1] @ xc)
+        xdamp = (xc @ rhs[l + 1]) / den if damp else 1.0
+        if trace is not None:
+            trace.append((l, xs[l].copy(), e.copy(), xdamp))
+        xs[l] += xdamp * e
+        gs(H_, l, rhs[l], xs[l], nu2, post_iadjoint)
+    return x
+
+
+def restated(H_, cycles, **kw):
+    x = np.zeros_like(H_.f)
+    den = np.abs(H_.f).sum()
+    its, hist = [], []
+    for _ in range(cycles):
+        vcycle(H_, x, **kw)
+        its.append(x.copy())
+        hist.append(np.abs(H_.f - H_.full[0] @ x).sum() / den)
+    return its, np.array(hist)
+
+
+def _oracle_runs(orc, inp, n, nlev, cycles):
+    _, inside = _interior_ids(n)
+    out, hist = [], None
+    for k in range(1, cycles + 1):
+        e, u, _, rw = orc.solve(inp, n, nlev, itmax=k, errtol=1e-300)
+        it, ie, _, h = orc.last_stats()
+        assert e == 1 and it == k and ie == 1
+        out.append(u[inside])
+        hist = h
+    return out, hist, rw
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+CASES = [(9, 2, 8, 2.0, 1), (17, 3, 30, 3.5, 2), (33, 3, 60, 6.0, 3)]
+
+
+@pytest.fixture(scope="module")
+def runs(orc):
+    out = {}
+    for n, nlev, natoms, R, seed in CASES:
+        inp = orc.pbe_synth(n, 0.5, natoms, R, 0.15, 0, seed)
+        its, hist, rw = _oracle_runs(orc, inp, n, nlev, 3)
+        Hh = Hier(orc, inp, n, nlev, rw)
+        out[n] = (inp, its, hist, Hh)
+    return out
+
+
+@pytest.mark.parametrize("n", [c[0] for c in CASES])
+def test_iterates_and_history_match_restatement(runs, orc, n):
+    inp, its, hist, Hh = runs[n]
+    mine, mhist = restated(Hh, 3)
+    errs = [_rel(a, b) for a, b in zip(mine, its)]
+    herr = np.max(np.abs(mhist - hist[:3]) / hist[:3])
+    print(f"n={n}: iterate errors {errs}, history {hist[:3]} err {herr:.1e}")
+    assert max(errs) <= 1e-12
+    assert herr <= 1e-12
+    assert hist[0] < 0.5 and np.all(np.diff(hist[:3]) < 0)
+
+
+@pytest.mark.parametrize("n", [c[0] for c in CASES])
+def test_galerkin_operators_match_oracle_planes(runs, orc, n):
+    """The restatement's coarse matrices P^T A P equal the oracle's
+    14-plane coarse operators (second pin of pmgo_buildG at these sizes)."""
+    inp, _, _, Hh = runs[n]
+    nlev = len(Hh.n)
+    _, _, _, rw = orc.solve(inp, n, nlev, itmax=1, errtol=1e-300)
+    import multigrid_amd as mg
+    sz = mg.Vmgsz(2, 0, 0, n, n, n, nlev)
+    off = sz["n_rpc"] + 2 * sz["narr"] + 27 * sz["narrc"] + 4 * sz["nf"]
+    for l in range(1, nlev):
+        nc = Hh.n[l]
+        ac = rw[off:off + 14 * nc ** 3]
+        off += 14 * nc ** 3
+        B = A_from_planes(nc, 27, ac)
+        D = (B - Hh.A[l]).toarray()
+        assert np.max(np.abs(D)) <= 1e-12 * np.max(np.abs(B.toarray()))
+
+
+@pytest.mark.parametrize("variant", ["post_not_reversed", "xdamp_one", "cg_50"])
+def test_negative_controls_differ(runs, variant):
+    """Each plausible mistake moves the iterates far beyond the 1e-12 bar;
+    the 50-iteration coarse CG shows only where the coarsest level (9^3,
+    343 unknowns) is not solved to rounding by 50 iterations."""
+    n = 33 if variant == "cg_50" else 17
+    _, its, hist, Hh = runs[n]
+    kw = {"post_not_reversed": dict(post_iadjoint=0), "xdamp_one": dict(damp=False),
+          "cg_50": dict(cg_its=50)}[variant]
+    mine, mhist = restated(Hh, 3, **kw)
+    err = max(_rel(a, b) for a, b in zip(mine, its))
+    print(f"{variant}: iterate error {err:.2e}")
+    assert err > (1e-10 if variant == "cg_50" else 1e-6)
+
+
+def test_xdamp_minimises_fine_energy(orc):
+    """Without a Helmholtz term the coarse operator is exactly P^T A P and
+    the coarse right-hand side P^T r, so (x_c . w0_c) / (x_c . A_c x_c) is
+    the step t minimising E(t) = 1/2 (x + t e)^T A (x + t e) - f^T (x + t e)
+    along e = P x_c.  The restatement's factor equals that minimiser
+    computed from the fine matrix alone, and the oracle's cycle matches the
+    restatement.  Three levels, so that the correction from level 2 is not
+    an exact solve there (an exact coarse solve makes the factor 1)."""
+    n, nlev = 17, 3
+    inp = orc.pbe_synth(n, 0.5, 20, 3.0, 0.0, 0, 9)
+    its, hist, rw = _oracle_runs(orc, inp, n, nlev, 1)
+    Hh = Hier(orc, inp, n, nlev, rw)
+    assert np.max(np.abs(Hh.cc[0])) == 0.0
+    trace = []
+    x = np.zeros_like(Hh.f)
+    vcycle(Hh, x, trace=trace)
+    assert _rel(x, its[0]) <= 1e-12
+    (l, x0, e, xdamp), = [t for t in trace if t[0] == 0]
+    A = Hh.full[0]
+    tstar = (e @ (Hh.f - A @ x0)) / (e @ (A @ e))
+    print(f"xdamp {xdamp:.15f} fine-energy minimiser {tstar:.15f}")
+    assert abs(xdamp - tstar) <= 1e-10 * abs(tstar)
+    assert abs(xdamp - 1.0) > 1e-6  # the factor is not trivially 1 here
