@@ -1,12 +1,21 @@
 """Benchmark: PB solves/sec + p99 latency at 1e-6 relative residual,
 mixed precision vs FP64 on one MI355X.
 
-Workload: the 161^3 linearised Poisson-Boltzmann solve (h = 0.4 A, 5 levels,
-1000 synthetic atoms in a 25 A sphere, 0.15 M salt), V(2,2) cycles with
-Galerkin coarsening, stop at 1e-6 L1 relative residual.  One step = one
-complete solve of a molecule resident in HBM: assembly of the grid inputs
-from the atom list on the GPU, operator assembly (VbuildA, VbuildP,
-VbuildG) and the V-cycles to tolerance.
+Headline workload: the serving stream of linearised Poisson-Boltzmann
+requests (grid sizes uniform over {65, 97, 129, 161}^3 at h = 0.5 A,
+100-3000 synthetic atoms each, 0.15 M salt, V(2,2) Galerkin cycles to 1e-6
+L1 relative residual, contexts within a 200 GB device-memory budget).  One
+step = one batch of --batch requests of the stream; the timed region
+submits the --steps batches at once and waits for the last result, so
+`value` is the serving loop's mixed-precision capacity in solves/s (each
+request: GPU input assembly from its atom list, operator build, V-cycles).
+The FP64 capacity, and p50/p99 latency of both precisions under Poisson
+arrivals at 0.8x the FP64 capacity, sit beside it.
+
+Further legs: the 161^3 and 289^3 single solves (p50/p99 over >= 10
+timed solves each), the 129^3 Newton solve, the banded coarsest solve,
+the class-style PCG; the roofline of the fine FP32 sweep at 289^3; the CPU
+oracle on the host cores.
 
 Prints one JSON line (see the contract at the end of main()).
 """
@@ -27,21 +36,22 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="c1")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--serve-requests", type=int, default=2000,
-                    help="requests of the serving stream (0: skip the serving leg)")
+    ap.add_argument("--steps", type=int, default=20,
+                    help="timed steps: batches of --batch requests of the serving stream")
+    ap.add_argument("--warmup", type=int, default=3,
+                    help="untimed warm-up batches (fill the context pool of every shape)")
+    ap.add_argument("--batch", type=int, default=100, help="requests per step")
     ap.add_argument("--serve-active", type=int, default=8,
                     help="solves in flight at once in the serving loop")
+    ap.add_argument("--single-steps", type=int, default=10,
+                    help="timed solves of each single-solve leg (161^3, 289^3)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extra", action="store_true",
-                    help="skip the 289^3 mixed-precision and 129^3 Newton legs")
-    ap.add_argument("--legs", default="large,newton,band,pcg",
-                    help="extra legs to run: large (289^3 mixed), newton (129^3), "
-                         "band (161^3 with the banded coarsest solve), pcg (161^3 "
-                         "with the class-style PCG)")
+                    help="skip the single-solve, Newton, band and PCG legs")
+    ap.add_argument("--legs", default="single,large,newton,band,pcg",
+                    help="extra legs: single (161^3), large (289^3, also the roofline), "
+                         "newton (129^3), band (161^3 with the banded coarsest solve), "
+                         "pcg (161^3 with the class-style PCG)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (a reference "
                          "kernel of known traffic for hardware-counter runs)")
@@ -76,8 +86,8 @@ def sweep_bytes(pts, sweeps, nu=(2, 2)):
 
 
 def measured_traffic(n):
-    """HBM bytes per launch of the fine sweep for an n^3 level, as measured
-    with rocprofv3 FETCH_SIZE / WRITE_SIZE counters and recorded in
+    """HBM bytes per launch of the fine FP32 sweep of an n^3 level, as
+    measured with rocprofv3 FETCH_SIZE / WRITE_SIZE counters and recorded in
     profiles/pb<n>_hbm_traffic.md (None when no such measurement exists)."""
     path = os.path.join(ROOT, "profiles", f"pb{n}_hbm_traffic.md")
     try:
@@ -89,27 +99,86 @@ def measured_traffic(n):
     return None
 
 
-def cpu_baseline(cfg, atoms, threads):
-    """The CPU oracle (C restatement of the reference pmgc, OpenMP) timed on
-    this host on one complete solve of the same molecule: the solve
-    (Vmgdriv: setup + V-cycles) is what is timed; the oracle's serial input
-    assembly is timed separately.  Returns the baseline and the oracle's
-    FP64 potential."""
-    if threads:
-        os.environ["OMP_NUM_THREADS"] = str(threads)
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _omp_threads(n):
+    """Set the OpenMP thread count of the (already loaded) oracle."""
+    import ctypes
+    for name in ("libgomp.so.1", "libgomp.so"):
+        try:
+            ctypes.CDLL(name).omp_set_num_threads(int(n))
+            return True
+        except OSError:
+            continue
+    return False
+
+
+def cpu_baseline(mg, stream_sizes, threads):
+    """The CPU oracle (C restatement of the reference pmgc, OpenMP over the
+    host cores) on this host: one request of each serving size (assembly +
+    setup + V-cycles to 1e-6, the same work a served request does), the
+    implied CPU capacity for the stream's size mix, a 1-thread solve of the
+    65^3 workload, and the 161^3 / 129^3 Newton / 289^3 workloads.  Returns
+    the baseline and the oracle's potentials for the error columns."""
     from tests import oracle as orc
 
-    n, nlev = cfg["n"], cfg["nlev"]
+    orc.olib()
+    nthreads = int(threads or os.environ.get("OMP_NUM_THREADS", 0) or os.cpu_count() or 1)
+    _omp_threads(nthreads)
+    per_size = {}
+    for n, natoms, seed in ((65, 1550, 1), (97, 1550, 2), (129, 1550, 3), (161, 1550, 4)):
+        half = (n - 1) * 0.5 / 2
+        R = min(14.0 * (natoms / 200.0) ** (1.0 / 3.0), 0.75 * half - 2.0)
+        atoms = mg.pbe_molecule(natoms, R, 0, 77 + seed)
+        t0 = time.perf_counter()
+        inp = orc.pbe_assemble(n, 0.5, 0.15, atoms)
+        t1 = time.perf_counter()
+        orc.solve(inp, n, mg.SERVE_LEVELS[n])
+        t2 = time.perf_counter()
+        per_size[n] = {"assembly_s": t1 - t0, "solve_s": t2 - t1,
+                       "request_s": t2 - t0, "vcycles": orc.last_stats()[0]}
+    # the stream draws its sizes uniformly: mean request time over its mix
+    mean_s = float(np.mean([per_size[n]["request_s"] for n in stream_sizes]))
+    out = {"value": 1.0 / mean_s, "unit": "solves/s", "cores": nthreads, "kind": "port",
+           "cpu_model": cpu_model(),
+           "sample": "one 1550-atom request of each serving size (65/97/129/161^3, "
+                     "h 0.5): oracle input assembly + Vmgdriv (setup + V-cycles to "
+                     "1e-6); value = 1 / mean request time over the stream's size mix",
+           "per_size": per_size}
+    us = {}
+    for name in ("c1", "c3", "c2"):
+        c = mg.CONFIGS[name]
+        atoms = mg.pbe_molecule(c["natoms"], c["sphere_r"], c["charge_mode"], c["seed"])
+        t0 = time.perf_counter()
+        inp = orc.pbe_assemble(c["n"], c["h"], c["ionic"], atoms)
+        t1 = time.perf_counter()
+        if name == "c3":
+            orc.mypdefinit([1.0, -1.0], [0.5, 0.5])
+            _, u, _, _ = orc.solve(inp, c["n"], c["nlev"], nonlin=True, errtol=1e-8)
+        else:
+            _, u, _, _ = orc.solve(inp, c["n"], c["nlev"])
+        t2 = time.perf_counter()
+        out[name] = {"assembly_s": t1 - t0, "solve_s": t2 - t1,
+                     "iters": orc.last_stats()[0]}
+        us[name] = (u, orc.last_stats()[0])
+    c = mg.CONFIGS["c0"]
+    atoms = mg.pbe_molecule(c["natoms"], c["sphere_r"], c["charge_mode"], c["seed"])
+    inp = orc.pbe_assemble(c["n"], c["h"], c["ionic"], atoms)
+    _omp_threads(1)
     t0 = time.perf_counter()
-    inp = orc.pbe_assemble(n, cfg["h"], cfg["ionic"], atoms)
-    ta = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    _, u, _, _ = orc.solve(inp, n, nlev)
-    dt = time.perf_counter() - t0
-    nthreads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": 1.0 / dt, "unit": "solves/s", "cores": nthreads, "kind": "port",
-            "sample": f"1 full {n}^3 solve (setup + V-cycles to 1e-6), {dt:.2f} s; "
-                      f"serial input assembly {ta:.2f} s not included"}, u
+    orc.solve(inp, c["n"], c["nlev"])
+    t1 = time.perf_counter()
+    _omp_threads(nthreads)
+    out["c0_one_thread"] = {"solve_s": t1 - t0, "vcycles": orc.last_stats()[0]}
+    return out, us
 
 
 def serve_requests(mg, count, seed=20261206):
@@ -132,7 +201,8 @@ def serve_requests(mg, count, seed=20261206):
 
 def serve_stream(mg, reqs, mode, rate, active, budget):
     """Submit the requests with Poisson arrivals at `rate` per second (all
-    at once when rate is None) and wait for all; returns the server stats."""
+    at once when rate is None) and wait for all; returns the server stats
+    and the wall time from the first submission to the last result."""
     srv = mg.Server(budget, active)
     params = {n: mg.make_params(n, nlev) for n, nlev in set((r[0], r[1]) for r in reqs)}
     rng = np.random.default_rng(7)
@@ -147,91 +217,101 @@ def serve_stream(mg, reqs, mode, rate, active, budget):
                 time.sleep(d)
         iparm, rparm = params[n]
         tickets.append(srv.submit(iparm, rparm, mode, 0.5, 0.15, atoms, want_u=False))
-    iters = [srv.wait(t)["iters"] for t in tickets]
+    res = [srv.wait(t) for t in tickets]
     wall = time.perf_counter() - t0
     st = srv.stats()
     srv.close()
     st["wall_s"] = wall
-    st["mean_iters"] = float(np.mean(iters))
+    st["mean_iters"] = float(np.mean([r["iters"] for r in res]))
+    st["max_relres"] = float(max(r["relres"] for r in res))
+    st["all_ok"] = all(r["rc"] == 0 for r in res)
     print(f"serving: {len(reqs)} requests mode={mode} rate={rate} -> "
           f"{st['solves_per_s']:.1f} solves/s p50={st['p50_ms']:.2f} ms "
-          f"p99={st['p99_ms']:.2f} ms", file=sys.stderr, flush=True)
+          f"p99={st['p99_ms']:.2f} ms wall={wall:.2f} s", file=sys.stderr, flush=True)
     return st
 
 
-def serving_leg(mg, count, active):
-    reqs = serve_requests(mg, count)
+KEEP = ("solves_per_s", "p50_ms", "p99_ms", "max_concurrent", "mean_iters", "max_relres",
+        "wall_s")
+
+
+def serving_leg(mg, steps, warmup, batch, active):
+    """The headline: the stream's --steps batches submitted at once, mixed
+    precision (timed: `value`) and FP64 (capacity beside it); then both
+    precisions under Poisson arrivals at 0.8x the FP64 capacity for
+    p50/p99 latency."""
     budget = 200 * (1 << 30)
-    # warm the context pool (allocation and graph capture of each shape)
-    for n in (65, 97, 129, 161):
-        first = [r for r in reqs if r[0] == n][: active]
-        serve_stream(mg, first, mg.PMG_FP64, None, active, budget)
-        serve_stream(mg, first, mg.PMG_MIXED, None, active, budget)
-    # FP64 capacity: the whole stream submitted at once
-    cap = serve_stream(mg, reqs, mg.PMG_FP64, None, active, budget)
-    rate = 0.8 * cap["solves_per_s"]
+    reqs = serve_requests(mg, steps * batch)
+    warm = serve_requests(mg, max(warmup, 1) * batch, seed=20261207)
+    for mode in (mg.PMG_FP64, mg.PMG_MIXED):  # context pool of every shape
+        serve_stream(mg, warm, mode, None, active, budget)
+    cap64 = serve_stream(mg, reqs, mg.PMG_FP64, None, active, budget)
+    capmx = serve_stream(mg, reqs, mg.PMG_MIXED, None, active, budget)
+    rate = 0.8 * cap64["solves_per_s"]
     mixed = serve_stream(mg, reqs, mg.PMG_MIXED, rate, active, budget)
     fp64 = serve_stream(mg, reqs, mg.PMG_FP64, rate, active, budget)
-    cap_mx = serve_stream(mg, reqs, mg.PMG_MIXED, None, active, budget)
-    keep = ("solves_per_s", "p50_ms", "p99_ms", "max_concurrent", "mean_iters", "wall_s")
     return {
-        "workload": f"{count} linearised PB requests, n uniform over {{65,97,129,161}}^3, "
-                    "h=0.5, 100-3000 atoms, stop 1e-6, Poisson arrivals",
+        "requests": len(reqs),
         "arrival_rate_per_s": rate,
         "max_active": active,
         "budget_bytes": budget,
         "results_on_device": True,
-        "mixed": {k: mixed[k] for k in keep},
-        "fp64": {k: fp64[k] for k in keep},
-        "capacity_solves_per_s": {"fp64": cap["solves_per_s"], "mixed": cap_mx["solves_per_s"]},
-    }
+        "capacity": {"mixed": {k: capmx[k] for k in KEEP}, "fp64": {k: cap64[k] for k in KEEP}},
+        "at_rate": {"mixed": {k: mixed[k] for k in KEEP}, "fp64": {k: fp64[k] for k in KEEP}},
+        "mixed_vs_fp64_capacity": capmx["solves_per_s"] / cap64["solves_per_s"],
+        "all_ok": capmx["all_ok"] and cap64["all_ok"] and mixed["all_ok"] and fp64["all_ok"],
+    }, capmx
 
 
 def rel_err(a, b):
     return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
 
 
-def large_leg(mg, cpu):
-    """The 289^3 workload (h = 0.35 A, 6 levels, 5000 atoms, charges over
-    {-1,-0.5,0,0.5,1}): one FP64 and one mixed-precision solve (assembly +
-    build + V-cycles), the FP32 fine sweep's bandwidth, and the error of the
-    mixed potential against the FP64 GPU solve and (with the CPU leg) the
-    FP64 oracle."""
-    cfg = mg.CONFIGS["c2"]
+def single_leg(mg, name, steps, oracle=None):
+    """One workload solved again and again on a resident context (each step:
+    GPU input assembly from the atom list, hierarchy build, V-cycles to
+    1e-6), FP64 and mixed precision, p50/p99 over `steps` timed solves; the
+    fine FP32 sweep timed with HIP events on the context stream over further
+    mixed solves; errors against the FP64 GPU solve and the oracle."""
+    cfg = mg.CONFIGS[name]
     n, nlev = cfg["n"], cfg["nlev"]
     atoms = mg.pbe_molecule(cfg["natoms"], cfg["sphere_r"], cfg["charge_mode"], cfg["seed"])
     mol = (cfg["h"], cfg["ionic"], atoms)
     iparm, rparm = mg.make_params(n, nlev)
     ctx = mg.Context(iparm, rparm, pbe=mol)
-    out = {"workload": f"linearised PBE {n}^3 h={cfg['h']} nlev={nlev} {cfg['natoms']} atoms"}
-    for name, mode in (("fp64", mg.PMG_FP64), ("mixed", mg.PMG_MIXED)):
-        ctx.load_pbe(*mol)
-        ctx.solve(mode)  # warm (graph capture)
-        t0 = time.perf_counter()
-        ctx.load_pbe(*mol)
-        ctx.solve(mode)
-        dt = time.perf_counter() - t0
+    out = {"workload": f"linearised PBE {n}^3 h={cfg['h']} nlev={nlev} {cfg['natoms']} atoms, "
+                       "V(2,2) Galerkin, stop 1e-6"}
+    us = {}
+    for key, mode in (("fp64", mg.PMG_FP64), ("mixed", mg.PMG_MIXED)):
+        wall, lat, its = timed_steps(ctx, mol, mode, steps, 2)
         st = ctx.stats()
-        out[name] = {"ms": dt * 1e3, "iters": st["iters"], "relres": st["relres"],
-                     "setup_ms": st["setup_ms"]}
-        out[name + "_u"] = ctx.u()
-    ctx.ktime_enable(4096)
-    ctx.solve(mg.PMG_MIXED)
+        out[key] = {"solves_per_s": steps / wall, "p50_ms": float(np.percentile(lat, 50)),
+                    "p99_ms": float(np.percentile(lat, 99)), "vcycles": its[-1],
+                    "relres": st["relres"], "setup_ms": st["setup_ms"],
+                    "solve_ms": st["solve_ms"]}
+        us[key] = ctx.u()
+    out["mixed_vs_fp64_speedup"] = out["mixed"]["solves_per_s"] / out["fp64"]["solves_per_s"]
+    ctx.ktime_enable(16384)
+    ksteps = 4
+    for _ in range(ksteps):
+        ctx.solve(mg.PMG_MIXED)
     kms, sweeps = ctx.ktime()
     ctx.close()
-    out["fp32_sweep_gbs"] = sweep_bytes((n - 2) ** 3, sweeps) / (kms / 1e3) / 1e9
-    u64, umx = out.pop("fp64_u"), out.pop("mixed_u")
-    out["mixed_rel_err_vs_fp64_gpu"] = rel_err(umx, u64)
-    if cpu:
-        from tests import oracle as orc
-        inp = orc.pbe_assemble(n, cfg["h"], cfg["ionic"], atoms)
-        _, u0, _, _ = orc.solve(inp, n, nlev)
-        out["fp64_rel_err_vs_oracle"] = rel_err(u64, u0)
-        out["mixed_rel_err_vs_oracle"] = rel_err(umx, u0)
+    pts = (n - 2) ** 3
+    avg_s = (kms / 1e3) / max(sweeps, 1)
+    achieved = sweep_bytes(pts, sweeps) / max(sweeps, 1) / avg_s / 1e9 if sweeps else 0.0
+    out["fp32_sweep"] = {"gbs": achieved, "avg_launch_us": avg_s * 1e6, "launches": sweeps,
+                         "bytes_per_launch": sweep_bytes(pts, sweeps) / max(sweeps, 1)}
+    out["mixed_rel_err_vs_fp64_gpu"] = rel_err(us["mixed"], us["fp64"])
+    if oracle is not None:
+        u0, it0 = oracle
+        out["oracle_vcycles"] = it0
+        out["fp64_rel_err_vs_oracle"] = rel_err(us["fp64"], u0)
+        out["mixed_rel_err_vs_oracle"] = rel_err(us["mixed"], u0)
     return out
 
 
-def newton_leg(mg, cpu):
+def newton_leg(mg, oracle=None):
     """The 129^3 nonlinear PBE (sinh ionic term, 0.3 M 1:1 salt, 800 atoms):
     Newton with FP64 and with mixed-precision inner V-cycles to 1e-8, and
     (with the CPU leg) the error against the FP64 oracle's Newton."""
@@ -255,12 +335,9 @@ def newton_leg(mg, cpu):
         us[name] = ctx.u()
     ctx.close()
     out["mixed_rel_err_vs_fp64_gpu"] = rel_err(us["mixed"], us["fp64"])
-    if cpu:
-        from tests import oracle as orc
-        orc.mypdefinit(*ions)
-        inp = orc.pbe_assemble(n, cfg["h"], cfg["ionic"], atoms)
-        _, u0, _, _ = orc.solve(inp, n, nlev, nonlin=True, errtol=1e-8)
-        out["oracle_newton_iters"] = orc.last_stats()[0]
+    if oracle is not None:
+        u0, it0 = oracle
+        out["oracle_newton_iters"] = it0
         out["fp64_rel_err_vs_oracle"] = rel_err(us["fp64"], u0)
         out["mixed_rel_err_vs_oracle"] = rel_err(us["mixed"], u0)
     return out
@@ -362,59 +439,43 @@ def main():
         torch.cuda.synchronize()
         del a, b
 
-    cfg = dict(mg.CONFIGS[args.config])
-    n, nlev = cfg["n"], cfg["nlev"]
-    atoms = mg.pbe_molecule(cfg["natoms"], cfg["sphere_r"], cfg["charge_mode"], cfg["seed"])
-    mol = (cfg["h"], cfg["ionic"], atoms)
-    iparm, rparm = mg.make_params(n, nlev)
-    ctx = mg.Context(iparm, rparm, pbe=mol)
+    # headline: the serving stream (the timed region is its mixed capacity run)
+    serving, capmx = serving_leg(mg, args.steps, args.warmup, args.batch, args.serve_active)
+    wall = capmx["wall_s"]
+    value = args.steps * args.batch / wall
 
-    # FP64 GPU path (the drop-in arithmetic)
-    wall64, lat64, it64 = timed_steps(ctx, mol, mg.PMG_FP64, args.steps, args.warmup)
-    u64 = ctx.u()
-    st64 = ctx.stats()
-    # mixed precision: FP32 V-cycles inside FP64 defect correction
-    wallmx, latmx, itmx = timed_steps(ctx, mol, mg.PMG_MIXED, args.steps, args.warmup)
-    umx = ctx.u()
-    st = ctx.stats()
-    # kernel timing: HIP events on the context stream around every fine
-    # smoothing launch of further mixed solves (direct launches; the timed
-    # steps above replay each iteration from a captured HIP graph)
-    ctx.ktime_enable(16384)
-    ksteps = max(3, args.steps // 4)
-    for _ in range(ksteps):
-        ctx.solve(mg.PMG_MIXED)
-    kms, ksweeps = ctx.ktime()
-    err = float(np.max(np.abs(umx - u64)) / np.max(np.abs(u64)))
-
-    # roofline of the dominant kernel: the fine-level fused red-black sweep
-    # (FP32 in the mixed path), one launch per full sweep.
-    pts = (n - 2) ** 3
-    launches = ksweeps
-    avg_launch_s = (kms / 1e3) / max(launches, 1)
-    bytes_launch = sweep_bytes(pts, launches) / max(launches, 1)
-    achieved = bytes_launch / avg_launch_s / 1e9 if launches else 0.0
-
-    ctx.close()
-    cpu = None
-    err_oracle = {}
+    cpu, oracle_u = None, {}
     if not args.no_cpu:
-        cpu, u_or = cpu_baseline(cfg, atoms, args.cpu_threads)
-        den = float(np.max(np.abs(u_or)))
-        err_oracle = {"fp64_potential_rel_err_vs_oracle": float(np.max(np.abs(u64 - u_or))) / den,
-                      "mixed_potential_rel_err_vs_oracle": float(np.max(np.abs(umx - u_or))) / den}
-
-    serving = serving_leg(mg, args.serve_requests, args.serve_active) \
-        if args.serve_requests > 0 else None
+        cpu, oracle_u = cpu_baseline(mg, (65, 97, 129, 161), 0)
+        print(f"cpu: {cpu}", file=sys.stderr, flush=True)
     extra = {}
     legs = set() if args.no_extra else set(args.legs.split(","))
-    for key, name, fn in (("large", "large_289", large_leg), ("newton", "newton_129", newton_leg),
-                          ("band", "band_161", band_leg), ("pcg", "pcg_161", pcg_leg)):
-        if key in legs:
-            extra[name] = fn(mg, not args.no_cpu)
-            print(f"{key}: {extra[name]}", file=sys.stderr, flush=True)
+    if "single" in legs:
+        extra["single_161"] = single_leg(mg, "c1", args.single_steps, oracle_u.get("c1"))
+    if "large" in legs:
+        extra["large_289"] = single_leg(mg, "c2", args.single_steps, oracle_u.get("c2"))
+    if "newton" in legs:
+        extra["newton_129"] = newton_leg(mg, oracle_u.get("c3"))
+    if "band" in legs:
+        extra["band_161"] = band_leg(mg, not args.no_cpu)
+    if "pcg" in legs:
+        extra["pcg_161"] = pcg_leg(mg, not args.no_cpu)
+    for k, v in extra.items():
+        print(f"{k}: {v}", file=sys.stderr, flush=True)
 
-    value = args.steps / wallmx
+    # roofline of the dominant kernel, the fine FP32 red-black sweep, on the
+    # largest grid (289^3); the 161^3 figure when that leg was skipped
+    rl = None
+    for key, n in (("large_289", 289), ("single_161", 161)):
+        if key in extra:
+            sw = extra[key]["fp32_sweep"]
+            rl = {"bound": "hbm", "achieved": sw["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                  "frac": sw["gbs"] / HBM_PEAK_GBS, "traffic": measured_traffic(n),
+                  "kernel": f"k_gs7_sweep<float> (fine fused red-black sweep, {n}^3)",
+                  "avg_launch_us": sw["avg_launch_us"], "timed_launches": sw["launches"],
+                  "algorithmic_bytes_per_launch": sw["bytes_per_launch"]}
+            break
+    at = serving["at_rate"]
     out = {
         "metric": "PB solves/sec + p99 latency at 1e-6 rel residual: mixed-precision vs FP64 GPU",
         "value": value,
@@ -422,30 +483,24 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wallmx / args.steps * 1e3,
+        "ms_per_step": wall / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32-in-f64",
         "data": "synthetic",
-        "config": {"workload": f"linearised PBE {n}^3 h={cfg['h']} nlev={nlev} "
-                               f"{cfg['natoms']} atoms, V(2,2) Galerkin, stop 1e-6",
-                   "n": n, "nlev": nlev, "batch": 1},
-        "p50_ms": float(np.percentile(latmx, 50)),
-        "p99_ms": float(np.percentile(latmx, 99)),
-        "fp64": {"value": args.steps / wall64, "p50_ms": float(np.percentile(lat64, 50)),
-                 "p99_ms": float(np.percentile(lat64, 99)), "vcycles": it64[-1],
-                 "solve_ms": st64["solve_ms"], "setup_ms": st64["setup_ms"]},
-        "mixed_solve_ms": st["solve_ms"],
-        "mixed_vs_fp64_speedup": (args.steps / wallmx) / (args.steps / wall64),
-        "mixed_outer_iters": itmx[-1],
-        "mixed_potential_rel_err_vs_fp64": err,
-        **err_oracle,
-        "setup_ms": st["setup_ms"],
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(n),
-                     "kernel": "k_gs7_sweep<float> (fine fused red-black sweep)",
-                     "avg_launch_us": avg_launch_s * 1e6, "timed_launches": launches},
+        "config": {"workload": f"serving stream of {args.steps * args.batch} linearised PB "
+                               "requests, n uniform over {65,97,129,161}^3, h=0.5, 100-3000 "
+                               "atoms, stop 1e-6, 200 GB budget; value = mixed-precision "
+                               "capacity (all requests submitted at once)",
+                   "requests_per_step": args.batch, "max_active": args.serve_active},
+        "p50_ms": at["mixed"]["p50_ms"],
+        "p99_ms": at["mixed"]["p99_ms"],
+        "fp64": {"value": serving["capacity"]["fp64"]["solves_per_s"],
+                 "p50_ms": at["fp64"]["p50_ms"], "p99_ms": at["fp64"]["p99_ms"]},
+        "latency_at_rate_per_s": serving["arrival_rate_per_s"],
+        "mixed_vs_fp64_capacity": serving["mixed_vs_fp64_capacity"],
+        "roofline": rl,
         "cpu_baseline": cpu,
         "serving": serving,
         **extra,
