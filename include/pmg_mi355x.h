@@ -11,7 +11,8 @@
  *   PMG_OK 0, PMG_ITMAX 1 (itmax reached), PMG_EBADPARM 2 (unsupported or
  *   invalid iparm/rparm), PMG_ENOTSPD 3 (coarse operator breakdown),
  *   PMG_ENOMEM 4 (device allocation failed), PMG_EDEVICE 5 (no usable GPU
- *   or a HIP runtime error), PMG_EBUSY 6 (serving admission refused).
+ *   or a HIP runtime error), PMG_EBUSY 6 (serving admission refused),
+ *   PMG_ESCRATCH 7 (beyond the fixed scratch of an allocation-free call).
  *
  * Host pointers are accepted; the library copies inputs to HBM, builds the
  * hierarchy there, solves, and copies results back.  The `*_ctx` calls keep
@@ -34,6 +35,7 @@ extern "C" {
 #define PMG_ENOMEM 4
 #define PMG_EDEVICE 5
 #define PMG_EBUSY 6
+#define PMG_ESCRATCH 7
 
 /* Replaces pmgc::Vmgsz (pmgc/mgdrvd.h:236-255, mgdrvd.cc:505-601).
  * out[12] = {nxc,nyc,nzc,nf,nc,narr,narrc,n_rpc,n_iz,n_ipc,iretot,iintot}.
@@ -228,6 +230,19 @@ int pmg_correct_f32(int nxc, int nyc, int nzc, int nxf, int nyf, int nzf,
 int pmg_mresid_mixed(int nx, int ny, int nz, const double *ac,
                      const double *cc, const double *fc, const double *x,
                      float *r, double *l1);
+/* pmg_mresid_mixed without any allocation (no counterpart in the
+ * reference; for callers that cannot allocate device memory: code under HIP
+ * graph capture, tight serving loops).  All pointers are DEVICE pointers
+ * the caller owns; l1 (required) receives the FP64 L1 norm; the work is
+ * enqueued on stream (NULL: the null stream) and the call does not wait.
+ * The norm's block partials use a fixed 1 MiB device array per GPU held
+ * until the process ends, so grids needing more than 131072 partials
+ * (beyond about 513^3) return PMG_ESCRATCH.  Calls that overlap on one GPU
+ * share that array and race. */
+int pmg_mresid_mixed_noalloc(int nx, int ny, int nz, const double *ac,
+                             const double *cc, const double *fc,
+                             const double *x, float *r, double *l1,
+                             void *stream);
 /* The update of the mixed-precision solve: u(interior) += e (FP64 += FP32,
  * Vxaxpy with alpha 1, mikpckd.cc:115-131). */
 int pmg_addcorr_f32(int nx, int ny, int nz, const float *e, double *u);

@@ -19,7 +19,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libpmg_mi355x.so")
 
-PMG_OK, PMG_ITMAX, PMG_EBADPARM, PMG_ENOTSPD, PMG_ENOMEM, PMG_EDEVICE, PMG_EBUSY = range(7)
+(PMG_OK, PMG_ITMAX, PMG_EBADPARM, PMG_ENOTSPD, PMG_ENOMEM, PMG_EDEVICE, PMG_EBUSY,
+ PMG_ESCRATCH) = range(8)
 PMG_FP64, PMG_MIXED = 0, 1
 _ERRS = {
     PMG_EBADPARM: "invalid or unsupported iparm/rparm or array shape",
@@ -27,6 +28,7 @@ _ERRS = {
     PMG_ENOMEM: "device memory allocation failed",
     PMG_EDEVICE: "no usable GPU or HIP runtime error",
     PMG_EBUSY: "admission refused",
+    PMG_ESCRATCH: "beyond the fixed scratch of an allocation-free call",
 }
 
 
@@ -98,6 +100,7 @@ def _declare(L):
         "pmg_correct_f32": (i, [i] * 7 + [fp] * 6),
         "pmg_mresid_mixed": (i, [i, i, i, dp, dp, dp, dp, fp, dp]),
         "pmg_addcorr_f32": (i, [i, i, i, fp, dp]),
+        "pmg_mresid_mixed_noalloc": (i, [i, i, i] + [vp] * 7),
         "pmg_buildP": (i, [i, i, i, i, i, i, i, i, dp, dp]),
         "pmg_buildG": (i, [i, i, i, i, i, i, i, dp, dp, dp]),
         "pmg_pbe_synth": (i, [i, d, i, d, d, i, ctypes.c_uint64] + [dp] * 13),

@@ -63,6 +63,9 @@ void gs27(const Dim &d, const T *ac, const T *cc, const T *fc, T *x,
 template <class T, class TO>
 void resid(const Dim &d, int numdia, const T *ac, const T *cc, const T *fc,
            const T *x, TO *r, const Red *red, hipStream_t s);
+// Number of block partials resid() writes (all of them, every call) when
+// given a Red.
+size_t resid_partials(const Dim &d);
 // Nonlinear residual r = f - A x - c_vec(cc, x) (pmgc/matvecd.cc:530-693),
 // 7-point fine operator only, FP64.
 void nmresid7(const Dim &d, const double *ac, const double *cc,

@@ -909,6 +909,12 @@ __global__ __launch_bounds__(256) void k_resid(
   }
 }
 
+size_t resid_partials(const Dim &d) {
+  const int kc = kchunk(d, 40);
+  return xcd_blocks(
+      dim3(cdiv(d.nx - 2, BX), cdiv(d.ny - 2, BY), cdiv(d.nz - 2, kc)));
+}
+
 template <class T, class TO>
 void resid(const Dim &d, int numdia, const T *ac, const T *cc, const T *fc,
            const T *x, TO *r, const Red *red, hipStream_t s) {

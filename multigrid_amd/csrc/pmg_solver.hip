@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -1266,6 +1267,38 @@ int finish(Scratch &S, T *host, const T *dev, size_t n) {
   return PMG_OK;
 }
 
+// ---- fixed device scratch of the allocation-free entry point ----
+// The block partials of pmg_mresid_mixed_noalloc's L1 norm live in this
+// array (one copy per GPU, held until the process ends).  Host code uses
+// only the device address hipGetSymbolAddress returns for it, looked up
+// once per device with that device current; the array's name itself is a
+// host-side stand-in, not a device address.  Calls that overlap on one GPU
+// share the array and race.
+constexpr size_t kScratchDoubles = (size_t)1 << 17;  // 1 MiB
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_dev[kMaxDevices] = {};
+
+// Device address of the current device's scratch (nullptr on failure).
+double *scratch_ptr() {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+    return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      fprintf(stderr, "pmg: no device address for the scratch array: %s\n",
+              hipGetErrorString(e));
+      return nullptr;
+    }
+    g_scratch_dev[dev] = (double *)p;
+  }
+  return g_scratch_dev[dev];
+}
+
 bool dims_ok(int nx, int ny, int nz) { return nx >= 3 && ny >= 3 && nz >= 3; }
 bool coarse_of(int f, int c) { return f == 2 * c - 1; }
 bool coarse_dims(int nxf, int nyf, int nzf, int nxc, int nyc, int nzc) {
@@ -1760,6 +1793,26 @@ int pmg_mresid_mixed(int nx, int ny, int nz, const double *ac,
           hipSuccess)
     rc = PMG_EDEVICE;
   return rc;
+}
+
+int pmg_mresid_mixed_noalloc(int nx, int ny, int nz, const double *ac,
+                             const double *cc, const double *fc,
+                             const double *x, float *r, double *l1,
+                             void *stream) {
+  if (!dims_ok(nx, ny, nz) || !ac || !cc || !fc || !x || !r || !l1)
+    return PMG_EBADPARM;
+  const Dim d{nx, ny, nz};
+  if (resid_partials(d) > kScratchDoubles) return PMG_ESCRATCH;
+  if (!device_ok()) return PMG_EDEVICE;
+  double *part = scratch_ptr();
+  if (!part) return PMG_EDEVICE;
+  // every partial the final sum reads is written by this call's residual
+  // kernel, so nothing left by an earlier call needs clearing
+  Red red;
+  red.part = part;
+  red.out = l1;
+  resid<double, float>(d, 7, ac, cc, fc, x, r, &red, (hipStream_t)stream);
+  return hipGetLastError() == hipSuccess ? PMG_OK : PMG_EDEVICE;
 }
 
 int pmg_addcorr_f32(int nx, int ny, int nz, const float *e, double *u) {
