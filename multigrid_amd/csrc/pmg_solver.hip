@@ -177,15 +177,26 @@ void smooth(pmg_ctx *c, const LV<T> &l, const T *f, T *x, int nu, int iadj,
 // Coarsest-level solve: Vcghs with 100 iterations and errtol = epsiln = 0
 // (mgsolv 0, D5), or the factored band (mgsolv 1, mgcsd.cc:391-407, which
 // ignores cc as the reference's band does).  FP64 inside either way.
+// errtol > 0: Vcghs's relative stopping test sqrt(r.r)/sqrt(r0.r0) <= errtol
+// with that tolerance instead of epsiln (at most 100 iterations).
 template <class T>
 void coarse_solve(pmg_ctx *c, int l, const double *ac64, const double *cc64,
-                  const T *f, T *x) {
+                  const T *f, T *x, double errtol = 0.0) {
   const Level &L = c->L[l];
   if (c->p.mgsolv == 1)
     band_solve<T>(L.d, L.numdia, c->band_rt, c->band_ct, f, x, c->s);
   else
-    cg_single<T>(L.d, L.numdia, ac64, cc64, f, x, 100, 0.0, c->s);
+    cg_single<T>(L.d, L.numdia, ac64, cc64, f, x, 100, errtol, c->s);
 }
+
+// Relative tolerance of the FP64 coarsest CG inside the FP32 V-cycle of the
+// linear mixed-precision solve.  The cycle is a correction step of the FP64
+// defect correction, whose stopping test is the FP64 residual; its coarse
+// correction is scaled by the Hackbusch-Reusken factor on the way up, and a
+// coarsest solve to 1e-3 gives the same contraction per cycle as 100
+// iterations (same outer iteration count on the benchmark workloads) in
+// about a tenth of the iterations.
+constexpr double kMixedCoarseTol = 1e-3;
 
 // The coarsest level fits the coarse solver of p.
 bool coarse_fits(pmg_ctx *c) {
@@ -200,7 +211,8 @@ bool coarse_fits(pmg_ctx *c) {
 // mixed solve); its boundary entries are zero already.
 template <class T>
 void vcycle(pmg_ctx *c, LV<T> *L, const T *frhs, const double *ac64c,
-            const double *cc64c, bool fine_zero = false) {
+            const double *cc64c, bool fine_zero = false,
+            double coarse_tol = 0.0) {
   const int nlev = c->nlev;
   const Params &p = c->p;
   hipStream_t s = c->s;
@@ -217,7 +229,8 @@ void vcycle(pmg_ctx *c, LV<T> *L, const T *frhs, const double *ac64c,
                   L[l].r, nullptr, s);
     }
   }
-  coarse_solve<T>(c, nlev - 1, ac64c, cc64c, L[nlev - 1].w0, L[nlev - 1].x);
+  coarse_solve<T>(c, nlev - 1, ac64c, cc64c, L[nlev - 1].w0, L[nlev - 1].x,
+                  coarse_tol);
   for (int l = nlev - 2; l >= 0; l--) {  // up (mgcsd.cc:418-473)
     const int cl = l + 1;
     const unsigned nb =
@@ -589,7 +602,8 @@ void step_body(pmg_ctx *c, int mode) {
       fill<float>(L32[0].x, F.d.n(), 0.0f, c->s);
       coarse_solve<float>(c, 0, F.ac, F.cc, r32, L32[0].x);
     } else {
-      vcycle<float>(c, L32, r32, c->L[nlev - 1].ac, cc64c, true);
+      vcycle<float>(c, L32, r32, c->L[nlev - 1].ac, cc64c, true,
+                    kMixedCoarseTol);
     }
     add_correction<float>(F.d, L32[0].x, c->u, c->s);
     resid<double, float>(F.d, F.numdia, F.ac, F.cc, F.fc, c->u, r32, &rnrm,

@@ -210,11 +210,10 @@ def test_ctx_fp64_and_mixed(mg, orc):
     assert s64["iters"] == it0 and _rel(u64, u0) < 1e-10
     assert smx["relres"] <= 1e-6 and smx["ierror"] == 0
     assert _rel(umx, u0) < 1e-6
-    # regression guard on the FP32 cycle's kernels (smoothers, transfers):
-    # it takes as many outer iterations as the FP64 cycle and lands within
-    # rounding distance of the FP64 potential (3.5e-14 measured), where a
-    # wrong FP32 smoother converges differently
-    assert smx["iters"] == s64["iters"] and _rel(umx, u64) < 1e-9
+    # the FP32 cycle (with its coarsest CG stopped at a relative 1e-3)
+    # contracts as the FP64 cycle does: as many outer iterations, and the
+    # potentials agree to the solver tolerance
+    assert smx["iters"] == s64["iters"] and _rel(umx, u64) < 1e-6
 
 
 def test_newdriv_parity(mg, orc):
