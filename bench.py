@@ -45,6 +45,9 @@ def parse():
                     help="solves in flight at once in the serving loop")
     ap.add_argument("--single-steps", type=int, default=10,
                     help="timed solves of each single-solve leg (161^3, 289^3)")
+    ap.add_argument("--no-serve", action="store_true",
+                    help="skip the serving stream (profiling runs of the other legs; "
+                         "no headline value then)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the single-solve, Newton, band and PCG legs")
@@ -440,9 +443,12 @@ def main():
         del a, b
 
     # headline: the serving stream (the timed region is its mixed capacity run)
-    serving, capmx = serving_leg(mg, args.steps, args.warmup, args.batch, args.serve_active)
-    wall = capmx["wall_s"]
-    value = args.steps * args.batch / wall
+    serving, wall, value = None, None, None
+    if not args.no_serve:
+        serving, capmx = serving_leg(mg, args.steps, args.warmup, args.batch,
+                                     args.serve_active)
+        wall = capmx["wall_s"]
+        value = args.steps * args.batch / wall
 
     cpu, oracle_u = None, {}
     if not args.no_cpu:
@@ -475,7 +481,9 @@ def main():
                   "avg_launch_us": sw["avg_launch_us"], "timed_launches": sw["launches"],
                   "algorithmic_bytes_per_launch": sw["bytes_per_launch"]}
             break
-    at = serving["at_rate"]
+    nothing = {"mixed": {"p50_ms": None, "p99_ms": None}, "fp64": {"p50_ms": None,
+                                                                  "p99_ms": None}}
+    at = serving["at_rate"] if serving else nothing
     out = {
         "metric": "PB solves/sec + p99 latency at 1e-6 rel residual: mixed-precision vs FP64 GPU",
         "value": value,
@@ -483,7 +491,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
+        "ms_per_step": wall / args.steps * 1e3 if wall else None,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -496,10 +504,10 @@ def main():
                    "requests_per_step": args.batch, "max_active": args.serve_active},
         "p50_ms": at["mixed"]["p50_ms"],
         "p99_ms": at["mixed"]["p99_ms"],
-        "fp64": {"value": serving["capacity"]["fp64"]["solves_per_s"],
+        "fp64": {"value": serving["capacity"]["fp64"]["solves_per_s"] if serving else None,
                  "p50_ms": at["fp64"]["p50_ms"], "p99_ms": at["fp64"]["p99_ms"]},
-        "latency_at_rate_per_s": serving["arrival_rate_per_s"],
-        "mixed_vs_fp64_capacity": serving["mixed_vs_fp64_capacity"],
+        "latency_at_rate_per_s": serving["arrival_rate_per_s"] if serving else None,
+        "mixed_vs_fp64_capacity": serving["mixed_vs_fp64_capacity"] if serving else None,
         "roofline": rl,
         "cpu_baseline": cpu,
         "serving": serving,

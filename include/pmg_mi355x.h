@@ -71,7 +71,12 @@ int pmg_maxlev(int nx, int ny, int nz);
  * a1cf..ccf, fcf and tcf are not overwritten.  iparm(21) = mgsolv 0 solves
  * the coarsest level by CG (cgd.cc:60-124), 1 by the factored band
  * (buildBd.cc:251-304, mlinpckd.cc:167-235), left in rwork at
- * ac(iz(7,nlev+1)); an indefinite band returns PMG_ENOTSPD. */
+ * ac(iz(7,nlev+1)); an indefinite band returns PMG_ENOTSPD.  iparm(11) =
+ * istop: 0 absolute and 1 relative L1 residual; 2 L1 difference of
+ * successive iterates (the first against tcf) over sqrt(nx ny nz); 3 and 4
+ * the relative 2-norm error against tcf; 5 the relative energy-norm error
+ * (mgcsd.cc:126-152, 497-526).  4 and 5 first replace fc by A tcf on every
+ * level, so that tcf is the solution (Vbuildalg, mgdrvd.cc:424-433). */
 int pmg_mgdriv(int *iparm, double *rparm, int *iwork, double *rwork,
                double *u, const double *xf, const double *yf, const double *zf,
                const double *gxcf, const double *gycf, const double *gzcf,
@@ -118,6 +123,12 @@ int pmg_Vmypdefinitnpbe(int *nion, double *charge, double *sconc);
  * pmg_mgdriv / pmg_newdriv call on this thread.  The reference computes
  * these and drops them (pmgc/mgdrvd.cc:194-195, 452-455). */
 int pmg_last_stats(int *iters, int *ierror, double *relres);
+/* The relative stopping measure after each V-cycle of the last pmg_mgdriv
+ * call on this thread (what the reference's Vprtstp reports per iteration,
+ * pmgc/mgsubd.cc:486-536; with iparm(12) iinfo > 0 the library also prints
+ * it, with the contraction number, to stdout).  Copies at most cap values;
+ * returns the number of cycles. */
+int pmg_last_history(double *relres, int cap);
 
 /* --- resident-hierarchy API (no reference counterpart: the reference
  * rebuilds everything per Vmgdriv call; these keep a solve's operators in
@@ -152,6 +163,15 @@ int pmg_ctx_stats(pmg_ctx *ctx, int *iters, int *ierror, double *relres,
                   double *setup_ms, double *solve_ms);
 /* Rebuild the hierarchy on the GPU from the resident inputs. */
 int pmg_ctx_rebuild(pmg_ctx *ctx);
+/* The comparison solution (tcf, fine level, nx*ny*nz) of the stopping
+ * tests iparm(11) = istop 2-5 (pmgc/mgcsd.cc:126-152, 497-526); needed
+ * before pmg_ctx_solve with those.  istop 4 and 5 also replace the
+ * right-hand side of every level by A tcf at the start of each solve (the
+ * reference's Vbuildalg, mgdrvd.cc:424-433), until the next rebuild. */
+int pmg_ctx_set_tru(pmg_ctx *ctx, const double *tcf);
+/* Per-cycle relative stopping measure of the last solve on ctx, as
+ * pmg_last_history. */
+int pmg_ctx_history(pmg_ctx *ctx, double *relres, int cap);
 /* Device bytes held by the context. */
 size_t pmg_ctx_bytes(const pmg_ctx *ctx);
 void pmg_ctx_destroy(pmg_ctx *ctx);

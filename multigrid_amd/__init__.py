@@ -75,6 +75,9 @@ def _declare(L):
         "pmg_ctx_pcg": (i, [vp, i, i]),
         "pmg_mypdefinit": (i, [i, dp, dp]),
         "pmg_last_stats": (i, [ip, ip, dp]),
+        "pmg_last_history": (i, [dp, i]),
+        "pmg_ctx_history": (i, [vp, dp, i]),
+        "pmg_ctx_set_tru": (i, [vp, dp]),
         "pmg_ctx_create": (i, [ctypes.POINTER(vp), ip, dp] + [dp] * 11),
         "pmg_ctx_solve": (i, [vp, i]),
         "pmg_ctx_get_u": (i, [vp, dp]),
@@ -240,6 +243,14 @@ def last_stats():
     return it.value, ie.value, rr.value
 
 
+def last_history(cap=4096):
+    """Relative stopping measure after each V-cycle of the last Vmgdriv call
+    on this thread (the reference's Vprtstp report, mgsubd.cc:486-536)."""
+    out = np.zeros(cap)
+    m = lib().pmg_last_history(_dp(out), cap)
+    return out[:min(m, cap)].copy()
+
+
 # ---------------------------------------------------------------------------
 # Resident-hierarchy context
 # ---------------------------------------------------------------------------
@@ -296,6 +307,17 @@ class Context:
 
     def rebuild(self):
         return _check(lib().pmg_ctx_rebuild(self._h), "pmg_ctx_rebuild")
+
+    def set_tru(self, tcf):
+        """The comparison solution of the stopping tests istop 2-5."""
+        t = np.ascontiguousarray(tcf, np.float64)
+        return _check(lib().pmg_ctx_set_tru(self._h, _dp(t)), "pmg_ctx_set_tru")
+
+    def history(self, cap=4096):
+        """Relative stopping measure after each iteration of the last solve."""
+        out = np.zeros(cap)
+        m = lib().pmg_ctx_history(self._h, _dp(out), cap)
+        return out[:min(m, cap)].copy()
 
     def nbytes(self):
         return int(lib().pmg_ctx_bytes(self._h))

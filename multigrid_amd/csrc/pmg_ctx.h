@@ -70,6 +70,11 @@ struct pmg_ctx {
          *gy = nullptr, *gz = nullptr, *a1 = nullptr, *a2 = nullptr,
          *a3 = nullptr, *ccf = nullptr, *fcf = nullptr;
   double *u = nullptr;       // fine solution (FP64)
+  // istop 2-5: the comparison solution tru (tcf, fine level; istop 2 then
+  // holds the previous iterate) and the relative stopping measure after
+  // each iteration of the last solve (Vprtstp's relres, mgsubd.cc:486-536)
+  double *tru = nullptr;
+  std::vector<double> hist;
   double *work[4] = {};      // fine FP64 scratch (Newton)
   double *pcg[4] = {};       // class-style PCG: r, p, A p, s (FP64)
   double *part = nullptr;    // reduction partials

@@ -103,6 +103,19 @@ template <class T>
 void interp_damped(const Dim &c, const Dim &f, const T *xc, const T *pc,
                    T *xf, const double *part, unsigned nblk, hipStream_t s);
 
+// ---- stopping tests istop 2-5 (pmgc/mgcsd.cc:126-152, 497-526) ----
+// d = tru - u on the interior.  kind 2: red.out[0] = sum |d|, then tru = u;
+// kind 3: red.out[0] = sum d^2; kind 5: w = d (no reduction).
+void stop_diff(const Dim &d, int kind, double *tru, const double *u,
+               double *w, const Red &red, hipStream_t s);
+// red.out[0] = w . (A w) over the interior (y receives A w).
+template <class T>
+void dot_energy(const Dim &d, int numdia, const T *ac, const T *cc,
+                const T *w, T *y, const Red &red, hipStream_t s);
+// Vextrac: coarse interior values by injection from the fine level.
+void extrac(const Dim &f, const Dim &c, const double *xf, double *xc,
+            hipStream_t s);
+
 // ---- reductions ----
 // Deterministic interior L1 norm into red.out[0].
 template <class T>

@@ -122,8 +122,9 @@ __device__ inline void red_store(const Red &r, double v0, double v1) {
   }
 }
 
-// out[0] = sum0 (MODE 0) or sum0 / sum1 (MODE 1; 0 when sum1 == 0, a zero
-// coarse correction).  Fixed order: thread t sums blocks t, t+1024, ...
+// out[0] = sum0 (MODE 0), sum0 / sum1 (MODE 1; 0 when sum1 == 0, a zero
+// coarse correction) or sum1 (MODE 2).  Fixed order: thread t sums blocks
+// t, t+1024, ...
 template <int NV, int MODE>
 __global__ __launch_bounds__(1024) void k_red_final(const double *part,
                                                     unsigned nblk,
@@ -139,6 +140,8 @@ __global__ __launch_bounds__(1024) void k_red_final(const double *part,
   if (threadIdx.x == 0) {
     if (MODE == 1)
       out[0] = sc != 0.0 ? sa / sc : 0.0;
+    else if (MODE == 2)
+      out[0] = sc;
     else
       out[0] = sa;
   }
@@ -1318,6 +1321,97 @@ void nrm1(const Dim &d, const T *x, const Red &red, hipStream_t s) {
   check_launch("nrm1");
 }
 
+// Stopping-test terms of istop 2-5 (mgcsd.cc:126-152, 497-526) on the
+// interior, d = tru - u: KIND 2 sums |d| and then sets tru = u (the
+// successive-iterate test); KIND 3 sums d^2; KIND 5 stores w = d (the
+// energy norm's vector, for dot_energy).
+template <int KIND>
+__global__ __launch_bounds__(256) void k_stopdiff(int nx, int ny, int nz,
+                                                  double *__restrict__ tru,
+                                                  const double *__restrict__ u,
+                                                  double *__restrict__ w,
+                                                  Red rd) {
+  __shared__ double red[4];
+  const int i = 2 + blockIdx.x * BX + threadIdx.x;
+  const int j = 2 + blockIdx.y * BY + threadIdx.y;
+  const int k0 = 2 + blockIdx.z * KCH;
+  double acc = 0.0;
+  if (i <= nx - 1 && j <= ny - 1)
+    for (int kk = 0; kk < KCH; kk++) {
+      const int k = k0 + kk;
+      if (k > nz - 1) break;
+      const size_t p = ix(nx, ny, i, j, k);
+      const double dv = tru[p] - u[p];
+      if (KIND == 2) {
+        acc += fabs(dv);
+        tru[p] = u[p];
+      } else if (KIND == 3) {
+        acc += dv * dv;
+      } else {
+        w[p] = dv;
+      }
+    }
+  if (KIND != 5) {
+    const double t = block_sum<256>(acc, red);
+    red_store<1>(rd, t, 0.0);
+  }
+}
+
+void stop_diff(const Dim &d, int kind, double *tru, const double *u,
+               double *w, const Red &red, hipStream_t s) {
+  dim3 g(cdiv(d.nx - 2, BX), cdiv(d.ny - 2, BY), cdiv(d.nz - 2, KCH));
+  dim3 b(BX, BY);
+  if (kind == 2) {
+    k_stopdiff<2><<<g, b, 0, s>>>(d.nx, d.ny, d.nz, tru, u, w, red);
+    red_final<1, 0>(red, g, s);
+  } else if (kind == 3) {
+    k_stopdiff<3><<<g, b, 0, s>>>(d.nx, d.ny, d.nz, tru, u, w, red);
+    red_final<1, 0>(red, g, s);
+  } else {
+    k_stopdiff<5><<<g, b, 0, s>>>(d.nx, d.ny, d.nz, tru, u, w, red);
+  }
+  check_launch("stop_diff");
+}
+
+template <class T>
+void dot_energy(const Dim &d, int numdia, const T *ac, const T *cc,
+                const T *w, T *y, const Red &red, hipStream_t s) {
+  const int kc = kchunk(d);
+  dim3 g(cdiv(d.nx - 2, BX), cdiv(d.ny - 2, BY), cdiv(d.nz - 2, kc));
+  dim3 b(BX, BY);
+  if (numdia == 7)
+    k_matvec_dots<T, 7><<<xcd_blocks(g), b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc,
+                                                    w, w, y, red, 1, kc,
+                                                    xg_of(g));
+  else
+    k_matvec_dots<T, 27><<<xcd_blocks(g), b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc,
+                                                     w, w, y, red, 1, kc,
+                                                     xg_of(g));
+  red_final<2, 2>(red, dim3(xcd_blocks(g)), s);
+  check_launch("dot_energy");
+}
+
+// Vextrac (matvecd.cc:997-1008): coarse interior by injection.
+__global__ __launch_bounds__(256) void k_extrac(int nxf, int nyf, int nxc,
+                                                int nyc, int nzc,
+                                                const double *__restrict__ xf,
+                                                double *__restrict__ xc) {
+  const int ii = 2 + blockIdx.x * BX + threadIdx.x;
+  const int jj = 2 + blockIdx.y * BY + threadIdx.y;
+  const int kk = 2 + blockIdx.z;
+  if (ii > nxc - 1 || jj > nyc - 1) return;
+  xc[ix(nxc, nyc, ii, jj, kk)] =
+      xf[ix(nxf, nyf, 2 * ii - 1, 2 * jj - 1, 2 * kk - 1)];
+}
+
+void extrac(const Dim &f, const Dim &c, const double *xf, double *xc,
+            hipStream_t s) {
+  if (c.nx < 3 || c.ny < 3 || c.nz < 3) return;
+  dim3 g(cdiv(c.nx - 2, BX), cdiv(c.ny - 2, BY), c.nz - 2);
+  k_extrac<<<g, dim3(BX, BY), 0, s>>>(f.nx, f.ny, c.nx, c.ny, c.nz, xf, xc);
+  check_launch("extrac");
+}
+
 // ---------------------------------------------------------------------------
 // Coarsest level: Vcghs (cgd.cc:60-124) in one 1024-thread workgroup.
 // The search direction p lives in LDS over the whole level grid; x, r and
@@ -2338,6 +2432,10 @@ void buildG(const Dim &f, const Dim &c, int numdia, const double *pc,
 // ---------------------------------------------------------------------------
 // explicit instantiations
 // ---------------------------------------------------------------------------
+template void dot_energy<double>(const Dim &, int, const double *,
+                                 const double *, const double *, double *,
+                                 const Red &, hipStream_t);
+
 #define INST(T)                                                               \
   template void gs7<T>(const Dim &, const T *, const T *, const T *, T *,     \
                        int, int, hipStream_t);                                \

@@ -347,8 +347,9 @@ int pmg_server_submit(pmg_server *S, const int *iparm, const double *rparm,
   Request *r = new (std::nothrow) Request;
   if (!r) return PMG_ENOMEM;
   int rc = decode(iparm, rparm, r->p);
-  if (!rc && (r->p.nx != r->p.ny || r->p.nx != r->p.nz || r->p.mode != 0))
-    rc = PMG_EBADPARM;  // cubic linear solves only
+  if (!rc && (r->p.nx != r->p.ny || r->p.nx != r->p.nz || r->p.mode != 0 ||
+              r->p.istop > 1))
+    rc = PMG_EBADPARM;  // cubic linear solves with a residual stopping test
   if (!rc && ctx_bytes_estimate(r->p, mode) > S->budget) rc = PMG_EBUSY;
   if (rc) {
     delete r;

@@ -94,7 +94,7 @@ int decode(const int *iparm, const double *rparm, Params &p) {
   if (p.nlev > 1 && p.nlev > maxlev(p.nx, p.ny, p.nz)) return PMG_EBADPARM;
   if (p.mgcoar != 2 || p.mgdisc != 0 || p.mgsmoo != 1 ||
       (p.mgsolv != 0 && p.mgsolv != 1) ||
-      (p.mgprol != 0 && p.mgprol != 1) || (p.istop != 0 && p.istop != 1) ||
+      (p.mgprol != 0 && p.mgprol != 1) || p.istop < 0 || p.istop > 5 ||
       p.iperf != 0 || p.ipkey != 0 || p.nu1 < 0 || p.nu2 < 0 || p.itmax < 0 ||
       (p.mgkey != 0 && p.mgkey != 1))
     return PMG_EBADPARM;
@@ -107,6 +107,7 @@ enum { S_RSDEN = 0, S_RSNRM = 1, S_XDAMP = 2, S_BINFO = 3, S_TMP = 4 };
 
 thread_local int t_iters = 0, t_ierror = 0;
 thread_local double t_relres = 0.0;
+thread_local std::vector<double> t_hist;
 
 int device_ok() {
   int n = 0;
@@ -588,6 +589,25 @@ void views(pmg_ctx *c, LV<double> *L64, LV<float> *L32, int mode) {
   L64[0].x = c->u;
 }
 
+// The stopping measure of istop 2-5 into the device scalar `slot`
+// (mgcsd.cc:497-526): 2 the L1 difference to tru, which then takes u; 3, 4
+// the squared 2-norm of tru - u; 5 its energy (tru-u) . A (tru-u) (the host
+// takes the square roots).  The fine FP64 scratch r and t hold tru - u and
+// its product with A.
+void stop_measure(pmg_ctx *c, int slot) {
+  Level &F = c->L[0];
+  const Red rd = red_to(c, slot);
+  const int istop = c->p.istop;
+  if (istop == 2) {
+    stop_diff(F.d, 2, c->tru, c->u, nullptr, rd, c->s);
+  } else if (istop == 3 || istop == 4) {
+    stop_diff(F.d, 3, c->tru, c->u, nullptr, rd, c->s);
+  } else {
+    stop_diff(F.d, 5, c->tru, c->u, F.r, rd, c->s);
+    dot_energy<double>(F.d, F.numdia, F.ac, F.cc, F.r, F.t, rd, c->s);
+  }
+}
+
 void step_body(pmg_ctx *c, int mode) {
   LV<double> L64[MAXLEV];
   LV<float> L32[MAXLEV];
@@ -596,6 +616,8 @@ void step_body(pmg_ctx *c, int mode) {
   const int nlev = c->nlev;
   const double *cc64c = c->L[nlev - 1].cc;
   const Red rnrm = red_to(c, S_RSNRM);
+  // istop 0, 1: the L1 norm of the FP64 residual; 2-5 their own measure
+  const bool resid_norm = c->p.istop <= 1;
   if (mode == PMG_MIXED) {
     float *r32 = F.w032;
     if (nlev == 1) {
@@ -606,15 +628,34 @@ void step_body(pmg_ctx *c, int mode) {
                     kMixedCoarseTol);
     }
     add_correction<float>(F.d, L32[0].x, c->u, c->s);
-    resid<double, float>(F.d, F.numdia, F.ac, F.cc, F.fc, c->u, r32, &rnrm,
-                         c->s);
+    resid<double, float>(F.d, F.numdia, F.ac, F.cc, F.fc, c->u, r32,
+                         resid_norm ? &rnrm : nullptr, c->s);
   } else {
     if (nlev == 1)
       coarse_solve<double>(c, 0, F.ac, F.cc, F.fc, c->u);
     else
       vcycle<double>(c, L64, F.fc, c->L[nlev - 1].ac, cc64c);
-    resid<double, double>(F.d, F.numdia, F.ac, F.cc, F.fc, c->u,
-                          (double *)nullptr, &rnrm, c->s);
+    if (resid_norm)
+      resid<double, double>(F.d, F.numdia, F.ac, F.cc, F.fc, c->u,
+                            (double *)nullptr, &rnrm, c->s);
+  }
+  if (!resid_norm) stop_measure(c, S_RSNRM);
+}
+
+// Vbuildalg (mgdrvd.cc:424-433, mgsubd.cc:891-950) for istop 4 and 5: the
+// right-hand side of every level becomes A tcf, tcf carried to the coarse
+// levels by injection (Vextrac in Vbuildops), so that tcf is the solution.
+// The coarse levels' r arrays hold the injected tcf.
+void build_alg(pmg_ctx *c) {
+  const double *t = c->tru;
+  for (int l = 0; l < c->nlev; l++) {
+    Level &L = c->L[l];
+    if (l > 0) {
+      extrac(c->L[l - 1].d, L.d, t, L.r, c->s);
+      t = L.r;
+    }
+    matvec_dots<double>(L.d, L.numdia, L.ac, L.cc, t, nullptr, L.fc, nullptr,
+                        c->s);
   }
 }
 
@@ -652,6 +693,7 @@ int lin_begin(pmg_ctx *c, int mode) {
     c->ops32_ok = true;
   }
   if (!coarse_fits(c)) return PMG_EBADPARM;
+  if (p.istop >= 2 && !c->tru) return PMG_EBADPARM;  // no tcf given
   if (!c->cyc_ev) {
     hipEventCreateWithFlags(&c->cyc_ev, hipEventDisableTiming);
     hipEventCreateWithFlags(&c->fin_ev, hipEventDisableTiming);
@@ -670,17 +712,27 @@ int lin_begin(pmg_ctx *c, int mode) {
   S.done = false;
   S.phase = 0;
   c->nev = 0;
+  c->hist.clear();
   hipEventRecord(c->t0_ev, c->s);
+  if (p.istop == 4 || p.istop == 5) build_alg(c);
   fill<double>(c->u, F.d.n(), 0.0, c->s);
   if (p.istop == 1) {
     const Red rden = red_to(c, S_RSDEN);
     nrm1<double>(F.d, F.fc, rden, c->s);
+  } else if (p.istop >= 3) {  // ||tru||_2^2, or tru . A tru (u = 0 here)
+    const Red rden = red_to(c, S_RSDEN);
+    if (p.istop == 5) {
+      stop_diff(F.d, 5, c->tru, c->u, F.r, rden, c->s);
+      dot_energy<double>(F.d, F.numdia, F.ac, F.cc, F.r, F.t, rden, c->s);
+    } else {
+      stop_diff(F.d, 3, c->tru, c->u, nullptr, rden, c->s);
+    }
   }
   if (mode == PMG_MIXED) {
     // residual of u = 0, the first right-hand side of the FP32 cycle
     const Red rnrm = red_to(c, S_RSNRM);
-    resid<double, float>(F.d, F.numdia, F.ac, F.cc, F.fc, c->u, F.w032, &rnrm,
-                         c->s);
+    resid<double, float>(F.d, F.numdia, F.ac, F.cc, F.fc, c->u, F.w032,
+                         p.istop <= 1 ? &rnrm : nullptr, c->s);
   }
   // the two scalars and the band factor's info (slot S_BINFO)
   hipMemcpyAsync(c->hscal, c->scal, 4 * sizeof(double), hipMemcpyDeviceToHost,
@@ -722,12 +774,15 @@ int lin_poll(pmg_ctx *c, bool wait) {
       S.rc = PMG_ENOTSPD;
       return 1;
     }
+    // the stopping-test denominator (mgcsd.cc:133-157)
     if (p.istop == 1) S.rsden = c->hscal[S_RSDEN];
+    else if (p.istop == 2) S.rsden = std::sqrt((double)c->L[0].d.n());
+    else if (p.istop >= 3) S.rsden = std::sqrt(c->hscal[S_RSDEN]);
     if (S.rsden == 0.0) S.rsden = 1.0;  // mgcsd.cc:154-157
-    S.rsnrm = S.mode == PMG_MIXED ? c->hscal[S_RSNRM] : S.rsden;
+    const bool mixed_res = S.mode == PMG_MIXED && p.istop <= 1;
+    S.rsnrm = mixed_res ? c->hscal[S_RSNRM] : S.rsden;
     // mixed: while (iters < itmax && relres > errtol) may do nothing
-    if (S.mode == PMG_MIXED &&
-        !(S.iters < p.itmax && S.rsnrm / S.rsden > p.errtol)) {
+    if (mixed_res && !(S.iters < p.itmax && S.rsnrm / S.rsden > p.errtol)) {
       S.done = true;
       S.rc = PMG_OK;
       return 1;
@@ -735,7 +790,13 @@ int lin_poll(pmg_ctx *c, bool wait) {
     return 0;
   }
   S.iters++;
-  S.rsnrm = c->hscal[S_RSNRM];
+  const double orsnrm = S.rsnrm;
+  S.rsnrm = p.istop >= 3 ? std::sqrt(c->hscal[S_RSNRM]) : c->hscal[S_RSNRM];
+  c->hist.push_back(S.rsnrm / S.rsden);
+  if (p.iinfo > 0)  // Vprtstp (mgsubd.cc:486-536)
+    printf("iteration = %d relative residual = %e contraction number = %e\n",
+           S.iters, S.rsnrm / S.rsden,
+           orsnrm != 0.0 ? S.rsnrm / orsnrm : 0.0);
   const bool more = (c->nlev > 1 || S.mode == PMG_MIXED) && S.iters < p.itmax &&
                     S.rsnrm / S.rsden > p.errtol;
   if (!more) {
@@ -846,6 +907,7 @@ double nm_norm(pmg_ctx *c, const double *x, double *r) {
 
 int ctx_newton(pmg_ctx *c, int mode) {
   const Params &p = c->p;
+  if (p.istop > 1) return PMG_EBADPARM;  // the residual tests only
   int rc = alloc_newton(c);
   if (rc) return rc;
   if (mode == PMG_MIXED) {
@@ -1120,6 +1182,19 @@ int ctx_pcg(pmg_ctx *c, int mode, int num_iters) {
   return ierr ? PMG_ITMAX : PMG_OK;
 }
 
+// Upload tcf (fine level) as the comparison solution of istop 2-5.
+int ctx_set_tru(pmg_ctx *c, const double *tcf) {
+  const size_t n = c->L[0].d.n();
+  if (!c->tru) {
+    c->tru = c->alloc<double>(n);
+    if (!c->ok) return PMG_ENOMEM;
+  }
+  if (hipMemcpyAsync(c->tru, tcf, n * sizeof(double), hipMemcpyHostToDevice,
+                     c->s) != hipSuccess)
+    return PMG_EDEVICE;
+  return hipStreamSynchronize(c->s) == hipSuccess ? PMG_OK : PMG_EDEVICE;
+}
+
 // Fill iwork/rwork as the reference leaves them (mgdrvd.cc:145-166,
 // mgsubd.cc:266-373): iz, ipc per level; rpc | cc | fc | pc | ac.
 int export_work(pmg_ctx *c, int *iwork, double *rwork, bool newton) {
@@ -1214,9 +1289,11 @@ int driver(int *iparm, double *rparm, int *iwork, double *rwork, double *u,
            const double *xf, const double *yf, const double *zf,
            const double *gxcf, const double *gycf, const double *gzcf,
            const double *a1cf, const double *a2cf, const double *a3cf,
-           const double *ccf, const double *fcf, int kind, int num_iters = 0) {
+           const double *ccf, const double *fcf, const double *tcf, int kind,
+           int num_iters = 0) {
   const bool newton = kind == 1;
   t_iters = 0, t_ierror = 0, t_relres = 0.0;
+  t_hist.clear();
   if (!iparm || !rparm || !iwork || !rwork || !u || !xf || !yf || !zf ||
       !gxcf || !gycf || !gzcf || !a1cf || !a2cf || !a3cf || !ccf || !fcf)
     return PMG_EBADPARM;
@@ -1226,12 +1303,20 @@ int driver(int *iparm, double *rparm, int *iwork, double *rwork, double *u,
   if (!newton && p.mode != 0) return PMG_EBADPARM;  // FAS not provided
   if (kind == 2 && (p.nlev < 2 || num_iters < 0)) return PMG_EBADPARM;
   if (newton && p.mgkey != 0) return PMG_EBADPARM;  // Vfnewton not provided
+  if (p.istop >= 2 && (kind != 0 || !tcf)) return PMG_EBADPARM;
   rc = check_sizes(p, newton);
   if (rc) return rc;
   pmg_ctx *c = nullptr;
   rc = ctx_create(&c, p, xf, yf, zf, gxcf, gycf, gzcf, a1cf, a2cf, a3cf, ccf,
                   fcf);
   if (rc) return rc;
+  if (p.istop >= 2) {  // the comparison solution of the stopping test
+    rc = ctx_set_tru(c, tcf);
+    if (rc) {
+      delete c;
+      return rc;
+    }
+  }
   int src = kind == 1   ? ctx_newton(c, PMG_FP64)
             : kind == 2 ? ctx_pcg(c, PMG_FP64, num_iters)
                         : ctx_solve(c, PMG_FP64);
@@ -1244,6 +1329,7 @@ int driver(int *iparm, double *rparm, int *iwork, double *rwork, double *u,
   t_iters = c->iters;
   t_ierror = c->ierror;
   t_relres = c->relres;
+  t_hist = c->hist;
   delete c;
   return src;
 }
@@ -1421,7 +1507,8 @@ int correct_t(int nxc, int nyc, int nzc, int nxf, int nyf, int nzf,
     *dxc = S.put(xc, c.n()), *dw0 = S.put(w0c, c.n()),
     *dpc = S.put(pc, 27 * c.n()), *dxf = S.put(xf, f.n()),
     *dy = S.put((const T *)nullptr, c.n());
-  double *part = S.put((const double *)nullptr, max_partials(c));
+  // two partials (x.w0, x.Ax) per block of the matvec
+  double *part = S.put((const double *)nullptr, 2 * max_partials(c));
   if (!S.ok) return PMG_ENOMEM;
   const unsigned nb =
       matvec_dots_part<T>(c, numdia, dac, dcc, dxc, dw0, dy, part, 0);
@@ -1493,9 +1580,8 @@ int pmg_mgdriv(int *iparm, double *rparm, int *iwork, double *rwork,
                const double *gxcf, const double *gycf, const double *gzcf,
                double *a1cf, double *a2cf, double *a3cf, double *ccf,
                double *fcf, double *tcf) {
-  (void)tcf;
   return driver(iparm, rparm, iwork, rwork, u, xf, yf, zf, gxcf, gycf, gzcf,
-                a1cf, a2cf, a3cf, ccf, fcf, 0);
+                a1cf, a2cf, a3cf, ccf, fcf, tcf, 0);
 }
 
 int pmg_pcgdriv(int *iparm, double *rparm, int *iwork, double *rwork,
@@ -1503,9 +1589,8 @@ int pmg_pcgdriv(int *iparm, double *rparm, int *iwork, double *rwork,
                 const double *zf, const double *gxcf, const double *gycf,
                 const double *gzcf, double *a1cf, double *a2cf, double *a3cf,
                 double *ccf, double *fcf, double *tcf, int num_iters) {
-  (void)tcf;
   return driver(iparm, rparm, iwork, rwork, u, xf, yf, zf, gxcf, gycf, gzcf,
-                a1cf, a2cf, a3cf, ccf, fcf, 2, num_iters);
+                a1cf, a2cf, a3cf, ccf, fcf, tcf, 2, num_iters);
 }
 
 int pmg_ctx_pcg(pmg_ctx *ctx, int mode, int num_iters) {
@@ -1519,9 +1604,8 @@ int pmg_newdriv(int *iparm, double *rparm, int *iwork, double *rwork,
                 const double *zf, const double *gxcf, const double *gycf,
                 const double *gzcf, double *a1cf, double *a2cf, double *a3cf,
                 double *ccf, double *fcf, double *tcf) {
-  (void)tcf;
   return driver(iparm, rparm, iwork, rwork, u, xf, yf, zf, gxcf, gycf, gzcf,
-                a1cf, a2cf, a3cf, ccf, fcf, 1);
+                a1cf, a2cf, a3cf, ccf, fcf, tcf, 1);
 }
 
 int pmg_mypdefinit(int nion, const double *charge, const double *sconc) {
@@ -1546,6 +1630,25 @@ int pmg_last_stats(int *iters, int *ierror, double *relres) {
   if (ierror) *ierror = t_ierror;
   if (relres) *relres = t_relres;
   return PMG_OK;
+}
+
+int pmg_last_history(double *relres, int cap) {
+  if (cap < 0 || (cap > 0 && !relres)) return -PMG_EBADPARM;
+  const int m = std::min(cap, (int)t_hist.size());
+  for (int q = 0; q < m; q++) relres[q] = t_hist[q];
+  return (int)t_hist.size();
+}
+
+int pmg_ctx_history(pmg_ctx *ctx, double *relres, int cap) {
+  if (!ctx || cap < 0 || (cap > 0 && !relres)) return -PMG_EBADPARM;
+  const int m = std::min(cap, (int)ctx->hist.size());
+  for (int q = 0; q < m; q++) relres[q] = ctx->hist[q];
+  return (int)ctx->hist.size();
+}
+
+int pmg_ctx_set_tru(pmg_ctx *ctx, const double *tcf) {
+  if (!ctx || !tcf) return PMG_EBADPARM;
+  return ctx_set_tru(ctx, tcf);
 }
 
 int pmg_ctx_create(pmg_ctx **out, const int *iparm, const double *rparm,

@@ -1014,10 +1014,13 @@ static double g_last_relres = 0.0;
 static double g_hist[HIST_CAP];
 static int g_nhist = 0;
 
-/* Stopping-test norm (mgcsd.cc:483-526), istop 0..3. */
+/* Stopping-test norm (mgcsd.cc:483-526): 0, 1 the L1 residual; 2 the L1
+ * difference to tru, which then takes x (successive iterates); 3, 4 the
+ * 2-norm of tru - x; 5 its energy norm sqrt((tru-x) . A (tru-x)) (D2: a
+ * true dot product).  w2 is scratch for istop 5. */
 static double stop_norm(const hier_t *h, const cyc_t *c, double *x,
                         const double *fc1, const double *cc1, double *w1,
-                        double *tru) {
+                        double *w2, double *tru) {
   int nx = h->nx[0], ny = h->ny[0], nz = h->nz[0];
   if (c->istop == 0 || c->istop == 1) {
     pmgo_mresid(nx, ny, nz, h->numdia[0], h->ac[0], cc1, fc1, x, w1);
@@ -1025,6 +1028,10 @@ static double stop_norm(const hier_t *h, const cyc_t *c, double *x,
   }
   pmgo_xcopy(nx, ny, nz, tru, w1);
   pmgo_xaxpy(nx, ny, nz, -1.0, x, w1);
+  if (c->istop == 5) {
+    pmgo_matvec(nx, ny, nz, h->numdia[0], h->ac[0], cc1, w1, w2);
+    return sqrt(pmgo_xdot(nx, ny, nz, w1, w2));
+  }
   double v = c->istop == 2 ? pmgo_xnrm1(nx, ny, nz, w1)
                            : pmgo_xnrm2(nx, ny, nz, w1);
   if (c->istop == 2) pmgo_xcopy(nx, ny, nz, x, tru);
@@ -1039,10 +1046,13 @@ static int mvcs(const hier_t *h, const cyc_t *c, double **x, double **w0,
   int nlev = h->nlev;
   double rsden, rsnrm;
   int nx0 = h->nx[0], ny0 = h->ny[0], nz0 = h->nz[0];
-  if (c->istop == 0) rsden = 1.0;
+  if (c->istop == 0) rsden = 1.0; /* mgcsd.cc:133-152 */
   else if (c->istop == 1) rsden = pmgo_xnrm1(nx0, ny0, nz0, fc1);
   else if (c->istop == 2) rsden = sqrt((double)nx0 * ny0 * nz0);
-  else rsden = pmgo_xnrm2(nx0, ny0, nz0, tru);
+  else if (c->istop == 5) {
+    pmgo_matvec(nx0, ny0, nz0, h->numdia[0], h->ac[0], ccl[0], tru, w1);
+    rsden = sqrt(pmgo_xdot(nx0, ny0, nz0, tru, w1));
+  } else rsden = pmgo_xnrm2(nx0, ny0, nz0, tru);
   if (rsden == 0.0) rsden = 1.0; /* mgcsd.cc:154-157 */
   rsnrm = rsden;
   if (record) g_nhist = 0;
@@ -1055,7 +1065,7 @@ static int mvcs(const hier_t *h, const cyc_t *c, double **x, double **w0,
     else
       pmgo_bandsolve(nx0, ny0, nz0, h->band, h->ipcb, fc1, x[0], h->bwk);
     *iters_out = 1;
-    rsnrm = stop_norm(h, c, x[0], fc1, ccl[0], w1, tru);
+    rsnrm = stop_norm(h, c, x[0], fc1, ccl[0], w1, w2, tru);
     if (record) {
       g_hist[0] = rsnrm / rsden;
       g_nhist = 1;
@@ -1109,7 +1119,7 @@ static int mvcs(const hier_t *h, const cyc_t *c, double **x, double **w0,
                 l == 0 ? fc1 : w0[l], x[l], c->nu2, 1);
     }
     iters++;
-    rsnrm = stop_norm(h, c, x[0], fc1, ccl[0], w1, tru); /* D6: always */
+    rsnrm = stop_norm(h, c, x[0], fc1, ccl[0], w1, w2, tru); /* D6: always */
     if (record && g_nhist < HIST_CAP) g_hist[g_nhist++] = rsnrm / rsden;
   } while (iters < c->itmax && rsnrm / rsden > c->errtol);
   *iters_out = iters;
@@ -1151,7 +1161,7 @@ static int decode(const int *iparm, const double *rparm, params_t *p) {
   if (p->nlev > pmgo_maxlev(p->nx, p->ny, p->nz) && p->nlev > 1) return 2;
   if (p->mgcoar != 2 || p->mgdisc != 0 || p->mgsmoo != 1 ||
       (p->mgsolv != 0 && p->mgsolv != 1) ||
-      (p->mgprol != 0 && p->mgprol != 1) || p->istop < 0 || p->istop > 3 ||
+      (p->mgprol != 0 && p->mgprol != 1) || p->istop < 0 || p->istop > 5 ||
       p->iperf != 0 || p->ipkey != 0 || p->nu1 < 0 || p->nu2 < 0)
     return 2;
   return 0;
@@ -1281,6 +1291,19 @@ int pmgo_mgdriv(int *iparm, double *rparm, int *iwork, double *rwork,
   x[0] = u;
   level_ptrs(&h, w0b, w0);
   for (int l = 0; l < h.nlev; l++) ccl[l] = h.cc[l];
+  if (p.istop == 4 || p.istop == 5) {
+    /* Vbuildalg (mgdrvd.cc:424-433, mgsubd.cc:891-950): the right-hand side
+     * of every level becomes A tcf, tcf carried to the coarse levels by
+     * injection (Vextrac in Vbuildops), so that tcf solves the system */
+    double *tl[32];
+    level_ptrs(&h, tru, tl);
+    for (int l = 1; l < h.nlev; l++)
+      pmgo_extrac(h.nx[l - 1], h.ny[l - 1], h.nz[l - 1], h.nx[l], h.ny[l],
+                  h.nz[l], tl[l - 1], tl[l]);
+    for (int l = 0; l < h.nlev; l++)
+      pmgo_matvec(h.nx[l], h.ny[l], h.nz[l], h.numdia[l], h.ac[l], h.cc[l],
+                  tl[l], h.fc[l]);
+  }
   memset(u, 0, nf * sizeof(double)); /* Vazeros mgdrvd.cc:419 */
   pmgo_fbound00(h.nx[0], h.ny[0], h.nz[0], u);
   cyc_t c = {p.istop, p.itmax, p.nu1, p.nu2, p.mgsolv, p.errtol, 0.0};
@@ -1476,6 +1499,7 @@ int pmgo_newdriv(int *iparm, double *rparm, int *iwork, double *rwork,
   params_t p;
   if (decode(iparm, rparm, &p)) return 2;
   if (p.mgkey != 0) return 2; /* Vfnewton (nested) not provided */
+  if (p.istop > 1) return 2;   /* Newton: the residual tests only */
   hier_t h;
   double *ccb, *fcb, *w1n, *w2n;
   int rc = setup(&p, iwork, rwork, 1, xf, yf, zf, gxcf, gycf, gzcf, a1cf,
