@@ -85,7 +85,12 @@ int pmg_mgdriv(int *iparm, double *rparm, int *iwork, double *rwork,
 
 /* Replaces pmgc::Vnewdriv (pmgc/newdrvd.h, newdrvd.cc:60-155): the
  * nonlinear (sinh) Poisson-Boltzmann Newton solve with PMG inner solves.
- * rwork needs iretot + 2*nf entries (newdrvd.cc:123). */
+ * rwork needs iretot + 2*nf entries (newdrvd.cc:123).  The interior of u
+ * is the starting iterate (its boundary is zeroed, newdrvd.cc:263).
+ * iparm(9) = mgkey 0 runs Vnewton, 1 the nested iteration Vfnewton
+ * (newtond.cc:60-138): Newton on each level from the coarsest up, each
+ * level's solution prolongated as the next finer level's start (the
+ * coarse iterates start from zero).  istop 0 and 1 only. */
 int pmg_newdriv(int *iparm, double *rparm, int *iwork, double *rwork,
                 double *u, const double *xf, const double *yf,
                 const double *zf, const double *gxcf, const double *gycf,

@@ -208,7 +208,10 @@ def Vmgdriv(iparm, rparm, iwork, rwork, u, xf, yf, zf, gxcf, gycf, gzcf,
 
 def Vnewdriv(iparm, rparm, iwork, rwork, u, xf, yf, zf, gxcf, gycf, gzcf,
              a1cf, a2cf, a3cf, ccf, fcf, tcf):
-    """Nonlinear PB Newton solve (pmgc/newdrvd.cc:60-155) on the GPU."""
+    """Nonlinear PB Newton solve (pmgc/newdrvd.cc:60-300) on the GPU, FP64.
+    The interior of u is the starting iterate (its boundary is zeroed);
+    iparm[8] = mgkey 1 runs the nested iteration (Vfnewton,
+    newtond.cc:60-138) from the coarsest level up."""
     return _driver(lib().pmg_newdriv, "Vnewdriv", iparm, rparm, iwork, rwork,
                    u, xf, yf, zf, gxcf, gycf, gzcf, a1cf, a2cf, a3cf, ccf, fcf, tcf)
 

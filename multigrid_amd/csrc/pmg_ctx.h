@@ -36,8 +36,9 @@ struct Level {
   double *x = nullptr, *w0 = nullptr, *r = nullptr, *t = nullptr;
   float *ac32 = nullptr, *cc32 = nullptr, *pc32 = nullptr, *x32 = nullptr,
         *w032 = nullptr, *r32 = nullptr, *t32 = nullptr;
-  // Newton: Jacobian diagonal term and right-hand side per level
-  double *cpr = nullptr, *rhs = nullptr;
+  // Newton: Jacobian diagonal term and right-hand side per level, and the
+  // Newton iterate of a coarse level (nested iteration)
+  double *cpr = nullptr, *rhs = nullptr, *xn = nullptr;
   float *cpr32 = nullptr;
 };
 
@@ -70,6 +71,7 @@ struct pmg_ctx {
          *gy = nullptr, *gz = nullptr, *a1 = nullptr, *a2 = nullptr,
          *a3 = nullptr, *ccf = nullptr, *fcf = nullptr;
   double *u = nullptr;       // fine solution (FP64)
+  bool u_given = false;      // u holds the next Newton solve's start
   // istop 2-5: the comparison solution tru (tcf, fine level; istop 2 then
   // holds the previous iterate) and the relative stopping measure after
   // each iteration of the last solve (Vprtstp's relres, mgsubd.cc:486-536)

@@ -67,10 +67,10 @@ void resid(const Dim &d, int numdia, const T *ac, const T *cc, const T *fc,
 // given a Red.
 size_t resid_partials(const Dim &d);
 // Nonlinear residual r = f - A x - c_vec(cc, x) (pmgc/matvecd.cc:530-693),
-// 7-point fine operator only, FP64.
-void nmresid7(const Dim &d, const double *ac, const double *cc,
-              const double *fc, const double *x, double *r, const Red *red,
-              hipStream_t s);
+// 7- or 27-point operator, FP64.
+void nmresid(const Dim &d, int numdia, const double *ac, const double *cc,
+             const double *fc, const double *x, double *r, const Red *red,
+             hipStream_t s);
 
 // ---- y = A x; with red, red->out[0] = (x.w0) / (x.y), the Hackbusch-
 // Reusken damping of the coarse correction (pmgc/mgcsd.cc:429-452) ----

@@ -984,7 +984,8 @@ void dc_vec(const double *coef, const double *u, double *out, size_t n,
   check_launch("dc_vec");
 }
 
-__global__ __launch_bounds__(256) void k_nmresid7(
+template <int ND>
+__global__ __launch_bounds__(256) void k_nmresid(
     int nx, int ny, int nz, const double *__restrict__ ac,
     const double *__restrict__ cc, const double *__restrict__ fc,
     const double *__restrict__ x, double *__restrict__ r, Red rd,
@@ -1004,10 +1005,16 @@ __global__ __launch_bounds__(256) void k_nmresid7(
       if (k > nz - 1) break;
       const size_t p = ix(nx, ny, i, j, k);
       const double w1 = ion_term(cc[p], x[p], 0);
-      const double v = fc[p] + oN[p] * x[p + sy] + oN[p - sy] * x[p - sy] +
-                       oE[p] * x[p + 1] + oE[p - 1] * x[p - 1] +
-                       uC[p - sz] * x[p - sz] + uC[p] * x[p + sz] -
-                       ac[p] * x[p] - w1;
+      double v;
+      if (ND == 7) {
+        v = fc[p] + oN[p] * x[p + sy] + oN[p - sy] * x[p - sy] +
+            oE[p] * x[p + 1] + oE[p - 1] * x[p - 1] + uC[p - sz] * x[p - sz] +
+            uC[p] * x[p + sz] - ac[p] * x[p] - w1;
+      } else {  // Vnmresid27_1s (matvecd.cc:578-693)
+        double sO, sU, sD;
+        sum27<double, double>(ac, n, sy, sz, p, x, sO, sU, sD);
+        v = sO + sU + sD + fc[p] - ac[p] * x[p] - w1;
+      }
       if (r) r[p] = v;
       acc += fabs(v);
     }
@@ -1018,15 +1025,19 @@ __global__ __launch_bounds__(256) void k_nmresid7(
   }
 }
 
-void nmresid7(const Dim &d, const double *ac, const double *cc,
-              const double *fc, const double *x, double *r, const Red *red,
-              hipStream_t s) {
+void nmresid(const Dim &d, int numdia, const double *ac, const double *cc,
+             const double *fc, const double *x, double *r, const Red *red,
+             hipStream_t s) {
   dim3 g(cdiv(d.nx - 2, BX), cdiv(d.ny - 2, BY), cdiv(d.nz - 2, KCH));
   const Red rd = red ? *red : Red{};
-  k_nmresid7<<<xcd_blocks(g), dim3(BX, BY), 0, s>>>(d.nx, d.ny, d.nz, ac, cc, fc, x, r,
-                                        rd, red ? 1 : 0, xg_of(g));
+  if (numdia == 7)
+    k_nmresid<7><<<xcd_blocks(g), dim3(BX, BY), 0, s>>>(
+        d.nx, d.ny, d.nz, ac, cc, fc, x, r, rd, red ? 1 : 0, xg_of(g));
+  else
+    k_nmresid<27><<<xcd_blocks(g), dim3(BX, BY), 0, s>>>(
+        d.nx, d.ny, d.nz, ac, cc, fc, x, r, rd, red ? 1 : 0, xg_of(g));
   if (red) red_final<1, 0>(rd, dim3(xcd_blocks(g)), s);
-  check_launch("nmresid7");
+  check_launch("nmresid");
 }
 
 // ---------------------------------------------------------------------------

@@ -210,11 +210,13 @@ bool coarse_fits(pmg_ctx *c) {
 // the coarsest level used by the FP64 coarse solve.
 // fine_zero: the fine iterate starts at zero (the correction cycle of the
 // mixed solve); its boundary entries are zero already.
+// nlevd: the cycle runs on the last nlevd levels of the hierarchy (L[0] is
+// the first of them; 0: all levels).
 template <class T>
 void vcycle(pmg_ctx *c, LV<T> *L, const T *frhs, const double *ac64c,
             const double *cc64c, bool fine_zero = false,
-            double coarse_tol = 0.0) {
-  const int nlev = c->nlev;
+            double coarse_tol = 0.0, int nlevd = 0) {
+  const int nlev = nlevd > 0 ? nlevd : c->nlev;
   const Params &p = c->p;
   hipStream_t s = c->s;
   // nu1 pre-smoothings on the fine grid, then the residual (D1)
@@ -230,8 +232,8 @@ void vcycle(pmg_ctx *c, LV<T> *L, const T *frhs, const double *ac64c,
                   L[l].r, nullptr, s);
     }
   }
-  coarse_solve<T>(c, nlev - 1, ac64c, cc64c, L[nlev - 1].w0, L[nlev - 1].x,
-                  coarse_tol);
+  coarse_solve<T>(c, c->nlev - 1, ac64c, cc64c, L[nlev - 1].w0,
+                  L[nlev - 1].x, coarse_tol);
   for (int l = nlev - 2; l >= 0; l--) {  // up (mgcsd.cc:418-473)
     const int cl = l + 1;
     const unsigned nb =
@@ -246,10 +248,11 @@ void vcycle(pmg_ctx *c, LV<T> *L, const T *frhs, const double *ac64c,
 // Linear solve with FP64 V-cycles.  L[0].x holds the iterate (zeroed by the
 // caller), frhs the fine right-hand side.  istop 0: absolute L1 residual,
 // 1: relative to ||frhs||_1 (mgcsd.cc:133-163, 483-527).
+// l0: the solve runs on levels l0.. of the hierarchy (L[0] is level l0).
 int linear_fp64(pmg_ctx *c, LV<double> *L, const double *frhs,
                 const double *cc64c, int istop, double errtol, int itmax,
-                int *iters_out, double *rel_out) {
-  Level &F = c->L[0];
+                int *iters_out, double *rel_out, int l0 = 0) {
+  Level &F = c->L[l0];
   double rsden = 1.0;
   const Red rden = red_to(c, S_RSDEN), rnrm = red_to(c, S_RSNRM);
   if (istop == 1) {
@@ -258,9 +261,9 @@ int linear_fp64(pmg_ctx *c, LV<double> *L, const double *frhs,
   }
   if (rsden == 0.0) rsden = 1.0;
   double rsnrm = rsden;
-  const int nlev = c->nlev;
+  const int nlev = c->nlev - l0;
   if (nlev == 1) {  // direct coarse solve on the only level (mgcsd.cc:170-291)
-    coarse_solve<double>(c, 0, F.ac, L[0].cc, frhs, L[0].x);
+    coarse_solve<double>(c, l0, F.ac, L[0].cc, frhs, L[0].x);
     resid<double, double>(F.d, F.numdia, F.ac, L[0].cc, frhs, L[0].x,
                           (double *)nullptr, &rnrm, c->s);
     rsnrm = host_scalar(c, S_RSNRM);
@@ -270,7 +273,7 @@ int linear_fp64(pmg_ctx *c, LV<double> *L, const double *frhs,
   }
   int iters = 0;
   do {
-    vcycle<double>(c, L, frhs, c->L[nlev - 1].ac, cc64c);
+    vcycle<double>(c, L, frhs, c->L[c->nlev - 1].ac, cc64c, false, 0.0, nlev);
     iters++;
     resid<double, double>(F.d, F.numdia, F.ac, L[0].cc, frhs, L[0].x,
                           (double *)nullptr, &rnrm, c->s);
@@ -288,8 +291,8 @@ int linear_fp64(pmg_ctx *c, LV<double> *L, const double *frhs,
 int linear_mixed(pmg_ctx *c, LV<double> *L64, LV<float> *L32,
                  const double *frhs, const double *cc64c, double *u,
                  int istop, double errtol, int itmax, int *iters_out,
-                 double *rel_out) {
-  Level &F = c->L[0];
+                 double *rel_out, int l0 = 0) {
+  Level &F = c->L[l0];
   double rsden = 1.0;
   const Red rden = red_to(c, S_RSDEN), rnrm = red_to(c, S_RSNRM);
   if (istop == 1) {
@@ -302,13 +305,14 @@ int linear_mixed(pmg_ctx *c, LV<double> *L64, LV<float> *L32,
                        c->s);
   double rsnrm = host_scalar(c, S_RSNRM);
   int iters = 0;
-  const int nlev = c->nlev;
+  const int nlev = c->nlev - l0;
   while (iters < itmax && rsnrm / rsden > errtol) {
     if (nlev == 1) {
       fill<float>(L32[0].x, F.d.n(), 0.0f, c->s);
-      coarse_solve<float>(c, 0, F.ac, L64[0].cc, r32, L32[0].x);
+      coarse_solve<float>(c, l0, F.ac, L64[0].cc, r32, L32[0].x);
     } else {
-      vcycle<float>(c, L32, r32, c->L[nlev - 1].ac, cc64c, true);
+      vcycle<float>(c, L32, r32, c->L[c->nlev - 1].ac, cc64c, true, 0.0,
+                    nlev);
     }
     add_correction<float>(F.d, L32[0].x, u, c->s);
     iters++;
@@ -891,6 +895,8 @@ int alloc_newton(pmg_ctx *c) {
     Level &L = c->L[l];
     L.cpr = c->alloc<double>(L.d.n());
     L.rhs = c->alloc<double>(L.d.n());
+    // the Newton iterate of a coarse level (nested iteration, mgkey 1)
+    if (l > 0 && c->p.mgkey == 1) L.xn = c->alloc<double>(L.d.n());
   }
   for (int q = 0; q < 4; q++) c->work[q] = c->alloc<double>(c->L[0].d.n());
   if (!c->ok) return PMG_ENOMEM;
@@ -898,13 +904,123 @@ int alloc_newton(pmg_ctx *c) {
   return PMG_OK;
 }
 
-double nm_norm(pmg_ctx *c, const double *x, double *r) {
-  Level &F = c->L[0];
+// L1 norm of the nonlinear residual of level l at x (Vnmresid + Vxnrm1),
+// r receives the residual.
+double nm_norm(pmg_ctx *c, int l, const double *x, double *r) {
+  Level &F = c->L[l];
   const Red rd = red_to(c, S_TMP);
-  nmresid7(F.d, F.ac, F.cc, F.fc, x, r, &rd, c->s);
+  nmresid(F.d, F.numdia, F.ac, F.cc, F.fc, x, r, &rd, c->s);
   return host_scalar(c, S_TMP);
 }
 
+// Vnewton (newtond.cc:140-486) on levels l0.. of the hierarchy (level l0 as
+// the finest) from the iterate x of level l0 as it stands: damped inexact
+// Newton whose inner solve (FP64, or FP32 cycles under FP64 defect
+// correction) runs to min(0.9 |r|, |r|^2) in the absolute L1 residual;
+// stopping test istop 0 / 1 on the nonlinear residual.
+int newton_level(pmg_ctx *c, int mode, int l0, double *x, int itmax,
+                 int *iters_out, double *rel_out) {
+  const Params &p = c->p;
+  Level &F = c->L[l0];
+  const Dim d = F.d;
+  const size_t nf = d.n();
+  hipStream_t s = c->s;
+  double *w0n = c->work[0], *w1 = c->work[1], *t1 = c->work[2],
+         *t2 = c->work[3];
+  for (int q = 0; q < 4; q++) fill<double>(c->work[q], nf, 0.0, s);
+  double rsden = 1.0;
+  if (p.istop == 1) rsden = nm_norm(c, l0, w1, t2);  // residual at zero
+  if (rsden == 0.0) rsden = 1.0;
+  double xold = nm_norm(c, l0, x, w0n);
+  double xnew = xold, xmed = xold, rsnrm = rsden;
+  int idamp = 1, iters = 0;
+  LV<double> L64[MAXLEV];
+  LV<float> L32[MAXLEV];
+  for (int l = l0; l < c->nlev; l++) {
+    L64[l] = view<double>(c->L[l]);
+    L64[l].cc = c->L[l].cpr;
+    if (mode == PMG_MIXED) {
+      L32[l] = view<float>(c->L[l]);
+      L32[l].cc = c->L[l].cpr32;
+    }
+  }
+  double *xtmp = F.x;  // iterate of the inner linear solve
+  L64[l0].x = xtmp;
+  for (;;) {
+    iters++;
+    // Vgetjac (newtond.cc:488-531)
+    copy_interior(d, w0n, F.rhs, s);
+    dc_vec(F.cc, x, F.cpr, nf, s);
+    for (int l = l0 + 1; l < c->nlev; l++) {
+      restrict_<double>(c->L[l - 1].d, c->L[l].d, c->L[l - 1].rhs,
+                        c->L[l].rhs, c->L[l].pc, s);
+      restrict_<double>(c->L[l - 1].d, c->L[l].d, c->L[l - 1].cpr,
+                        c->L[l].cpr, c->L[l].pc, s);
+    }
+    if (mode == PMG_MIXED)
+      for (int l = l0; l < c->nlev; l++)
+        convert<double, float>(c->L[l].cpr, c->L[l].cpr32, c->L[l].d.n(), s);
+    const double errtol_s = std::min(0.9 * xold, 1.0 * std::pow(xold, 2.0));
+    fill<double>(xtmp, nf, 0.0, s);
+    int its = 0;
+    double rel = 0.0;
+    const double *cc64c = c->L[c->nlev - 1].cpr;
+    if (mode == PMG_MIXED)
+      linear_mixed(c, L64 + l0, L32 + l0, F.rhs, cc64c, xtmp, 0, errtol_s,
+                   1000, &its, &rel, l0);
+    else
+      linear_fp64(c, L64 + l0, F.rhs, cc64c, 0, errtol_s, 1000, &its, &rel,
+                  l0);
+    if (idamp == 1) {  // damping (newtond.cc:313-376)
+      xpay_interior(d, x, 1.0, xtmp, w1, s);
+      xnew = nm_norm(c, l0, w1, w0n);
+      double damp = 1.0;
+      int iter_d = 0, dmode = 0;
+      while (iter_d < 10) {
+        if (dmode == 0) {
+          if (xnew < xold) dmode = 1;
+        } else if (xnew > xmed) {
+          break;
+        }
+        copy_interior(d, w1, t1, s);
+        copy_interior(d, w0n, t2, s);
+        xmed = xnew;
+        damp = damp / 2.0;
+        xpay_interior(d, x, damp, xtmp, w1, s);
+        xnew = nm_norm(c, l0, w1, w0n);
+        iter_d++;
+      }
+      copy_interior(d, t1, x, s);
+      copy_interior(d, t2, w0n, s);
+      xnew = xmed;
+      xold = xnew;
+      if (iter_d - 1 == 0) idamp = 0;
+    } else {
+      axpy_interior<double>(d, 1.0, xtmp, x, s);
+      xnew = nm_norm(c, l0, x, w0n);
+      xold = xnew;
+    }
+    rsnrm = xnew;
+    if (l0 == 0) {
+      c->hist.push_back(rsnrm / rsden);
+      if (p.iinfo > 0)  // Vprtstp (mgsubd.cc:486-536)
+        printf("iteration = %d relative residual = %e\n", iters,
+               rsnrm / rsden);
+    }
+    if (rsnrm / rsden <= p.errtol) break;
+    if (iters >= itmax) break;
+  }
+  *iters_out = iters;
+  *rel_out = rsnrm / rsden;
+  return *rel_out <= p.errtol ? PMG_OK : PMG_ITMAX;
+}
+
+// Vnewton (mgkey 0) or Vfnewton (mgkey 1, newtond.cc:60-138 with D9):
+// Newton on each level from the coarsest up, each level's solution
+// prolongated (VinterpPMG) as the next finer level's starting iterate; the
+// coarse iterates start from zero.  The fine iterate starts from c->u when
+// u_given is set (the caller's u of pmg_newdriv, boundary zeroed,
+// VfboundPMG00 newdrvd.cc:263), else from zero.
 int ctx_newton(pmg_ctx *c, int mode) {
   const Params &p = c->p;
   if (p.istop > 1) return PMG_EBADPARM;  // the residual tests only
@@ -923,91 +1039,29 @@ int ctx_newton(pmg_ctx *c, int mode) {
   if (p.mgsolv == 1 && host_scalar(c, S_BINFO) != 0.0) return PMG_ENOTSPD;
   Level &F = c->L[0];
   const Dim d = F.d;
-  const size_t nf = d.n();
   hipStream_t s = c->s;
-  double *w0n = c->work[0], *w1 = c->work[1], *t1 = c->work[2],
-         *t2 = c->work[3];
   hipEvent_t e0, e1;
   hipEventCreate(&e0);
   hipEventCreate(&e1);
   hipEventRecord(e0, s);
-  fill<double>(c->u, nf, 0.0, s);
-  for (int q = 0; q < 4; q++) fill<double>(c->work[q], nf, 0.0, s);
-  double rsden = 1.0;
-  if (p.istop == 1) rsden = nm_norm(c, w1, t2);  // residual at u = 0
-  if (rsden == 0.0) rsden = 1.0;
-  double xold = nm_norm(c, c->u, w0n);
-  double xnew = xold, xmed = xold, rsnrm = rsden;
-  int idamp = 1, iters = 0;
-  LV<double> L64[MAXLEV];
-  LV<float> L32[MAXLEV];
-  for (int l = 0; l < c->nlev; l++) {
-    L64[l] = view<double>(c->L[l]);
-    L64[l].cc = c->L[l].cpr;
-    if (mode == PMG_MIXED) {
-      L32[l] = view<float>(c->L[l]);
-      L32[l].cc = c->L[l].cpr32;
+  c->hist.clear();
+  if (c->u_given)
+    fbound00(d, c->u, s);
+  else
+    fill<double>(c->u, d.n(), 0.0, s);
+  c->u_given = false;
+  int iters = 0;
+  double rel = 0.0;
+  if (p.mgkey == 1) {
+    for (int l = 1; l < c->nlev; l++)
+      fill<double>(c->L[l].xn, c->L[l].d.n(), 0.0, s);
+    for (int l = c->nlev - 1; l >= 1; l--) {
+      newton_level(c, mode, l, c->L[l].xn, 1000, &iters, &rel);
+      interp<double>(c->L[l].d, c->L[l - 1].d, c->L[l].xn, c->L[l].pc,
+                     l - 1 == 0 ? c->u : c->L[l - 1].xn, nullptr, s);
     }
   }
-  double *xtmp = F.x;  // fine iterate of the inner linear solve
-  L64[0].x = xtmp;
-  for (;;) {
-    iters++;
-    // Vgetjac (newtond.cc:488-531)
-    copy_interior(d, w0n, F.rhs, s);
-    dc_vec(F.cc, c->u, F.cpr, nf, s);
-    for (int l = 1; l < c->nlev; l++) {
-      restrict_<double>(c->L[l - 1].d, c->L[l].d, c->L[l - 1].rhs,
-                        c->L[l].rhs, c->L[l].pc, s);
-      restrict_<double>(c->L[l - 1].d, c->L[l].d, c->L[l - 1].cpr,
-                        c->L[l].cpr, c->L[l].pc, s);
-    }
-    if (mode == PMG_MIXED)
-      for (int l = 0; l < c->nlev; l++)
-        convert<double, float>(c->L[l].cpr, c->L[l].cpr32, c->L[l].d.n(), s);
-    const double errtol_s = std::min(0.9 * xold, 1.0 * std::pow(xold, 2.0));
-    fill<double>(xtmp, nf, 0.0, s);
-    int its = 0;
-    double rel = 0.0;
-    const double *cc64c = c->L[c->nlev - 1].cpr;
-    if (mode == PMG_MIXED)
-      linear_mixed(c, L64, L32, F.rhs, cc64c, xtmp, 0, errtol_s, 1000, &its,
-                   &rel);
-    else
-      linear_fp64(c, L64, F.rhs, cc64c, 0, errtol_s, 1000, &its, &rel);
-    if (idamp == 1) {  // damping (newtond.cc:313-376)
-      xpay_interior(d, c->u, 1.0, xtmp, w1, s);
-      xnew = nm_norm(c, w1, w0n);
-      double damp = 1.0;
-      int iter_d = 0, dmode = 0;
-      while (iter_d < 10) {
-        if (dmode == 0) {
-          if (xnew < xold) dmode = 1;
-        } else if (xnew > xmed) {
-          break;
-        }
-        copy_interior(d, w1, t1, s);
-        copy_interior(d, w0n, t2, s);
-        xmed = xnew;
-        damp = damp / 2.0;
-        xpay_interior(d, c->u, damp, xtmp, w1, s);
-        xnew = nm_norm(c, w1, w0n);
-        iter_d++;
-      }
-      copy_interior(d, t1, c->u, s);
-      copy_interior(d, t2, w0n, s);
-      xnew = xmed;
-      xold = xnew;
-      if (iter_d - 1 == 0) idamp = 0;
-    } else {
-      axpy_interior<double>(d, 1.0, xtmp, c->u, s);
-      xnew = nm_norm(c, c->u, w0n);
-      xold = xnew;
-    }
-    rsnrm = xnew;
-    if (rsnrm / rsden <= p.errtol) break;
-    if (iters >= p.itmax) break;
-  }
+  rc = newton_level(c, mode, 0, c->u, p.itmax, &iters, &rel);
   fbound_dirichlet(d, c->u, c->gx, c->gy, c->gz, s);
   hipEventRecord(e1, s);
   hipError_t se = hipStreamSynchronize(s);
@@ -1018,9 +1072,9 @@ int ctx_newton(pmg_ctx *c, int mode) {
   hipEventDestroy(e1);
   if (se != hipSuccess) return PMG_EDEVICE;
   c->iters = iters;
-  c->relres = rsnrm / rsden;
-  c->ierror = c->relres <= p.errtol ? 0 : 1;
-  return c->ierror ? PMG_ITMAX : PMG_OK;
+  c->relres = rel;
+  c->ierror = rc == PMG_OK ? 0 : 1;
+  return rc;
 }
 
 // ---- class-style solver (PBE_pmgc_class.h:1070-1133, CG_Solver.h:45-242)
@@ -1302,7 +1356,7 @@ int driver(int *iparm, double *rparm, int *iwork, double *rwork, double *u,
   if (rc) return rc;
   if (!newton && p.mode != 0) return PMG_EBADPARM;  // FAS not provided
   if (kind == 2 && (p.nlev < 2 || num_iters < 0)) return PMG_EBADPARM;
-  if (newton && p.mgkey != 0) return PMG_EBADPARM;  // Vfnewton not provided
+
   if (p.istop >= 2 && (kind != 0 || !tcf)) return PMG_EBADPARM;
   rc = check_sizes(p, newton);
   if (rc) return rc;
@@ -1310,6 +1364,14 @@ int driver(int *iparm, double *rparm, int *iwork, double *rwork, double *u,
   rc = ctx_create(&c, p, xf, yf, zf, gxcf, gycf, gzcf, a1cf, a2cf, a3cf, ccf,
                   fcf);
   if (rc) return rc;
+  if (newton) {  // the starting iterate (interior of the caller's u)
+    if (hipMemcpyAsync(c->u, u, c->L[0].d.n() * sizeof(double),
+                       hipMemcpyHostToDevice, c->s) != hipSuccess) {
+      delete c;
+      return PMG_EDEVICE;
+    }
+    c->u_given = true;
+  }
   if (p.istop >= 2) {  // the comparison solution of the stopping test
     rc = ctx_set_tru(c, tcf);
     if (rc) {

@@ -1491,22 +1491,25 @@ int pmgo_pcgdriv(int *iparm, double *rparm, int *iwork, double *rwork,
 }
 
 /* Vnewton (newtond.cc:140-486) with Vgetjac (:488-531). */
-int pmgo_newdriv(int *iparm, double *rparm, int *iwork, double *rwork,
-                 double *u, const double *xf, const double *yf,
-                 const double *zf, const double *gxcf, const double *gycf,
-                 const double *gzcf, double *a1cf, double *a2cf, double *a3cf,
-                 double *ccf, double *fcf, double *tcf) {
-  params_t p;
-  if (decode(iparm, rparm, &p)) return 2;
-  if (p.mgkey != 0) return 2; /* Vfnewton (nested) not provided */
-  if (p.istop > 1) return 2;   /* Newton: the residual tests only */
-  hier_t h;
-  double *ccb, *fcb, *w1n, *w2n;
-  int rc = setup(&p, iwork, rwork, 1, xf, yf, zf, gxcf, gycf, gzcf, a1cf,
-                 a2cf, a3cf, ccf, fcf, &h, &ccb, &fcb, &w1n, &w2n);
-  if (rc) return rc;
-  (void)w1n, (void)w2n;
-  int nx = h.nx[0], ny = h.ny[0], nz = h.nz[0];
+/* Vnewton (newtond.cc:140-486) on the levels L..nlev-1 of h0 (level L as
+ * the finest), from the iterate x of level L as given: damped inexact
+ * Newton with the Vmvcs inner solve to min(0.9 |r|, |r|^2) (absolute L1,
+ * istop 0, at most 1000 cycles), stopping test istop 0 / 1 on the L1
+ * nonlinear residual.  record: keep the per-iteration history and the
+ * final statistics (the finest Newton of a driver call). */
+static int newton_at(const hier_t *h0, int L, int istop, int itmax,
+                     double errtol, int nu1, int nu2, int mgsolv, double *x0,
+                     int record) {
+  hier_t h = *h0; /* levels L.. of h0 as levels 0.. */
+  h.nlev = h0->nlev - L;
+  for (int l = 0; l < h.nlev; l++) {
+    h.nx[l] = h0->nx[L + l], h.ny[l] = h0->ny[L + l], h.nz[l] = h0->nz[L + l];
+    h.n[l] = h0->n[L + l];
+    h.numdia[l] = h0->numdia[L + l];
+    h.ac[l] = h0->ac[L + l], h.cc[l] = h0->cc[L + l], h.fc[l] = h0->fc[L + l];
+    h.pc[l] = l + 1 < h.nlev ? h0->pc[L + l] : 0;
+  }
+  int nx = h.nx[0], ny = h.ny[0], nz = h.nz[0], nd = h.numdia[0];
   size_t nf = h.n[0], narr = 0;
   for (int l = 0; l < h.nlev; l++) narr += h.n[l];
   double *w0b = (double *)calloc(narr, sizeof(double));
@@ -1519,40 +1522,32 @@ int pmgo_newdriv(int *iparm, double *rparm, int *iwork, double *rwork,
   double *xtmpb = (double *)calloc(narr, sizeof(double));
   double *t1 = (double *)calloc(nf, sizeof(double));
   double *t2 = (double *)calloc(nf, sizeof(double));
-  double *bwk = (double *)calloc(h.n[h.nlev - 1], sizeof(double));
-  h.bwk = bwk;
-  double *x[32], *w0[32], *cpr[32], *rhs[32], *xtmp[32];
-  x[0] = u;
+  double *w0[32], *cpr[32], *rhs[32], *xtmp[32];
   level_ptrs(&h, w0b, w0);
   level_ptrs(&h, cprb, cpr);
   level_ptrs(&h, rhsb, rhs);
   level_ptrs(&h, xtmpb, xtmp);
-  memset(u, 0, nf * sizeof(double));
-  pmgo_fbound00(nx, ny, nz, u);
   const double *cc1 = h.cc[0], *fc1 = h.fc[0];
-  int istop = p.istop;
   double rsden;
   if (istop == 0) {
     rsden = 1.0;
-  } else if (istop == 1) { /* newtond.cc:191-202 */
+  } else { /* istop 1: residual at a zero guess (newtond.cc:191-202) */
     memset(w1, 0, nf * sizeof(double));
-    pmgo_nmresid(nx, ny, nz, 7, h.ac[0], cc1, fc1, w1, w2, w3);
+    pmgo_nmresid(nx, ny, nz, nd, h.ac[0], cc1, fc1, w1, w2, w3);
     rsden = pmgo_xnrm1(nx, ny, nz, w2);
-  } else {
-    rsden = pmgo_xnrm2(nx, ny, nz, tcf);
   }
   if (rsden == 0.0) rsden = 1.0;
   double rsnrm = rsden;
-  pmgo_nmresid(nx, ny, nz, 7, h.ac[0], cc1, fc1, x[0], w0[0], w2);
+  pmgo_nmresid(nx, ny, nz, nd, h.ac[0], cc1, fc1, x0, w0[0], w2);
   double xnorm_old = pmgo_xnrm1(nx, ny, nz, w0[0]);
-  double xnorm_den = rsden, xnorm_new = xnorm_old, xnorm_med = xnorm_old;
+  double xnorm_new = xnorm_old, xnorm_med = xnorm_old;
   int idamp = 1, iters = 0;
-  g_nhist = 0;
+  if (record) g_nhist = 0;
   for (;;) {
     iters++;
     /* Vgetjac: rhs = r, cprime = dc_vec(cc, x), restricted to all levels */
     pmgo_xcopy(nx, ny, nz, w0[0], rhs[0]);
-    pmgo_dc_vec(cc1, x[0], cpr[0], nx, ny, nz);
+    pmgo_dc_vec(cc1, x0, cpr[0], nx, ny, nz);
     for (int l = 1; l < h.nlev; l++) {
       pmgo_restrc(h.nx[l - 1], h.ny[l - 1], h.nz[l - 1], h.nx[l], h.ny[l],
                   h.nz[l], rhs[l - 1], rhs[l], h.pc[l - 1]);
@@ -1561,13 +1556,13 @@ int pmgo_newdriv(int *iparm, double *rparm, int *iwork, double *rwork,
     }
     double errtol_s = fmin(0.9 * xnorm_old, 1.0 * pow(xnorm_old, 2.0));
     memset(xtmp[0], 0, nf * sizeof(double));
-    cyc_t c = {0, 1000, p.nu1, p.nu2, p.mgsolv, errtol_s, 0.0};
+    cyc_t c = {0, 1000, nu1, nu2, mgsolv, errtol_s, 0.0};
     int its = 0;
     mvcs(&h, &c, xtmp, w0, cpr, rhs[0], w1, w2, w3, tru, &its, 0);
     if (idamp == 1) { /* newtond.cc:313-376 */
-      pmgo_xcopy(nx, ny, nz, x[0], w1);
+      pmgo_xcopy(nx, ny, nz, x0, w1);
       pmgo_xaxpy(nx, ny, nz, 1.0, xtmp[0], w1);
-      pmgo_nmresid(nx, ny, nz, 7, h.ac[0], cc1, fc1, w1, w0[0], rhs[0]);
+      pmgo_nmresid(nx, ny, nz, nd, h.ac[0], cc1, fc1, w1, w0[0], rhs[0]);
       xnorm_new = pmgo_xnrm1(nx, ny, nz, w0[0]);
       double damp = 1.0;
       int iter_d = 0, mode = 0;
@@ -1580,38 +1575,86 @@ int pmgo_newdriv(int *iparm, double *rparm, int *iwork, double *rwork,
         pmgo_xcopy(nx, ny, nz, w1, t1);
         pmgo_xcopy(nx, ny, nz, w0[0], t2);
         xnorm_med = xnorm_new;
-        pmgo_xcopy(nx, ny, nz, x[0], w1);
+        pmgo_xcopy(nx, ny, nz, x0, w1);
         damp = damp / 2.0;
         pmgo_xaxpy(nx, ny, nz, damp, xtmp[0], w1);
-        pmgo_nmresid(nx, ny, nz, 7, h.ac[0], cc1, fc1, w1, w0[0], rhs[0]);
+        pmgo_nmresid(nx, ny, nz, nd, h.ac[0], cc1, fc1, w1, w0[0], rhs[0]);
         xnorm_new = pmgo_xnrm1(nx, ny, nz, w0[0]);
         iter_d++;
       }
-      pmgo_xcopy(nx, ny, nz, t1, x[0]);
+      pmgo_xcopy(nx, ny, nz, t1, x0);
       pmgo_xcopy(nx, ny, nz, t2, w0[0]);
       xnorm_new = xnorm_med;
       xnorm_old = xnorm_new;
       if (iter_d - 1 == 0) idamp = 0;
     } else {
-      pmgo_xaxpy(nx, ny, nz, 1.0, xtmp[0], x[0]);
-      pmgo_nmresid(nx, ny, nz, 7, h.ac[0], cc1, fc1, x[0], w0[0], rhs[0]);
+      pmgo_xaxpy(nx, ny, nz, 1.0, xtmp[0], x0);
+      pmgo_nmresid(nx, ny, nz, nd, h.ac[0], cc1, fc1, x0, w0[0], rhs[0]);
       xnorm_new = pmgo_xnrm1(nx, ny, nz, w0[0]);
       xnorm_old = xnorm_new;
     }
     rsnrm = xnorm_new; /* istop 0/1 (newtond.cc:396-401); D6 */
-    if (g_nhist < HIST_CAP) g_hist[g_nhist++] = rsnrm / rsden;
-    if (rsnrm / rsden <= p.errtol) break;
-    if (iters >= p.itmax) break;
+    if (record && g_nhist < HIST_CAP) g_hist[g_nhist++] = rsnrm / rsden;
+    if (rsnrm / rsden <= errtol) break;
+    if (iters >= itmax) break;
   }
-  (void)xnorm_den;
-  pmgo_fbound(1, nx, ny, nz, u, gxcf, gycf, gzcf);
-  g_last_iters = iters;
-  g_last_ierror = rsnrm / rsden <= p.errtol ? 0 : 1;
-  g_last_relres = rsnrm / rsden;
+  const int ierr = rsnrm / rsden <= errtol ? 0 : 1;
+  if (record) {
+    g_last_iters = iters;
+    g_last_ierror = ierr;
+    g_last_relres = rsnrm / rsden;
+  }
   free(w0b), free(w1), free(w2), free(w3), free(tru), free(cprb), free(rhsb),
-      free(xtmpb), free(t1), free(t2), free(bwk);
+      free(xtmpb), free(t1), free(t2);
+  return ierr;
+}
+
+/* Vnewdriv / Vnewdriv2 (newdrvd.cc:60-300): mgkey 0 runs Vnewton on the
+ * fine level from the caller's u (interior as given, boundary zeroed,
+ * VfboundPMG00 newdrvd.cc:263); mgkey 1 runs Vfnewton (newtond.cc:60-138,
+ * D9): Newton on the coarsest level, prolongated (VinterpPMG) as the
+ * starting guess of the next finer level's Newton, up to the finest level.
+ * The coarse-level iterates start from zero (the reference reads them from
+ * the coarse part of a narr-long u, which this interface does not carry). */
+int pmgo_newdriv(int *iparm, double *rparm, int *iwork, double *rwork,
+                 double *u, const double *xf, const double *yf,
+                 const double *zf, const double *gxcf, const double *gycf,
+                 const double *gzcf, double *a1cf, double *a2cf, double *a3cf,
+                 double *ccf, double *fcf, double *tcf) {
+  params_t p;
+  if (decode(iparm, rparm, &p)) return 2;
+  if (p.mgkey != 0 && p.mgkey != 1) return 2;
+  if (p.istop > 1) return 2; /* Newton: the residual tests only */
+  hier_t h;
+  double *ccb, *fcb, *w1n, *w2n;
+  int rc = setup(&p, iwork, rwork, 1, xf, yf, zf, gxcf, gycf, gzcf, a1cf,
+                 a2cf, a3cf, ccf, fcf, &h, &ccb, &fcb, &w1n, &w2n);
+  if (rc) return rc;
+  (void)w1n, (void)w2n, (void)tcf;
+  int nx = h.nx[0], ny = h.ny[0], nz = h.nz[0];
+  double *bwk = (double *)calloc(h.n[h.nlev - 1], sizeof(double));
+  h.bwk = bwk;
+  pmgo_fbound00(nx, ny, nz, u);
+  if (p.mgkey == 1) {
+    size_t narr = 0;
+    for (int l = 0; l < h.nlev; l++) narr += h.n[l];
+    double *xb = (double *)calloc(narr, sizeof(double)), *xl[32];
+    level_ptrs(&h, xb, xl);
+    xl[0] = u;
+    for (int L = h.nlev - 1; L >= 1; L--) {
+      newton_at(&h, L, p.istop, 1000, p.errtol, p.nu1, p.nu2, p.mgsolv, xl[L],
+                0);
+      pmgo_interp(h.nx[L], h.ny[L], h.nz[L], h.nx[L - 1], h.ny[L - 1],
+                  h.nz[L - 1], xl[L], xl[L - 1], h.pc[L - 1]);
+    }
+    free(xb);
+  }
+  int ierr = newton_at(&h, 0, p.istop, p.itmax, p.errtol, p.nu1, p.nu2,
+                       p.mgsolv, u, 1);
+  pmgo_fbound(1, nx, ny, nz, u, gxcf, gycf, gzcf);
+  free(bwk);
   (void)a1cf, (void)a2cf, (void)a3cf, (void)ccf, (void)fcf;
-  return g_last_ierror;
+  return ierr;
 }
 
 int pmgo_last_stats(int *iters, int *ierror, double *relres, double *hist,

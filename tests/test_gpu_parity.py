@@ -232,6 +232,77 @@ def test_newdriv_parity(mg, orc):
     assert _rel(u1, u0) < 1e-9
 
 
+def _newton_problem(mg, orc, n, seed):
+    ions = ([1.0, -1.0], [0.5, 0.5])
+    mg.Vmypdefinitnpbe(2, *ions)
+    orc.mypdefinit(*ions)
+    return orc.pbe_synth(n, 0.5, 40, 6.0, 0.3, 0, seed)
+
+
+@pytest.mark.parametrize("n,nlev", [(33, 3), (65, 4)])
+def test_nested_newton_parity(mg, orc, n, nlev):
+    """mgkey 1 (Vfnewton, newtond.cc:60-138): Newton on each level from the
+    coarsest up; the same Newton iteration count and per-iteration relative
+    residuals as the oracle, the potential within 1e-9."""
+    print(f"nested newton parity n={n} nlev={nlev}", flush=True)
+    inp = _newton_problem(mg, orc, n, 5)
+    e1, u1, _, _ = _run_driver(mg, mg.Vnewdriv, inp, n, nlev, nonlin=True, errtol=1e-8, mgkey=1)
+    it1, h1 = mg.last_stats()[0], mg.last_history()
+    e0, u0, _, _ = orc.solve(inp, n, nlev, nonlin=True, errtol=1e-8, mgkey=1)
+    it0, _, _, h0 = orc.last_stats()
+    print(f"  nested newton iters gpu={it1} oracle={it0} err={_rel(u1, u0):.2e}", flush=True)
+    assert e1 == e0 == 0 and it1 == it0 == len(h1) == len(h0)
+    assert np.allclose(h1, h0, rtol=1e-6, atol=1e-9 * h0[0])
+    assert _rel(u1, u0) < 1e-9
+
+
+def test_newton_from_callers_u(mg, orc):
+    """Vnewdriv starts from the interior of the caller's u (boundary zeroed,
+    VfboundPMG00 newdrvd.cc:263): a perturbed converged potential takes the
+    oracle's Newton iterations from that start, to the same potential."""
+    print("newton from caller's u n=33", flush=True)
+    n, nlev = 33, 3
+    inp = _newton_problem(mg, orc, n, 6)
+    _, u0, _, _ = orc.solve(inp, n, nlev, nonlin=True, errtol=1e-10)
+    it_zero = orc.last_stats()[0]
+    rng = np.random.default_rng(7)
+    start = u0 * (1.0 + 1e-3 * rng.standard_normal(u0.size))
+    iparm, rparm = mg.make_params(n, nlev, nonlin=1, errtol=1e-10)
+    sz = mg.Vmgsz(2, 0, 0, n, n, n, nlev)
+    runs = []
+    for fn in (mg.Vnewdriv, orc.newdriv):
+        a = {k: inp[k].copy() for k in mg.PBE_KEYS}
+        u = start.copy()
+        e = fn(iparm.copy(), rparm.copy(), np.zeros(sz["iintot"], np.int32),
+               np.zeros(int(iparm[0])), u, *(a[k] for k in mg.PBE_KEYS))
+        runs.append((e, u, (mg.last_stats() if fn is mg.Vnewdriv else orc.last_stats())[0]))
+    (e1, u1, it1), (e0, u0b, it0) = runs
+    print(f"  iters gpu={it1} oracle={it0} (from zero {it_zero})", flush=True)
+    assert e1 == e0 == 0 and it1 == it0 < it_zero
+    assert _rel(u1, u0b) < 1e-9
+
+
+def test_ctx_nested_newton_mixed(mg, orc):
+    """The resident context's Newton (nested, mgkey 1) in FP64 and with FP32
+    inner cycles: FP64 as the drop-in, mixed within 1e-6."""
+    print("ctx nested newton fp64 + mixed n=33", flush=True)
+    n, nlev = 33, 3
+    inp = _newton_problem(mg, orc, n, 5)
+    iparm, rparm = mg.make_params(n, nlev, nonlin=1, errtol=1e-8, mgkey=1)
+    ctx = mg.Context(iparm, rparm, inp)
+    ctx.solve(mg.PMG_FP64)
+    u64, s64 = ctx.u(), ctx.stats()
+    ctx.solve(mg.PMG_MIXED)
+    umx, smx = ctx.u(), ctx.stats()
+    ctx.close()
+    _, u0, _, _ = orc.solve(inp, n, nlev, nonlin=True, errtol=1e-8, mgkey=1)
+    it0 = orc.last_stats()[0]
+    print(f"  fp64 iters={s64['iters']} mixed iters={smx['iters']} oracle={it0} "
+          f"mixed err={_rel(umx, u0):.2e}", flush=True)
+    assert s64["iters"] == it0 and _rel(u64, u0) < 1e-9
+    assert smx["ierror"] == 0 and _rel(umx, u0) < 1e-6
+
+
 def test_error_paths(mg, orc):
     print("error paths", flush=True)
     n, nlev = 17, 3

@@ -119,3 +119,64 @@ def test_vcycle_contracts_on_pb_problem(orc, mg):
     assert ierr == 0 and rel <= 1e-6 and 2 <= it <= 30
     rho = hist[1:] / hist[:-1]
     assert np.all(rho < 0.5)
+
+
+def _newton_inputs(orc, n, h, seed):
+    rng = np.random.default_rng(seed)
+    orc.mypdefinit([1.0, -1.0], [0.5, 0.5])
+    inp = uniform_inputs(n, h, lambda x, y, z: 0.0)
+    N = n ** 3
+    for k in ("a1cf", "a2cf", "a3cf"):
+        inp[k] = rng.choice([2.0, 78.54], N)
+    inp["ccf"] = rng.uniform(0.0, 5.0, N)
+    ac, cc, _, _ = orc.buildA_fv(n, inp)
+    us = H.from_int(n, 3.0 * rng.standard_normal((n - 2) ** 3))
+    rhs = orc.matvec(n, 7, ac, np.zeros(N), us) + orc.c_vec(cc, us)
+    inp["fcf"] = H.from_int(n, H.to_int(n, rhs)) / h ** 3
+    return inp, us
+
+
+def test_nested_newton_manufactured(orc):
+    """mgkey 1 (Vfnewton, newtond.cc:60-138 with D9): the nested iteration
+    recovers the manufactured solution as the plain Newton does."""
+    n, nlev, h = 17, 3, 0.5
+    inp, us = _newton_inputs(orc, n, h, 21)
+    ierr, u, _, _ = orc.solve(inp, n, nlev, nonlin=True, errtol=1e-12, itmax=50, mgkey=1)
+    assert ierr == 0
+    assert np.max(np.abs(u - us)) < 1e-8
+
+
+def test_nested_newton_starts_from_coarse_solution(orc):
+    """The finest Newton of the nested iteration starts from the prolongated
+    coarse-level solutions: its first step leaves a smaller nonlinear
+    residual than the plain Newton's first step from zero, it needs no more
+    iterations, and both reach the same potential."""
+    n, nlev = 33, 3
+    orc.mypdefinit([1.0, -1.0], [0.5, 0.5])
+    inp = orc.pbe_synth(n, 0.5, 40, 6.0, 0.3, 0, 5)
+    e0, u0, _, _ = orc.solve(inp, n, nlev, nonlin=True, errtol=1e-8)
+    it0, _, _, h0 = orc.last_stats()
+    e1, u1, _, _ = orc.solve(inp, n, nlev, nonlin=True, errtol=1e-8, mgkey=1)
+    it1, _, _, h1 = orc.last_stats()
+    assert e0 == e1 == 0
+    assert h1[0] < h0[0] and it1 <= it0
+    assert np.max(np.abs(u1 - u0)) <= 1e-7 * np.max(np.abs(u0))
+
+
+def test_newton_starts_from_callers_u(orc, mg):
+    """Vnewdriv2 zeroes only the boundary of u (VfboundPMG00,
+    newdrvd.cc:263): the interior given is the starting iterate, so the
+    converged potential given back converges in one Newton step."""
+    n, nlev = 17, 3
+    orc.mypdefinit([1.0, -1.0], [0.5, 0.5])
+    inp = orc.pbe_synth(n, 0.5, 12, 3.0, 0.3, 0, 6)
+    _, u0, _, _ = orc.solve(inp, n, nlev, nonlin=True, errtol=1e-10)
+    it0 = orc.last_stats()[0]
+    iparm, rparm = mg.make_params(n, nlev, nonlin=1, errtol=1e-10)
+    sz = orc.mgsz(2, 0, 0, n, n, n, nlev)[1]
+    a = {k: np.ascontiguousarray(inp[k], np.float64).copy() for k in orc.PBE_KEYS}
+    u = u0.copy()
+    e = orc.newdriv(iparm, rparm, np.zeros(int(sz[11]), np.int32), np.zeros(int(iparm[0])),
+                    u, *(a[k] for k in orc.PBE_KEYS))
+    it = orc.last_stats()[0]
+    assert e == 0 and it0 > 2 and it == 1
