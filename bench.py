@@ -202,11 +202,12 @@ def serve_requests(mg, count, seed=20261206):
     return out
 
 
-def serve_stream(mg, reqs, mode, rate, active, budget):
+def serve_stream(mg, srv, reqs, mode, rate):
     """Submit the requests with Poisson arrivals at `rate` per second (all
-    at once when rate is None) and wait for all; returns the server stats
-    and the wall time from the first submission to the last result."""
-    srv = mg.Server(budget, active)
+    at once when rate is None) to the running server and wait for all;
+    returns the server stats of this stream and the wall time from the
+    first submission to the last result."""
+    srv.reset_stats()
     params = {n: mg.make_params(n, nlev) for n, nlev in set((r[0], r[1]) for r in reqs)}
     rng = np.random.default_rng(7)
     tickets = []
@@ -223,7 +224,6 @@ def serve_stream(mg, reqs, mode, rate, active, budget):
     res = [srv.wait(t) for t in tickets]
     wall = time.perf_counter() - t0
     st = srv.stats()
-    srv.close()
     st["wall_s"] = wall
     st["mean_iters"] = float(np.mean([r["iters"] for r in res]))
     st["max_relres"] = float(max(r["relres"] for r in res))
@@ -246,13 +246,17 @@ def serving_leg(mg, steps, warmup, batch, active):
     budget = 200 * (1 << 30)
     reqs = serve_requests(mg, steps * batch)
     warm = serve_requests(mg, max(warmup, 1) * batch, seed=20261207)
-    for mode in (mg.PMG_FP64, mg.PMG_MIXED):  # context pool of every shape
-        serve_stream(mg, warm, mode, None, active, budget)
-    cap64 = serve_stream(mg, reqs, mg.PMG_FP64, None, active, budget)
-    capmx = serve_stream(mg, reqs, mg.PMG_MIXED, None, active, budget)
+    # one server for every stream: the warm-up fills its context pool with
+    # every shape and precision, as a running service holds it
+    srv = mg.Server(budget, active)
+    for mode in (mg.PMG_FP64, mg.PMG_MIXED):
+        serve_stream(mg, srv, warm, mode, None)
+    cap64 = serve_stream(mg, srv, reqs, mg.PMG_FP64, None)
+    capmx = serve_stream(mg, srv, reqs, mg.PMG_MIXED, None)
     rate = 0.8 * cap64["solves_per_s"]
-    mixed = serve_stream(mg, reqs, mg.PMG_MIXED, rate, active, budget)
-    fp64 = serve_stream(mg, reqs, mg.PMG_FP64, rate, active, budget)
+    mixed = serve_stream(mg, srv, reqs, mg.PMG_MIXED, rate)
+    fp64 = serve_stream(mg, srv, reqs, mg.PMG_FP64, rate)
+    srv.close()
     return {
         "requests": len(reqs),
         "arrival_rate_per_s": rate,
@@ -424,6 +428,35 @@ def pcg_leg(mg, cpu, steps=5):
     return out
 
 
+def copy_probe(mg, mib):
+    """Three device-to-device copies of `mib` MiB through the HIP runtime the
+    library itself uses (a second runtime, such as torch's, in the same
+    process breaks the library's calls): the reference kernel of known
+    traffic that scales the hardware counters."""
+    import ctypes
+
+    mg.lib()  # the library's HIP runtime is the one loaded
+    h = ctypes.CDLL("libamdhip64.so.7")
+    h.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    h.hipMemset.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t]
+    h.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    h.hipFree.argtypes = [ctypes.c_void_p]
+    nb = mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if h.hipMalloc(ctypes.byref(a), nb) or h.hipMalloc(ctypes.byref(b), nb):
+        raise RuntimeError("copy probe: hipMalloc failed")
+    try:
+        if h.hipMemset(a, 1, nb):
+            raise RuntimeError("copy probe: hipMemset failed")
+        for _ in range(3):
+            if h.hipMemcpy(b, a, nb, 3):  # hipMemcpyDeviceToDevice
+                raise RuntimeError("copy probe: hipMemcpy failed")
+        h.hipDeviceSynchronize()
+    finally:
+        h.hipFree(a)
+        h.hipFree(b)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -433,14 +466,7 @@ def main():
     import multigrid_amd as mg
 
     if args.copy_probe_mib:
-        import torch
-        nel = args.copy_probe_mib * (1 << 20) // 4
-        a = torch.ones(nel, dtype=torch.float32, device="cuda")
-        b = torch.empty_like(a)
-        for _ in range(3):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        del a, b
+        copy_probe(mg, args.copy_probe_mib)
 
     # headline: the serving stream (the timed region is its mixed capacity run)
     serving, wall, value = None, None, None

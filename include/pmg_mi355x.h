@@ -129,10 +129,11 @@ int pmg_Vmypdefinitnpbe(int *nion, double *charge, double *sconc);
  * these and drops them (pmgc/mgdrvd.cc:194-195, 452-455). */
 int pmg_last_stats(int *iters, int *ierror, double *relres);
 /* The relative stopping measure after each V-cycle of the last pmg_mgdriv
- * call on this thread (what the reference's Vprtstp reports per iteration,
- * pmgc/mgsubd.cc:486-536; with iparm(12) iinfo > 0 the library also prints
- * it, with the contraction number, to stdout).  Copies at most cap values;
- * returns the number of cycles. */
+ * call on this thread, or after each finest-level Newton iteration of the
+ * last pmg_newdriv call (what the reference's Vprtstp reports per
+ * iteration, pmgc/mgsubd.cc:486-536; with iparm(12) iinfo > 0 the library
+ * also prints it, with the contraction number for V-cycles, to stdout).
+ * Copies at most cap values; returns the number of iterations. */
 int pmg_last_history(double *relres, int cap);
 
 /* --- resident-hierarchy API (no reference counterpart: the reference
@@ -156,6 +157,14 @@ int pmg_ctx_create(pmg_ctx **out, const int *iparm, const double *rparm,
 /* Solve on the resident hierarchy (mode PMG_FP64 or PMG_MIXED).  The
  * solution stays in HBM until pmg_ctx_get_u. */
 int pmg_ctx_solve(pmg_ctx *ctx, int mode);
+/* Solve n (<= 64) distinct contexts together: their V-cycle iterations run
+ * in lockstep with every kernel launched once for all of them (the
+ * grouped kernels of the serving loop), each context leaving the group when
+ * it meets its stopping test.  Each result, iteration count and history is
+ * bit-identical to pmg_ctx_solve of that context alone.  Linear contexts
+ * with istop 0 or 1, nlev >= 2, all with the same nu1 and nu2
+ * (PMG_EBADPARM otherwise).  No reference counterpart. */
+int pmg_ctx_solve_group(pmg_ctx *const *ctx, int n, int mode);
 /* The class-style PCG of pmg_pcgdriv on a resident context; PMG_MIXED runs
  * the V-cycle preconditioner in FP32 under the FP64 CG.  Statistics as
  * pmg_ctx_solve (iterations are CG iterations). */
@@ -364,6 +373,10 @@ int pmg_server_wait(pmg_server *srv, int64_t ticket, int timeout_ms,
 int pmg_server_stats(pmg_server *srv, long long *completed, double *p50_ms,
                      double *p99_ms, double *solves_per_s, size_t *mem_used,
                      int *max_concurrent);
+/* Start the statistics of pmg_server_stats afresh (completed requests,
+ * latencies, the throughput window, the concurrency high-water mark); the
+ * context pool is kept.  PMG_EBUSY while requests are in flight. */
+int pmg_server_reset_stats(pmg_server *srv);
 /* Finish every submitted request, then free the server. */
 void pmg_server_destroy(pmg_server *srv);
 

@@ -80,6 +80,7 @@ def _declare(L):
         "pmg_ctx_set_tru": (i, [vp, dp]),
         "pmg_ctx_create": (i, [ctypes.POINTER(vp), ip, dp] + [dp] * 11),
         "pmg_ctx_solve": (i, [vp, i]),
+        "pmg_ctx_solve_group": (i, [ctypes.POINTER(vp), i, i]),
         "pmg_ctx_get_u": (i, [vp, dp]),
         "pmg_ctx_stats": (i, [vp, ip, ip, dp, dp, dp]),
         "pmg_ctx_rebuild": (i, [vp]),
@@ -120,6 +121,7 @@ def _declare(L):
         "pmg_server_wait": (i, [vp, ctypes.c_int64, i, ip, dp, dp]),
         "pmg_server_stats": (i, [vp, ctypes.POINTER(ctypes.c_longlong), dp, dp, dp,
                                  ctypes.POINTER(ctypes.c_size_t), ip]),
+        "pmg_server_reset_stats": (i, [vp]),
         "pmg_server_destroy": (None, [vp]),
         "pmg_version": (ctypes.c_char_p, []),
     }
@@ -257,6 +259,15 @@ def last_history(cap=4096):
 # ---------------------------------------------------------------------------
 # Resident-hierarchy context
 # ---------------------------------------------------------------------------
+def solve_group(contexts, mode=PMG_FP64):
+    """Solve several resident contexts together (pmg_ctx_solve_group): their
+    iterations run in lockstep, each kernel launched once for all of them;
+    every result is bit-identical to that context's own solve."""
+    arr = (ctypes.c_void_p * len(contexts))(*[c._h.value for c in contexts])
+    return _check(lib().pmg_ctx_solve_group(arr, len(contexts), mode),
+                  "pmg_ctx_solve_group")
+
+
 class Context:
     """A solve whose inputs and hierarchy stay resident in HBM."""
 
@@ -408,6 +419,11 @@ class Server:
                                ctypes.byref(mx))
         return dict(completed=c.value, p50_ms=p50.value, p99_ms=p99.value,
                     solves_per_s=sps.value, mem_used=mem.value, max_concurrent=mx.value)
+
+    def reset_stats(self):
+        """Start the statistics afresh (no request may be in flight); the
+        pooled contexts stay."""
+        _check(lib().pmg_server_reset_stats(self._h), "pmg_server_reset_stats")
 
     def close(self):
         if self._h:

@@ -178,6 +178,18 @@ void lin_finish(pmg_ctx *c, double *host_u);
 int lin_finished(pmg_ctx *c, bool wait);
 // Statistics of the finished session; returns its status code.
 int lin_end(pmg_ctx *c);
+// An iteration of the session has landed with its stopping scalar in
+// c->hscal[S_RSNRM] (slot 1): count, record, decide.  1: finished.
+int ses_cycle_done(pmg_ctx *c);
+
+// Grouped iterations (the serving loop): contexts whose sessions may run
+// their iterations through group_step, and one iteration of n such
+// sessions (same mode, same nu1 and nu2) on stream s, each kernel launched
+// once for all of them; context q's stopping scalar also goes to norm[q]
+// (device memory).  Bit-identical to each context's own iteration.
+bool group_ok(const pmg_ctx *c);
+void group_step(pmg_ctx *const *c, int n, int mode, hipStream_t s,
+                double *norm);
 
 }  // namespace drv
 }  // namespace pmg

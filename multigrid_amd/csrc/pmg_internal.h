@@ -228,4 +228,83 @@ int pbe_assemble(int n, double h, double ionic, int natoms,
                  hipStream_t s, double eps_in = 2.0, double eps_out = 78.54,
                  double shell = 2.0);
 
+// ---- grouped launches (the serving loop): the same step for up to GMAX
+// independent problems per launch, each problem running exactly the
+// workgroups and arithmetic of its own launch (bit-identical results).
+// Problems a grouped kernel does not take run through the single launcher.
+constexpr int GMAX = 8;
+template <class T>
+struct GSmooth {  // one level's smoothing (gs7_fused / gs27 arguments)
+  Dim d;
+  const T *ac, *cc, *f;
+  T *x, *tmp;
+};
+template <class T>
+void g_gs7_fused(const GSmooth<T> *j, int n, int nsweeps, int iadjoint,
+                 bool zero_init, hipStream_t s);
+template <class T>
+void g_gs27(const GSmooth<T> *j, int n, int nsweeps, int iadjoint,
+            bool zero_init, hipStream_t s);
+template <class T, class TO>
+struct GResid {  // resid(); out2, when non-null, also receives the norm
+  Dim d;
+  int numdia;
+  const T *ac, *cc, *fc, *x;
+  TO *r;
+  const Red *red;
+  double *out2;
+};
+template <class T, class TO>
+void g_resid(const GResid<T, TO> *j, int n, hipStream_t s);
+template <class T>
+struct GRestrict {
+  Dim f, c;
+  const T *xin;
+  T *xout;
+  const T *pc;
+};
+template <class T>
+void g_restrict(const GRestrict<T> *j, int n, hipStream_t s);
+template <class T>
+struct GMatvec {  // matvec_dots_part()
+  Dim d;
+  int numdia;
+  const T *ac, *cc, *x, *w0;
+  T *y;
+  double *part;
+};
+// nb[q] receives problem q's number of partials.
+template <class T>
+void g_matvec_dots_part(const GMatvec<T> *j, int n, unsigned *nb,
+                        hipStream_t s);
+template <class T>
+struct GInterp {  // interp_damped()
+  Dim c, f;
+  const T *xc, *pc;
+  T *xf;
+  const double *part;
+  unsigned nblk;
+};
+template <class T>
+void g_interp_damped(const GInterp<T> *j, int n, hipStream_t s);
+template <class T>
+struct GCoarse {  // cg_single()
+  Dim d;
+  int numdia;
+  const double *ac, *cc;
+  const T *f;
+  T *x;
+};
+template <class T>
+void g_cg(const GCoarse<T> *j, int n, int itmax, double errtol,
+          hipStream_t s);
+template <class T>
+struct GAddcorr {  // add_correction()
+  Dim d;
+  const T *e;
+  double *u;
+};
+template <class T>
+void g_addcorr(const GAddcorr<T> *j, int n, hipStream_t s);
+
 }  // namespace pmg

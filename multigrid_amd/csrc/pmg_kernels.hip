@@ -43,13 +43,19 @@ inline XG xg_of(const dim3 &g) { return XG{(int)g.x, (int)g.y, (int)g.z}; }
 inline unsigned xcd_blocks(const dim3 &g) {
   return 8u * ((g.x * g.y * g.z + 7u) / 8u);
 }
-__device__ inline bool xcd_tile(const XG &g, int &bx, int &by, int &bz) {
+// bid: the workgroup's index in the launch of one problem (blockIdx.x, or
+// its index within its problem's share of a grouped launch).
+__device__ inline bool xcd_tile_at(const XG &g, unsigned bid, int &bx,
+                                   int &by, int &bz) {
   const int nt = g.gx * g.gy * g.gz, per = (nt + 7) / 8;
-  const int t = (int)(blockIdx.x % 8u) * per + (int)(blockIdx.x / 8u);
+  const int t = (int)(bid % 8u) * per + (int)(bid / 8u);
   bx = t % g.gx;
   by = (t / g.gx) % g.gy;
   bz = t / (g.gx * g.gy);
   return t < nt;
+}
+__device__ inline bool xcd_tile(const XG &g, int &bx, int &by, int &bz) {
+  return xcd_tile_at(g, blockIdx.x, bx, by, bz);
 }
 
 static void check_launch(const char *what) {
@@ -112,23 +118,27 @@ __device__ inline double block_sum(double v, double *red) {
 // device-scope atomic per block; across the eight XCDs, whose L2 caches are
 // not coherent, those serialise and cost more than the extra launch.)
 template <int NV>
-__device__ inline void red_store(const Red &r, double v0, double v1) {
+__device__ inline void red_store_at(const Red &r, size_t b, double v0,
+                                   double v1) {
   const int tid = threadIdx.x + threadIdx.y * blockDim.x;
   if (tid == 0) {
-    const size_t b =
-        blockIdx.x + gridDim.x * (blockIdx.y + (size_t)gridDim.y * blockIdx.z);
     r.part[b * NV] = v0;
     if (NV == 2) r.part[b * NV + 1] = v1;
   }
+}
+template <int NV>
+__device__ inline void red_store(const Red &r, double v0, double v1) {
+  red_store_at<NV>(r, blockIdx.x + gridDim.x * (blockIdx.y +
+                                               (size_t)gridDim.y * blockIdx.z),
+                   v0, v1);
 }
 
 // out[0] = sum0 (MODE 0), sum0 / sum1 (MODE 1; 0 when sum1 == 0, a zero
 // coarse correction) or sum1 (MODE 2).  Fixed order: thread t sums blocks
 // t, t+1024, ...
 template <int NV, int MODE>
-__global__ __launch_bounds__(1024) void k_red_final(const double *part,
-                                                    unsigned nblk,
-                                                    double *out) {
+__device__ __forceinline__ void red_final_body(const double *part,
+                                               unsigned nblk, double *out) {
   __shared__ double lds[16];
   double a = 0.0, c = 0.0;
   for (unsigned q = threadIdx.x; q < nblk; q += 1024) {
@@ -145,6 +155,12 @@ __global__ __launch_bounds__(1024) void k_red_final(const double *part,
     else
       out[0] = sa;
   }
+}
+template <int NV, int MODE>
+__global__ __launch_bounds__(1024) void k_red_final(const double *part,
+                                                    unsigned nblk,
+                                                    double *out) {
+  red_final_body<NV, MODE>(part, nblk, out);
 }
 
 template <int NV, int MODE>
@@ -284,10 +300,10 @@ struct SwCo {  // own-point operator values of one plane
 
 // ZX: xin is zero everywhere (the sweep after Vazeros) and is not read.
 template <class T, bool ZX>
-__global__ __launch_bounds__(SW_TX *SW_TY) void k_gs7_sweep(
+__device__ __forceinline__ void gs7_sweep_body(
     int nx, int ny, int nz, const T *__restrict__ ac, const T *__restrict__ cc,
     const T *__restrict__ fc, const T *__restrict__ xin, T *__restrict__ xout,
-    int c1, int kc, int gx, int gy, int gz) {
+    int c1, int kc, int gx, int gy, int gz, unsigned bid) {
   __shared__ T SX[2][SW_TY][SW_TX], SE[3][SW_TY][SW_TX], SN[3][SW_TY][SW_TX],
       SM[3][SW_TY][SW_TX];
   // XCD-aware tile order: workgroups are dispatched round-robin over the 8
@@ -295,7 +311,7 @@ __global__ __launch_bounds__(SW_TX *SW_TY) void k_gs7_sweep(
   // of tiles (x fastest, then y, then k chunk): neighbouring tiles, which
   // read each other's halo rows and columns, share that XCD's L2.
   const int ntile = gx * gy * gz, per = (ntile + 7) / 8;
-  const int t = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+  const int t = (int)(bid % 8) * per + (int)(bid / 8);
   if (t >= ntile) return;  // whole workgroup, before any barrier
   const int tx = threadIdx.x, ty = threadIdx.y;
   const int i = (t % gx) * (SW_TX - 4) + tx;
@@ -419,6 +435,15 @@ __global__ __launch_bounds__(SW_TX *SW_TY) void k_gs7_sweep(
   if (k + 1 <= ke) step(k + 1, I1{});
   if (k + 2 <= ke) step(k + 2, I2{});
   if (k + 3 <= ke) step(k + 3, I3{});
+}
+
+template <class T, bool ZX>
+__global__ __launch_bounds__(SW_TX *SW_TY) void k_gs7_sweep(
+    int nx, int ny, int nz, const T *__restrict__ ac, const T *__restrict__ cc,
+    const T *__restrict__ fc, const T *__restrict__ xin, T *__restrict__ xout,
+    int c1, int kc, int gx, int gy, int gz) {
+  gs7_sweep_body<T, ZX>(nx, ny, nz, ac, cc, fc, xin, xout, c1, kc, gx, gy,
+                        gz, blockIdx.x);
 }
 
 // Compute units of the current device (cached per device).
@@ -733,13 +758,13 @@ __global__ __launch_bounds__(256) void k_gs27_rows(
 // value another workgroup writes in the same launch, except the duplicated
 // edge rows, which both write with the same bits.
 template <class T, int ADJ, int PPL, int RS2>
-__global__ __launch_bounds__(64 * (RS2 + 1)) void k_gs27_pair(
+__device__ __forceinline__ void gs27_pair_body(
     int nx, int ny, int nz, const T *__restrict__ ac, const T *__restrict__ cc,
     const T *__restrict__ fc, const T *cur, T *nxt, const T *po, int pz,
-    int bp, int nplanes, int nstrips, int zmask) {
+    int bp, int nplanes, int nstrips, int zmask, unsigned bid) {
   // XCD-contiguous runs of strips, planes outer
   const int nt = nplanes * nstrips, per = (nt + 7) / 8;
-  const int t = (int)(blockIdx.x % 8u) * per + (int)(blockIdx.x / 8u);
+  const int t = (int)(bid % 8u) * per + (int)(bid / 8u);
   if (t >= nt) return;  // whole workgroup, before the barrier
   const int k = 2 + pz + 2 * (t / nstrips), st = t % nstrips;
   const int w = threadIdx.x >> 6;
@@ -756,6 +781,15 @@ __global__ __launch_bounds__(64 * (RS2 + 1)) void k_gs27_pair(
     gs27_row<T, ADJ, PPL>(nx, ny, nz, ac, cc, fc, pz, bp, j2, k,
                           zmask & ~(1 << (2 * pz + 1 - bp)), cur, nxt, po, po,
                           nxt);
+}
+
+template <class T, int ADJ, int PPL, int RS2>
+__global__ __launch_bounds__(64 * (RS2 + 1)) void k_gs27_pair(
+    int nx, int ny, int nz, const T *__restrict__ ac, const T *__restrict__ cc,
+    const T *__restrict__ fc, const T *cur, T *nxt, const T *po, int pz,
+    int bp, int nplanes, int nstrips, int zmask) {
+  gs27_pair_body<T, ADJ, PPL, RS2>(nx, ny, nz, ac, cc, fc, cur, nxt, po, pz,
+                                   bp, nplanes, nstrips, zmask, blockIdx.x);
 }
 
 static int count_parity(int n, int par) {  // #{v in [2, n-1] : v%2 == par}
@@ -871,15 +905,15 @@ void gs27(const Dim &d, const T *ac, const T *cc, const T *fc, T *x,
 // Residual (Vmresid7_1s / Vmresid27_1s) with optional L1 partial sums
 // ---------------------------------------------------------------------------
 template <class T, class TO, int ND>
-__global__ __launch_bounds__(256) void k_resid(
+__device__ __forceinline__ void resid_body(
     int nx, int ny, int nz, const T *__restrict__ ac,
     const T *__restrict__ cc, const T *__restrict__ fc,
     const T *__restrict__ x, TO *__restrict__ r, Red rd, int do_red, int kc,
-    XG xg) {
+    XG xg, unsigned bid) {
   __shared__ double red[4];
   const size_t n = (size_t)nx * ny * nz, sy = nx, sz = (size_t)nx * ny;
   int bx, by, bz;
-  const bool tok = xcd_tile(xg, bx, by, bz);
+  const bool tok = xcd_tile_at(xg, bid, bx, by, bz);
   const int i = 2 + bx * BX + threadIdx.x;
   const int j = 2 + by * BY + threadIdx.y;
   const int k0 = 2 + bz * kc;
@@ -908,8 +942,18 @@ __global__ __launch_bounds__(256) void k_resid(
   }
   if (do_red) {
     const double t = block_sum<256>(acc, red);
-    red_store<1>(rd, t, 0.0);
+    red_store_at<1>(rd, bid, t, 0.0);
   }
+}
+
+template <class T, class TO, int ND>
+__global__ __launch_bounds__(256) void k_resid(
+    int nx, int ny, int nz, const T *__restrict__ ac,
+    const T *__restrict__ cc, const T *__restrict__ fc,
+    const T *__restrict__ x, TO *__restrict__ r, Red rd, int do_red, int kc,
+    XG xg) {
+  resid_body<T, TO, ND>(nx, ny, nz, ac, cc, fc, x, r, rd, do_red, kc, xg,
+                        blockIdx.x);
 }
 
 size_t resid_partials(const Dim &d) {
@@ -1044,15 +1088,15 @@ void nmresid(const Dim &d, int numdia, const double *ac, const double *cc,
 // y = A x with partial sums of x.w0 and x.y (Vmatvec + 2 Vxdot, D2)
 // ---------------------------------------------------------------------------
 template <class T, int ND>
-__global__ __launch_bounds__(256) void k_matvec_dots(
+__device__ __forceinline__ void matvec_dots_body(
     int nx, int ny, int nz, const T *__restrict__ ac,
     const T *__restrict__ cc, const T *__restrict__ x,
     const T *__restrict__ w0, T *__restrict__ y, Red rd, int do_red, int kc,
-    XG xg) {
+    XG xg, unsigned bid) {
   __shared__ double red[4];
   const size_t n = (size_t)nx * ny * nz, sy = nx, sz = (size_t)nx * ny;
   int bx, by, bz;
-  const bool tok = xcd_tile(xg, bx, by, bz);
+  const bool tok = xcd_tile_at(xg, bid, bx, by, bz);
   const int i = 2 + bx * BX + threadIdx.x;
   const int j = 2 + by * BY + threadIdx.y;
   const int k0 = 2 + bz * kc;
@@ -1081,8 +1125,18 @@ __global__ __launch_bounds__(256) void k_matvec_dots(
   if (do_red) {
     const double t0 = block_sum<256>(a0, red);
     const double t1 = block_sum<256>(a1, red);
-    red_store<2>(rd, t0, t1);
+    red_store_at<2>(rd, bid, t0, t1);
   }
+}
+
+template <class T, int ND>
+__global__ __launch_bounds__(256) void k_matvec_dots(
+    int nx, int ny, int nz, const T *__restrict__ ac,
+    const T *__restrict__ cc, const T *__restrict__ x,
+    const T *__restrict__ w0, T *__restrict__ y, Red rd, int do_red, int kc,
+    XG xg) {
+  matvec_dots_body<T, ND>(nx, ny, nz, ac, cc, x, w0, y, rd, do_red, kc, xg,
+                          blockIdx.x);
 }
 
 template <class T>
@@ -1126,19 +1180,20 @@ unsigned matvec_dots_part(const Dim &d, int numdia, const T *ac, const T *cc,
 // Grid transfer: Vrestrc2 (matvecd.cc:755-795), VinterpPMG2 (:878-960)
 // ---------------------------------------------------------------------------
 template <class T>
-__global__ __launch_bounds__(256) void k_restrict(int nxf, int nyf, int nxc,
-                                                  int nyc, int nzc,
-                                                  const T *__restrict__ xin,
-                                                  T *__restrict__ xout,
-                                                  const T *__restrict__ pc, XG xg,
-                                                  float *__restrict__ out32) {
+__device__ __forceinline__ void restrict_body(int nxf, int nyf, int nxc,
+                                              int nyc, int nzc,
+                                              const T *__restrict__ xin,
+                                              T *__restrict__ xout,
+                                              const T *__restrict__ pc, XG xg,
+                                              float *__restrict__ out32,
+                                              unsigned bid) {
   // no multiply-add contraction: the setup restricts cc and fc with this
   // kernel, and a source of both signs cancels in those sums, so the
   // coarse cc and fc are the oracle's to the bit only with the same
   // roundings (same term order, separate multiply and add)
 #pragma clang fp contract(off)
   int bx, by, bz;
-  if (!xcd_tile(xg, bx, by, bz)) return;
+  if (!xcd_tile_at(xg, bid, bx, by, bz)) return;
   const int ii = 2 + bx * BX + threadIdx.x;
   const int jj = 2 + by * BY + threadIdx.y;
   const int kk = 2 + bz;
@@ -1172,6 +1227,17 @@ __global__ __launch_bounds__(256) void k_restrict(int nxf, int nyf, int nxc,
 }
 
 template <class T>
+__global__ __launch_bounds__(256) void k_restrict(int nxf, int nyf, int nxc,
+                                                  int nyc, int nzc,
+                                                  const T *__restrict__ xin,
+                                                  T *__restrict__ xout,
+                                                  const T *__restrict__ pc, XG xg,
+                                                  float *__restrict__ out32) {
+  restrict_body<T>(nxf, nyf, nxc, nyc, nzc, xin, xout, pc, xg, out32,
+                   blockIdx.x);
+}
+
+template <class T>
 void restrict_(const Dim &f, const Dim &c, const T *xin, T *xout, const T *pc,
                hipStream_t s, float *out32) {
   if (c.nx < 3 || c.ny < 3 || c.nz < 3) return;
@@ -1195,16 +1261,17 @@ enum : int {
 // the cell, as VinterpPMG2.  ADD: xf += xdamp * value on fine interior
 // points (D3); otherwise xf = value (boundary zeroed separately, D8).
 template <class T, bool ADD>
-__global__ __launch_bounds__(256) void k_interp(int nxc, int nyc, int nzc,
-                                                int nxf, int nyf, int nzf,
-                                                const T *__restrict__ xc,
-                                                const T *__restrict__ pc,
-                                                T *__restrict__ xf,
-                                                const double *xdamp,
-                                                const double *part,
-                                                unsigned nblk, XG xg) {
+__device__ __forceinline__ void interp_body(int nxc, int nyc, int nzc,
+                                            int nxf, int nyf, int nzf,
+                                            const T *__restrict__ xc,
+                                            const T *__restrict__ pc,
+                                            T *__restrict__ xf,
+                                            const double *xdamp,
+                                            const double *part,
+                                            unsigned nblk, XG xg,
+                                            unsigned bid) {
   int bx, by, bz;
-  const bool tok = xcd_tile(xg, bx, by, bz);
+  const bool tok = xcd_tile_at(xg, bid, bx, by, bz);
   const int ii = 1 + bx * BX + threadIdx.x;
   const int jj = 1 + by * BY + threadIdx.y;
   const int kk = 1 + bz;
@@ -1265,6 +1332,19 @@ __global__ __launch_bounds__(256) void k_interp(int nxc, int nyc, int nzc,
       xf[p] = v[q];
     }
   }
+}
+
+template <class T, bool ADD>
+__global__ __launch_bounds__(256) void k_interp(int nxc, int nyc, int nzc,
+                                                int nxf, int nyf, int nzf,
+                                                const T *__restrict__ xc,
+                                                const T *__restrict__ pc,
+                                                T *__restrict__ xf,
+                                                const double *xdamp,
+                                                const double *part,
+                                                unsigned nblk, XG xg) {
+  interp_body<T, ADD>(nxc, nyc, nzc, nxf, nyf, nzf, xc, pc, xf, xdamp, part,
+                      nblk, xg, blockIdx.x);
 }
 
 template <class T>
@@ -1624,13 +1704,14 @@ constexpr int CG_R = 3, CG_RNT = 512;
 // criterion (else Vcghs's relative one); ZT: errtol is 0, so Vcghs's test
 // sqrt(r.r)/sqrt(r0.r0) <= errtol is r.r == 0 (no square root or divide in
 // the loop).  The PMG cycle uses <T, 0, 0, 1> (errtol = epsiln = 0).
+extern __shared__ double cg_lds[];
 template <class T, bool WARM, bool ABS, bool ZT>
-__global__ __launch_bounds__(CG_RNT) void k_cg_rows(
+__device__ __forceinline__ void cg_rows_body(
     int nx, int ny, int nz, const double *__restrict__ ac,
     const double *__restrict__ cc, const T *__restrict__ f,
     T *__restrict__ xout, int itmax, double errtol) {
   constexpr int R = CG_R;
-  extern __shared__ double P[];
+  double *P = cg_lds;
   __shared__ double rbuf[2][CG_RNT / 64];
   cg_zero_slots<CG_RNT / 64>(rbuf);
   const int tid = threadIdx.x, NTB = blockDim.x;
@@ -1792,6 +1873,14 @@ __global__ __launch_bounds__(CG_RNT) void k_cg_rows(
 #pragma unroll
   for (int r = 0; r < R; r++)
     if (val[r]) xout[pidx[r]] = (T)xr[r];
+}
+
+template <class T, bool WARM, bool ABS, bool ZT>
+__global__ __launch_bounds__(CG_RNT) void k_cg_rows(
+    int nx, int ny, int nz, const double *__restrict__ ac,
+    const double *__restrict__ cc, const T *__restrict__ f,
+    T *__restrict__ xout, int itmax, double errtol) {
+  cg_rows_body<T, WARM, ABS, ZT>(nx, ny, nz, ac, cc, f, xout, itmax, errtol);
 }
 
 template <class T, int ND>
@@ -1995,12 +2084,13 @@ void axpy_interior(const Dim &d, double alpha, const T *x, T *y,
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void k_addcorr(int nx, int ny, int nz,
-                                                 const T *__restrict__ e,
-                                                 double *__restrict__ u) {
-  const int i = 2 + blockIdx.x * BX + threadIdx.x;
-  const int j = 2 + blockIdx.y * BY + threadIdx.y;
-  const int k0 = 2 + blockIdx.z * KCH;
+__device__ __forceinline__ void addcorr_body(int nx, int ny, int nz,
+                                             const T *__restrict__ e,
+                                             double *__restrict__ u, int bx,
+                                             int by, int bz) {
+  const int i = 2 + bx * BX + threadIdx.x;
+  const int j = 2 + by * BY + threadIdx.y;
+  const int k0 = 2 + bz * KCH;
   if (i > nx - 1 || j > ny - 1) return;
   for (int kk = 0; kk < KCH; kk++) {
     const int k = k0 + kk;
@@ -2008,6 +2098,12 @@ __global__ __launch_bounds__(256) void k_addcorr(int nx, int ny, int nz,
     const size_t p = ix(nx, ny, i, j, k);
     u[p] += (double)e[p];
   }
+}
+template <class T>
+__global__ __launch_bounds__(256) void k_addcorr(int nx, int ny, int nz,
+                                                 const T *__restrict__ e,
+                                                 double *__restrict__ u) {
+  addcorr_body<T>(nx, ny, nz, e, u, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 template <class T>
 void add_correction(const Dim &d, const T *e, double *u, hipStream_t s) {
@@ -2441,6 +2537,522 @@ void buildG(const Dim &f, const Dim &c, int numdia, const double *pc,
 }
 
 // ---------------------------------------------------------------------------
+// Grouped launches (serving): one launch runs the same step for up to GMAX
+// independent problems.  Problem q's arguments sit in slot q of a table
+// passed by value, blockIdx.y selects the slot, and the slot's workgroups
+// are those with blockIdx.x below its own launch's workgroup count: every
+// problem runs exactly the workgroups, with the same indices and the same
+// arithmetic, as its own launch would, so its results are bit-identical to
+// solving it alone.  The launchers split the problems by kernel variant
+// and into launches of at most GMAX.
+// ---------------------------------------------------------------------------
+template <class A>
+struct GTab {
+  A a[GMAX];
+};
+
+// Launch f(tab, count) for each run of at most GMAX consecutive entries of
+// the problems whose key equals key (entries built by mk(q) into the tab).
+template <class A, class KeyF, class MkF, class LaunchF>
+static void g_split(int n, KeyF key, MkF mk, LaunchF launch) {
+  std::vector<char> done(n, 0);
+  for (int q0 = 0; q0 < n; q0++) {
+    if (done[q0]) continue;
+    const auto k0 = key(q0);
+    GTab<A> t{};
+    int m = 0;
+    for (int q = q0; q < n; q++) {
+      if (done[q] || !(key(q) == k0)) continue;
+      done[q] = 1;
+      t.a[m++] = mk(q);
+      if (m == GMAX) {
+        launch(t, m, k0);
+        m = 0;
+      }
+    }
+    if (m) launch(t, m, k0);
+  }
+}
+
+template <class T>
+struct A_gs7 {
+  int nx, ny, nz;
+  const T *ac, *cc, *fc, *xin;
+  T *xout;
+  int c1, kc, gx, gy, gz;
+  unsigned nblk;
+};
+template <class T, bool ZX>
+__global__ __launch_bounds__(SW_TX *SW_TY) void kb_gs7_sweep(
+    const GTab<A_gs7<T>> t) {
+  const A_gs7<T> &a = t.a[blockIdx.y];
+  if (blockIdx.x >= a.nblk) return;
+  gs7_sweep_body<T, ZX>(a.nx, a.ny, a.nz, a.ac, a.cc, a.fc, a.xin, a.xout,
+                        a.c1, a.kc, a.gx, a.gy, a.gz, blockIdx.x);
+}
+
+template <class T>
+void g_gs7_fused(const GSmooth<T> *j, int n, int nsweeps, int iadjoint,
+                 bool zero_init, hipStream_t s) {
+  if (nsweeps <= 0 || n <= 0) return;
+  struct G {
+    int kc, gx, gy, gz;
+    unsigned nblk;
+    T *src, *dst;
+  };
+  std::vector<G> g(n);
+  const int cus = device_cus();
+  for (int q = 0; q < n; q++) {  // the launch shape of gs7_fused
+    const Dim &d = j[q].d;
+    G &e = g[q];
+    e.gx = cdiv(d.nx - 2, SW_TX - 4), e.gy = cdiv(d.ny - 2, SW_TY - 4);
+    const int zb = std::max(1, cus / (e.gx * e.gy));
+    e.kc = std::max(8, cdiv(d.nz - 2, zb));
+    e.gz = cdiv(d.nz - 2, e.kc);
+    e.nblk = 8 * cdiv(e.gx * e.gy * e.gz, 8);
+    e.src = j[q].x, e.dst = j[q].tmp;
+  }
+  const int c1 = iadjoint ? 1 : 0;
+  for (int it = 0; it < nsweeps; it++) {
+    const bool zx = zero_init && it == 0;
+    g_split<A_gs7<T>>(
+        n, [&](int) { return 0; },
+        [&](int q) {
+          const Dim &d = j[q].d;
+          return A_gs7<T>{d.nx,    d.ny,     d.nz,     j[q].ac, j[q].cc,
+                          j[q].f,  g[q].src, g[q].dst, c1,      g[q].kc,
+                          g[q].gx, g[q].gy,  g[q].gz,  g[q].nblk};
+        },
+        [&](const GTab<A_gs7<T>> &t, int m, int) {
+          unsigned mx = 0;
+          for (int q = 0; q < m; q++) mx = std::max(mx, t.a[q].nblk);
+          dim3 grid(mx, m), b(SW_TX, SW_TY);
+          if (zx)
+            kb_gs7_sweep<T, true><<<grid, b, 0, s>>>(t);
+          else
+            kb_gs7_sweep<T, false><<<grid, b, 0, s>>>(t);
+        });
+    for (int q = 0; q < n; q++) std::swap(g[q].src, g[q].dst);
+  }
+  for (int q = 0; q < n; q++)
+    if (g[q].src != j[q].x)
+      hipMemcpyAsync(j[q].x, g[q].src, j[q].d.n() * sizeof(T),
+                     hipMemcpyDeviceToDevice, s);
+  check_launch("g_gs7_fused");
+}
+
+template <class T>
+struct A_g27 {
+  int nx, ny, nz;
+  const T *ac, *cc, *fc, *cur;
+  T *nxt;
+  const T *po;
+  int pz, bp, nplanes, nstrips, zmask;
+  unsigned nblk;
+};
+template <class T, int ADJ, int PPL, int RS2>
+__global__ __launch_bounds__(64 * (RS2 + 1)) void kb_gs27_pair(
+    const GTab<A_g27<T>> t) {
+  const A_g27<T> &a = t.a[blockIdx.y];
+  if (blockIdx.x >= a.nblk) return;
+  gs27_pair_body<T, ADJ, PPL, RS2>(a.nx, a.ny, a.nz, a.ac, a.cc, a.fc, a.cur,
+                                   a.nxt, a.po, a.pz, a.bp, a.nplanes,
+                                   a.nstrips, a.zmask, blockIdx.x);
+}
+
+// Problems the paired launches of gs27 take (an even number of sweeps,
+// rows of at most ROWS_MAXN points, a second buffer).
+template <class T>
+static bool g27_pairs_ok(const GSmooth<T> &j, int nsweeps) {
+  const Dim &d = j.d;
+  return d.nx >= 3 && d.ny >= 4 && d.nz >= 3 && d.nx <= ROWS_MAXN && j.tmp &&
+         j.tmp != j.x && nsweeps % 2 == 0;
+}
+
+template <class T>
+void g_gs27(const GSmooth<T> *j, int n, int nsweeps, int iadjoint,
+            bool zero_init, hipStream_t s) {
+  if (nsweeps <= 0 || n <= 0) return;
+  std::vector<int> idx;  // problems taken here; the others run alone
+  for (int q = 0; q < n; q++) {
+    if (g27_pairs_ok(j[q], nsweeps))
+      idx.push_back(q);
+    else
+      gs27<T>(j[q].d, j[q].ac, j[q].cc, j[q].f, j[q].x, nsweeps, iadjoint, s,
+              zero_init, j[q].tmp);
+  }
+  const int m = (int)idx.size();
+  if (!m) return;
+  const int bp = iadjoint ? 0 : 1;
+  struct G {
+    int ppl;
+    bool big;
+    int nstrips;
+    T *cur, *nxt;
+  };
+  std::vector<G> g(m);
+  for (int r = 0; r < m; r++) {  // the launch shape of gs27's paired sweeps
+    const GSmooth<T> &J = j[idx[r]];
+    g[r].ppl = J.d.nx <= 129 ? 1 : (J.d.nx <= 257 ? 2 : 4);
+    g[r].big = J.d.nx >= 65;
+    g[r].nstrips = cdiv(count_parity(J.d.ny, bp), g[r].big ? 7 : 3);
+    g[r].cur = J.x, g[r].nxt = J.tmp;
+  }
+  for (int done = 0; done < nsweeps; done++) {
+    int zmask = (zero_init && done == 0) ? 0xF : 0;
+    for (int h = 0; h < 2; h++) {
+      const int pz = iadjoint ? 1 - h : h;
+      std::vector<int> sub;  // problems with planes of this parity
+      for (int r = 0; r < m; r++)
+        if (count_parity(j[idx[r]].d.nz, pz) > 0) sub.push_back(r);
+      g_split<A_g27<T>>(
+          (int)sub.size(),
+          [&](int q) { return g[sub[q]].ppl * 2 + (g[sub[q]].big ? 1 : 0); },
+          [&](int q) {
+            const int r = sub[q];
+            const GSmooth<T> &J = j[idx[r]];
+            const int npl = count_parity(J.d.nz, pz);
+            const T *po = h == 0 ? g[r].cur : g[r].nxt;
+            return A_g27<T>{J.d.nx,     J.d.ny,     J.d.nz,  J.ac, J.cc,
+                            J.f,        g[r].cur,   g[r].nxt, po,  pz,
+                            bp,         npl,        g[r].nstrips, zmask,
+                            (unsigned)(8 * cdiv(npl * g[r].nstrips, 8))};
+          },
+          [&](const GTab<A_g27<T>> &t, int cnt, int key) {
+            unsigned mx = 0;
+            for (int q = 0; q < cnt; q++) mx = std::max(mx, t.a[q].nblk);
+            const int ppl = key / 2;
+            const bool big = key & 1;
+            const dim3 grid(mx, cnt);
+#define GPAIR(A, Q)                                                         \
+  do {                                                                      \
+    if (big)                                                                \
+      kb_gs27_pair<T, A, Q, 7><<<grid, 64 * 8, 0, s>>>(t);                  \
+    else                                                                    \
+      kb_gs27_pair<T, A, Q, 3><<<grid, 64 * 4, 0, s>>>(t);                  \
+  } while (0)
+            if (iadjoint) {
+              if (ppl == 1) GPAIR(1, 1);
+              else if (ppl == 2) GPAIR(1, 2);
+              else GPAIR(1, 4);
+            } else {
+              if (ppl == 1) GPAIR(0, 1);
+              else if (ppl == 2) GPAIR(0, 2);
+              else GPAIR(0, 4);
+            }
+#undef GPAIR
+          });
+      zmask &= ~(0x3 << (2 * pz));
+    }
+    for (int r = 0; r < m; r++) std::swap(g[r].cur, g[r].nxt);
+  }
+  check_launch("g_gs27");
+}
+
+template <class T, class TO>
+struct A_res {
+  int nx, ny, nz;
+  const T *ac, *cc, *fc, *x;
+  TO *r;
+  Red rd;
+  int do_red, kc;
+  XG xg;
+  unsigned nblk;
+};
+template <class T, class TO, int ND>
+__global__ __launch_bounds__(256) void kb_resid(const GTab<A_res<T, TO>> t) {
+  const A_res<T, TO> &a = t.a[blockIdx.y];
+  if (blockIdx.x >= a.nblk) return;
+  resid_body<T, TO, ND>(a.nx, a.ny, a.nz, a.ac, a.cc, a.fc, a.x, a.r, a.rd,
+                        a.do_red, a.kc, a.xg, blockIdx.x);
+}
+
+struct A_rf {
+  const double *part;
+  unsigned nblk;
+  double *out, *out2;
+};
+template <int NV, int MODE>
+__global__ __launch_bounds__(1024) void kb_red_final(const GTab<A_rf> t) {
+  const A_rf &a = t.a[blockIdx.y];
+  red_final_body<NV, MODE>(a.part, a.nblk, a.out);
+  if (threadIdx.x == 0 && a.out2) a.out2[0] = a.out[0];
+}
+
+template <class T, class TO>
+void g_resid(const GResid<T, TO> *j, int n, hipStream_t s) {
+  if (n <= 0) return;
+  std::vector<A_res<T, TO>> a(n);
+  for (int q = 0; q < n; q++) {  // the launch shape of resid
+    const Dim &d = j[q].d;
+    const bool red = j[q].red != nullptr;
+    const int kc = kchunk(d, red ? 40 : 80);
+    const dim3 g(cdiv(d.nx - 2, BX), cdiv(d.ny - 2, BY), cdiv(d.nz - 2, kc));
+    a[q] = A_res<T, TO>{d.nx,    d.ny,   d.nz,    j[q].ac,
+                        j[q].cc, j[q].fc, j[q].x, j[q].r,
+                        red ? *j[q].red : Red{}, red ? 1 : 0, kc,
+                        xg_of(g), xcd_blocks(g)};
+  }
+  g_split<A_res<T, TO>>(
+      n, [&](int q) { return j[q].numdia; }, [&](int q) { return a[q]; },
+      [&](const GTab<A_res<T, TO>> &t, int m, int nd) {
+        unsigned mx = 0;
+        for (int q = 0; q < m; q++) mx = std::max(mx, t.a[q].nblk);
+        const dim3 grid(mx, m), b(BX, BY);
+        if (nd == 7)
+          kb_resid<T, TO, 7><<<grid, b, 0, s>>>(t);
+        else
+          kb_resid<T, TO, 27><<<grid, b, 0, s>>>(t);
+      });
+  std::vector<int> rq;
+  for (int q = 0; q < n; q++)
+    if (j[q].red) rq.push_back(q);
+  g_split<A_rf>(
+      (int)rq.size(), [&](int) { return 0; },
+      [&](int q) {
+        const int r = rq[q];
+        return A_rf{j[r].red->part, a[r].nblk, j[r].red->out, j[r].out2};
+      },
+      [&](const GTab<A_rf> &t, int m, int) {
+        kb_red_final<1, 0><<<dim3(1, m), 1024, 0, s>>>(t);
+      });
+  check_launch("g_resid");
+}
+
+template <class T>
+struct A_rs {
+  int nxf, nyf, nxc, nyc, nzc;
+  const T *xin;
+  T *xout;
+  const T *pc;
+  XG xg;
+  unsigned nblk;
+};
+template <class T>
+__global__ __launch_bounds__(256) void kb_restrict(const GTab<A_rs<T>> t) {
+  const A_rs<T> &a = t.a[blockIdx.y];
+  if (blockIdx.x >= a.nblk) return;
+  restrict_body<T>(a.nxf, a.nyf, a.nxc, a.nyc, a.nzc, a.xin, a.xout, a.pc,
+                   a.xg, nullptr, blockIdx.x);
+}
+
+template <class T>
+void g_restrict(const GRestrict<T> *j, int n, hipStream_t s) {
+  std::vector<int> ok;
+  for (int q = 0; q < n; q++) {
+    const Dim &f = j[q].f, &c = j[q].c;
+    if (c.nx < 3 || c.ny < 3 || c.nz < 3) continue;
+    if (f.nx != 2 * c.nx - 1 || f.ny != 2 * c.ny - 1 || f.nz != 2 * c.nz - 1) {
+      fprintf(stderr, "pmg: restrict shape mismatch\n");
+      continue;
+    }
+    ok.push_back(q);
+  }
+  g_split<A_rs<T>>(
+      (int)ok.size(), [&](int) { return 0; },
+      [&](int q) {
+        const GRestrict<T> &J = j[ok[q]];
+        const dim3 g(cdiv(J.c.nx - 2, BX), cdiv(J.c.ny - 2, BY), J.c.nz - 2);
+        return A_rs<T>{J.f.nx, J.f.ny, J.c.nx,   J.c.ny,   J.c.nz,
+                       J.xin,  J.xout, J.pc,     xg_of(g), xcd_blocks(g)};
+      },
+      [&](const GTab<A_rs<T>> &t, int m, int) {
+        unsigned mx = 0;
+        for (int q = 0; q < m; q++) mx = std::max(mx, t.a[q].nblk);
+        kb_restrict<T><<<dim3(mx, m), dim3(BX, BY), 0, s>>>(t);
+      });
+  check_launch("g_restrict");
+}
+
+template <class T>
+struct A_mv {
+  int nx, ny, nz;
+  const T *ac, *cc, *x, *w0;
+  T *y;
+  Red rd;
+  int kc;
+  XG xg;
+  unsigned nblk;
+};
+template <class T, int ND>
+__global__ __launch_bounds__(256) void kb_matvec_dots(const GTab<A_mv<T>> t) {
+  const A_mv<T> &a = t.a[blockIdx.y];
+  if (blockIdx.x >= a.nblk) return;
+  matvec_dots_body<T, ND>(a.nx, a.ny, a.nz, a.ac, a.cc, a.x, a.w0, a.y, a.rd,
+                          1, a.kc, a.xg, blockIdx.x);
+}
+
+template <class T>
+void g_matvec_dots_part(const GMatvec<T> *j, int n, unsigned *nb,
+                        hipStream_t s) {
+  std::vector<A_mv<T>> a(n);
+  for (int q = 0; q < n; q++) {  // the launch shape of matvec_dots_part
+    const Dim &d = j[q].d;
+    const int kc = kchunk(d);
+    const dim3 g(cdiv(d.nx - 2, BX), cdiv(d.ny - 2, BY), cdiv(d.nz - 2, kc));
+    Red rd;
+    rd.part = j[q].part;
+    a[q] = A_mv<T>{d.nx,   d.ny,    d.nz, j[q].ac,  j[q].cc,      j[q].x,
+                   j[q].w0, j[q].y, rd,   kc,       xg_of(g),     xcd_blocks(g)};
+    nb[q] = a[q].nblk;
+  }
+  g_split<A_mv<T>>(
+      n, [&](int q) { return j[q].numdia; }, [&](int q) { return a[q]; },
+      [&](const GTab<A_mv<T>> &t, int m, int nd) {
+        unsigned mx = 0;
+        for (int q = 0; q < m; q++) mx = std::max(mx, t.a[q].nblk);
+        const dim3 grid(mx, m), b(BX, BY);
+        if (nd == 7)
+          kb_matvec_dots<T, 7><<<grid, b, 0, s>>>(t);
+        else
+          kb_matvec_dots<T, 27><<<grid, b, 0, s>>>(t);
+      });
+  check_launch("g_matvec_dots");
+}
+
+template <class T>
+struct A_ip {
+  int nxc, nyc, nzc, nxf, nyf, nzf;
+  const T *xc, *pc;
+  T *xf;
+  const double *part;
+  unsigned npart;
+  XG xg;
+  unsigned nblk;
+};
+template <class T>
+__global__ __launch_bounds__(256) void kb_interp_damped(const GTab<A_ip<T>> t) {
+  const A_ip<T> &a = t.a[blockIdx.y];
+  if (blockIdx.x >= a.nblk) return;
+  interp_body<T, true>(a.nxc, a.nyc, a.nzc, a.nxf, a.nyf, a.nzf, a.xc, a.pc,
+                       a.xf, nullptr, a.part, a.npart, a.xg, blockIdx.x);
+}
+
+template <class T>
+void g_interp_damped(const GInterp<T> *j, int n, hipStream_t s) {
+  std::vector<int> ok;
+  for (int q = 0; q < n; q++) {
+    const Dim &c = j[q].c, &f = j[q].f;
+    if (f.nx != 2 * c.nx - 1 || f.ny != 2 * c.ny - 1 || f.nz != 2 * c.nz - 1) {
+      fprintf(stderr, "pmg: interp shape mismatch\n");
+      continue;
+    }
+    ok.push_back(q);
+  }
+  g_split<A_ip<T>>(
+      (int)ok.size(), [&](int) { return 0; },
+      [&](int q) {
+        const GInterp<T> &J = j[ok[q]];
+        const dim3 g(cdiv(J.c.nx - 1, BX), cdiv(J.c.ny - 1, BY), J.c.nz - 1);
+        return A_ip<T>{J.c.nx, J.c.ny, J.c.nz, J.f.nx, J.f.ny,
+                       J.f.nz, J.xc,   J.pc,   J.xf,   J.part,
+                       J.nblk, xg_of(g), xcd_blocks(g)};
+      },
+      [&](const GTab<A_ip<T>> &t, int m, int) {
+        unsigned mx = 0;
+        for (int q = 0; q < m; q++) mx = std::max(mx, t.a[q].nblk);
+        kb_interp_damped<T><<<dim3(mx, m), dim3(BX, BY), 0, s>>>(t);
+      });
+  check_launch("g_interp_damped");
+}
+
+template <class T>
+struct A_cg {
+  int nx, ny, nz;
+  const double *ac, *cc;
+  const T *f;
+  T *x;
+  int itmax;
+  double errtol;
+};
+template <class T, bool ZT>
+__global__ __launch_bounds__(CG_RNT) void kb_cg_rows(const GTab<A_cg<T>> t) {
+  const A_cg<T> &a = t.a[blockIdx.y];
+  cg_rows_body<T, false, false, ZT>(a.nx, a.ny, a.nz, a.ac, a.cc, a.f, a.x,
+                                    a.itmax, a.errtol);
+}
+
+// The 27-point coarsest levels k_cg_rows takes (cg_launch's first case).
+static bool g_cg_rows_ok(const Dim &d, int numdia) {
+  const int nrow = (int)((d.nx - 2 + CG_R - 1) / CG_R) * (d.ny - 2) * (d.nz - 2);
+  return numdia == 27 && nrow <= CG_RNT && cg_fits(d);
+}
+
+template <class T>
+void g_cg(const GCoarse<T> *j, int n, int itmax, double errtol,
+          hipStream_t s) {
+  std::vector<int> ok;
+  for (int q = 0; q < n; q++) {
+    if (g_cg_rows_ok(j[q].d, j[q].numdia))
+      ok.push_back(q);
+    else
+      cg_single<T>(j[q].d, j[q].numdia, j[q].ac, j[q].cc, j[q].f, j[q].x,
+                   itmax, errtol, s);
+  }
+  // one launch per coarsest shape: the workgroup is as wide as the shape's
+  // own launch, and its LDS holds the shape's grid
+  g_split<A_cg<T>>(
+      (int)ok.size(),
+      [&](int q) {
+        const Dim &d = j[ok[q]].d;
+        return (long long)d.nx * 1000000 + d.ny * 1000 + d.nz;
+      },
+      [&](int q) {
+        const GCoarse<T> &J = j[ok[q]];
+        return A_cg<T>{J.d.nx, J.d.ny, J.d.nz, J.ac, J.cc, J.f, J.x, itmax,
+                       errtol};
+      },
+      [&](const GTab<A_cg<T>> &t, int m, long long) {
+        const Dim d{t.a[0].nx, t.a[0].ny, t.a[0].nz};
+        const int nrow =
+            (int)((d.nx - 2 + CG_R - 1) / CG_R) * (d.ny - 2) * (d.nz - 2);
+        const int nt = (nrow + 63) / 64 * 64;
+        const size_t lds = d.n() * sizeof(double) + (CG_R + 2) * sizeof(double);
+        if (errtol == 0.0)
+          kb_cg_rows<T, true><<<dim3(1, m), nt, lds, s>>>(t);
+        else
+          kb_cg_rows<T, false><<<dim3(1, m), nt, lds, s>>>(t);
+      });
+  check_launch("g_cg");
+}
+
+template <class T>
+struct A_acr {
+  int nx, ny, nz;
+  const T *e;
+  double *u;
+  int gx, gy;
+  unsigned nblk;
+};
+template <class T>
+__global__ __launch_bounds__(256) void kb_addcorr(const GTab<A_acr<T>> t) {
+  const A_acr<T> &a = t.a[blockIdx.y];
+  if (blockIdx.x >= a.nblk) return;
+  const int b = (int)blockIdx.x;
+  addcorr_body<T>(a.nx, a.ny, a.nz, a.e, a.u, b % a.gx, (b / a.gx) % a.gy,
+                  b / (a.gx * a.gy));
+}
+
+template <class T>
+void g_addcorr(const GAddcorr<T> *j, int n, hipStream_t s) {
+  g_split<A_acr<T>>(
+      n, [&](int) { return 0; },
+      [&](int q) {
+        const Dim &d = j[q].d;
+        const int gx = cdiv(d.nx - 2, BX), gy = cdiv(d.ny - 2, BY),
+                  gz = cdiv(d.nz - 2, KCH);
+        return A_acr<T>{d.nx, d.ny, d.nz, j[q].e, j[q].u, gx, gy,
+                        (unsigned)(gx * gy * gz)};
+      },
+      [&](const GTab<A_acr<T>> &t, int m, int) {
+        unsigned mx = 0;
+        for (int q = 0; q < m; q++) mx = std::max(mx, t.a[q].nblk);
+        kb_addcorr<T><<<dim3(mx, m), dim3(BX, BY), 0, s>>>(t);
+      });
+  check_launch("g_addcorr");
+}
+
+// ---------------------------------------------------------------------------
 // explicit instantiations
 // ---------------------------------------------------------------------------
 template void dot_energy<double>(const Dim &, int, const double *,
@@ -2497,5 +3109,26 @@ template void convert<double, float>(const double *, float *, size_t,
 template void convert<float, double>(const float *, double *, size_t,
                                      hipStream_t);
 template void fill<int>(int *, size_t, int, hipStream_t);
+
+#define GINST(T)                                                              \
+  template void g_gs7_fused<T>(const GSmooth<T> *, int, int, int, bool,       \
+                               hipStream_t);                                  \
+  template void g_gs27<T>(const GSmooth<T> *, int, int, int, bool,            \
+                          hipStream_t);                                       \
+  template void g_restrict<T>(const GRestrict<T> *, int, hipStream_t);        \
+  template void g_matvec_dots_part<T>(const GMatvec<T> *, int, unsigned *,    \
+                                      hipStream_t);                           \
+  template void g_interp_damped<T>(const GInterp<T> *, int, hipStream_t);     \
+  template void g_cg<T>(const GCoarse<T> *, int, int, double, hipStream_t);   \
+  template void g_addcorr<T>(const GAddcorr<T> *, int, hipStream_t);
+GINST(double)
+GINST(float)
+#undef GINST
+template void g_resid<double, double>(const GResid<double, double> *, int,
+                                      hipStream_t);
+template void g_resid<double, float>(const GResid<double, float> *, int,
+                                     hipStream_t);
+template void g_resid<float, float>(const GResid<float, float> *, int,
+                                    hipStream_t);
 
 }  // namespace pmg

@@ -431,6 +431,17 @@ int pmg_server_stats(pmg_server *S, long long *completed, double *p50_ms,
   return PMG_OK;
 }
 
+int pmg_server_reset_stats(pmg_server *S) {
+  if (!S) return PMG_EBADPARM;
+  std::lock_guard<std::mutex> lk(S->mu);
+  if (S->in_flight) return PMG_EBUSY;
+  S->lat_ms.clear();
+  S->completed = 0;
+  S->have_t0 = false;
+  S->max_seen = 0;
+  return PMG_OK;
+}
+
 void pmg_server_destroy(pmg_server *S) {
   if (!S) return;
   {

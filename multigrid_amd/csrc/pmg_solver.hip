@@ -793,6 +793,15 @@ int lin_poll(pmg_ctx *c, bool wait) {
     }
     return 0;
   }
+  return ses_cycle_done(c);
+}
+
+// An iteration of the session has landed, its stopping scalar in
+// c->hscal[S_RSNRM]: count it, record and report it, decide whether the
+// session goes on (mgcsd.cc:483-527).  Returns 1 when it has finished.
+int ses_cycle_done(pmg_ctx *c) {
+  auto &S = c->ses;
+  const Params &p = c->p;
   S.iters++;
   const double orsnrm = S.rsnrm;
   S.rsnrm = p.istop >= 3 ? std::sqrt(c->hscal[S_RSNRM]) : c->hscal[S_RSNRM];
@@ -850,6 +859,255 @@ int lin_end(pmg_ctx *c) {
   c->relres = S.rsnrm / S.rsden;
   c->ierror = S.rc == PMG_ITMAX ? 1 : 0;
   return S.rc;
+}
+
+// ---- grouped iterations (the serving loop) ----
+// One iteration of several sessions at once: the step of step_body for
+// every context, each kernel launched once for all of them (g_* launchers),
+// so that a level's work for up to GMAX requests costs one launch.  Every
+// context runs exactly the kernels, workgroups and arithmetic of its own
+// step, in its own order: results are bit-identical to solving it alone.
+
+// Contexts whose iterations can run grouped: a residual stopping test, two
+// or more levels, no per-kernel timing.
+bool group_ok(const pmg_ctx *c) {
+  return c->p.istop <= 1 && c->nlev >= 2 && !c->ktime;
+}
+
+// smooth() of level lev for the contexts qs (x = L[lev].x, f[k] its
+// right-hand side).
+template <class T>
+void g_smooth_level(LV<T> *const *L, const std::vector<int> &qs, int lev,
+                    const std::vector<const T *> &f, int nu, int iadj,
+                    bool zero_init, bool bnd_zero, hipStream_t s) {
+  if (qs.empty()) return;
+  if (nu <= 0) {
+    if (zero_init)
+      for (int q : qs) fill<T>(L[q][lev].x, L[q][lev].d.n(), (T)0, s);
+    return;
+  }
+  std::vector<GSmooth<T>> j7, j27;
+  for (size_t k = 0; k < qs.size(); k++) {
+    const LV<T> &l = L[qs[k]][lev];
+    T *x = l.x;
+    if (l.numdia == 7) {
+      if (l.t && l.t != x) {
+        if (zero_init && !bnd_zero) fill<T>(x, l.d.n(), (T)0, s);
+        j7.push_back(GSmooth<T>{l.d, l.ac, l.cc, f[k], x, l.t});
+      } else {
+        if (zero_init) fill<T>(x, l.d.n(), (T)0, s);
+        gs7<T>(l.d, l.ac, l.cc, f[k], x, nu, iadj, s);
+      }
+    } else {
+      j27.push_back(GSmooth<T>{l.d, l.ac, l.cc, f[k], x,
+                               l.t != x ? l.t : nullptr});
+    }
+  }
+  g_gs7_fused<T>(j7.data(), (int)j7.size(), nu, iadj, zero_init && bnd_zero,
+                 s);
+  g_gs27<T>(j27.data(), (int)j27.size(), nu, iadj, zero_init, s);
+}
+
+// vcycle() for n contexts at once (same nu1, nu2): level by level from the
+// top; a context takes part in the levels it has, and its coarsest solve
+// runs once every context has restricted to its own coarsest level.
+template <class T>
+void g_vcycle(pmg_ctx *const *c, int n, LV<T> *const *L,
+              const std::vector<const T *> &frhs, bool fine_zero,
+              double coarse_tol, hipStream_t s) {
+  const int nu1 = c[0]->p.nu1, nu2 = c[0]->p.nu2;
+  int lmax = 0;
+  for (int q = 0; q < n; q++) lmax = std::max(lmax, c[q]->nlev);
+  std::vector<int> all(n);
+  for (int q = 0; q < n; q++) all[q] = q;
+  g_smooth_level<T>(L, all, 0, frhs, nu1, 0, fine_zero, fine_zero, s);
+  {
+    std::vector<GResid<T, T>> j;
+    for (int q = 0; q < n; q++) {
+      const LV<T> &l = L[q][0];
+      j.push_back(GResid<T, T>{l.d, l.numdia, l.ac, l.cc, frhs[q], l.x, l.r,
+                               nullptr, nullptr});
+    }
+    g_resid<T, T>(j.data(), n, s);
+  }
+  for (int l = 1; l < lmax; l++) {  // down
+    std::vector<GRestrict<T>> jr;
+    std::vector<int> mid;
+    std::vector<const T *> fm;
+    for (int q = 0; q < n; q++) {
+      if (c[q]->nlev <= l) continue;
+      jr.push_back(GRestrict<T>{L[q][l - 1].d, L[q][l].d, L[q][l - 1].r,
+                                L[q][l].w0, L[q][l].pc});
+      if (l != c[q]->nlev - 1) {
+        mid.push_back(q);
+        fm.push_back(L[q][l].w0);
+      }
+    }
+    g_restrict<T>(jr.data(), (int)jr.size(), s);
+    g_smooth_level<T>(L, mid, l, fm, nu1, 0, true, false, s);
+    std::vector<GResid<T, T>> j;
+    for (int q : mid) {
+      const LV<T> &v = L[q][l];
+      j.push_back(GResid<T, T>{v.d, v.numdia, v.ac, v.cc, v.w0, v.x, v.r,
+                               nullptr, nullptr});
+    }
+    g_resid<T, T>(j.data(), (int)j.size(), s);
+  }
+  {  // coarsest levels
+    std::vector<GCoarse<T>> jc;
+    for (int q = 0; q < n; q++) {
+      const int lc = c[q]->nlev - 1;
+      const Level &C = c[q]->L[lc];
+      if (c[q]->p.mgsolv == 1) {
+        band_solve<T>(C.d, C.numdia, c[q]->band_rt, c[q]->band_ct,
+                      L[q][lc].w0, L[q][lc].x, s);
+      } else {
+        jc.push_back(GCoarse<T>{C.d, C.numdia, C.ac, C.cc, L[q][lc].w0,
+                                L[q][lc].x});
+      }
+    }
+    g_cg<T>(jc.data(), (int)jc.size(), 100, coarse_tol, s);
+  }
+  for (int l = lmax - 2; l >= 0; l--) {  // up
+    std::vector<int> qs;
+    for (int q = 0; q < n; q++)
+      if (c[q]->nlev - 2 >= l) qs.push_back(q);
+    if (qs.empty()) continue;
+    const int cl = l + 1;
+    std::vector<GMatvec<T>> jm;
+    for (int q : qs) {
+      const LV<T> &v = L[q][cl];
+      jm.push_back(GMatvec<T>{v.d, v.numdia, v.ac, v.cc, v.x, v.w0, v.t,
+                              c[q]->part});
+    }
+    std::vector<unsigned> nb(qs.size());
+    g_matvec_dots_part<T>(jm.data(), (int)jm.size(), nb.data(), s);
+    std::vector<GInterp<T>> ji;
+    std::vector<const T *> fu;
+    for (size_t k = 0; k < qs.size(); k++) {
+      const int q = qs[k];
+      ji.push_back(GInterp<T>{L[q][cl].d, L[q][l].d, L[q][cl].x, L[q][cl].pc,
+                              L[q][l].x, c[q]->part, nb[k]});
+      fu.push_back(l == 0 ? frhs[q] : L[q][l].w0);
+    }
+    g_interp_damped<T>(ji.data(), (int)ji.size(), s);
+    g_smooth_level<T>(L, qs, l, fu, nu2, 1, false, false, s);
+  }
+}
+
+// One iteration (step_body) of the sessions of c[0..n-1], all in mode and
+// with the same nu1, nu2, on stream s; each context's stopping norm goes
+// to its device scalar and to norm[q].
+void group_step(pmg_ctx *const *c, int n, int mode, hipStream_t s,
+                double *norm) {
+  std::vector<LV<double>> v64((size_t)n * MAXLEV);
+  std::vector<LV<float>> v32((size_t)n * MAXLEV);
+  std::vector<LV<double> *> L64(n);
+  std::vector<LV<float> *> L32(n);
+  std::vector<Red> rn(n);
+  for (int q = 0; q < n; q++) {
+    L64[q] = &v64[(size_t)q * MAXLEV];
+    L32[q] = &v32[(size_t)q * MAXLEV];
+    views(c[q], L64[q], L32[q], mode);
+    rn[q] = red_to(c[q], S_RSNRM);
+  }
+  if (mode == PMG_MIXED) {
+    std::vector<const float *> f(n);
+    for (int q = 0; q < n; q++) f[q] = c[q]->L[0].w032;
+    g_vcycle<float>(c, n, L32.data(), f, true, kMixedCoarseTol, s);
+    std::vector<GAddcorr<float>> ja;
+    std::vector<GResid<double, float>> jr;
+    for (int q = 0; q < n; q++) {
+      Level &F = c[q]->L[0];
+      ja.push_back(GAddcorr<float>{F.d, L32[q][0].x, c[q]->u});
+      jr.push_back(GResid<double, float>{F.d, F.numdia, F.ac, F.cc, F.fc,
+                                         c[q]->u, F.w032, &rn[q], norm + q});
+    }
+    g_addcorr<float>(ja.data(), n, s);
+    g_resid<double, float>(jr.data(), n, s);
+  } else {
+    std::vector<const double *> f(n);
+    for (int q = 0; q < n; q++) f[q] = c[q]->L[0].fc;
+    g_vcycle<double>(c, n, L64.data(), f, false, 0.0, s);
+    std::vector<GResid<double, double>> jr;
+    for (int q = 0; q < n; q++) {
+      Level &F = c[q]->L[0];
+      jr.push_back(GResid<double, double>{F.d, F.numdia, F.ac, F.cc, F.fc,
+                                          c[q]->u, nullptr, &rn[q],
+                                          norm + q});
+    }
+    g_resid<double, double>(jr.data(), n, s);
+  }
+}
+
+// Solve the contexts' linear systems together, their iterations grouped
+// (group_step) until each has met its stopping test; a context leaves the
+// group when it finishes.  Blocking.
+int ctx_solve_group(pmg_ctx *const *cs, int n, int mode) {
+  for (int q = 0; q < n; q++)
+    if (!cs[q] || cs[q]->p.mode != 0 || !group_ok(cs[q]) ||
+        cs[q]->p.nu1 != cs[0]->p.nu1 || cs[q]->p.nu2 != cs[0]->p.nu2)
+      return PMG_EBADPARM;
+  for (int q = 0; q < n; q++)
+    for (int r = 0; r < q; r++)
+      if (cs[q] == cs[r]) return PMG_EBADPARM;
+  int rc = PMG_OK;
+  std::vector<pmg_ctx *> act;
+  for (int q = 0; q < n; q++) {
+    const int r = lin_begin(cs[q], mode);
+    if (r) return r;  // contexts started before stay unfinished; no work
+  }
+  for (int q = 0; q < n; q++) {
+    if (lin_poll(cs[q], true))
+      lin_finish(cs[q], nullptr);
+    else
+      act.push_back(cs[q]);
+  }
+  hipStream_t gs = nullptr;
+  hipEvent_t ev = nullptr;
+  double *dn = nullptr, *hn = nullptr;
+  if (hipStreamCreateWithFlags(&gs, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess ||
+      hipMalloc((void **)&dn, sizeof(double) * (n + 1)) != hipSuccess ||
+      hipHostMalloc((void **)&hn, sizeof(double) * (n + 1)) != hipSuccess)
+    rc = PMG_EDEVICE;
+  while (!rc && !act.empty()) {
+    group_step(act.data(), (int)act.size(), mode, gs, dn);
+    hipMemcpyAsync(hn, dn, sizeof(double) * act.size(), hipMemcpyDeviceToHost,
+                   gs);
+    hipEventRecord(ev, gs);
+    if (hipEventSynchronize(ev) != hipSuccess) {
+      rc = PMG_EDEVICE;
+      break;
+    }
+    std::vector<pmg_ctx *> next;
+    for (size_t k = 0; k < act.size(); k++) {
+      pmg_ctx *c = act[k];
+      c->hscal[S_RSNRM] = hn[k];
+      c->ses.pending = false;
+      if (ses_cycle_done(c)) {
+        hipStreamWaitEvent(c->s, ev, 0);
+        lin_finish(c, nullptr);
+      } else {
+        next.push_back(c);
+      }
+    }
+    act.swap(next);
+  }
+  for (pmg_ctx *c : act) {  // only after a device error
+    hipStreamWaitEvent(c->s, ev, 0);
+    lin_finish(c, nullptr);
+  }
+  if (gs) hipStreamSynchronize(gs);
+  for (int q = 0; q < n; q++) {
+    const int r = lin_end(cs[q]);
+    if (!rc && r != PMG_OK && r != PMG_ITMAX) rc = r;
+  }
+  if (ev) hipEventDestroy(ev);
+  if (dn) hipFree(dn);
+  if (hn) hipHostFree(hn);
+  if (gs) hipStreamDestroy(gs);
+  return rc;
 }
 
 int ctx_solve(pmg_ctx *c, int mode) {
@@ -1733,6 +1991,12 @@ int pmg_ctx_solve(pmg_ctx *ctx, int mode) {
   if (!ctx || (mode != PMG_FP64 && mode != PMG_MIXED)) return PMG_EBADPARM;
   if (ctx->p.mode == 1) return ctx_newton(ctx, mode);
   return ctx_solve(ctx, mode);
+}
+
+int pmg_ctx_solve_group(pmg_ctx *const *ctx, int n, int mode) {
+  if (!ctx || n < 1 || n > 64 || (mode != PMG_FP64 && mode != PMG_MIXED))
+    return PMG_EBADPARM;
+  return ctx_solve_group(ctx, n, mode);
 }
 
 int pmg_ctx_get_u(pmg_ctx *ctx, double *u) {
