@@ -43,6 +43,9 @@ def parse():
     ap.add_argument("--batch", type=int, default=100, help="requests per step")
     ap.add_argument("--serve-active", type=int, default=8,
                     help="solves in flight at once in the serving loop")
+    ap.add_argument("--serve-groups", type=int, default=2,
+                    help="also measure the stream's capacity with grouped iterations in "
+                         "this many groups (0: skip)")
     ap.add_argument("--single-steps", type=int, default=10,
                     help="timed solves of each single-solve leg (161^3, 289^3)")
     ap.add_argument("--no-serve", action="store_true",
@@ -238,7 +241,7 @@ KEEP = ("solves_per_s", "p50_ms", "p99_ms", "max_concurrent", "mean_iters", "max
         "wall_s")
 
 
-def serving_leg(mg, steps, warmup, batch, active):
+def serving_leg(mg, steps, warmup, batch, active, groups=2):
     """The headline: the stream's --steps batches submitted at once, mixed
     precision (timed: `value`) and FP64 (capacity beside it); then both
     precisions under Poisson arrivals at 0.8x the FP64 capacity for
@@ -257,6 +260,17 @@ def serving_leg(mg, steps, warmup, batch, active):
     mixed = serve_stream(mg, srv, reqs, mg.PMG_MIXED, rate)
     fp64 = serve_stream(mg, srv, reqs, mg.PMG_FP64, rate)
     srv.close()
+    # the same stream with grouped iterations (two groups, each kernel of an
+    # iteration launched once for up to 8 requests)
+    grouped = {}
+    if groups:
+        gsrv = mg.Server(budget, active, groups=groups)
+        for mode in (mg.PMG_FP64, mg.PMG_MIXED):
+            serve_stream(mg, gsrv, warm, mode, None)
+        for key, mode in (("fp64", mg.PMG_FP64), ("mixed", mg.PMG_MIXED)):
+            st = serve_stream(mg, gsrv, reqs, mode, None)
+            grouped[key] = {k: st[k] for k in KEEP}
+        gsrv.close()
     return {
         "requests": len(reqs),
         "arrival_rate_per_s": rate,
@@ -266,6 +280,7 @@ def serving_leg(mg, steps, warmup, batch, active):
         "capacity": {"mixed": {k: capmx[k] for k in KEEP}, "fp64": {k: cap64[k] for k in KEEP}},
         "at_rate": {"mixed": {k: mixed[k] for k in KEEP}, "fp64": {k: fp64[k] for k in KEEP}},
         "mixed_vs_fp64_capacity": capmx["solves_per_s"] / cap64["solves_per_s"],
+        "grouped_capacity": dict(groups=groups, **grouped) if groups else None,
         "all_ok": capmx["all_ok"] and cap64["all_ok"] and mixed["all_ok"] and fp64["all_ok"],
     }, capmx
 
@@ -472,7 +487,7 @@ def main():
     serving, wall, value = None, None, None
     if not args.no_serve:
         serving, capmx = serving_leg(mg, args.steps, args.warmup, args.batch,
-                                     args.serve_active)
+                                     args.serve_active, args.serve_groups)
         wall = capmx["wall_s"]
         value = args.steps * args.batch / wall
 

@@ -353,6 +353,13 @@ typedef struct pmg_server pmg_server;
 /* Start a server on the current GPU: at most max_active solves in flight,
  * contexts within mem_budget bytes of HBM (0: 90% of the free memory). */
 int pmg_server_create(pmg_server **out, size_t mem_budget, int max_active);
+/* The same with grouped iterations: the V-cycle iterations of the solves in
+ * flight run in `groups` groups (0: each solve on its own stream, as
+ * pmg_server_create), a group launching each kernel of an iteration once
+ * for up to 8 solves; solves join and leave a group between iterations.
+ * Results are bit-identical either way (pmg_ctx_solve_group). */
+int pmg_server_create_grouped(pmg_server **out, size_t mem_budget,
+                              int max_active, int groups);
 /* Queue the solve of an atom list (natoms x {x, y, z, radius, charge},
  * copied) on the cubic grid iparm/rparm describe (linear: iparm(16) = 0),
  * spacing h, 1:1 salt of ionic_strength mol/L, mode PMG_FP64 or PMG_MIXED.

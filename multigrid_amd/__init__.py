@@ -122,6 +122,7 @@ def _declare(L):
         "pmg_server_stats": (i, [vp, ctypes.POINTER(ctypes.c_longlong), dp, dp, dp,
                                  ctypes.POINTER(ctypes.c_size_t), ip]),
         "pmg_server_reset_stats": (i, [vp]),
+        "pmg_server_create_grouped": (i, [ctypes.POINTER(vp), ctypes.c_size_t, i, i]),
         "pmg_server_destroy": (None, [vp]),
         "pmg_version": (ctypes.c_char_p, []),
     }
@@ -383,10 +384,13 @@ class Context:
 class Server:
     """Continuous-batching PB solve server (pmg_server_*)."""
 
-    def __init__(self, mem_budget=0, max_active=8):
+    def __init__(self, mem_budget=0, max_active=8, groups=0):
+        """groups > 0: the solves' iterations run grouped
+        (pmg_server_create_grouped); 0: each on its own stream."""
         self._h = ctypes.c_void_p()
-        _check(lib().pmg_server_create(ctypes.byref(self._h), mem_budget, max_active),
-               "pmg_server_create")
+        _check(lib().pmg_server_create_grouped(ctypes.byref(self._h), mem_budget,
+                                               max_active, int(groups)),
+               "pmg_server_create_grouped")
         self._out = {}
 
     def submit(self, iparm, rparm, mode, h, ionic_strength, atoms, want_u=True):

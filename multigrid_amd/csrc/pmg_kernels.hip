@@ -2989,25 +2989,29 @@ void g_cg(const GCoarse<T> *j, int n, int itmax, double errtol,
       cg_single<T>(j[q].d, j[q].numdia, j[q].ac, j[q].cc, j[q].f, j[q].x,
                    itmax, errtol, s);
   }
-  // one launch per coarsest shape: the workgroup is as wide as the shape's
-  // own launch, and its LDS holds the shape's grid
+  // One launch for every coarsest shape: the workgroup is as wide, and its
+  // LDS as large, as the largest shape needs.  k_cg_rows gives the same
+  // bits at any width: threads past a shape's rows hold no unknowns and add
+  // exact zeros to wavefront slots the reductions sum in a fixed order
+  // (cg_allsum), whether those wavefronts exist or not.
   g_split<A_cg<T>>(
-      (int)ok.size(),
-      [&](int q) {
-        const Dim &d = j[ok[q]].d;
-        return (long long)d.nx * 1000000 + d.ny * 1000 + d.nz;
-      },
+      (int)ok.size(), [&](int) { return 0; },
       [&](int q) {
         const GCoarse<T> &J = j[ok[q]];
         return A_cg<T>{J.d.nx, J.d.ny, J.d.nz, J.ac, J.cc, J.f, J.x, itmax,
                        errtol};
       },
-      [&](const GTab<A_cg<T>> &t, int m, long long) {
-        const Dim d{t.a[0].nx, t.a[0].ny, t.a[0].nz};
-        const int nrow =
-            (int)((d.nx - 2 + CG_R - 1) / CG_R) * (d.ny - 2) * (d.nz - 2);
-        const int nt = (nrow + 63) / 64 * 64;
-        const size_t lds = d.n() * sizeof(double) + (CG_R + 2) * sizeof(double);
+      [&](const GTab<A_cg<T>> &t, int m, int) {
+        int nt = 64;
+        size_t lds = 0;
+        for (int q = 0; q < m; q++) {
+          const Dim d{t.a[q].nx, t.a[q].ny, t.a[q].nz};
+          const int nrow =
+              (int)((d.nx - 2 + CG_R - 1) / CG_R) * (d.ny - 2) * (d.nz - 2);
+          nt = std::max(nt, (nrow + 63) / 64 * 64);
+          lds = std::max(lds, d.n() * sizeof(double) +
+                                  (CG_R + 2) * sizeof(double));
+        }
         if (errtol == 0.0)
           kb_cg_rows<T, true><<<dim3(1, m), nt, lds, s>>>(t);
         else

@@ -61,6 +61,19 @@ struct Request {
   pmg_ctx *ctx = nullptr;
   size_t bytes = 0;  // budget charged for a newly created context
   int passed = 0;    // later requests admitted while this one did not fit
+  bool grouped = false;  // iterating in (or waiting for) a group
+};
+
+// Grouped iterations: the iterations of up to GMAX sessions run together
+// on the group's stream (group_step), one launch per kernel for all of
+// them; members join and leave between iterations.
+struct Group {
+  hipStream_t s = nullptr;
+  hipEvent_t ev = nullptr;
+  double *dn = nullptr, *hn = nullptr;  // stopping norms (device, pinned)
+  std::vector<Request *> mem;
+  int mode = PMG_MIXED;
+  bool busy = false;
 };
 
 // Contexts are interchangeable when everything their buffers and captured
@@ -86,6 +99,7 @@ struct pmg_server {
   int device = 0;
   size_t budget = 0, used = 0;
   int max_active = 1;
+  int ngroups = 0;  // 0: every solve iterates on its own stream
   std::mutex mu;
   std::condition_variable cv_work, cv_done, cv_copy;
   std::deque<Request *> queue;
@@ -167,8 +181,71 @@ void copier_main(pmg_server *S) {
   }
 }
 
+// A group's iteration has landed: every member takes its stopping norm;
+// the finished ones restore their boundary values on their own streams,
+// after the iteration, and leave the group.
+void group_land(Group &g) {
+  std::vector<Request *> keep;
+  for (size_t k = 0; k < g.mem.size(); k++) {
+    Request *r = g.mem[k];
+    pmg_ctx *c = r->ctx;
+    c->hscal[1] = g.hn[k];  // S_RSNRM
+    c->ses.pending = false;
+    if (ses_cycle_done(c)) {
+      hipStreamWaitEvent(c->s, g.ev, 0);
+      lin_finish(c, r->u_out ? c->u_pin : nullptr);
+      r->grouped = false;
+    } else {
+      keep.push_back(r);
+    }
+  }
+  g.mem.swap(keep);
+  g.busy = false;
+}
+
+// Start the group's next iteration with the members it kept and ready
+// sessions of the same mode and smoothing counts (at most GMAX).
+void group_launch(Group &g, std::deque<Request *> &ready) {
+  for (auto it = ready.begin();
+       it != ready.end() && (int)g.mem.size() < pmg::GMAX;) {
+    Request *r = *it;
+    const Params &p0 = g.mem.empty() ? r->ctx->p : g.mem[0]->ctx->p;
+    const int m0 = g.mem.empty() ? r->mode : g.mode;
+    if (r->mode == m0 && r->ctx->p.nu1 == p0.nu1 && r->ctx->p.nu2 == p0.nu2) {
+      g.mode = m0;
+      g.mem.push_back(r);
+      it = ready.erase(it);
+    } else {
+      ++it;
+    }
+  }
+  if (g.mem.empty()) return;
+  std::vector<pmg_ctx *> cs;
+  for (Request *r : g.mem) {
+    cs.push_back(r->ctx);
+    r->ctx->ses.pending = true;
+  }
+  group_step(cs.data(), (int)cs.size(), g.mode, g.s, g.dn);
+  hipMemcpyAsync(g.hn, g.dn, sizeof(double) * cs.size(),
+                 hipMemcpyDeviceToHost, g.s);
+  hipEventRecord(g.ev, g.s);
+  g.busy = true;
+}
+
 void worker_main(pmg_server *S) {
   hipSetDevice(S->device);
+  std::vector<Group> groups(S->ngroups);
+  for (Group &g : groups) {
+    if (hipStreamCreateWithFlags(&g.s, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&g.ev, hipEventDisableTiming) != hipSuccess ||
+        hipMalloc((void **)&g.dn, sizeof(double) * pmg::GMAX) != hipSuccess ||
+        hipHostMalloc((void **)&g.hn, sizeof(double) * pmg::GMAX) !=
+            hipSuccess) {
+      groups.clear();  // iterate every solve on its own stream instead
+      break;
+    }
+  }
+  std::deque<Request *> ready;  // sessions waiting for a group
   std::vector<Request *> active;
   std::unique_lock<std::mutex> lk(S->mu);
   for (;;) {
@@ -202,6 +279,15 @@ void worker_main(pmg_server *S) {
         S->in_flight == 0) {
       S->worker_done = true;
       S->cv_copy.notify_all();
+      lk.unlock();
+      for (Group &g : groups) {
+        if (g.s) hipStreamSynchronize(g.s);
+        if (g.ev) hipEventDestroy(g.ev);
+        if (g.dn) hipFree(g.dn);
+        if (g.hn) hipHostFree(g.hn);
+        if (g.s) hipStreamDestroy(g.s);
+      }
+      lk.lock();
       break;
     }
     lk.unlock();
@@ -214,6 +300,9 @@ void worker_main(pmg_server *S) {
       if (!r->ctx) {
         rc = ctx_new(&r->ctx, r->p);
         fresh = rc == PMG_OK;
+        // its iterations will run in groups: no per-context graph to capture
+        if (fresh && !groups.empty() && group_ok(r->ctx))
+          r->ctx->graphs = false;
       } else {
         r->ctx->p = r->p;  // stopping parameters of this request
       }
@@ -263,16 +352,32 @@ void worker_main(pmg_server *S) {
       }
       lk.unlock();
     }
+    // grouped iterations that have landed
+    for (Group &g : groups) {
+      if (g.busy && hipEventQuery(g.ev) != hipErrorNotReady) {
+        group_land(g);
+        progressed = true;
+      }
+    }
     // advance every session by what has landed
     for (size_t q = 0; q < active.size();) {
       Request *r = active[q];
       pmg_ctx *c = r->ctx;
+      if (r->grouped) {
+        q++;
+        continue;
+      }
       if (c->ses.phase < 2) {
         if (lin_poll(c, false)) {
           lin_finish(c, r->u_out ? c->u_pin : nullptr);
           progressed = true;
         } else if (!c->ses.pending) {
-          lin_enqueue(c);
+          if (!groups.empty() && group_ok(c)) {
+            r->grouped = true;  // its iterations run in a group from now
+            ready.push_back(r);
+          } else {
+            lin_enqueue(c);
+          }
           progressed = true;
         }
         q++;
@@ -301,6 +406,13 @@ void worker_main(pmg_server *S) {
       }
       lk.unlock();
     }
+    // the next iteration of every idle group, with the sessions ready
+    for (Group &g : groups) {
+      if (!g.busy && (!g.mem.empty() || !ready.empty())) {
+        group_launch(g, ready);
+        progressed = progressed || g.busy;
+      }
+    }
     lk.lock();
     if (!progressed) {
       // nothing landed: wait briefly (new submissions wake the worker)
@@ -314,7 +426,13 @@ void worker_main(pmg_server *S) {
 extern "C" {
 
 int pmg_server_create(pmg_server **out, size_t mem_budget, int max_active) {
-  if (!out || max_active < 1) return PMG_EBADPARM;
+  return pmg_server_create_grouped(out, mem_budget, max_active, 0);
+}
+
+int pmg_server_create_grouped(pmg_server **out, size_t mem_budget,
+                              int max_active, int groups) {
+  if (!out || max_active < 1 || groups < 0 || groups > 16)
+    return PMG_EBADPARM;
   *out = nullptr;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return PMG_EDEVICE;
@@ -331,6 +449,7 @@ int pmg_server_create(pmg_server **out, size_t mem_budget, int max_active) {
   }
   S->budget = mem_budget;
   S->max_active = max_active;
+  S->ngroups = groups;
   S->worker = std::thread(worker_main, S);
   S->copier = std::thread(copier_main, S);
   *out = S;

@@ -48,6 +48,30 @@ def test_same_alone_and_in_mix(mg):
         assert np.array_equal(a["u"], b["u"])
 
 
+@pytest.mark.parametrize("groups", [1, 2])
+def test_grouped_server_same_results(mg, groups):
+    """Grouped iterations (pmg_server_create_grouped): every request of the
+    uneven mix, both precisions in flight at once, gives the bits, the
+    iteration count and the residual of the ungrouped server."""
+    reqs = _requests(mg) * 2
+    res = {}
+    for g in (0, groups):
+        print(f"serve: {len(reqs)} requests, groups={g}", flush=True)
+        srv = mg.Server(0, 8, groups=g)
+        tickets = []
+        for (n, nlev, mode, atoms) in reqs:
+            iparm, rparm = mg.make_params(n, nlev)
+            tickets.append(srv.submit(iparm, rparm, mode, 0.5, 0.15, atoms))
+        res[g] = [srv.wait(t) for t in tickets]
+        st = srv.stats()
+        srv.close()
+        assert st["completed"] == len(reqs)
+    for a, b in zip(res[0], res[groups]):
+        assert a["rc"] == b["rc"] == 0
+        assert a["iters"] == b["iters"] and a["relres"] == b["relres"] <= 1e-6
+        assert np.array_equal(a["u"], b["u"])
+
+
 def test_matches_context_solve_and_oracle(mg, orc):
     """A served solve equals the resident-context solve of the same molecule
     bit for bit, and (mixed precision) the FP64 oracle to 1e-6."""
