@@ -193,11 +193,12 @@ void coarse_solve(pmg_ctx *c, int l, const double *ac64, const double *cc64,
 // Relative tolerance of the FP64 coarsest CG inside the FP32 V-cycle of the
 // linear mixed-precision solve.  The cycle is a correction step of the FP64
 // defect correction, whose stopping test is the FP64 residual; its coarse
-// correction is scaled by the Hackbusch-Reusken factor on the way up, and a
-// coarsest solve to 1e-3 gives the same contraction per cycle as 100
-// iterations (same outer iteration count on the benchmark workloads) in
-// about a tenth of the iterations.
-constexpr double kMixedCoarseTol = 1e-3;
+// correction is scaled by the Hackbusch-Reusken factor on the way up.  On
+// the benchmark hierarchies (coarsest 7^3 to 11^3) the residual history of
+// the cycle is the same to three digits whether the coarsest level is
+// solved to 1e-1, 1e-2, 1e-3 or by 100 iterations; 1e-2 takes 7 to 11
+// iterations.
+constexpr double kMixedCoarseTol = 1e-2;
 
 // The coarsest level fits the coarse solver of p.
 bool coarse_fits(pmg_ctx *c) {
