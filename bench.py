@@ -83,12 +83,12 @@ def timed_steps(ctx, mol, mode, steps, warmup):
 
 def sweep_bytes(pts, sweeps, nu=(2, 2)):
     """Algorithmic bytes of `sweeps` timed FP32 fine sweeps of a mixed solve
-    (V(nu1, nu2) cycles).  A sweep reads oC, cc, fc, oE, oN, uC and x and
-    writes x: 8 arrays x 4 B per interior point.  The first pre-smoothing
-    sweep of each correction cycle starts from x = 0 and does not read x:
-    7 arrays."""
+    (V(nu1, nu2) cycles).  A sweep reads the diagonal oC + cc (precomputed
+    once per setup), fc, oE, oN, uC and x and writes x: 7 arrays x 4 B per
+    interior point.  The first pre-smoothing sweep of each correction cycle
+    starts from x = 0 and does not read x: 6 arrays."""
     zero_start = sweeps // (nu[0] + nu[1])
-    return 4 * pts * (8 * sweeps - zero_start)
+    return 4 * pts * (7 * sweeps - zero_start)
 
 
 def measured_traffic(n):

@@ -23,6 +23,11 @@ namespace drv {
 
 constexpr int MAXLEV = 16;
 
+// device scalar slots of a context (pmg_ctx::scal, mirrored in hscal):
+// stopping-test denominator and norm, xdamp, the band factor's Vdpbfa info
+// (mgsolv 1), a temporary
+enum { S_RSDEN = 0, S_RSNRM = 1, S_XDAMP = 2, S_BINFO = 3, S_TMP = 4 };
+
 struct Params {
   int nrwk, niwk, nx, ny, nz, nlev, nu1, nu2, mgkey, itmax, istop, iinfo,
       ipkey, mode, mgprol, mgcoar, mgdisc, mgsmoo, mgsolv, iperf;
@@ -40,6 +45,10 @@ struct Level {
   // Newton iterate of a coarse level (nested iteration)
   double *cpr = nullptr, *rhs = nullptr, *xn = nullptr;
   float *cpr32 = nullptr;
+  // 7-point (fine) level: the diagonal oC + cc, precomputed for the sweeps
+  // and residuals (FP64; FP32 as the sum of the FP32 copies)
+  double *dg = nullptr;
+  float *dg32 = nullptr;
 };
 
 template <class T>
@@ -48,6 +57,7 @@ struct LV {
   int numdia;
   const T *ac, *cc, *pc;
   T *x, *w0, *r, *t;
+  const T *dg;  // oC + cc when it holds for ac and cc above, else null
 };
 
 
@@ -179,7 +189,7 @@ int lin_finished(pmg_ctx *c, bool wait);
 // Statistics of the finished session; returns its status code.
 int lin_end(pmg_ctx *c);
 // An iteration of the session has landed with its stopping scalar in
-// c->hscal[S_RSNRM] (slot 1): count, record, decide.  1: finished.
+// c->hscal[S_RSNRM]: count, record, decide.  1: finished.
 int ses_cycle_done(pmg_ctx *c);
 
 // Grouped iterations (the serving loop): contexts whose sessions may run

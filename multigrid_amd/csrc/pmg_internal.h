@@ -44,10 +44,16 @@ void gs7(const Dim &d, const T *ac, const T *cc, const T *fc, T *x,
 // ping-pong; tmp must share x's boundary values).  Same result as gs7.
 // zero_init: the sweeps start from x = 0 (Vazeros); x's interior is not
 // read, its boundary entries must already be zero.
+// dg (may be null): the diagonal oC + cc of the operator, precomputed
+// (diag_sum); the sweep then reads it instead of oC and cc.
 template <class T>
 void gs7_fused(const Dim &d, const T *ac, const T *cc, const T *fc, T *x,
                T *tmp, int nsweeps, int iadjoint, hipStream_t s,
-               bool zero_init = false);
+               bool zero_init = false, const T *dg = nullptr);
+// dg[p] = ac[p] + cc[p] for the n points of a 7-point level (ac: its oC
+// plane), the diagonal the 7-point sweeps and residuals use.
+template <class T>
+void diag_sum(const T *ac, const T *cc, T *dg, size_t n, hipStream_t s);
 // zero_init: x starts from 0 (the caller need not clear it).  tmp (may be
 // null): an array of the level's size with the same boundary values as x,
 // the second buffer of the two-launch sweeps; its interior is overwritten.
@@ -60,9 +66,11 @@ void gs27(const Dim &d, const T *ac, const T *cc, const T *fc, T *x,
 // TO is the output type (FP64 operator may write an FP32 residual for the
 // mixed-precision inner solve); r may be null.  If red != nullptr the L1
 // norm of the residual (in double) is written to red->out[0].
+// dg (7-point, may be null): the precomputed diagonal oC + cc.
 template <class T, class TO>
 void resid(const Dim &d, int numdia, const T *ac, const T *cc, const T *fc,
-           const T *x, TO *r, const Red *red, hipStream_t s);
+           const T *x, TO *r, const Red *red, hipStream_t s,
+           const T *dg = nullptr);
 // Number of block partials resid() writes (all of them, every call) when
 // given a Red.
 size_t resid_partials(const Dim &d);
@@ -238,6 +246,7 @@ struct GSmooth {  // one level's smoothing (gs7_fused / gs27 arguments)
   Dim d;
   const T *ac, *cc, *f;
   T *x, *tmp;
+  const T *dg = nullptr;
 };
 template <class T>
 void g_gs7_fused(const GSmooth<T> *j, int n, int nsweeps, int iadjoint,
@@ -253,6 +262,7 @@ struct GResid {  // resid(); out2, when non-null, also receives the norm
   TO *r;
   const Red *red;
   double *out2;
+  const T *dg = nullptr;
 };
 template <class T, class TO>
 void g_resid(const GResid<T, TO> *j, int n, hipStream_t s);

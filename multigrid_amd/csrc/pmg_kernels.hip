@@ -299,11 +299,15 @@ struct SwCo {  // own-point operator values of one plane
 };
 
 // ZX: xin is zero everywhere (the sweep after Vazeros) and is not read.
-template <class T, bool ZX>
+// DG: the diagonal oC + cc comes precomputed in dg (the same sum, so the
+// same bits; one array less to read); ac and cc then serve only for the
+// couplings.
+template <class T, bool ZX, bool DG>
 __device__ __forceinline__ void gs7_sweep_body(
     int nx, int ny, int nz, const T *__restrict__ ac, const T *__restrict__ cc,
     const T *__restrict__ fc, const T *__restrict__ xin, T *__restrict__ xout,
-    int c1, int kc, int gx, int gy, int gz, unsigned bid) {
+    int c1, int kc, int gx, int gy, int gz, unsigned bid,
+    const T *__restrict__ dg) {
   __shared__ T SX[2][SW_TY][SW_TX], SE[3][SW_TY][SW_TX], SN[3][SW_TY][SW_TX],
       SM[3][SW_TY][SW_TX];
   // XCD-aware tile order: workgroups are dispatched round-robin over the 8
@@ -330,8 +334,12 @@ __device__ __forceinline__ void gs7_sweep_body(
   auto at = [&](int q) { return p0 + (size_t)(min(max(q, 1), nz) - 1) * sz; };
   auto load = [&](int q, SwCo<T> &P) {
     const size_t p = at(q);
-    P.oC = ac[p], P.cc = cc[p], P.fc = fc[p], P.oE = oE[p], P.oN = oN[p],
-    P.uC = uC[p];
+    if constexpr (DG) {
+      P.oC = dg[p], P.cc = (T)0;  // oC + cc == dg (dg >= 0)
+    } else {
+      P.oC = ac[p], P.cc = cc[p];
+    }
+    P.fc = fc[p], P.oE = oE[p], P.oN = oN[p], P.uC = uC[p];
   };
   auto xat = [&](int q) -> T { return ZX ? (T)0 : xin[at(q)]; };
   auto first = [&](int q) {
@@ -437,13 +445,13 @@ __device__ __forceinline__ void gs7_sweep_body(
   if (k + 3 <= ke) step(k + 3, I3{});
 }
 
-template <class T, bool ZX>
+template <class T, bool ZX, bool DG>
 __global__ __launch_bounds__(SW_TX *SW_TY) void k_gs7_sweep(
     int nx, int ny, int nz, const T *__restrict__ ac, const T *__restrict__ cc,
     const T *__restrict__ fc, const T *__restrict__ xin, T *__restrict__ xout,
-    int c1, int kc, int gx, int gy, int gz) {
-  gs7_sweep_body<T, ZX>(nx, ny, nz, ac, cc, fc, xin, xout, c1, kc, gx, gy,
-                        gz, blockIdx.x);
+    int c1, int kc, int gx, int gy, int gz, const T *__restrict__ dg) {
+  gs7_sweep_body<T, ZX, DG>(nx, ny, nz, ac, cc, fc, xin, xout, c1, kc, gx, gy,
+                            gz, blockIdx.x, dg);
 }
 
 // Compute units of the current device (cached per device).
@@ -469,7 +477,7 @@ static int device_cus() {
 template <class T>
 void gs7_fused(const Dim &d, const T *ac, const T *cc, const T *fc, T *x,
                T *tmp, int nsweeps, int iadjoint, hipStream_t s,
-               bool zero_init) {
+               bool zero_init, const T *dg) {
   if (d.nx < 3 || d.ny < 3 || d.nz < 3 || nsweeps <= 0) return;
   const int gx = cdiv(d.nx - 2, SW_TX - 4), gy = cdiv(d.ny - 2, SW_TY - 4);
   const int zb = std::max(1, device_cus() / (gx * gy));
@@ -480,12 +488,17 @@ void gs7_fused(const Dim &d, const T *ac, const T *cc, const T *fc, T *x,
   const int c1 = iadjoint ? 1 : 0;
   T *src = x, *dst = tmp;
   for (int it = 0; it < nsweeps; it++) {
-    if (zero_init && it == 0)
-      k_gs7_sweep<T, true><<<nblk, b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, fc,
-                                              src, dst, c1, kc, gx, gy, gz);
-    else
-      k_gs7_sweep<T, false><<<nblk, b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, fc,
-                                               src, dst, c1, kc, gx, gy, gz);
+#define SWEEP(ZX, DG)                                                         \
+  k_gs7_sweep<T, ZX, DG><<<nblk, b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, fc, src, \
+                                            dst, c1, kc, gx, gy, gz, dg)
+    if (zero_init && it == 0) {
+      if (dg) SWEEP(true, true);
+      else SWEEP(true, false);
+    } else {
+      if (dg) SWEEP(false, true);
+      else SWEEP(false, false);
+    }
+#undef SWEEP
     T *t = src;
     src = dst;
     dst = t;
@@ -904,12 +917,14 @@ void gs27(const Dim &d, const T *ac, const T *cc, const T *fc, T *x,
 // ---------------------------------------------------------------------------
 // Residual (Vmresid7_1s / Vmresid27_1s) with optional L1 partial sums
 // ---------------------------------------------------------------------------
-template <class T, class TO, int ND>
+// DG (7-point only): the diagonal oC + cc comes precomputed in dg (same
+// bits, one array less to read).
+template <class T, class TO, int ND, bool DG = false>
 __device__ __forceinline__ void resid_body(
     int nx, int ny, int nz, const T *__restrict__ ac,
     const T *__restrict__ cc, const T *__restrict__ fc,
     const T *__restrict__ x, TO *__restrict__ r, Red rd, int do_red, int kc,
-    XG xg, unsigned bid) {
+    XG xg, unsigned bid, const T *__restrict__ dg = nullptr) {
   __shared__ double red[4];
   const size_t n = (size_t)nx * ny * nz, sy = nx, sz = (size_t)nx * ny;
   int bx, by, bz;
@@ -926,10 +941,10 @@ __device__ __forceinline__ void resid_body(
       T v;
       if (ND == 7) {
         const T *oE = ac + n, *oN = ac + 2 * n, *uC = ac + 3 * n;
+        const T dgp = DG ? dg[p] : ac[p] + cc[p];
         v = fc[p] + oN[p] * x[p + sy] + oN[p - sy] * x[p - sy] +
             oE[p] * x[p + 1] + oE[p - 1] * x[p - 1] +
-            uC[p - sz] * x[p - sz] + uC[p] * x[p + sz] -
-            (ac[p] + cc[p]) * x[p];
+            uC[p - sz] * x[p - sz] + uC[p] * x[p + sz] - dgp * x[p];
       } else {
         double sO, sU, sD;
         sum27<T, T>(ac, n, sy, sz, p, x, sO, sU, sD);
@@ -946,14 +961,14 @@ __device__ __forceinline__ void resid_body(
   }
 }
 
-template <class T, class TO, int ND>
+template <class T, class TO, int ND, bool DG>
 __global__ __launch_bounds__(256) void k_resid(
     int nx, int ny, int nz, const T *__restrict__ ac,
     const T *__restrict__ cc, const T *__restrict__ fc,
     const T *__restrict__ x, TO *__restrict__ r, Red rd, int do_red, int kc,
-    XG xg) {
-  resid_body<T, TO, ND>(nx, ny, nz, ac, cc, fc, x, r, rd, do_red, kc, xg,
-                        blockIdx.x);
+    XG xg, const T *__restrict__ dg) {
+  resid_body<T, TO, ND, DG>(nx, ny, nz, ac, cc, fc, x, r, rd, do_red, kc, xg,
+                            blockIdx.x, dg);
 }
 
 size_t resid_partials(const Dim &d) {
@@ -964,17 +979,22 @@ size_t resid_partials(const Dim &d) {
 
 template <class T, class TO>
 void resid(const Dim &d, int numdia, const T *ac, const T *cc, const T *fc,
-           const T *x, TO *r, const Red *red, hipStream_t s) {
+           const T *x, TO *r, const Red *red, hipStream_t s, const T *dg) {
   const int kc = kchunk(d, red ? 40 : 80);
   dim3 g(cdiv(d.nx - 2, BX), cdiv(d.ny - 2, BY), cdiv(d.nz - 2, kc));
   dim3 b(BX, BY);
   const Red rd = red ? *red : Red{};
-  if (numdia == 7)
-    k_resid<T, TO, 7><<<xcd_blocks(g), b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, fc, x, r, rd,
-                                      red ? 1 : 0, kc, xg_of(g));
+  if (numdia == 7 && dg)
+    k_resid<T, TO, 7, true><<<xcd_blocks(g), b, 0, s>>>(
+        d.nx, d.ny, d.nz, ac, cc, fc, x, r, rd, red ? 1 : 0, kc, xg_of(g), dg);
+  else if (numdia == 7)
+    k_resid<T, TO, 7, false><<<xcd_blocks(g), b, 0, s>>>(
+        d.nx, d.ny, d.nz, ac, cc, fc, x, r, rd, red ? 1 : 0, kc, xg_of(g),
+        nullptr);
   else
-    k_resid<T, TO, 27><<<xcd_blocks(g), b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, fc, x, r, rd,
-                                       red ? 1 : 0, kc, xg_of(g));
+    k_resid<T, TO, 27, false><<<xcd_blocks(g), b, 0, s>>>(
+        d.nx, d.ny, d.nz, ac, cc, fc, x, r, rd, red ? 1 : 0, kc, xg_of(g),
+        nullptr);
   if (red) red_final<1, 0>(rd, dim3(xcd_blocks(g)), s);
   check_launch("resid");
 }
@@ -2030,6 +2050,20 @@ void pcg_p(const Dim &d, double *p, const double *sv, const double *rsn,
 // BLAS-1 helpers
 // ---------------------------------------------------------------------------
 template <class T>
+__global__ void k_diag(const T *__restrict__ ac, const T *__restrict__ cc,
+                       T *__restrict__ dg, size_t n) {
+  for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n;
+       p += (size_t)gridDim.x * blockDim.x)
+    dg[p] = ac[p] + cc[p];
+}
+template <class T>
+void diag_sum(const T *ac, const T *cc, T *dg, size_t n, hipStream_t s) {
+  const unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+  k_diag<T><<<nb, 256, 0, s>>>(ac, cc, dg, n);
+  check_launch("diag_sum");
+}
+
+template <class T>
 __global__ void k_fill(T *x, size_t n, T v) {
   for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n;
        t += (size_t)gridDim.x * blockDim.x)
@@ -2581,14 +2615,15 @@ struct A_gs7 {
   T *xout;
   int c1, kc, gx, gy, gz;
   unsigned nblk;
+  const T *dg;
 };
-template <class T, bool ZX>
+template <class T, bool ZX, bool DG>
 __global__ __launch_bounds__(SW_TX *SW_TY) void kb_gs7_sweep(
     const GTab<A_gs7<T>> t) {
   const A_gs7<T> &a = t.a[blockIdx.y];
   if (blockIdx.x >= a.nblk) return;
-  gs7_sweep_body<T, ZX>(a.nx, a.ny, a.nz, a.ac, a.cc, a.fc, a.xin, a.xout,
-                        a.c1, a.kc, a.gx, a.gy, a.gz, blockIdx.x);
+  gs7_sweep_body<T, ZX, DG>(a.nx, a.ny, a.nz, a.ac, a.cc, a.fc, a.xin, a.xout,
+                            a.c1, a.kc, a.gx, a.gy, a.gz, blockIdx.x, a.dg);
 }
 
 template <class T>
@@ -2616,21 +2651,25 @@ void g_gs7_fused(const GSmooth<T> *j, int n, int nsweeps, int iadjoint,
   for (int it = 0; it < nsweeps; it++) {
     const bool zx = zero_init && it == 0;
     g_split<A_gs7<T>>(
-        n, [&](int) { return 0; },
+        n, [&](int q) { return j[q].dg ? 1 : 0; },
         [&](int q) {
           const Dim &d = j[q].d;
-          return A_gs7<T>{d.nx,    d.ny,     d.nz,     j[q].ac, j[q].cc,
-                          j[q].f,  g[q].src, g[q].dst, c1,      g[q].kc,
-                          g[q].gx, g[q].gy,  g[q].gz,  g[q].nblk};
+          return A_gs7<T>{d.nx,    d.ny,     d.nz,     j[q].ac,   j[q].cc,
+                          j[q].f,  g[q].src, g[q].dst, c1,        g[q].kc,
+                          g[q].gx, g[q].gy,  g[q].gz,  g[q].nblk, j[q].dg};
         },
-        [&](const GTab<A_gs7<T>> &t, int m, int) {
+        [&](const GTab<A_gs7<T>> &t, int m, int dgk) {
           unsigned mx = 0;
           for (int q = 0; q < m; q++) mx = std::max(mx, t.a[q].nblk);
           dim3 grid(mx, m), b(SW_TX, SW_TY);
-          if (zx)
-            kb_gs7_sweep<T, true><<<grid, b, 0, s>>>(t);
+          if (zx && dgk)
+            kb_gs7_sweep<T, true, true><<<grid, b, 0, s>>>(t);
+          else if (zx)
+            kb_gs7_sweep<T, true, false><<<grid, b, 0, s>>>(t);
+          else if (dgk)
+            kb_gs7_sweep<T, false, true><<<grid, b, 0, s>>>(t);
           else
-            kb_gs7_sweep<T, false><<<grid, b, 0, s>>>(t);
+            kb_gs7_sweep<T, false, false><<<grid, b, 0, s>>>(t);
         });
     for (int q = 0; q < n; q++) std::swap(g[q].src, g[q].dst);
   }
@@ -2758,13 +2797,14 @@ struct A_res {
   int do_red, kc;
   XG xg;
   unsigned nblk;
+  const T *dg;
 };
-template <class T, class TO, int ND>
+template <class T, class TO, int ND, bool DG>
 __global__ __launch_bounds__(256) void kb_resid(const GTab<A_res<T, TO>> t) {
   const A_res<T, TO> &a = t.a[blockIdx.y];
   if (blockIdx.x >= a.nblk) return;
-  resid_body<T, TO, ND>(a.nx, a.ny, a.nz, a.ac, a.cc, a.fc, a.x, a.r, a.rd,
-                        a.do_red, a.kc, a.xg, blockIdx.x);
+  resid_body<T, TO, ND, DG>(a.nx, a.ny, a.nz, a.ac, a.cc, a.fc, a.x, a.r,
+                            a.rd, a.do_red, a.kc, a.xg, blockIdx.x, a.dg);
 }
 
 struct A_rf {
@@ -2788,21 +2828,26 @@ void g_resid(const GResid<T, TO> *j, int n, hipStream_t s) {
     const bool red = j[q].red != nullptr;
     const int kc = kchunk(d, red ? 40 : 80);
     const dim3 g(cdiv(d.nx - 2, BX), cdiv(d.ny - 2, BY), cdiv(d.nz - 2, kc));
+    const T *dg = j[q].numdia == 7 ? j[q].dg : nullptr;
     a[q] = A_res<T, TO>{d.nx,    d.ny,   d.nz,    j[q].ac,
                         j[q].cc, j[q].fc, j[q].x, j[q].r,
                         red ? *j[q].red : Red{}, red ? 1 : 0, kc,
-                        xg_of(g), xcd_blocks(g)};
+                        xg_of(g), xcd_blocks(g), dg};
   }
+  // variant key: 27 (27-point), 7 (7-point), 8 (7-point with dg)
   g_split<A_res<T, TO>>(
-      n, [&](int q) { return j[q].numdia; }, [&](int q) { return a[q]; },
-      [&](const GTab<A_res<T, TO>> &t, int m, int nd) {
+      n, [&](int q) { return j[q].numdia == 7 && a[q].dg ? 8 : j[q].numdia; },
+      [&](int q) { return a[q]; },
+      [&](const GTab<A_res<T, TO>> &t, int m, int key) {
         unsigned mx = 0;
         for (int q = 0; q < m; q++) mx = std::max(mx, t.a[q].nblk);
         const dim3 grid(mx, m), b(BX, BY);
-        if (nd == 7)
-          kb_resid<T, TO, 7><<<grid, b, 0, s>>>(t);
+        if (key == 8)
+          kb_resid<T, TO, 7, true><<<grid, b, 0, s>>>(t);
+        else if (key == 7)
+          kb_resid<T, TO, 7, false><<<grid, b, 0, s>>>(t);
         else
-          kb_resid<T, TO, 27><<<grid, b, 0, s>>>(t);
+          kb_resid<T, TO, 27, false><<<grid, b, 0, s>>>(t);
       });
   std::vector<int> rq;
   for (int q = 0; q < n; q++)
@@ -3067,7 +3112,9 @@ template void dot_energy<double>(const Dim &, int, const double *,
   template void gs7<T>(const Dim &, const T *, const T *, const T *, T *,     \
                        int, int, hipStream_t);                                \
   template void gs7_fused<T>(const Dim &, const T *, const T *, const T *,    \
-                             T *, T *, int, int, hipStream_t, bool);          \
+                             T *, T *, int, int, hipStream_t, bool,           \
+                             const T *);                                      \
+  template void diag_sum<T>(const T *, const T *, T *, size_t, hipStream_t);  \
   template void gs27<T>(const Dim &, const T *, const T *, const T *, T *,    \
                         int, int, hipStream_t, bool, T *);                    \
   template void matvec_dots<T>(const Dim &, int, const T *, const T *,        \
@@ -3100,14 +3147,15 @@ INST(float)
 template void resid<double, double>(const Dim &, int, const double *,
                                     const double *, const double *,
                                     const double *, double *, const Red *,
-                                    hipStream_t);
+                                    hipStream_t, const double *);
 template void resid<double, float>(const Dim &, int, const double *,
                                    const double *, const double *,
                                    const double *, float *, const Red *,
-                                   hipStream_t);
+                                   hipStream_t, const double *);
 template void resid<float, float>(const Dim &, int, const float *,
                                   const float *, const float *, const float *,
-                                  float *, const Red *, hipStream_t);
+                                  float *, const Red *, hipStream_t,
+                                  const float *);
 template void convert<double, float>(const double *, float *, size_t,
                                      hipStream_t);
 template void convert<float, double>(const float *, double *, size_t,

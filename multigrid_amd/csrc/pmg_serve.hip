@@ -189,7 +189,7 @@ void group_land(Group &g) {
   for (size_t k = 0; k < g.mem.size(); k++) {
     Request *r = g.mem[k];
     pmg_ctx *c = r->ctx;
-    c->hscal[1] = g.hn[k];  // S_RSNRM
+    c->hscal[S_RSNRM] = g.hn[k];
     c->ses.pending = false;
     if (ses_cycle_done(c)) {
       hipStreamWaitEvent(c->s, g.ev, 0);
@@ -203,21 +203,23 @@ void group_land(Group &g) {
   g.busy = false;
 }
 
-// Start the group's next iteration with the members it kept and ready
-// sessions of the same mode and smoothing counts (at most GMAX).
+// Start the group's next iteration with the members it kept and the
+// ready sessions, oldest first, of the same mode and smoothing counts (at
+// most GMAX).  A ready session that does not match stops the admission
+// (the group drains and the next empty group takes it), so that none waits
+// behind a stream of later matching ones.
 void group_launch(Group &g, std::deque<Request *> &ready) {
-  for (auto it = ready.begin();
-       it != ready.end() && (int)g.mem.size() < pmg::GMAX;) {
-    Request *r = *it;
-    const Params &p0 = g.mem.empty() ? r->ctx->p : g.mem[0]->ctx->p;
-    const int m0 = g.mem.empty() ? r->mode : g.mode;
-    if (r->mode == m0 && r->ctx->p.nu1 == p0.nu1 && r->ctx->p.nu2 == p0.nu2) {
-      g.mode = m0;
-      g.mem.push_back(r);
-      it = ready.erase(it);
-    } else {
-      ++it;
+  while (!ready.empty() && (int)g.mem.size() < pmg::GMAX) {
+    Request *r = ready.front();
+    if (!g.mem.empty()) {
+      const Params &p0 = g.mem[0]->ctx->p;
+      if (r->mode != g.mode || r->ctx->p.nu1 != p0.nu1 ||
+          r->ctx->p.nu2 != p0.nu2)
+        break;
     }
+    g.mode = r->mode;
+    g.mem.push_back(r);
+    ready.pop_front();
   }
   if (g.mem.empty()) return;
   std::vector<pmg_ctx *> cs;

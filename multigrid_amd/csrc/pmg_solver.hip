@@ -102,8 +102,6 @@ int decode(const int *iparm, const double *rparm, Params &p) {
 }
 
 
-// device scalar slots; S_BINFO: Vdpbfa info of the band factor (mgsolv 1)
-enum { S_RSDEN = 0, S_RSNRM = 1, S_XDAMP = 2, S_BINFO = 3, S_TMP = 4 };
 
 thread_local int t_iters = 0, t_ierror = 0;
 thread_local double t_relres = 0.0;
@@ -134,11 +132,12 @@ template <class T>
 LV<T> view(Level &l);
 template <>
 LV<double> view<double>(Level &l) {
-  return {l.d, l.numdia, l.ac, l.cc, l.pc, l.x, l.w0, l.r, l.t};
+  return {l.d, l.numdia, l.ac, l.cc, l.pc, l.x, l.w0, l.r, l.t, l.dg};
 }
 template <>
 LV<float> view<float>(Level &l) {
-  return {l.d, l.numdia, l.ac32, l.cc32, l.pc32, l.x32, l.w032, l.r32, l.t32};
+  return {l.d,    l.numdia, l.ac32, l.cc32, l.pc32,
+          l.x32,  l.w032,   l.r32,  l.t32,  l.dg32};
 }
 
 // zero_init: x starts at zero (Vazeros).  bnd_zero: x's boundary entries
@@ -158,7 +157,7 @@ void smooth(pmg_ctx *c, const LV<T> &l, const T *f, T *x, int nu, int iadj,
     if (l.t && l.t != x) {
       if (zero_init && !bnd_zero) fill<T>(x, l.d.n(), (T)0, c->s);
       gs7_fused<T>(l.d, l.ac, l.cc, f, x, l.t, nu, iadj, c->s,
-                   zero_init && bnd_zero);
+                   zero_init && bnd_zero, l.dg);
     } else {
       if (zero_init) fill<T>(x, l.d.n(), (T)0, c->s);
       gs7<T>(l.d, l.ac, l.cc, f, x, nu, iadj, c->s);
@@ -223,7 +222,7 @@ void vcycle(pmg_ctx *c, LV<T> *L, const T *frhs, const double *ac64c,
   // nu1 pre-smoothings on the fine grid, then the residual (D1)
   smooth<T>(c, L[0], frhs, L[0].x, p.nu1, 0, true, fine_zero, fine_zero);
   resid<T, T>(L[0].d, L[0].numdia, L[0].ac, L[0].cc, frhs, L[0].x, L[0].r,
-              nullptr, s);
+              nullptr, s, L[0].dg);
   for (int l = 1; l < nlev; l++) {  // down (mgcsd.cc:325-360)
     restrict_<T>(L[l - 1].d, L[l].d, L[l - 1].r, L[l].w0, L[l].pc, s);
     if (l != nlev - 1) {
@@ -266,7 +265,7 @@ int linear_fp64(pmg_ctx *c, LV<double> *L, const double *frhs,
   if (nlev == 1) {  // direct coarse solve on the only level (mgcsd.cc:170-291)
     coarse_solve<double>(c, l0, F.ac, L[0].cc, frhs, L[0].x);
     resid<double, double>(F.d, F.numdia, F.ac, L[0].cc, frhs, L[0].x,
-                          (double *)nullptr, &rnrm, c->s);
+                          (double *)nullptr, &rnrm, c->s, L[0].dg);
     rsnrm = host_scalar(c, S_RSNRM);
     *iters_out = 1;
     *rel_out = rsnrm / rsden;
@@ -277,7 +276,7 @@ int linear_fp64(pmg_ctx *c, LV<double> *L, const double *frhs,
     vcycle<double>(c, L, frhs, c->L[c->nlev - 1].ac, cc64c, false, 0.0, nlev);
     iters++;
     resid<double, double>(F.d, F.numdia, F.ac, L[0].cc, frhs, L[0].x,
-                          (double *)nullptr, &rnrm, c->s);
+                          (double *)nullptr, &rnrm, c->s, L[0].dg);
     rsnrm = host_scalar(c, S_RSNRM);
   } while (iters < itmax && rsnrm / rsden > errtol);
   *iters_out = iters;
@@ -303,7 +302,7 @@ int linear_mixed(pmg_ctx *c, LV<double> *L64, LV<float> *L32,
   if (rsden == 0.0) rsden = 1.0;
   float *r32 = F.w032;  // fine right-hand side of the FP32 cycle
   resid<double, float>(F.d, F.numdia, F.ac, L64[0].cc, frhs, u, r32, &rnrm,
-                       c->s);
+                       c->s, L64[0].dg);
   double rsnrm = host_scalar(c, S_RSNRM);
   int iters = 0;
   const int nlev = c->nlev - l0;
@@ -318,7 +317,7 @@ int linear_mixed(pmg_ctx *c, LV<double> *L64, LV<float> *L32,
     add_correction<float>(F.d, L32[0].x, u, c->s);
     iters++;
     resid<double, float>(F.d, F.numdia, F.ac, L64[0].cc, frhs, u, r32, &rnrm,
-                         c->s);
+                         c->s, L64[0].dg);
     rsnrm = host_scalar(c, S_RSNRM);
   }
   *iters_out = iters;
@@ -338,6 +337,7 @@ int alloc_hierarchy(pmg_ctx *c) {
     L.ac = c->alloc<double>((l == 0 ? 4 : 14) * n);
     L.cc = c->alloc<double>(n);
     L.fc = c->alloc<double>(n);
+    if (l == 0) L.dg = c->alloc<double>(n);
     if (l > 0) L.pc = c->alloc<double>(27 * n);
     L.x = c->alloc<double>(n);
     L.w0 = c->alloc<double>(n);
@@ -369,6 +369,7 @@ int alloc_fp32(pmg_ctx *c) {
     size_t n = L.d.n();
     L.ac32 = c->alloc<float>((l == 0 ? 4 : 14) * n);
     L.cc32 = c->alloc<float>(n);
+    if (l == 0) L.dg32 = c->alloc<float>(n);
     if (l > 0) L.pc32 = c->alloc<float>(27 * n);
     L.x32 = c->alloc<float>(n);
     L.w032 = c->alloc<float>(n);
@@ -386,6 +387,7 @@ void convert_ops32(pmg_ctx *c) {
     size_t n = L.d.n();
     convert<double, float>(L.ac, L.ac32, (l == 0 ? 4 : 14) * n, c->s);
     convert<double, float>(L.cc, L.cc32, n, c->s);
+    if (l == 0) diag_sum<float>(L.ac32, L.cc32, L.dg32, n, c->s);
     if (l > 0) convert<double, float>(L.pc, L.pc32, 27 * n, c->s);
   }
 }
@@ -401,6 +403,8 @@ void build_ops(pmg_ctx *c) {
   buildA(F.d, F.ac, F.cc, F.fc, c->xf, c->yf, c->zf, c->gx, c->gy, c->gz,
          c->a1, c->a2, c->a3, c->ccf, c->fcf, s, w32 ? F.ac32 : nullptr,
          w32 ? F.cc32 : nullptr);
+  diag_sum<double>(F.ac, F.cc, F.dg, F.d.n(), s);
+  if (w32) diag_sum<float>(F.ac32, F.cc32, F.dg32, F.d.n(), s);
   for (int l = 1; l < c->nlev; l++) {
     Level &Lf = c->L[l - 1], &Lc = c->L[l];
     buildP(Lf.d, Lc.d, c->p.mgprol, Lf.numdia, Lf.ac, Lc.pc, s,
@@ -634,7 +638,7 @@ void step_body(pmg_ctx *c, int mode) {
     }
     add_correction<float>(F.d, L32[0].x, c->u, c->s);
     resid<double, float>(F.d, F.numdia, F.ac, F.cc, F.fc, c->u, r32,
-                         resid_norm ? &rnrm : nullptr, c->s);
+                         resid_norm ? &rnrm : nullptr, c->s, F.dg);
   } else {
     if (nlev == 1)
       coarse_solve<double>(c, 0, F.ac, F.cc, F.fc, c->u);
@@ -642,7 +646,7 @@ void step_body(pmg_ctx *c, int mode) {
       vcycle<double>(c, L64, F.fc, c->L[nlev - 1].ac, cc64c);
     if (resid_norm)
       resid<double, double>(F.d, F.numdia, F.ac, F.cc, F.fc, c->u,
-                            (double *)nullptr, &rnrm, c->s);
+                            (double *)nullptr, &rnrm, c->s, F.dg);
   }
   if (!resid_norm) stop_measure(c, S_RSNRM);
 }
@@ -737,7 +741,7 @@ int lin_begin(pmg_ctx *c, int mode) {
     // residual of u = 0, the first right-hand side of the FP32 cycle
     const Red rnrm = red_to(c, S_RSNRM);
     resid<double, float>(F.d, F.numdia, F.ac, F.cc, F.fc, c->u, F.w032,
-                         p.istop <= 1 ? &rnrm : nullptr, c->s);
+                         p.istop <= 1 ? &rnrm : nullptr, c->s, F.dg);
   }
   // the two scalars and the band factor's info (slot S_BINFO)
   hipMemcpyAsync(c->hscal, c->scal, 4 * sizeof(double), hipMemcpyDeviceToHost,
@@ -894,7 +898,7 @@ void g_smooth_level(LV<T> *const *L, const std::vector<int> &qs, int lev,
     if (l.numdia == 7) {
       if (l.t && l.t != x) {
         if (zero_init && !bnd_zero) fill<T>(x, l.d.n(), (T)0, s);
-        j7.push_back(GSmooth<T>{l.d, l.ac, l.cc, f[k], x, l.t});
+        j7.push_back(GSmooth<T>{l.d, l.ac, l.cc, f[k], x, l.t, l.dg});
       } else {
         if (zero_init) fill<T>(x, l.d.n(), (T)0, s);
         gs7<T>(l.d, l.ac, l.cc, f[k], x, nu, iadj, s);
@@ -927,7 +931,7 @@ void g_vcycle(pmg_ctx *const *c, int n, LV<T> *const *L,
     for (int q = 0; q < n; q++) {
       const LV<T> &l = L[q][0];
       j.push_back(GResid<T, T>{l.d, l.numdia, l.ac, l.cc, frhs[q], l.x, l.r,
-                               nullptr, nullptr});
+                               nullptr, nullptr, l.dg});
     }
     g_resid<T, T>(j.data(), n, s);
   }
@@ -1022,7 +1026,8 @@ void group_step(pmg_ctx *const *c, int n, int mode, hipStream_t s,
       Level &F = c[q]->L[0];
       ja.push_back(GAddcorr<float>{F.d, L32[q][0].x, c[q]->u});
       jr.push_back(GResid<double, float>{F.d, F.numdia, F.ac, F.cc, F.fc,
-                                         c[q]->u, F.w032, &rn[q], norm + q});
+                                         c[q]->u, F.w032, &rn[q], norm + q,
+                                         F.dg});
     }
     g_addcorr<float>(ja.data(), n, s);
     g_resid<double, float>(jr.data(), n, s);
@@ -1034,8 +1039,8 @@ void group_step(pmg_ctx *const *c, int n, int mode, hipStream_t s,
     for (int q = 0; q < n; q++) {
       Level &F = c[q]->L[0];
       jr.push_back(GResid<double, double>{F.d, F.numdia, F.ac, F.cc, F.fc,
-                                          c[q]->u, nullptr, &rn[q],
-                                          norm + q});
+                                          c[q]->u, nullptr, &rn[q], norm + q,
+                                          F.dg});
     }
     g_resid<double, double>(jr.data(), n, s);
   }
@@ -1056,7 +1061,14 @@ int ctx_solve_group(pmg_ctx *const *cs, int n, int mode) {
   std::vector<pmg_ctx *> act;
   for (int q = 0; q < n; q++) {
     const int r = lin_begin(cs[q], mode);
-    if (r) return r;  // contexts started before stay unfinished; no work
+    if (r) {  // close the sessions already started, then report
+      for (int k = 0; k < q; k++) {
+        lin_poll(cs[k], true);
+        lin_finish(cs[k], nullptr);
+        lin_end(cs[k]);
+      }
+      return r;
+    }
   }
   for (int q = 0; q < n; q++) {
     if (lin_poll(cs[q], true))
@@ -1197,10 +1209,12 @@ int newton_level(pmg_ctx *c, int mode, int l0, double *x, int itmax,
   LV<float> L32[MAXLEV];
   for (int l = l0; l < c->nlev; l++) {
     L64[l] = view<double>(c->L[l]);
-    L64[l].cc = c->L[l].cpr;
+    L64[l].cc = c->L[l].cpr;  // the Jacobian term: no precomputed diagonal
+    L64[l].dg = nullptr;
     if (mode == PMG_MIXED) {
       L32[l] = view<float>(c->L[l]);
       L32[l].cc = c->L[l].cpr32;
+      L32[l].dg = nullptr;
     }
   }
   double *xtmp = F.x;  // iterate of the inner linear solve
