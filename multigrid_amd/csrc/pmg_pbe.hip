@@ -185,6 +185,83 @@ __global__ __launch_bounds__(256) void k_pb_nodes(
   fcf[p] = acc;
 }
 
+// exp(x) for -708 <= x <= 0 by Tang's table-driven method (P. T. P. Tang,
+// "Table-driven implementation of the exponential function in IEEE
+// floating-point arithmetic", ACM TOMS 15(2):144-157, 1989):
+// x = (64 m + j) ln2/64 + R with |R| <= ln2/128, exp(x) = 2^m 2^(j/64) e^R,
+// e^R - 1 by its Taylor polynomial of degree 6 (truncation error below
+// 3e-20), ln2/64 split as L1 + L2 with N L1 exact, 2^(j/64) as lead + trail
+// (tables in LDS).  Within one ulp of exp (checked against the C library on
+// 2e6 arguments); about half the instructions of the general routine.
+constexpr double EXP_INV_L = 0x1.71547652b82fep+6;  // 64 / ln2
+constexpr double EXP_L1 = 0x1.62e42fef00000p-7;      // ln2/64, low 20 bits zero
+constexpr double EXP_L2 = 0x1.473de6af278edp-40;      // ln2/64 - L1
+// 2^(j/64), j = 0..63: the double nearest to it (lead) and the double
+// nearest to the remainder (trail), computed from the definition
+__constant__ double c_exp2_lead[64] = {
+    0x1.0000000000000p+0, 0x1.02c9a3e778061p+0, 0x1.059b0d3158574p+0,
+    0x1.0874518759bc8p+0, 0x1.0b5586cf9890fp+0, 0x1.0e3ec32d3d1a2p+0,
+    0x1.11301d0125b51p+0, 0x1.1429aaea92de0p+0, 0x1.172b83c7d517bp+0,
+    0x1.1a35beb6fcb75p+0, 0x1.1d4873168b9aap+0, 0x1.2063b88628cd6p+0,
+    0x1.2387a6e756238p+0, 0x1.26b4565e27cddp+0, 0x1.29e9df51fdee1p+0,
+    0x1.2d285a6e4030bp+0, 0x1.306fe0a31b715p+0, 0x1.33c08b26416ffp+0,
+    0x1.371a7373aa9cbp+0, 0x1.3a7db34e59ff7p+0, 0x1.3dea64c123422p+0,
+    0x1.4160a21f72e2ap+0, 0x1.44e086061892dp+0, 0x1.486a2b5c13cd0p+0,
+    0x1.4bfdad5362a27p+0, 0x1.4f9b2769d2ca7p+0, 0x1.5342b569d4f82p+0,
+    0x1.56f4736b527dap+0, 0x1.5ab07dd485429p+0, 0x1.5e76f15ad2148p+0,
+    0x1.6247eb03a5585p+0, 0x1.6623882552225p+0, 0x1.6a09e667f3bcdp+0,
+    0x1.6dfb23c651a2fp+0, 0x1.71f75e8ec5f74p+0, 0x1.75feb564267c9p+0,
+    0x1.7a11473eb0187p+0, 0x1.7e2f336cf4e62p+0, 0x1.82589994cce13p+0,
+    0x1.868d99b4492edp+0, 0x1.8ace5422aa0dbp+0, 0x1.8f1ae99157736p+0,
+    0x1.93737b0cdc5e5p+0, 0x1.97d829fde4e50p+0, 0x1.9c49182a3f090p+0,
+    0x1.a0c667b5de565p+0, 0x1.a5503b23e255dp+0, 0x1.a9e6b5579fdbfp+0,
+    0x1.ae89f995ad3adp+0, 0x1.b33a2b84f15fbp+0, 0x1.b7f76f2fb5e47p+0,
+    0x1.bcc1e904bc1d2p+0, 0x1.c199bdd85529cp+0, 0x1.c67f12e57d14bp+0,
+    0x1.cb720dcef9069p+0, 0x1.d072d4a07897cp+0, 0x1.d5818dcfba487p+0,
+    0x1.da9e603db3285p+0, 0x1.dfc97337b9b5fp+0, 0x1.e502ee78b3ff6p+0,
+    0x1.ea4afa2a490dap+0, 0x1.efa1bee615a27p+0, 0x1.f50765b6e4540p+0,
+    0x1.fa7c1819e90d8p+0,
+};
+__constant__ double c_exp2_trail[64] = {
+    0x0.0p+0, -0x1.19083535b085dp-56, 0x1.d73e2a475b465p-55,
+    0x1.186be4bb284ffp-57, 0x1.8a62e4adc610bp-54, 0x1.03a1727c57b53p-59,
+    -0x1.6c51039449b3ap-54, -0x1.32fbf9af1369ep-54, -0x1.19041b9d78a76p-55,
+    0x1.e5b4c7b4968e4p-55, 0x1.e016e00a2643cp-54, 0x1.dc775814a8495p-55,
+    0x1.9b07eb6c70573p-54, 0x1.2bd339940e9d9p-55, 0x1.612e8afad1255p-55,
+    0x1.0024754db41d5p-54, 0x1.6f46ad23182e4p-55, 0x1.32721843659a6p-54,
+    -0x1.63aeabf42eae2p-54, -0x1.5e436d661f5e3p-56, 0x1.ada0911f09ebcp-55,
+    -0x1.ef3691c309278p-58, 0x1.89b7a04ef80d0p-59, 0x1.3c1a3b69062f0p-56,
+    0x1.d4397afec42e2p-56, -0x1.4b309d25957e3p-54, -0x1.07abe1db13cadp-55,
+    0x1.9bb2c011d93adp-54, 0x1.6324c054647adp-54, 0x1.ba6f93080e65ep-54,
+    -0x1.383c17e40b497p-54, -0x1.bb60987591c34p-54, -0x1.bdd3413b26456p-54,
+    -0x1.bbe3a683c88abp-57, -0x1.16e4786887a99p-55, -0x1.0245957316dd3p-54,
+    -0x1.41577ee04992fp-55, 0x1.05d02ba15797ep-56, -0x1.d4c1dd41532d8p-54,
+    -0x1.fc6f89bd4f6bap-54, 0x1.6e9f156864b27p-54, 0x1.5cc13a2e3976cp-55,
+    -0x1.75fc781b57ebcp-57, -0x1.d185b7c1b85d1p-54, 0x1.c7c46b071f2bep-56,
+    -0x1.359495d1cd533p-54, -0x1.d2f6edb8d41e1p-54, 0x1.0fac90ef7fd31p-54,
+    0x1.7a1cd345dcc81p-54, -0x1.2805e3084d708p-57, -0x1.5584f7e54ac3bp-56,
+    0x1.23dd07a2d9e84p-55, 0x1.11065895048ddp-55, 0x1.2884dff483cadp-54,
+    0x1.503cbd1e949dbp-56, -0x1.cbc3743797a9cp-54, 0x1.2ed02d75b3707p-55,
+    0x1.c2300696db532p-54, -0x1.1a5cd4f184b5cp-54, 0x1.39e8980a9cc8fp-55,
+    -0x1.e9c23179c2893p-54, 0x1.dc7f486a4b6b0p-54, 0x1.9d3e12dd8a18bp-54,
+    0x1.74853f3a5931ep-55,
+};
+
+__device__ __forceinline__ double exp_tab(double x, const double *lead,
+                                          const double *trail) {
+  const double N = rint(x * EXP_INV_L);
+  const int n = (int)N;
+  const double R = fma(-N, EXP_L2, fma(-N, EXP_L1, x));
+  double q = fma(R, 1.0 / 720.0, 1.0 / 120.0);
+  q = fma(R, q, 1.0 / 24.0);
+  q = fma(R, q, 1.0 / 6.0);
+  q = fma(R, q, 0.5);
+  const double p = fma(R * R, q, R);  // e^R - 1
+  const int j = n & 63, m = n >> 6;   // n = 64 m + j, 0 <= j < 64
+  const double sl = lead[j], st = trail[j];
+  return ldexp(sl + fma(sl + st, p, st), m);
+}
+
 constexpr int FACE_NT = 256, FACE_NODES = 64, FACE_SPLIT = FACE_NT / FACE_NODES;
 
 // Atoms [t0, t1) of the staged chunk for the node pair at in-plane
@@ -196,6 +273,7 @@ __device__ __forceinline__ void face_terms(int t0, int t1, double va, double vb,
                                            double lo, double hi, double kap,
                                            const double *su, const double *sw,
                                            const double *sn, const double *sc,
+                                           const double *tl, const double *tt,
                                            double &s0, double &s1) {
 #pragma clang fp contract(off)
   for (int t = t0; t < t1; t++) {
@@ -207,8 +285,8 @@ __device__ __forceinline__ void face_terms(int t0, int t1, double va, double vb,
     const double r1 = AX == 0 ? d1s + dus + dws
                               : (AX == 1 ? dus + d1s + dws : dus + dws + d1s);
     const double i0 = rsqrt(r0), i1 = rsqrt(r1);
-    s0 += sc[t] * exp(-kap * (r0 * i0)) * i0;
-    s1 += sc[t] * exp(-kap * (r1 * i1)) * i1;
+    s0 += sc[t] * exp_tab(-kap * (r0 * i0), tl, tt) * i0;
+    s1 += sc[t] * exp_tab(-kap * (r1 * i1), tl, tt) * i1;
   }
 }
 
@@ -219,8 +297,9 @@ __device__ __forceinline__ void face_terms(int t0, int t1, double va, double vb,
 // of atoms (all lanes of a wavefront read the same atom: an LDS broadcast),
 // and the four partial sums are added in a fixed order.  Per term the atom
 // factor lambda_B q exp(kappa r) / (eps_w (1 + kappa r)) is staged once,
-// leaving one exp and one reciprocal square root per (node, atom).  The sum
-// is deterministic; its order differs from the serial one by rounding only.
+// leaving one exp (exp_tab) and one reciprocal square root per (node,
+// atom).  The sum is deterministic; its order differs from the serial one
+// by rounding only.
 __global__ __launch_bounds__(FACE_NT) void k_pb_faces(
     int n, double c0, double h, double lb, double kap, double eps_out,
     int natoms,
@@ -228,6 +307,11 @@ __global__ __launch_bounds__(FACE_NT) void k_pb_faces(
 #pragma clang fp contract(off)
   __shared__ double sx[FACE_NT], sy[FACE_NT], sz[FACE_NT], sc[FACE_NT];
   __shared__ double part[2][FACE_SPLIT][FACE_NODES];
+  __shared__ double tl[64], tt[64];  // exp_tab's table (read after a barrier)
+  if (threadIdx.x < 64) {
+    tl[threadIdx.x] = c_exp2_lead[threadIdx.x];
+    tt[threadIdx.x] = c_exp2_trail[threadIdx.x];
+  }
   const int axis = blockIdx.y;
   const int lane = threadIdx.x % FACE_NODES, split = threadIdx.x / FACE_NODES;
   const long long nn = (long long)n * n;
@@ -252,11 +336,14 @@ __global__ __launch_bounds__(FACE_NT) void k_pb_faces(
     const int per = (m + FACE_SPLIT - 1) / FACE_SPLIT;
     const int t0 = split * per, t1 = min(m, t0 + per);
     if (axis == 0)
-      face_terms<0>(t0, t1, va, vb, lo, hi, kap, sy, sz, sx, sc, s0, s1);
+      face_terms<0>(t0, t1, va, vb, lo, hi, kap, sy, sz, sx, sc, tl, tt, s0,
+                    s1);
     else if (axis == 1)
-      face_terms<1>(t0, t1, va, vb, lo, hi, kap, sx, sz, sy, sc, s0, s1);
+      face_terms<1>(t0, t1, va, vb, lo, hi, kap, sx, sz, sy, sc, tl, tt, s0,
+                    s1);
     else
-      face_terms<2>(t0, t1, va, vb, lo, hi, kap, sx, sy, sz, sc, s0, s1);
+      face_terms<2>(t0, t1, va, vb, lo, hi, kap, sx, sy, sz, sc, tl, tt, s0,
+                    s1);
   }
   part[0][split][lane] = s0;
   part[1][split][lane] = s1;
