@@ -822,10 +822,11 @@ void gs27(const Dim &d, const T *ac, const T *cc, const T *fc, T *x,
   if (tmp && tmp != x && d.nx <= ROWS_MAXN && d.ny >= 4) {
     const int bp = iadjoint ? 0 : 1;
     constexpr int RSA = 3, RSB = 7;
-    // strips of 7 rows (8 wavefronts) on levels of 65^3 and more, of 3 rows
+    // strips of 7 rows (8 wavefronts) on levels of 65^3 to 129^3, of 3 rows
     // (4 wavefronts, more workgroups) below, where the launch is latency-
-    // rather than bandwidth-bound (measured at 81^3, 41^3, 21^3)
-    const bool big = d.nx >= 65;
+    // rather than bandwidth-bound, and above, where the rows of 7-row strips
+    // overflow the L2 (measured at 145^3, 81^3, 41^3, 21^3)
+    const bool big = d.nx >= 65 && d.nx <= 129;
     const int nstrips = cdiv(count_parity(d.ny, bp), big ? RSB : RSA);
     T *cur = x, *nxt = tmp;
     for (; done < (nsweeps & ~1); done++) {
@@ -2733,7 +2734,7 @@ void g_gs27(const GSmooth<T> *j, int n, int nsweeps, int iadjoint,
   for (int r = 0; r < m; r++) {  // the launch shape of gs27's paired sweeps
     const GSmooth<T> &J = j[idx[r]];
     g[r].ppl = J.d.nx <= 129 ? 1 : (J.d.nx <= 257 ? 2 : 4);
-    g[r].big = J.d.nx >= 65;
+    g[r].big = J.d.nx >= 65 && J.d.nx <= 129;  // as gs27
     g[r].nstrips = cdiv(count_parity(J.d.ny, bp), g[r].big ? 7 : 3);
     g[r].cur = J.x, g[r].nxt = J.tmp;
   }
