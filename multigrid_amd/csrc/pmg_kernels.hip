@@ -1106,7 +1106,8 @@ void nmresid(const Dim &d, int numdia, const double *ac, const double *cc,
 }
 
 // ---------------------------------------------------------------------------
-// y = A x with partial sums of x.w0 and x.y (Vmatvec + 2 Vxdot, D2)
+// y = A x with partial sums of x.w0 and x.y (Vmatvec + 2 Vxdot, Vxdot
+// being x . y; the reference's squares x)
 // ---------------------------------------------------------------------------
 template <class T, int ND>
 __device__ __forceinline__ void matvec_dots_body(
@@ -1280,7 +1281,8 @@ enum : int {
 
 // One thread per coarse cell (ii, jj, kk) in 1..nc-1: the 8 fine points of
 // the cell, as VinterpPMG2.  ADD: xf += xdamp * value on fine interior
-// points (D3); otherwise xf = value (boundary zeroed separately, D8).
+// points (Vxaxpy over the interior); otherwise xf = value (the fine
+// boundary is zeroed separately).
 template <class T, bool ADD>
 __device__ __forceinline__ void interp_body(int nxc, int nyc, int nzc,
                                             int nxf, int nyf, int nzf,
@@ -2192,7 +2194,8 @@ void copy_interior(const Dim &d, const double *x, double *y, hipStream_t s) {
   check_launch("copy_interior");
 }
 
-// Faces written i, then j, then k (VfboundPMG, mikpckd.cc:266-336, D4);
+// Faces written i, then j, then k (VfboundPMG, mikpckd.cc:266-336, on the
+// faces it intends);
 // three launches keep the edge precedence of the reference.
 __global__ void k_face(int nx, int ny, int nz, int axis, double *x,
                        const double *g) {

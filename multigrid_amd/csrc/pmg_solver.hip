@@ -175,7 +175,7 @@ void smooth(pmg_ctx *c, const LV<T> &l, const T *f, T *x, int nu, int iadj,
 }
 
 // Coarsest-level solve: Vcghs with 100 iterations and errtol = epsiln = 0
-// (mgsolv 0, D5), or the factored band (mgsolv 1, mgcsd.cc:391-407, which
+// (mgsolv 0; epsiln 0 gives exactly 100 iterations), or the factored band (mgsolv 1, mgcsd.cc:391-407, which
 // ignores cc as the reference's band does).  FP64 inside either way.
 // errtol > 0: Vcghs's relative stopping test sqrt(r.r)/sqrt(r0.r0) <= errtol
 // with that tolerance instead of epsiln (at most 100 iterations).
@@ -219,7 +219,8 @@ void vcycle(pmg_ctx *c, LV<T> *L, const T *frhs, const double *ac64c,
   const int nlev = nlevd > 0 ? nlevd : c->nlev;
   const Params &p = c->p;
   hipStream_t s = c->s;
-  // nu1 pre-smoothings on the fine grid, then the residual (D1)
+  // nu1 pre-smoothings on the fine grid, then the residual f - A x that
+  // Vmvcs restricts (the reference's Vgsrb drops its iresid output)
   smooth<T>(c, L[0], frhs, L[0].x, p.nu1, 0, true, fine_zero, fine_zero);
   resid<T, T>(L[0].d, L[0].numdia, L[0].ac, L[0].cc, frhs, L[0].x, L[0].r,
               nullptr, s, L[0].dg);
@@ -1288,7 +1289,8 @@ int newton_level(pmg_ctx *c, int mode, int l0, double *x, int itmax,
   return *rel_out <= p.errtol ? PMG_OK : PMG_ITMAX;
 }
 
-// Vnewton (mgkey 0) or Vfnewton (mgkey 1, newtond.cc:60-138 with D9):
+// Vnewton (mgkey 0) or Vfnewton (mgkey 1, newtond.cc:60-138, its level loop
+// run from the coarsest level up, as its level-- intends):
 // Newton on each level from the coarsest up, each level's solution
 // prolongated (VinterpPMG) as the next finer level's starting iterate; the
 // coarse iterates start from zero.  The fine iterate starts from c->u when

@@ -4,8 +4,8 @@
  * TEST INFRASTRUCTURE ONLY (see pmg_oracle.h).  A fresh C restatement of
  * the reference's pmgc algorithm with APBS semantics; every function cites
  * the reference lines it follows.  Deviations from the literal reference
- * text (the fix list D1-D13) are applied where the reference's driver files
- * would otherwise not run; each is marked "Dn" at the place it applies.
+ * text are applied where the reference's driver files would otherwise not
+ * run (each is described in a comment where it applies).
  *
  * Reductions are deterministic: partial sums per k-plane (i fastest, then
  * j), then a sequential sum over planes, independent of the thread count.
@@ -207,7 +207,7 @@ void pmgo_buildA_fv(int nx, int ny, int nz, int ipkey, int *ipc, double *ac,
   ipc[9] = ipkey; /* VAT(ipc,10) */
   ipc[10] = 7;    /* VAT(ipc,11) */
   ipc[11] = 1;    /* VAT(ipc,12) */
-  /* D7: the reference's z-range uses ny (buildAd.cc:124); nz is used. */
+  /* the reference's z-range uses ny (buildAd.cc:124); nz is used. */
 #pragma omp parallel for schedule(static)
   for (int k = 2; k <= nz - 1; k++)
     for (int j = 2; j <= ny - 1; j++)
@@ -484,7 +484,7 @@ void pmgo_mypdefinit(int nion, const double *charge, const double *sconc) {
 }
 
 /* Vc_vecpmg (mypdec.cc:136-212) and Vdc_vecpmg (:348-414), ipkey 0.
- * D10: evaluated on whole arrays exactly as the reference host loops. */
+ * Evaluated on whole arrays exactly as the reference host loops. */
 static void ion_term(const double *coef, const double *uin, double *uout,
                      size_t n, int deriv) {
   const double ZSMALL = 1.0e-20, ZLARGE = 1.0e20, SMIN = -85.0, SMAX = 85.0;
@@ -564,7 +564,8 @@ void pmgo_gsrb(int nx, int ny, int nz, int numdia, const double *ac,
                      (oC[p] + cc[p]);
             }
       }
-    } else { /* Vgsrb27x gsd.cc:160-211 (8 colours, D11) */
+    } else { /* Vgsrb27x gsd.cc:160-211 (8 colours; the reference's driver
+              * reaches it for 27-point levels) */
       for (int c = 0; c < 8; c++) {
         int col = c * (1 - iadjoint) + iadjoint * (7 - c);
 #pragma omp parallel for schedule(static)
@@ -664,7 +665,7 @@ void pmgo_interp(int nxc, int nyc, int nzc, int nxf, int nyf, int nzf,
 #undef PW
 #undef XC
 #undef XF
-  /* D8: VfboundPMG00 on the result (matvecd.cc:963 is commented out). */
+  /* VfboundPMG00 on the result (matvecd.cc:963 is commented out). */
   pmgo_fbound00(nxf, nyf, nzf, xout);
 }
 
@@ -711,12 +712,12 @@ double pmgo_xnrm2(int nx, int ny, int nz, const double *x) {
   return sqrt(reduce_interior(nx, ny, nz, x, 0, 1)); /* Vxnrm2 :161 */
 }
 double pmgo_xdot(int nx, int ny, int nz, const double *x, const double *y) {
-  /* Vxdot mikpckd.cc:192-222; D2: x.y (the reference squares x, :216) */
+  /* Vxdot mikpckd.cc:192-222: x.y (the reference squares x, :216) */
   return reduce_interior(nx, ny, nz, x, y, 2);
 }
 void pmgo_xaxpy(int nx, int ny, int nz, double alpha, const double *x,
                 double *y) {
-  /* Vxaxpy mikpckd.cc:115-131; D3: interior range. */
+  /* Vxaxpy mikpckd.cc:115-131 over the interior range. */
 #pragma omp parallel for schedule(static)
   for (int k = 2; k <= nz - 1; k++)
     for (int j = 2; j <= ny - 1; j++)
@@ -743,7 +744,7 @@ static void xscal(int nx, int ny, int nz, double fac, double *x) {
 }
 
 void pmgo_fbound00(int nx, int ny, int nz, double *x) {
-  /* VfboundPMG00 mikpckd.cc:234-264; D4: intended faces. */
+  /* VfboundPMG00 mikpckd.cc:234-264 on the faces it intends. */
   for (int k = 1; k <= nz; k++)
     for (int j = 1; j <= ny; j++) {
       x[IX(nx, ny, 1, j, k)] = 0.0;
@@ -763,7 +764,7 @@ void pmgo_fbound00(int nx, int ny, int nz, double *x) {
 
 void pmgo_fbound(int ibound, int nx, int ny, int nz, double *x,
                  const double *gxc, const double *gyc, const double *gzc) {
-  /* VfboundPMG mikpckd.cc:266-336; D4: tests *ibound, intended faces.
+  /* VfboundPMG mikpckd.cc:266-336: tests *ibound, on the faces it intends.
    * Faces are written i, then j, then k (edges take the later face). */
   if (ibound == 0) {
     pmgo_fbound00(nx, ny, nz, x);
@@ -803,7 +804,7 @@ int pmgo_cghs(int nx, int ny, int nz, int numdia, const double *ac,
   double denom = pmgo_xnrm2(nx, ny, nz, r);
   if (denom == 0.0) return iters;
   for (;;) {
-    double rhok2 = pmgo_xdot(nx, ny, nz, r, r); /* D2 */
+    double rhok2 = pmgo_xdot(nx, ny, nz, r, r); /* r . r */
     double rsnrm = sqrt(rhok2);
     if (rsnrm / denom <= errtol) break;
     if (iters >= itmax) break;
@@ -1016,8 +1017,8 @@ static int g_nhist = 0;
 
 /* Stopping-test norm (mgcsd.cc:483-526): 0, 1 the L1 residual; 2 the L1
  * difference to tru, which then takes x (successive iterates); 3, 4 the
- * 2-norm of tru - x; 5 its energy norm sqrt((tru-x) . A (tru-x)) (D2: a
- * true dot product).  w2 is scratch for istop 5. */
+ * 2-norm of tru - x; 5 its energy norm sqrt((tru-x) . A (tru-x)) (Vxdot
+ * as a true dot product; the reference's squares x).  w2 is scratch for istop 5. */
 static double stop_norm(const hier_t *h, const cyc_t *c, double *x,
                         const double *fc1, const double *cc1, double *w1,
                         double *w2, double *tru) {
@@ -1076,7 +1077,8 @@ static int mvcs(const hier_t *h, const cyc_t *c, double **x, double **w0,
 
   int iters = 0;
   do {
-    /* fine pre-smoothing; D1: residual into w1 */
+    /* fine pre-smoothing, then the residual into w1 (the reference's Vgsrb
+     * drops its iresid output) */
     pmgo_gsrb(nx0, ny0, nz0, h->numdia[0], h->ac[0], ccl[0], fc1, x[0],
               c->nu1, 0);
     pmgo_mresid(nx0, ny0, nz0, h->numdia[0], h->ac[0], ccl[0], fc1, x[0], w1);
@@ -1092,7 +1094,7 @@ static int mvcs(const hier_t *h, const cyc_t *c, double **x, double **w0,
                     ccl[l], w0[l], x[l], w1);
       }
     }
-    { /* coarsest: Vcghs, 100 iterations, errtol = epsiln (D5), or the
+    { /* coarsest: Vcghs, 100 iterations, errtol = epsiln = 0, or the
        * factored band (mgcsd.cc:391-407) */
       int l = nlev - 1;
       memset(x[l], 0, h->n[l] * sizeof(double));
@@ -1111,7 +1113,7 @@ static int mvcs(const hier_t *h, const cyc_t *c, double **x, double **w0,
                   ccl[cl], x[cl], w2);
       double xnum = pmgo_xdot(h->nx[cl], h->ny[cl], h->nz[cl], x[cl], w0[cl]);
       double xden = pmgo_xdot(h->nx[cl], h->ny[cl], h->nz[cl], x[cl], w2);
-      /* D14: a zero coarse correction (xden == 0) gives xdamp = 0 where the
+      /* a zero coarse correction (xden == 0) gives xdamp = 0 where the
        * reference divides 0/0 (mgcsd.cc:452) and spreads NaN. */
       double xdamp = xden != 0.0 ? xnum / xden : 0.0;
       pmgo_xaxpy(h->nx[l], h->ny[l], h->nz[l], xdamp, w1, x[l]);
@@ -1119,7 +1121,7 @@ static int mvcs(const hier_t *h, const cyc_t *c, double **x, double **w0,
                 l == 0 ? fc1 : w0[l], x[l], c->nu2, 1);
     }
     iters++;
-    rsnrm = stop_norm(h, c, x[0], fc1, ccl[0], w1, w2, tru); /* D6: always */
+    rsnrm = stop_norm(h, c, x[0], fc1, ccl[0], w1, w2, tru); /* every cycle */
     if (record && g_nhist < HIST_CAP) g_hist[g_nhist++] = rsnrm / rsden;
   } while (iters < c->itmax && rsnrm / rsden > c->errtol);
   *iters_out = iters;
@@ -1274,7 +1276,7 @@ int pmgo_mgdriv(int *iparm, double *rparm, int *iwork, double *rwork,
   size_t narr = 0;
   for (int l = 0; l < h.nlev; l++) narr += h.n[l];
   size_t nf = h.n[0];
-  /* D13: private workspaces instead of clobbering a1cf..ccf, tcf */
+  /* private workspaces instead of clobbering a1cf..ccf, tcf */
   double *w0b = (double *)calloc(narr, sizeof(double));
   double *w1 = (double *)calloc(nf, sizeof(double));
   double *w2 = (double *)calloc(nf, sizeof(double));
@@ -1593,7 +1595,7 @@ static int newton_at(const hier_t *h0, int L, int istop, int itmax,
       xnorm_new = pmgo_xnrm1(nx, ny, nz, w0[0]);
       xnorm_old = xnorm_new;
     }
-    rsnrm = xnorm_new; /* istop 0/1 (newtond.cc:396-401); D6 */
+    rsnrm = xnorm_new; /* istop 0/1 (newtond.cc:396-401), every iteration */
     if (record && g_nhist < HIST_CAP) g_hist[g_nhist++] = rsnrm / rsden;
     if (rsnrm / rsden <= errtol) break;
     if (iters >= itmax) break;
@@ -1612,7 +1614,8 @@ static int newton_at(const hier_t *h0, int L, int istop, int itmax,
 /* Vnewdriv / Vnewdriv2 (newdrvd.cc:60-300): mgkey 0 runs Vnewton on the
  * fine level from the caller's u (interior as given, boundary zeroed,
  * VfboundPMG00 newdrvd.cc:263); mgkey 1 runs Vfnewton (newtond.cc:60-138,
- * D9): Newton on the coarsest level, prolongated (VinterpPMG) as the
+ * its level loop run upward as its level-- intends): Newton on the coarsest
+ * level, prolongated (VinterpPMG) as the
  * starting guess of the next finer level's Newton, up to the finest level.
  * The coarse-level iterates start from zero (the reference reads them from
  * the coarse part of a narr-long u, which this interface does not carry). */

@@ -217,7 +217,7 @@ def test_coarse_correction(mg, orc, nf, nc, numdia):
     _call(mg, "pmg_correct_f32", nc, nc, nc, nf, nf, nf, 27, *(fp(v) for v in a32[:5]),
           fp(out32))
     assert _rel(_back(out32), ref32) <= 1e-5
-    # a zero coarse solution gives a zero correction (xdamp 0/0 -> 0, D14)
+    # a zero coarse solution gives a zero correction (xdamp 0/0 -> 0)
     z = np.zeros(nc ** 3)
     out = xf.copy()
     _call(mg, "pmg_correct", nc, nc, nc, nf, nf, nf, 27, dp(acc), dp(ccc), dp(z), dp(w0c),

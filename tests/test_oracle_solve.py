@@ -137,7 +137,7 @@ def _newton_inputs(orc, n, h, seed):
 
 
 def test_nested_newton_manufactured(orc):
-    """mgkey 1 (Vfnewton, newtond.cc:60-138 with D9): the nested iteration
+    """mgkey 1 (Vfnewton, newtond.cc:60-138): the nested iteration
     recovers the manufactured solution as the plain Newton does."""
     n, nlev, h = 17, 3, 0.5
     inp, us = _newton_inputs(orc, n, h, 21)

@@ -13,9 +13,9 @@ form, written from the reference text:
   (i%2)+2(j%2)+4(k%2) on the 27-point levels, order reversed with iadjoint
   (gsd.cc:85-93, 161-169);
 * the cycle of mgcsd.cc:299-480: nu1 sweeps with iadjoint 0, the
-  residual restricted (D1), zero start and nu1 sweeps on every level above
+  residual restricted, zero start and nu1 sweeps on every level above
   the coarsest, 100 CG iterations from zero with errtol 0 on the coarsest
-  (Vcghs, cgd.cc:60-124, D5), and on the way up the prolongated coarse
+  (Vcghs, cgd.cc:60-124), and on the way up the prolongated coarse
   solution added with the Hackbusch-Reusken factor
   (x_c . w0_c) / (x_c . A_c x_c) (mgcsd.cc:429-458), then nu2 sweeps with
   iadjoint 1; the L1 relative residual after every cycle.
