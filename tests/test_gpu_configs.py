@@ -213,3 +213,61 @@ def test_c3_129_newton_vs_oracle(mg, orc):
     assert it1 == it0 and smx["iters"] == it0
     assert _rel(u1, u0) <= 1e-9
     assert _rel(umx, u0) <= 1e-9
+
+
+def test_c3_129_nested_newton_vs_oracle(mg, orc):
+    """The c3 workload with the nested iteration (mgkey 1, Vfnewton
+    newtond.cc:60-138): the same finest-level Newton iterations and
+    per-iteration residuals as the oracle, the potential within 1e-9, in the
+    drop-in and through a resident context with FP32 inner cycles."""
+    c = mg.CONFIGS["c3"]
+    n, nlev = c["n"], c["nlev"]
+    ions = ([1.0, -1.0], [0.5, 0.5])
+    mg.Vmypdefinitnpbe(2, *ions)
+    orc.mypdefinit(*ions)
+    atoms = mg.pbe_molecule(c["natoms"], c["sphere_r"], c["charge_mode"], c["seed"])
+    inp = orc.pbe_assemble(n, c["h"], c["ionic"], atoms)
+    iparm, rparm = mg.make_params(n, nlev, nonlin=1, errtol=1e-8, mgkey=1)
+    print(f"c3 nested: {n}^3 Newton from the coarsest level, FP64 drop-in", flush=True)
+    e1, u1, _, _ = _drv(mg, mg.Vnewdriv, inp, iparm, rparm)
+    it1, h1 = mg.last_stats()[0], mg.last_history()
+    print("c3 nested: FP32 inner V-cycles", flush=True)
+    ctx = mg.Context(iparm, rparm, inp)
+    ctx.solve(mg.PMG_MIXED)
+    umx, smx = ctx.u(), ctx.stats()
+    ctx.close()
+    print("c3 nested: oracle", flush=True)
+    e0, u0, _, _ = orc.solve(inp, n, nlev, nonlin=True, errtol=1e-8, mgkey=1)
+    it0, _, _, h0 = orc.last_stats()
+    print(f"  Newton iters gpu={it1} mixed={smx['iters']} oracle={it0} err fp64 "
+          f"{_rel(u1, u0):.2e} mixed {_rel(umx, u0):.2e}", flush=True)
+    assert e1 == e0 == 0 and smx["ierror"] == 0
+    assert it1 == it0 == len(h1) == len(h0)
+    assert np.allclose(h1, h0, rtol=1e-6, atol=1e-9 * h0[0])
+    assert _rel(u1, u0) <= 1e-9
+    assert _rel(umx, u0) <= 1e-9
+
+
+def test_grouped_workloads_equal_alone(mg, orc):
+    """The c0 (65^3) and c1 (161^3) workloads and a 129^3 linear solve of the
+    c3 molecule solved together with grouped iterations
+    (pmg_ctx_solve_group): each potential, iteration count and history is
+    that of its own solve, bit for bit, in both precisions."""
+    specs = []
+    for name in ("c0", "c1", "c3"):
+        c = mg.CONFIGS[name]
+        atoms = mg.pbe_molecule(c["natoms"], c["sphere_r"], c["charge_mode"], c["seed"])
+        specs.append((c["n"], c["nlev"], (c["h"], c["ionic"], atoms)))
+    for mode in (mg.PMG_MIXED, mg.PMG_FP64):
+        ctxs = [mg.Context(*mg.make_params(n, nlev), pbe=mol) for n, nlev, mol in specs]
+        alone = []
+        for c in ctxs:
+            c.solve(mode)
+            alone.append((c.u(), c.stats()["iters"], c.history()))
+        print(f"grouped workloads mode={mode}", flush=True)
+        mg.solve_group(ctxs, mode)
+        for (n, _, _), c, (u0, it0, h0) in zip(specs, ctxs, alone):
+            assert c.stats()["iters"] == it0 and np.array_equal(c.history(), h0)
+            assert np.array_equal(c.u(), u0), f"{n}^3 grouped potential differs"
+        for c in ctxs:
+            c.close()
