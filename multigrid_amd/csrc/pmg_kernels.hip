@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <type_traits>
+#include <utility>
 #include <cmath>
 #include <cstdio>
 
@@ -2469,108 +2470,176 @@ __host__ __device__ constexpr int gdelta(int s, int a) {
                    : (a == 2 ? 1 : -1);
 }
 
-// One stored entry XA(C, C + delta) (delta = output plane S):
-//   sum over fine F = C_f + d in the support of P_C and stencil neighbours
-//   G = F + e in the support of P_{C+delta} of P_C(d) A(F, G) P_{C+delta}(q),
-//   q = d + e - 2 delta.
-// All (d, e) pairs are pruned at compile time.
-template <int ND, int S>
-__device__ inline double rap_entry(const double *__restrict__ pc,
-                                   const double *__restrict__ ac, size_t nf,
-                                   size_t nc, size_t pC, size_t pC2,
-                                   size_t pCf, long long sy, long long sz) {
-  // no multiply-add contraction (the sums cancel; see k_buildP)
-#pragma clang fp contract(off)
-  double acc = 0.0;
-#pragma unroll
-  for (int a = 0; a < 27; a++) {
-    const int dx = a % 3 - 1, dy = (a / 3) % 3 - 1, dz = a / 9 - 1;
-    bool anyd = false;
-#pragma unroll
-    for (int b = 0; b < 27; b++) {
-      const int ex = b % 3 - 1, ey = (b / 3) % 3 - 1, ez = b / 9 - 1;
-      const int qx = dx + ex - 2 * gdelta(S, 0), qy = dy + ey - 2 * gdelta(S, 1),
-                qz = dz + ez - 2 * gdelta(S, 2);
-      const bool ok = (ND == 27 || (ex != 0) + (ey != 0) + (ez != 0) <= 1) &&
-                      qx >= -1 && qx <= 1 && qy >= -1 && qy <= 1 &&
-                      qz >= -1 && qz <= 1;
-      anyd |= ok;
-    }
-    if (!anyd) continue;
-    const double wC = pc[(size_t)pplane(dx, dy, dz) * nc + pC];
-    const size_t pF = (size_t)((long long)pCf + dx + dy * sy + dz * sz);
-    double part = 0.0;
-#pragma unroll
-    for (int b = 0; b < 27; b++) {
-      const int ex = b % 3 - 1, ey = (b / 3) % 3 - 1, ez = b / 9 - 1;
-      const int ne = (ex != 0) + (ey != 0) + (ez != 0);
-      if (ND == 7 && ne > 1) continue;
-      const int qx = dx + ex - 2 * gdelta(S, 0), qy = dy + ey - 2 * gdelta(S, 1),
-                qz = dz + ez - 2 * gdelta(S, 2);
-      if (qx < -1 || qx > 1 || qy < -1 || qy > 1 || qz < -1 || qz > 1)
-        continue;
-      const double A =
-          ne == 0 ? ac[pF]
-                  : -coupling<ND>(ac, nf, pF, ex + ey * sy + ez * sz, ex, ey, ez);
-      part += A * pc[(size_t)pplane(qx, qy, qz) * nc + pC2];
-    }
-    acc += wC * part;
-  }
-  return acc;
+// XA(C, C + delta) = P_C^T A P_{C+delta} is computed through w = A P_C:
+//   XA(C, C + delta) = sum over q in [-1, 1]^3 of P_{C+delta}(q) w(2 delta + q),
+//   w(r) = sum over d in [-1, 1]^3 of A(C_f + r, C_f + d) P_C(d),
+// offsets taken from C_f, the fine point of C.  w is supported on the stencil
+// neighbours of the box, r in [-2, 2]^3, and the stored (upper) directions
+// have delta_z in {0, 1}, so only the fine planes r_z = -1 .. 2 contribute.
+// Per coarse point this loads each fine coupling about once (135 values for
+// a 7-point fine operator, 729 for a 27-point one) and 27 + 151 (185) values
+// of P, against 1011 (2725) loads when each entry is summed over its (d, e)
+// pairs on its own.
+
+// Calls f(std::integral_constant<int, I>) for I = 0 .. N-1, so that every
+// index-derived offset, plane and test below is a constant expression (a
+// loop under #pragma unroll is not expanded when the body is this large).
+template <class F, int... I>
+__device__ __forceinline__ void static_for_(F &&f,
+                                            std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F &&f) {
+  static_for_(f, std::make_integer_sequence<int, N>{});
 }
 
-// Grid: (64-point runs of a coarse plane's interior, taken in i-fastest
-// order across row ends so that every lane has a point, coarse k x 14
-// entries); one thread per (coarse point, stored entry), so each coarse
-// point's 14 entries run in 14 different wavefronts (enough of them in
-// flight to hide the gather latency).
+__host__ __device__ constexpr bool in_box(int x, int y, int z) {
+  return x >= -1 && x <= 1 && y >= -1 && y <= 1 && z >= -1 && z <= 1;
+}
+// Fine offset r = (RX, RY, RZ) is 2 delta + q (q in the box) for some stored
+// direction delta.
+__host__ __device__ constexpr bool rap_used(int rx, int ry, int rz) {
+  bool u = false;
+  for (int s = 0; s < 14; s++)
+    u = u || in_box(rx - 2 * gdelta(s, 0), ry - 2 * gdelta(s, 1),
+                    rz - 2 * gdelta(s, 2));
+  return u;
+}
+// e is a stencil direction of an ND-point operator (or 0).
+__host__ __device__ constexpr bool stencil_dir(int nd, int ex, int ey, int ez) {
+  return in_box(ex, ey, ez) &&
+         (nd == 27 || (ex != 0) + (ey != 0) + (ez != 0) <= 1);
+}
+// w(r) has a term: r is a stencil neighbour of some d in the box.
+__host__ __device__ constexpr bool rap_any(int nd, int rx, int ry, int rz) {
+  bool a = false;
+  for (int d = 0; d < 27; d++)
+    a = a || stencil_dir(nd, d % 3 - 1 - rx, (d / 3) % 3 - 1 - ry,
+                         d / 9 - 1 - rz);
+  return a;
+}
+
+// A(F, F + e) at fine point F = pF: the diagonal, or the negated stored
+// coupling, held at F for an upper direction and at F + e otherwise.
+template <int EX, int EY, int EZ>
+__device__ __forceinline__ double afine(const double *__restrict__ ac,
+                                        size_t nf, size_t pF, long long sy,
+                                        long long sz) {
+  if constexpr (EX == 0 && EY == 0 && EZ == 0) {
+    return ac[pF];
+  } else if constexpr (upper_plane(EX, EY, EZ) >= 0) {
+    return -ac[(size_t)upper_plane(EX, EY, EZ) * nf + pF];
+  } else {
+    return -ac[(size_t)upper_plane(-EX, -EY, -EZ) * nf +
+               (size_t)((long long)pF + EX + EY * sy + EZ * sz)];
+  }
+}
+
+// Contributions of fine plane r_z = RZ to the 14 stored entries.
+template <int ND, int RZ>
+__device__ __forceinline__ void rap_plane(const double *__restrict__ pc,
+                                          const double *__restrict__ ac,
+                                          size_t nf, size_t nc, size_t pC,
+                                          size_t pCf, long long sy,
+                                          long long sz, long long cy,
+                                          long long cz, double (&acc)[14]) {
+  // no multiply-add contraction: the next level's prolongation divides
+  // differences of these entries that cancel where the row sums vanish, so
+  // they match the oracle only with the same roundings
+#pragma clang fp contract(off)
+  static_for<25>([&](auto tc) {
+    constexpr int t = decltype(tc)::value;
+    constexpr int rx = t % 5 - 2, ry = t / 5 - 2;
+    if constexpr (rap_used(rx, ry, RZ) && rap_any(ND, rx, ry, RZ)) {
+      const size_t pF = (size_t)((long long)pCf + rx + ry * sy + RZ * sz);
+      double w = 0.0;
+      static_for<27>([&](auto ac_) {
+        constexpr int a = decltype(ac_)::value;
+        constexpr int dx = a % 3 - 1, dy = (a / 3) % 3 - 1, dz = a / 9 - 1;
+        if constexpr (stencil_dir(ND, dx - rx, dy - ry, dz - RZ)) {
+#pragma clang fp contract(off)
+          w += afine<dx - rx, dy - ry, dz - RZ>(ac, nf, pF, sy, sz) *
+               pc[(size_t)pplane(dx, dy, dz) * nc + pC];
+        }
+      });
+      static_for<14>([&](auto sc) {
+        constexpr int s = decltype(sc)::value;
+        constexpr int qx = rx - 2 * gdelta(s, 0), qy = ry - 2 * gdelta(s, 1),
+                      qz = RZ - 2 * gdelta(s, 2);
+        if constexpr (in_box(qx, qy, qz)) {
+#pragma clang fp contract(off)
+          const size_t pC2 = (size_t)((long long)pC + gdelta(s, 0) +
+                                      gdelta(s, 1) * cy + gdelta(s, 2) * cz);
+          acc[s] += pc[(size_t)pplane(qx, qy, qz) * nc + pC2] * w;
+        }
+      });
+    }
+  });
+}
+
+// Workgroup: 64 consecutive interior points of a coarse plane (i fastest
+// across row ends, so short rows leave no lanes idle) x 4 wavefronts, wave
+// w summing the fine plane r_z = w - 1; the four partial sums of each entry
+// are added through LDS in plane order.  Tiles are XCD-contiguous (see
+// xcd_tile), so the fine planes that neighbouring coarse rows share are
+// fetched into one L2.
 template <int ND>
-__global__ __launch_bounds__(64) void k_buildG(int nxf, int nyf, int nzf,
-                                               int nxc, int nyc, int nzc,
-                                               const double *__restrict__ pc,
-                                               const double *__restrict__ ac,
-                                               double *__restrict__ acc,
-                                               float *__restrict__ acc32) {
-  const int q = blockIdx.x * 64 + threadIdx.x;
-  if (q >= (nxc - 2) * (nyc - 2)) return;
+__global__ __launch_bounds__(256) void k_buildG(int nxf, int nyf, int nzf,
+                                                int nxc, int nyc, int nzc,
+                                                const double *__restrict__ pc,
+                                                const double *__restrict__ ac,
+                                                double *__restrict__ acc,
+                                                float *__restrict__ acc32,
+                                                XG xg) {
+  __shared__ double part[4][14][64];
+  int bx, by, bz;
+  if (!xcd_tile(xg, bx, by, bz)) return;  // uniform over the workgroup
+  const int lane = threadIdx.x, w = threadIdx.y;
+  const int q = bx * 64 + lane;
+  const bool live = q < (nxc - 2) * (nyc - 2);
   const int ii = 2 + q % (nxc - 2);
   const int jj = 2 + q / (nxc - 2);
-  const int kk = 2 + blockIdx.z / 14;
-  const int sidx = blockIdx.z % 14;
+  const int kk = 2 + bz;
   const size_t nf = (size_t)nxf * nyf * nzf, nc = (size_t)nxc * nyc * nzc;
   const size_t pC = ix(nxc, nyc, ii, jj, kk);
   const size_t pCf = ix(nxf, nyf, 2 * ii - 1, 2 * jj - 1, 2 * kk - 1);
   const long long sy = nxf, sz = (long long)nxf * nyf;
   const long long cy = nxc, cz = (long long)nxc * nyc;
-  double v = 0.0;
-#define RAP_CASE(S)                                                          \
-  case S: {                                                                  \
-    const size_t pC2 = (size_t)((long long)pC + gdelta(S, 0) +              \
-                                gdelta(S, 1) * cy + gdelta(S, 2) * cz);      \
-    v = rap_entry<ND, S>(pc, ac, nf, nc, pC, pC2, pCf, sy, sz);              \
-    break;                                                                   \
+  double a[14];
+#pragma unroll
+  for (int s = 0; s < 14; s++) a[s] = 0.0;
+  if (live) {
+    switch (w) {
+      case 0: rap_plane<ND, -1>(pc, ac, nf, nc, pC, pCf, sy, sz, cy, cz, a); break;
+      case 1: rap_plane<ND, 0>(pc, ac, nf, nc, pC, pCf, sy, sz, cy, cz, a); break;
+      case 2: rap_plane<ND, 1>(pc, ac, nf, nc, pC, pCf, sy, sz, cy, cz, a); break;
+      default: rap_plane<ND, 2>(pc, ac, nf, nc, pC, pCf, sy, sz, cy, cz, a); break;
+    }
   }
-  switch (sidx) {
-    RAP_CASE(0) RAP_CASE(1) RAP_CASE(2) RAP_CASE(3) RAP_CASE(4) RAP_CASE(5)
-    RAP_CASE(6) RAP_CASE(7) RAP_CASE(8) RAP_CASE(9) RAP_CASE(10)
-    RAP_CASE(11) RAP_CASE(12) RAP_CASE(13)
+#pragma unroll
+  for (int s = 0; s < 14; s++) part[w][s][lane] = a[s];
+  __syncthreads();
+  if (!live) return;
+  for (int s = w; s < 14; s += 4) {
+    const double v = ((part[0][s][lane] + part[1][s][lane]) + part[2][s][lane]) +
+                     part[3][s][lane];
+    const double out = s == 0 ? v : -v;
+    acc[(size_t)s * nc + pC] = out;
+    if (acc32) acc32[(size_t)s * nc + pC] = (float)out;
   }
-#undef RAP_CASE
-  const double out = sidx == 0 ? v : -v;
-  acc[(size_t)sidx * nc + pC] = out;
-  if (acc32) acc32[(size_t)sidx * nc + pC] = (float)out;
 }
 
 void buildG(const Dim &f, const Dim &c, int numdia, const double *pc,
             const double *acf, double *acc, hipStream_t s, float *acc32) {
   if (c.nx < 3 || c.ny < 3 || c.nz < 3) return;
-  dim3 g(cdiv((c.nx - 2) * (c.ny - 2), 64), 1, (c.nz - 2) * 14);
+  dim3 g(cdiv((c.nx - 2) * (c.ny - 2), 64), 1, c.nz - 2);
   if (numdia == 7)
-    k_buildG<7><<<g, 64, 0, s>>>(f.nx, f.ny, f.nz, c.nx, c.ny, c.nz, pc, acf,
-                                 acc, acc32);
+    k_buildG<7><<<xcd_blocks(g), dim3(64, 4), 0, s>>>(
+        f.nx, f.ny, f.nz, c.nx, c.ny, c.nz, pc, acf, acc, acc32, xg_of(g));
   else
-    k_buildG<27><<<g, 64, 0, s>>>(f.nx, f.ny, f.nz, c.nx, c.ny, c.nz, pc, acf,
-                                  acc, acc32);
+    k_buildG<27><<<xcd_blocks(g), dim3(64, 4), 0, s>>>(
+        f.nx, f.ny, f.nz, c.nx, c.ny, c.nz, pc, acf, acc, acc32, xg_of(g));
   check_launch("buildG");
 }
 

@@ -352,13 +352,20 @@ void pmgo_buildP(int nxf, int nyf, int nzf, int nxc, int nyc, int nzc,
 /* The reference's VbuildG_7 / VbuildG_27 are expanded (Maple-generated)
  * forms of XA(C, C+delta) = sum_p P_C(p) (A P_{C+delta})(p) over the 27-point
  * supports of P; stored off-diagonals are -XA (buildGd.cc:527-1360,
- * 1449-3273).  Here the same sum is enumerated, grouped by the fine point
- * F = C_f + d of P_C's support: XA = sum_d P_C(d) * (sum_e A(F, F+e)
- * P_{C+delta}(d + e - 2 delta)) over the e with |d + e - 2 delta| <= 1 per
- * axis (d and e each in dz, dy, dx order, dx fastest).  The Maple forms fix
- * no summation order of their own that a restatement could follow term for
- * term; this grouping is the one the GPU kernel uses, so the two agree to
- * the last bit where neither contracts multiply-adds. */
+ * 1449-3273).  Here the same sum is taken through w = A P_C (A symmetric):
+ *   w(r) = sum_d A(C_f + r, C_f + d) P_C(d)      (d in [-1,1]^3, dz, dy, dx
+ *                                                 order, dx fastest),
+ *   XA(C, C+delta) = sum_q P_{C+delta}(q) w(2 delta + q),
+ * offsets from C_f, the fine point of C.  The second sum is grouped by the
+ * fine plane r_z = 2 delta_z + q_z (-1 .. 2 for the stored directions, which
+ * have delta_z in {0, 1}): within a plane over (r_y, r_x) ascending, then the
+ * four plane sums added in plane order.  The Maple forms fix no summation
+ * order of their own that a restatement could follow term for term; this
+ * grouping is the one the GPU kernel uses (one wavefront per plane), so the
+ * two agree to the last bit where neither contracts multiply-adds.  The
+ * prolongation of the next level divides differences of these entries that
+ * cancel where the row sums vanish, so any other grouping moves the deeper
+ * levels' P and Galerkin entries by ~1e-11 relative. */
 void pmgo_buildG(int nxf, int nyf, int nzf, int nxc, int nyc, int nzc,
                  int numdia, const double *pc, const double *acf,
                  double *acc) {
@@ -376,40 +383,54 @@ void pmgo_buildG(int nxf, int nyf, int nzf, int nxc, int nyc, int nzc,
       for (int ii = 2; ii <= nxc - 1; ii++) {
         size_t pC = IX(nxc, nyc, ii, jj, kk);
         size_t pCf = IX(nxf, nyf, 2 * ii - 1, 2 * jj - 1, 2 * kk - 1);
+        double w[5][5][5];
+        int has[5][5][5];
+        for (int rz = -2; rz <= 2; rz++)
+          for (int ry = -2; ry <= 2; ry++)
+            for (int rx = -2; rx <= 2; rx++) {
+              size_t pF = (size_t)((long long)pCf + rx * sx + ry * sy + rz * sz);
+              double v = 0.0;
+              int any = 0;
+              for (int dz = -1; dz <= 1; dz++)
+                for (int dy = -1; dy <= 1; dy++)
+                  for (int dx = -1; dx <= 1; dx++) {
+                    int ex = dx - rx, ey = dy - ry, ez = dz - rz;
+                    if (ex < -1 || ex > 1 || ey < -1 || ey > 1 || ez < -1 ||
+                        ez > 1)
+                      continue;
+                    int ne = (ex != 0) + (ey != 0) + (ez != 0);
+                    if (numdia == 7 && ne > 1) continue;
+                    any = 1;
+                    double A = ne == 0 ? acf[pF]
+                                       : -coupling(acf, nf, numdia, pF,
+                                                   ex * sx + ey * sy + ez * sz,
+                                                   ex, ey, ez);
+                    v += A * pc[(size_t)pplane(dx, dy, dz) * nc + pC];
+                  }
+              w[rz + 2][ry + 2][rx + 2] = v;
+              has[rz + 2][ry + 2][rx + 2] = any;
+            }
         for (int s = 0; s < 14; s++) {
           const int *dl = g_delta[s];
           size_t pC2 = (size_t)((long long)pC + dl[0] * cx + dl[1] * cy +
                                 dl[2] * cz);
-          double acc_v = 0.0;
-          for (int dz = -1; dz <= 1; dz++)
-            for (int dy = -1; dy <= 1; dy++)
-              for (int dx = -1; dx <= 1; dx++) {
-                size_t pF =
-                    (size_t)((long long)pCf + dx * sx + dy * sy + dz * sz);
-                double part = 0.0;
-                int any = 0;
-                for (int ez = -1; ez <= 1; ez++)
-                  for (int ey = -1; ey <= 1; ey++)
-                    for (int ex = -1; ex <= 1; ex++) {
-                      int qx = dx + ex - 2 * dl[0], qy = dy + ey - 2 * dl[1],
-                          qz = dz + ez - 2 * dl[2];
-                      if (qx < -1 || qx > 1 || qy < -1 || qy > 1 || qz < -1 ||
-                          qz > 1)
-                        continue;
-                      int ne = (ex != 0) + (ey != 0) + (ez != 0);
-                      if (numdia == 7 && ne > 1) continue;
-                      any = 1;
-                      double A = ne == 0 ? acf[pF]
-                                         : -coupling(acf, nf, numdia, pF,
-                                                     ex * sx + ey * sy +
-                                                         ez * sz,
-                                                     ex, ey, ez);
-                      part += A * pc[(size_t)pplane(qx, qy, qz) * nc + pC2];
-                    }
-                if (!any) continue;
-                acc_v += pc[(size_t)pplane(dx, dy, dz) * nc + pC] * part;
+          double plane[4];
+          for (int rz = -1; rz <= 2; rz++) {
+            double t = 0.0;
+            for (int ry = -2; ry <= 2; ry++)
+              for (int rx = -2; rx <= 2; rx++) {
+                int qx = rx - 2 * dl[0], qy = ry - 2 * dl[1],
+                    qz = rz - 2 * dl[2];
+                if (qx < -1 || qx > 1 || qy < -1 || qy > 1 || qz < -1 || qz > 1)
+                  continue;
+                if (!has[rz + 2][ry + 2][rx + 2]) continue;
+                t += pc[(size_t)pplane(qx, qy, qz) * nc + pC2] *
+                     w[rz + 2][ry + 2][rx + 2];
               }
-          acc[(size_t)s * nc + pC] = (s == 0) ? acc_v : -acc_v;
+            plane[rz + 1] = t;
+          }
+          double v = ((plane[0] + plane[1]) + plane[2]) + plane[3];
+          acc[(size_t)s * nc + pC] = (s == 0) ? v : -v;
         }
       }
 }
