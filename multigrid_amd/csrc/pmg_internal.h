@@ -211,7 +211,13 @@ void buildA(const Dim &d, double *ac, double *cc, double *fc,
             const double *gx, const double *gy, const double *gz,
             const double *a1, const double *a2, const double *a3,
             const double *ccf, const double *fcf, hipStream_t s,
-            float *ac32 = nullptr, float *cc32 = nullptr);
+            double *dg, float *ac32 = nullptr, float *cc32 = nullptr,
+            float *dg32 = nullptr);
+// The two restrictions of a level's setup (cc and fc, Vbuildgaler0
+// mgsubd.cc:370-372) in one pass over P, each sum in restrict_'s term order.
+void restrict2(const Dim &f, const Dim &c, const double *a, const double *b,
+               double *ao, double *bo, const double *pc, hipStream_t s,
+               float *ao32);
 void buildP(const Dim &f, const Dim &c, int mgprol, int numdia,
             const double *acf, double *pc, hipStream_t s,
             float *pc32 = nullptr);

@@ -1274,6 +1274,68 @@ void restrict_(const Dim &f, const Dim &c, const T *xin, T *xout, const T *pc,
   check_launch("restrict");
 }
 
+// Two sources restricted in one pass (the setup's cc and fc): every P plane
+// is loaded once for both sums, each of which keeps restrict_body's term
+// order, so the results are those of two k_restrict launches to the bit.
+__global__ __launch_bounds__(256) void k_restrict2(
+    int nxf, int nyf, int nxc, int nyc, int nzc, const double *__restrict__ a,
+    const double *__restrict__ b, double *__restrict__ ao,
+    double *__restrict__ bo, const double *__restrict__ pc, XG xg,
+    float *__restrict__ ao32) {
+#pragma clang fp contract(off)
+  int bx, by, bz;
+  if (!xcd_tile(xg, bx, by, bz)) return;
+  const int ii = 2 + bx * BX + threadIdx.x;
+  const int jj = 2 + by * BY + threadIdx.y;
+  const int kk = 2 + bz;
+  if (ii > nxc - 1 || jj > nyc - 1) return;
+  const size_t nc = (size_t)nxc * nyc * nzc, pC = ix(nxc, nyc, ii, jj, kk);
+  const size_t sy = nxf, sz = (size_t)nxf * nyf;
+  const size_t f = ix(nxf, nyf, 2 * ii - 1, 2 * jj - 1, 2 * kk - 1);
+  const long long off[9] = {0,
+                            (long long)sy,
+                            -(long long)sy,
+                            1,
+                            -1,
+                            1 + (long long)sy,
+                            -1 + (long long)sy,
+                            1 - (long long)sy,
+                            -1 - (long long)sy};
+  const long long zo[3] = {0, (long long)sz, -(long long)sz};
+  double ta[3], tb[3];
+#pragma unroll
+  for (int z = 0; z < 3; z++) {
+    const double w0 = pc[(size_t)(9 * z) * nc + pC];
+    double sa = w0 * a[f + zo[z]], sb = w0 * b[f + zo[z]];
+#pragma unroll
+    for (int q = 1; q < 9; q++) {
+      const double w = pc[(size_t)(9 * z + q) * nc + pC];
+      sa += w * a[f + zo[z] + off[q]];
+      sb += w * b[f + zo[z] + off[q]];
+    }
+    ta[z] = sa;
+    tb[z] = sb;
+  }
+  const double va = ta[0] + ta[1] + ta[2];
+  ao[pC] = va;
+  if (ao32) ao32[pC] = (float)va;
+  bo[pC] = tb[0] + tb[1] + tb[2];
+}
+
+void restrict2(const Dim &f, const Dim &c, const double *a, const double *b,
+               double *ao, double *bo, const double *pc, hipStream_t s,
+               float *ao32) {
+  if (c.nx < 3 || c.ny < 3 || c.nz < 3) return;
+  if (f.nx != 2 * c.nx - 1 || f.ny != 2 * c.ny - 1 || f.nz != 2 * c.nz - 1) {
+    fprintf(stderr, "pmg: restrict shape mismatch\n");
+    return;
+  }
+  dim3 g(cdiv(c.nx - 2, BX), cdiv(c.ny - 2, BY), c.nz - 2);
+  k_restrict2<<<xcd_blocks(g), dim3(BX, BY), 0, s>>>(
+      f.nx, f.ny, c.nx, c.ny, c.nz, a, b, ao, bo, pc, xg_of(g), ao32);
+  check_launch("restrict2");
+}
+
 enum : int {
   oPC, oPN, oPS, oPE, oPW, oPNE, oPNW, oPSE, oPSW,
   uPC, uPN, uPS, uPE, uPW, uPNE, uPNW, uPSE, uPSW,
@@ -2241,7 +2303,8 @@ __global__ __launch_bounds__(256) void k_buildA(
     const double *__restrict__ gz, const double *__restrict__ a1,
     const double *__restrict__ a2, const double *__restrict__ a3,
     const double *__restrict__ ccf, const double *__restrict__ fcf,
-    float *__restrict__ ac32, float *__restrict__ cc32) {
+    double *__restrict__ dg, float *__restrict__ ac32,
+    float *__restrict__ cc32, float *__restrict__ dg32) {
   // no multiply-add contraction: the operator is the oracle's to the bit
 #pragma clang fp contract(off)
   // BX*BY consecutive interior points of plane k, i fastest across row ends
@@ -2270,6 +2333,7 @@ __global__ __launch_bounds__(256) void k_buildA(
                     cU * a3[p] + cUm * a3[pzm];
   cc[p] = vcc;
   oC[p] = vC;
+  dg[p] = vC + vcc;  // the sweeps' and residuals' diagonal oC + cc
   const int ike = (i == nx - 1) ? 0 : 1;
   const double vE = ike * cE * a1[p];
   oE[p] = vE;
@@ -2295,6 +2359,7 @@ __global__ __launch_bounds__(256) void k_buildA(
   if (ac32) {  // the FP32 copies, rounded as a separate conversion would
     cc32[p] = (float)vcc;
     ac32[p] = (float)vC;
+    dg32[p] = (float)vC + (float)vcc;  // the sum of the FP32 copies
     ac32[n + p] = (float)vE;
     ac32[2 * n + p] = (float)vN;
     ac32[3 * n + p] = (float)vU;
@@ -2306,11 +2371,11 @@ void buildA(const Dim &d, double *ac, double *cc, double *fc,
             const double *gx, const double *gy, const double *gz,
             const double *a1, const double *a2, const double *a3,
             const double *ccf, const double *fcf, hipStream_t s,
-            float *ac32, float *cc32) {
+            double *dg, float *ac32, float *cc32, float *dg32) {
   dim3 g(cdiv((d.nx - 2) * (d.ny - 2), BX * BY), 1, d.nz - 2);
   k_buildA<<<g, dim3(BX, BY), 0, s>>>(d.nx, d.ny, d.nz, ac, cc, fc, xf, yf,
                                       zf, gx, gy, gz, a1, a2, a3, ccf, fcf,
-                                      ac32, cc32);
+                                      dg, ac32, cc32, dg32);
   check_launch("buildA");
 }
 

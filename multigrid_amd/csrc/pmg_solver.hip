@@ -402,19 +402,17 @@ void build_ops(pmg_ctx *c) {
   // kernels do not write is zero in both precisions).
   const bool w32 = c->have32;
   buildA(F.d, F.ac, F.cc, F.fc, c->xf, c->yf, c->zf, c->gx, c->gy, c->gz,
-         c->a1, c->a2, c->a3, c->ccf, c->fcf, s, w32 ? F.ac32 : nullptr,
-         w32 ? F.cc32 : nullptr);
-  diag_sum<double>(F.ac, F.cc, F.dg, F.d.n(), s);
-  if (w32) diag_sum<float>(F.ac32, F.cc32, F.dg32, F.d.n(), s);
+         c->a1, c->a2, c->a3, c->ccf, c->fcf, s, F.dg,
+         w32 ? F.ac32 : nullptr, w32 ? F.cc32 : nullptr,
+         w32 ? F.dg32 : nullptr);
   for (int l = 1; l < c->nlev; l++) {
     Level &Lf = c->L[l - 1], &Lc = c->L[l];
     buildP(Lf.d, Lc.d, c->p.mgprol, Lf.numdia, Lf.ac, Lc.pc, s,
            w32 ? Lc.pc32 : nullptr);
     buildG(Lf.d, Lc.d, Lf.numdia, Lc.pc, Lf.ac, Lc.ac, s,
            w32 ? Lc.ac32 : nullptr);
-    restrict_<double>(Lf.d, Lc.d, Lf.cc, Lc.cc, Lc.pc, s,
-                      w32 ? Lc.cc32 : nullptr);
-    restrict_<double>(Lf.d, Lc.d, Lf.fc, Lc.fc, Lc.pc, s);
+    restrict2(Lf.d, Lc.d, Lf.cc, Lf.fc, Lc.cc, Lc.fc, Lc.pc, s,
+              w32 ? Lc.cc32 : nullptr);
   }
   if (c->p.mgsolv == 1) {  // Vbuildband + Vdpbfa (mgsubd.cc:249-262)
     const Level &C = c->L[c->nlev - 1];
