@@ -247,8 +247,10 @@ __constant__ double c_exp2_trail[64] = {
     0x1.74853f3a5931ep-55,
 };
 
+// lead, trail: 2^(j/64) as lead + trail; sum: lead + trail, added once.
 __device__ __forceinline__ double exp_tab(double x, const double *lead,
-                                          const double *trail) {
+                                          const double *trail,
+                                          const double *sum) {
   const double N = rint(x * EXP_INV_L);
   const int n = (int)N;
   const double R = fma(-N, EXP_L2, fma(-N, EXP_L1, x));
@@ -259,7 +261,18 @@ __device__ __forceinline__ double exp_tab(double x, const double *lead,
   const double p = fma(R * R, q, R);  // e^R - 1
   const int j = n & 63, m = n >> 6;   // n = 64 m + j, 0 <= j < 64
   const double sl = lead[j], st = trail[j];
-  return ldexp(sl + fma(sl + st, p, st), m);
+  return ldexp(sl + fma(sum[j], p, st), m);
+}
+
+// 1/sqrt(x) for a positive, finite x: the hardware estimate y0 with one
+// second-order (Householder) refinement y0 (1 + t/2 + 3t^2/8),
+// t = 1 - x y0^2.  The library's rsqrt takes the same steps and then
+// re-selects y0 for zero, infinite and NaN estimates, a test a squared
+// distance to an atom never needs (three instructions per term saved).
+__device__ __forceinline__ double rsqrt_pos(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  const double t = fma(y * -x, y, 1.0);
+  return fma(y * t, fma(t, 0.375, 0.5), y);
 }
 
 constexpr int FACE_NT = 256, FACE_NODES = 64, FACE_SPLIT = FACE_NT / FACE_NODES;
@@ -274,7 +287,8 @@ __device__ __forceinline__ void face_terms(int t0, int t1, double va, double vb,
                                            const double *su, const double *sw,
                                            const double *sn, const double *sc,
                                            const double *tl, const double *tt,
-                                           double &s0, double &s1) {
+                                           const double *ts, double &s0,
+                                           double &s1) {
 #pragma clang fp contract(off)
   for (int t = t0; t < t1; t++) {
     const double du = va - su[t], dw = vb - sw[t];
@@ -284,9 +298,9 @@ __device__ __forceinline__ void face_terms(int t0, int t1, double va, double vb,
                               : (AX == 1 ? dus + d0s + dws : dus + dws + d0s);
     const double r1 = AX == 0 ? d1s + dus + dws
                               : (AX == 1 ? dus + d1s + dws : dus + dws + d1s);
-    const double i0 = rsqrt(r0), i1 = rsqrt(r1);
-    s0 += sc[t] * exp_tab(-kap * (r0 * i0), tl, tt) * i0;
-    s1 += sc[t] * exp_tab(-kap * (r1 * i1), tl, tt) * i1;
+    const double i0 = rsqrt_pos(r0), i1 = rsqrt_pos(r1);
+    s0 += sc[t] * exp_tab(-kap * (r0 * i0), tl, tt, ts) * i0;
+    s1 += sc[t] * exp_tab(-kap * (r1 * i1), tl, tt, ts) * i1;
   }
 }
 
@@ -307,10 +321,12 @@ __global__ __launch_bounds__(FACE_NT) void k_pb_faces(
 #pragma clang fp contract(off)
   __shared__ double sx[FACE_NT], sy[FACE_NT], sz[FACE_NT], sc[FACE_NT];
   __shared__ double part[2][FACE_SPLIT][FACE_NODES];
-  __shared__ double tl[64], tt[64];  // exp_tab's table (read after a barrier)
+  // exp_tab's tables (read after a barrier)
+  __shared__ double tl[64], tt[64], ts[64];
   if (threadIdx.x < 64) {
     tl[threadIdx.x] = c_exp2_lead[threadIdx.x];
     tt[threadIdx.x] = c_exp2_trail[threadIdx.x];
+    ts[threadIdx.x] = c_exp2_lead[threadIdx.x] + c_exp2_trail[threadIdx.x];
   }
   const int axis = blockIdx.y;
   const int lane = threadIdx.x % FACE_NODES, split = threadIdx.x / FACE_NODES;
@@ -336,13 +352,13 @@ __global__ __launch_bounds__(FACE_NT) void k_pb_faces(
     const int per = (m + FACE_SPLIT - 1) / FACE_SPLIT;
     const int t0 = split * per, t1 = min(m, t0 + per);
     if (axis == 0)
-      face_terms<0>(t0, t1, va, vb, lo, hi, kap, sy, sz, sx, sc, tl, tt, s0,
+      face_terms<0>(t0, t1, va, vb, lo, hi, kap, sy, sz, sx, sc, tl, tt, ts, s0,
                     s1);
     else if (axis == 1)
-      face_terms<1>(t0, t1, va, vb, lo, hi, kap, sx, sz, sy, sc, tl, tt, s0,
+      face_terms<1>(t0, t1, va, vb, lo, hi, kap, sx, sz, sy, sc, tl, tt, ts, s0,
                     s1);
     else
-      face_terms<2>(t0, t1, va, vb, lo, hi, kap, sx, sy, sz, sc, tl, tt, s0,
+      face_terms<2>(t0, t1, va, vb, lo, hi, kap, sx, sy, sz, sc, tl, tt, ts, s0,
                     s1);
   }
   part[0][split][lane] = s0;
