@@ -129,6 +129,30 @@ def test_buildP_buildG_noncubic(mg, orc, numdia, mgprol):
     assert np.abs(acco).max() > 0 and _rel(acc, acco) < 1e-12
 
 
+@pytest.mark.parametrize("df", [(5, 5, 5), (7, 5, 9), (9, 17, 5), (131, 7, 7)])
+@pytest.mark.parametrize("numdia", [7, 27])
+def test_buildG_shapes_bitwise(mg, orc, numdia, df):
+    """The Galerkin product (w = A P_C, one wavefront per fine plane, partial
+    sums added through LDS) on the smallest and most uneven coarsenings: one
+    coarse interior point, fewer coarse points per plane than lanes, 128
+    coarse points per plane (two full workgroups).  Same grouping and
+    roundings as the oracle, so the entries are equal to the bit."""
+    print(f"buildG shapes numdia={numdia} df={df}", flush=True)
+    dc = tuple((v - 1) // 2 + 1 for v in df)
+    Nf, Nc = df[0] * df[1] * df[2], dc[0] * dc[1] * dc[2]
+    rng = np.random.default_rng(sum(df) + numdia)
+    nplanes = 4 if numdia == 7 else 14
+    ac = rng.uniform(0.1, 1.0, nplanes * Nf)
+    ac[:Nf] += 2.0 * (nplanes - 1)
+    pc = np.zeros(27 * Nc)
+    orc.olib().pmgo_buildP(*df, *dc, 1, numdia, dp(ac), dp(pc))
+    acc, acco = np.zeros(14 * Nc), np.zeros(14 * Nc)
+    _call(mg, "pmg_buildG", *df, *dc, numdia, dp(pc), dp(ac), dp(acc))
+    orc.olib().pmgo_buildG(*df, *dc, numdia, dp(pc), dp(ac), dp(acco))
+    assert np.abs(acco).max() > 0
+    assert np.array_equal(acc, acco)
+
+
 @pytest.mark.parametrize("mgprol", [0, 1])
 @pytest.mark.parametrize("numdia", [7, 27])
 def test_buildP_buildG_transfer(mg, orc, numdia, mgprol):
