@@ -160,13 +160,13 @@ def cpu_baseline(mg, stream_sizes, threads):
                      "1e-6); value = 1 / mean request time over the stream's size mix",
            "per_size": per_size}
     us = {}
-    for name in ("c1", "c3", "c2"):
+    for name in ("pb161", "newton129", "pb289"):
         c = mg.CONFIGS[name]
         atoms = mg.pbe_molecule(c["natoms"], c["sphere_r"], c["charge_mode"], c["seed"])
         t0 = time.perf_counter()
         inp = orc.pbe_assemble(c["n"], c["h"], c["ionic"], atoms)
         t1 = time.perf_counter()
-        if name == "c3":
+        if name == "newton129":
             orc.mypdefinit([1.0, -1.0], [0.5, 0.5])
             _, u, _, _ = orc.solve(inp, c["n"], c["nlev"], nonlin=True, errtol=1e-8)
         else:
@@ -175,7 +175,7 @@ def cpu_baseline(mg, stream_sizes, threads):
         out[name] = {"assembly_s": t1 - t0, "solve_s": t2 - t1,
                      "iters": orc.last_stats()[0]}
         us[name] = (u, orc.last_stats()[0])
-    c = mg.CONFIGS["c0"]
+    c = mg.CONFIGS["pb65"]
     atoms = mg.pbe_molecule(c["natoms"], c["sphere_r"], c["charge_mode"], c["seed"])
     inp = orc.pbe_assemble(c["n"], c["h"], c["ionic"], atoms)
     _omp_threads(1)
@@ -183,7 +183,7 @@ def cpu_baseline(mg, stream_sizes, threads):
     orc.solve(inp, c["n"], c["nlev"])
     t1 = time.perf_counter()
     _omp_threads(nthreads)
-    out["c0_one_thread"] = {"solve_s": t1 - t0, "vcycles": orc.last_stats()[0]}
+    out["pb65_one_thread"] = {"solve_s": t1 - t0, "vcycles": orc.last_stats()[0]}
     return out, us
 
 
@@ -337,7 +337,7 @@ def newton_leg(mg, oracle=None):
     """The 129^3 nonlinear PBE (sinh ionic term, 0.3 M 1:1 salt, 800 atoms):
     Newton with FP64 and with mixed-precision inner V-cycles to 1e-8, and
     (with the CPU leg) the error against the FP64 oracle's Newton."""
-    cfg = mg.CONFIGS["c3"]
+    cfg = mg.CONFIGS["newton129"]
     n, nlev = cfg["n"], cfg["nlev"]
     atoms = mg.pbe_molecule(cfg["natoms"], cfg["sphere_r"], cfg["charge_mode"], cfg["seed"])
     ions = ([1.0, -1.0], [0.5, 0.5])
@@ -370,7 +370,7 @@ def band_leg(mg, cpu, steps=5):
     Vbuildband + Vdpbfa at setup, Vdpbsl per V-cycle) instead of 100 CG
     iterations: FP64 and mixed solves (assembly + build + V-cycles) and,
     with the CPU leg, the error against the oracle's mgsolv = 1 solve."""
-    cfg = mg.CONFIGS["c1"]
+    cfg = mg.CONFIGS["pb161"]
     n, nlev = cfg["n"], cfg["nlev"]
     atoms = mg.pbe_molecule(cfg["natoms"], cfg["sphere_r"], cfg["charge_mode"], cfg["seed"])
     mol = (cfg["h"], cfg["ionic"], atoms)
@@ -405,7 +405,7 @@ def pcg_leg(mg, cpu, steps=5):
     the same hierarchy, to the same 1e-6 relative L1 residual: FP64 and with
     the FP32 V-cycle preconditioner; with the CPU leg, the error against the
     oracle's class-style solve."""
-    cfg = mg.CONFIGS["c1"]
+    cfg = mg.CONFIGS["pb161"]
     n, nlev = cfg["n"], cfg["nlev"]
     atoms = mg.pbe_molecule(cfg["natoms"], cfg["sphere_r"], cfg["charge_mode"], cfg["seed"])
     mol = (cfg["h"], cfg["ionic"], atoms)
@@ -498,11 +498,11 @@ def main():
     extra = {}
     legs = set() if args.no_extra else set(args.legs.split(","))
     if "single" in legs:
-        extra["single_161"] = single_leg(mg, "c1", args.single_steps, oracle_u.get("c1"))
+        extra["single_161"] = single_leg(mg, "pb161", args.single_steps, oracle_u.get("pb161"))
     if "large" in legs:
-        extra["large_289"] = single_leg(mg, "c2", args.single_steps, oracle_u.get("c2"))
+        extra["large_289"] = single_leg(mg, "pb289", args.single_steps, oracle_u.get("pb289"))
     if "newton" in legs:
-        extra["newton_129"] = newton_leg(mg, oracle_u.get("c3"))
+        extra["newton_129"] = newton_leg(mg, oracle_u.get("newton129"))
     if "band" in legs:
         extra["band_161"] = band_leg(mg, not args.no_cpu)
     if "pcg" in legs:

@@ -527,10 +527,10 @@ def dx_read(path):
 # Benchmark configurations (sizes and molecule parameters of the five
 # workloads this solver is measured on).
 CONFIGS = {
-    "c0": dict(n=65, h=0.5, nlev=4, natoms=200, sphere_r=14.0, ionic=0.15, charge_mode=0, seed=20261202),
-    "c1": dict(n=161, h=0.4, nlev=5, natoms=1000, sphere_r=25.0, ionic=0.15, charge_mode=0, seed=20261203),
-    "c2": dict(n=289, h=0.35, nlev=6, natoms=5000, sphere_r=45.0, ionic=0.15, charge_mode=1, seed=20261204),
-    "c3": dict(n=129, h=0.5, nlev=5, natoms=800, sphere_r=20.0, ionic=0.30, charge_mode=0, seed=20261205),
+    "pb65": dict(n=65, h=0.5, nlev=4, natoms=200, sphere_r=14.0, ionic=0.15, charge_mode=0, seed=20261202),
+    "pb161": dict(n=161, h=0.4, nlev=5, natoms=1000, sphere_r=25.0, ionic=0.15, charge_mode=0, seed=20261203),
+    "pb289": dict(n=289, h=0.35, nlev=6, natoms=5000, sphere_r=45.0, ionic=0.15, charge_mode=1, seed=20261204),
+    "newton129": dict(n=129, h=0.5, nlev=5, natoms=800, sphere_r=20.0, ionic=0.30, charge_mode=0, seed=20261205),
 }
 
 

@@ -149,8 +149,8 @@ def _linear_config(mg, orc, name):
                 umx=umx, smx=smx, e0=e0, u0=u0, iw0=iw0, rw0=rw0, it0=it0, rr0=rr0)
 
 
-def test_c1_161_fp64_and_mixed_vs_oracle(mg, orc):
-    r = _linear_config(mg, orc, "c1")
+def test_pb161_fp64_and_mixed_vs_oracle(mg, orc):
+    r = _linear_config(mg, orc, "pb161")
     n, nlev = r["n"], r["nlev"]
     worst = _check_operators(mg, orc, n, nlev, r["rw1"], r["rw0"])
     print(f"  V-cycles gpu={r['it1']} oracle={r['it0']} potential err "
@@ -167,8 +167,8 @@ def test_c1_161_fp64_and_mixed_vs_oracle(mg, orc):
     assert _rel(r["umx"], r["u0"]) <= 1e-6
 
 
-def test_c2_289_fp64_and_mixed_vs_oracle(mg, orc):
-    r = _linear_config(mg, orc, "c2")
+def test_pb289_fp64_and_mixed_vs_oracle(mg, orc):
+    r = _linear_config(mg, orc, "pb289")
     n, nlev = r["n"], r["nlev"]
     worst = _check_operators(mg, orc, n, nlev, r["rw1"], r["rw0"])
     host64 = _host_relres(orc, n, r["inp"], r["u1"])
@@ -187,8 +187,8 @@ def test_c2_289_fp64_and_mixed_vs_oracle(mg, orc):
     assert abs(host64 - r["rr1"]) <= 1e-9 and abs(hostmx - r["smx"]["relres"]) <= 1e-9
 
 
-def test_c3_129_newton_vs_oracle(mg, orc):
-    c = mg.CONFIGS["c3"]
+def test_newton129_vs_oracle(mg, orc):
+    c = mg.CONFIGS["newton129"]
     n, nlev = c["n"], c["nlev"]
     ions = ([1.0, -1.0], [0.5, 0.5])
     mg.Vmypdefinitnpbe(2, *ions)
@@ -196,15 +196,15 @@ def test_c3_129_newton_vs_oracle(mg, orc):
     atoms = mg.pbe_molecule(c["natoms"], c["sphere_r"], c["charge_mode"], c["seed"])
     inp = orc.pbe_assemble(n, c["h"], c["ionic"], atoms)
     iparm, rparm = mg.make_params(n, nlev, nonlin=1, errtol=1e-8)
-    print(f"c3: {n}^3 Newton FP64 drop-in on the GPU", flush=True)
+    print(f"newton129: {n}^3 Newton FP64 drop-in on the GPU", flush=True)
     e1, u1, _, _ = _drv(mg, mg.Vnewdriv, inp, iparm, rparm)
     it1 = mg.last_stats()[0]
-    print("c3: Newton with FP32 inner V-cycles", flush=True)
+    print("newton129: Newton with FP32 inner V-cycles", flush=True)
     ctx = mg.Context(iparm, rparm, inp)
     ctx.solve(mg.PMG_MIXED)
     umx, smx = ctx.u(), ctx.stats()
     ctx.close()
-    print("c3: oracle", flush=True)
+    print("newton129: oracle", flush=True)
     e0, u0, _, _ = orc.solve(inp, n, nlev, nonlin=True, errtol=1e-8)
     it0 = orc.last_stats()[0]
     print(f"  Newton iters gpu={it1} mixed={smx['iters']} oracle={it0} err fp64 "
@@ -215,12 +215,12 @@ def test_c3_129_newton_vs_oracle(mg, orc):
     assert _rel(umx, u0) <= 1e-9
 
 
-def test_c3_129_nested_newton_vs_oracle(mg, orc):
-    """The c3 workload with the nested iteration (mgkey 1, Vfnewton
+def test_newton129_nested_vs_oracle(mg, orc):
+    """The 129^3 Newton workload with the nested iteration (mgkey 1, Vfnewton
     newtond.cc:60-138): the same finest-level Newton iterations and
     per-iteration residuals as the oracle, the potential within 1e-9, in the
     drop-in and through a resident context with FP32 inner cycles."""
-    c = mg.CONFIGS["c3"]
+    c = mg.CONFIGS["newton129"]
     n, nlev = c["n"], c["nlev"]
     ions = ([1.0, -1.0], [0.5, 0.5])
     mg.Vmypdefinitnpbe(2, *ions)
@@ -228,15 +228,15 @@ def test_c3_129_nested_newton_vs_oracle(mg, orc):
     atoms = mg.pbe_molecule(c["natoms"], c["sphere_r"], c["charge_mode"], c["seed"])
     inp = orc.pbe_assemble(n, c["h"], c["ionic"], atoms)
     iparm, rparm = mg.make_params(n, nlev, nonlin=1, errtol=1e-8, mgkey=1)
-    print(f"c3 nested: {n}^3 Newton from the coarsest level, FP64 drop-in", flush=True)
+    print(f"newton129 nested: {n}^3 Newton from the coarsest level, FP64 drop-in", flush=True)
     e1, u1, _, _ = _drv(mg, mg.Vnewdriv, inp, iparm, rparm)
     it1, h1 = mg.last_stats()[0], mg.last_history()
-    print("c3 nested: FP32 inner V-cycles", flush=True)
+    print("newton129 nested: FP32 inner V-cycles", flush=True)
     ctx = mg.Context(iparm, rparm, inp)
     ctx.solve(mg.PMG_MIXED)
     umx, smx = ctx.u(), ctx.stats()
     ctx.close()
-    print("c3 nested: oracle", flush=True)
+    print("newton129 nested: oracle", flush=True)
     e0, u0, _, _ = orc.solve(inp, n, nlev, nonlin=True, errtol=1e-8, mgkey=1)
     it0, _, _, h0 = orc.last_stats()
     print(f"  Newton iters gpu={it1} mixed={smx['iters']} oracle={it0} err fp64 "
@@ -249,12 +249,12 @@ def test_c3_129_nested_newton_vs_oracle(mg, orc):
 
 
 def test_grouped_workloads_equal_alone(mg, orc):
-    """The c0 (65^3) and c1 (161^3) workloads and a 129^3 linear solve of the
-    c3 molecule solved together with grouped iterations
+    """The 65^3 and 161^3 workloads and a 129^3 linear solve of the Newton
+    workload's molecule solved together with grouped iterations
     (pmg_ctx_solve_group): each potential, iteration count and history is
     that of its own solve, bit for bit, in both precisions."""
     specs = []
-    for name in ("c0", "c1", "c3"):
+    for name in ("pb65", "pb161", "newton129"):
         c = mg.CONFIGS[name]
         atoms = mg.pbe_molecule(c["natoms"], c["sphere_r"], c["charge_mode"], c["seed"])
         specs.append((c["n"], c["nlev"], (c["h"], c["ionic"], atoms)))

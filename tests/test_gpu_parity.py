@@ -344,12 +344,12 @@ def test_error_paths(mg, orc):
     assert e == 0 and np.all(u == 0.0)
 
 
-def test_full_size_c1_properties(mg, orc):
+def test_full_size_pb161_properties(mg, orc):
     """161^3 / 5 levels: the FP64 GPU solution's L1 relative residual,
     recomputed by the oracle on the host, meets 1e-6; the mixed-precision
     solution agrees with it to 1e-6."""
-    print("c1 161^3 full-size properties", flush=True)
-    c = mg.CONFIGS["c1"]
+    print("pb161 161^3 full-size properties", flush=True)
+    c = mg.CONFIGS["pb161"]
     n, nlev = c["n"], c["nlev"]
     atoms = mg.pbe_molecule(c["natoms"], c["sphere_r"], c["charge_mode"], c["seed"])
     inp = orc.pbe_assemble(n, c["h"], c["ionic"], atoms)

@@ -35,7 +35,7 @@ def _clustered(n, h, seed):
 
 CASES = [
     ("tiny", 17, 0.5, 0.15, lambda mg: mg.pbe_molecule(8, 2.0, 0, 1)),
-    ("c0-like", 65, 0.5, 0.15, lambda mg: mg.pbe_molecule(200, 14.0, 0, 20261202)),
+    ("pb65-like", 65, 0.5, 0.15, lambda mg: mg.pbe_molecule(200, 14.0, 0, 20261202)),
     ("mode1", 33, 0.35, 0.3, lambda mg: mg.pbe_molecule(120, 4.5, 1, 11)),
     ("clustered", 21, 0.4, 0.15, lambda mg: _clustered(21, 0.4, 5)),
     ("no atoms", 9, 0.5, 0.15, lambda mg: np.zeros((0, 5))),

@@ -31,7 +31,7 @@ def _drive(mg, inp, n, nlev, num_iters=0, **kw):
 @pytest.mark.parametrize("n,nlev,natoms,R", [(33, 3, 40, 6.0), (65, 4, 200, 14.0)])
 def test_pcgdriv_parity(mg, orc, n, nlev, natoms, R):
     print(f"pcg parity n={n} nlev={nlev}", flush=True)
-    c, inp = orc.config_inputs("c0", n=n, natoms=natoms, sphere_r=R)
+    c, inp = orc.config_inputs("pb65", n=n, natoms=natoms, sphere_r=R)
     e1, u1, iw1, rw1 = _drive(mg, inp, n, nlev)
     it1, ie1, rr1 = mg.last_stats()
     e0, u0, iw0, rw0 = orc.pcg(inp, n, nlev)
@@ -50,7 +50,7 @@ def test_pcg_counting_criterion(mg, orc):
     reference's counting_criterion (CG_Solver.h:38-43)."""
     print("pcg counting n=33", flush=True)
     n, nlev = 33, 3
-    c, inp = orc.config_inputs("c0", n=n, natoms=40, sphere_r=6.0)
+    c, inp = orc.config_inputs("pb65", n=n, natoms=40, sphere_r=6.0)
     e1, u1, _, _ = _drive(mg, inp, n, nlev, num_iters=3)
     it1 = mg.last_stats()[0]
     e0, u0, _, _ = orc.pcg(inp, n, nlev, num_iters=3)
@@ -62,7 +62,7 @@ def test_pcg_counting_criterion(mg, orc):
 def test_ctx_pcg_fp64_and_mixed(mg, orc):
     print("ctx pcg fp64 + mixed n=65", flush=True)
     n, nlev = 65, 4
-    c, inp = orc.config_inputs("c0", n=n, natoms=200, sphere_r=14.0)
+    c, inp = orc.config_inputs("pb65", n=n, natoms=200, sphere_r=14.0)
     iparm, rparm = mg.make_params(n, nlev)
     ctx = mg.Context(iparm, rparm, inp)
     ctx.pcg(mg.PMG_FP64)
@@ -86,7 +86,7 @@ def test_ctx_pcg_fp64_and_mixed(mg, orc):
 def test_pcg_needs_two_levels(mg, orc):
     print("pcg nlev=1 refused", flush=True)
     n = 9
-    c, inp = orc.config_inputs("c0", n=n, natoms=6, sphere_r=2.5, h=1.0)
+    c, inp = orc.config_inputs("pb65", n=n, natoms=6, sphere_r=2.5, h=1.0)
     with pytest.raises(mg.PMGError) as ex:
         _drive(mg, inp, n, 1)
     assert ex.value.code == mg.PMG_EBADPARM

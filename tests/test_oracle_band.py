@@ -203,7 +203,7 @@ def test_driver_nlev1_direct_solve_is_exact_without_salt(orc):
     (mgcsd.cc:195-212); with no ions cc = 0, so the result is the exact
     discrete solution of the 7-point operator."""
     n = 9
-    c, inp = orc.config_inputs("c0", n=n, h=1.0, natoms=6, sphere_r=2.5, ionic=0.0)
+    c, inp = orc.config_inputs("pb65", n=n, h=1.0, natoms=6, sphere_r=2.5, ionic=0.0)
     ierr, u, iw, rw = orc.solve(inp, n, 1, mgsolv=1)
     assert ierr == 0
     it, ie, rr, hist = orc.last_stats()
@@ -214,7 +214,7 @@ def test_driver_nlev1_band_omits_cc(orc):
     """With ions the band still carries oC alone, so the one-level direct
     solve returns (A - diag(cc))^{-1} fc on the interior."""
     n = 9
-    c, inp = orc.config_inputs("c0", n=n, h=1.0, natoms=6, sphere_r=2.5)
+    c, inp = orc.config_inputs("pb65", n=n, h=1.0, natoms=6, sphere_r=2.5)
     ierr, u, iw, rw = orc.solve(inp, n, 1, mgsolv=1)
     sz = orc.mgsz(2, 0, 1, n, n, n, 1)[1]
     nrpc, narr = int(sz[7]), int(sz[5])
@@ -268,7 +268,7 @@ def test_driver_reports_indefinite_band(orc):
 
 def test_mgsolv_out_of_range_rejected(orc):
     n = 17
-    c, inp = orc.config_inputs("c0", n=n, natoms=10, sphere_r=3.0)
+    c, inp = orc.config_inputs("pb65", n=n, natoms=10, sphere_r=3.0)
     import multigrid_amd as mg
     iparm, rparm = mg.make_params(n, 2)
     iparm[20] = 2
@@ -284,6 +284,6 @@ def test_band_refused_with_one_interior_column(orc):
     """A 3-point coarsest grid (one interior point per direction) would put
     band offsets on the diagonal and past the band column: refused."""
     n = 17
-    c, inp = orc.config_inputs("c0", n=n, natoms=10, sphere_r=3.0)
+    c, inp = orc.config_inputs("pb65", n=n, natoms=10, sphere_r=3.0)
     ierr, *_ = orc.solve(inp, n, 4, mgsolv=1)
     assert ierr == 2

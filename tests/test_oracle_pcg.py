@@ -59,7 +59,7 @@ def test_pcg_agrees_with_pmg(orc, n, nlev):
     """To a relative L1 residual of 1e-6 the class-style PCG and the PMG
     V-cycle driver give the same potential to the tolerance, the PCG in
     fewer outer iterations than PMG has V-cycles (each costs one V-cycle)."""
-    c, inp = orc.config_inputs("c0", n=n, natoms=40 if n == 33 else 200,
+    c, inp = orc.config_inputs("pb65", n=n, natoms=40 if n == 33 else 200,
                                sphere_r=6.0 if n == 33 else 14.0)
     e0, u0, _, _ = orc.solve(inp, n, nlev)
     it0 = orc.last_stats()[0]

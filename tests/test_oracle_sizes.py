@@ -25,7 +25,7 @@ def test_mgsz_closed_form(orc, n, nlev):
     assert out[11] == 150 * (nlev + 1)
 
 
-def test_mgsz_survey_numbers(orc):
+def test_mgsz_workload_numbers(orc):
     """Sizes quoted for the 65^3 / 161^3 / 289^3 workloads."""
     _, o = orc.mgsz(2, 0, 0, 65, 65, 65, 4)
     assert (o[3], o[5], o[6], o[10]) == (274625, 316204, 41579, 3436147)
