@@ -675,22 +675,41 @@ __device__ __forceinline__ void gs27_row(
     n1[q] = a1[q];
   }
   const T xb1 = xO[rb];  // i = 1 boundary
-  // One lane pair per lane (rows up to 129 points): every load of both
-  // points is issued before any arithmetic, so the row waits for memory
-  // once; only the in-row terms wait for the first colour.  Longer rows
-  // load each point's operands just before its update.
+  // One lane pair per lane (rows up to 129 points), and two in FP32 (rows
+  // up to 257 points, 4 x 53 values): every load of the lane's points is
+  // issued before any arithmetic, so the row waits for memory once; only
+  // the in-row terms wait for the first colour.  FP64 rows of up to 257
+  // points load each colour's two points together (two round trips instead
+  // of four); longer rows load each point's operands just before its
+  // update.  Same arithmetic in every case.
   G27Part<T> g0[PPL], g1[PPL];
-  if (PPL == 1) {
+  constexpr bool PRE = PPL == 1 || (PPL == 2 && sizeof(T) == 4);
+  if constexpr (PPL == 1) {
     const G27Raw<T> r0 = load(i0[0]), r1 = load(i1[0]);
     __builtin_amdgcn_sched_barrier(0);
     g0[0] = part(r0);
     g1[0] = part(r1);
+  } else if constexpr (PRE) {
+    const G27Raw<T> r00 = load(i0[0]), r10 = load(i1[0]), r01 = load(i0[1]),
+                    r11 = load(i1[1]);
+    __builtin_amdgcn_sched_barrier(0);
+    g0[0] = part(r00);
+    g1[0] = part(r10);
+    g0[1] = part(r01);
+    g1[1] = part(r11);
   }
+  constexpr bool PRE1 = !PRE && PPL == 2;
   auto part0 = [&](int q) -> G27Part<T> {
-    return PPL == 1 ? g0[0] : part(load(i0[q]));
+    return PRE ? g0[q] : part(load(i0[q]));
   };
   auto part1 = [&](int q) -> G27Part<T> {
-    return PPL == 1 ? g1[0] : part(load(i1[q]));
+    return PRE ? g1[q] : part(load(i1[q]));
+  };
+  auto batch = [&](const int *ii, G27Part<T> *g) {
+    const G27Raw<T> ra = load(ii[0]), rb = load(ii[1]);
+    __builtin_amdgcn_sched_barrier(0);
+    g[0] = part(ra);
+    g[1] = part(rb);
   };
   // value of the previous pair's second point / next pair's first point
   // (every lane takes part in the shuffles)
@@ -707,29 +726,33 @@ __device__ __forceinline__ void gs27_row(
   };
   if (ADJ == 0) {
     // colour i%2 = 0 (points i0) from the old odd values, then i%2 = 1
+    if constexpr (PRE1) batch(i0, g0);
 #pragma unroll
     for (int q = 0; q < PPL; q++) {
       const T w = west(a1, q);
-      const G27Part<T> g = part0(q);
+      const G27Part<T> g = PRE1 ? g0[q] : part0(q);
       if (v0[q]) n0[q] = fin(g, w, a1[q]);
     }
+    if constexpr (PRE1) batch(i1, g1);
 #pragma unroll
     for (int q = 0; q < PPL; q++) {
       const T e = east(n0, q);
-      const G27Part<T> g = part1(q);
+      const G27Part<T> g = PRE1 ? g1[q] : part1(q);
       if (v1[q]) n1[q] = fin(g, n0[q], e);
     }
   } else {
+    if constexpr (PRE1) batch(i1, g1);
 #pragma unroll
     for (int q = 0; q < PPL; q++) {
       const T e = east(a0, q);
-      const G27Part<T> g = part1(q);
+      const G27Part<T> g = PRE1 ? g1[q] : part1(q);
       if (v1[q]) n1[q] = fin(g, a0[q], e);
     }
+    if constexpr (PRE1) batch(i0, g0);
 #pragma unroll
     for (int q = 0; q < PPL; q++) {
       const T w = west(n1, q);
-      const G27Part<T> g = part0(q);
+      const G27Part<T> g = PRE1 ? g0[q] : part0(q);
       if (v0[q]) n0[q] = fin(g, w, n1[q]);
     }
   }

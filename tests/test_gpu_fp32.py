@@ -86,8 +86,10 @@ def _abs_matvec(orc, dims, numdia, ac, cc, x):
     return y
 
 
+# (145, 37, 21), 27 points: rows longer than 129 points (two point pairs per
+# lane, all four points' operands loaded before any update)
 CASES = [((17, 17, 17), 7), ((17, 17, 17), 27), ((97, 97, 97), 7), ((97, 97, 97), 27),
-         ((33, 25, 19), 7), ((33, 25, 19), 27)]
+         ((33, 25, 19), 7), ((33, 25, 19), 27), ((145, 37, 21), 27)]
 
 
 @pytest.mark.parametrize("dims,numdia", CASES)
