@@ -958,7 +958,27 @@ __device__ __forceinline__ void resid_body(
   const int j = 2 + by * BY + threadIdx.y;
   const int k0 = 2 + bz * kc;
   double acc = 0.0;
-  if (tok && i <= nx - 1 && j <= ny - 1) {
+  if (ND == 7 && kc == KCH && tok && i <= nx - 1 && j <= ny - 1) {
+    // the chunk's planes with a fixed trip count and clamped (always valid)
+    // loads, so every plane's loads can be issued before the first result
+    T vv[KCH];
+#pragma unroll
+    for (int kk = 0; kk < KCH; kk++) {
+      const int k = min(k0 + kk, nz - 1);
+      const size_t p = ix(nx, ny, i, j, k);
+      const T *oE = ac + n, *oN = ac + 2 * n, *uC = ac + 3 * n;
+      const T dgp = DG ? dg[p] : ac[p] + cc[p];
+      vv[kk] = fc[p] + oN[p] * x[p + sy] + oN[p - sy] * x[p - sy] +
+               oE[p] * x[p + 1] + oE[p - 1] * x[p - 1] +
+               uC[p - sz] * x[p - sz] + uC[p] * x[p + sz] - dgp * x[p];
+    }
+#pragma unroll
+    for (int kk = 0; kk < KCH; kk++) {
+      if (k0 + kk > nz - 1) break;
+      if (r) r[ix(nx, ny, i, j, k0 + kk)] = (TO)vv[kk];
+      acc += fabs((double)vv[kk]);
+    }
+  } else if (tok && i <= nx - 1 && j <= ny - 1) {
     for (int kk = 0; kk < kc; kk++) {
       const int k = k0 + kk;
       if (k > nz - 1) break;
