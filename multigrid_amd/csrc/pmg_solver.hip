@@ -190,8 +190,9 @@ void coarse_solve(pmg_ctx *c, int l, const double *ac64, const double *cc64,
 }
 
 // Relative tolerance of the FP64 coarsest CG inside the FP32 V-cycle of the
-// linear mixed-precision solve.  The cycle is a correction step of the FP64
-// defect correction, whose stopping test is the FP64 residual; its coarse
+// mixed-precision solves (the linear solve and Newton's inner linear
+// solves).  The cycle is a correction step of the FP64 defect correction,
+// whose stopping test is the FP64 residual; its coarse
 // correction is scaled by the Hackbusch-Reusken factor on the way up.  On
 // the benchmark hierarchies (coarsest 7^3 to 11^3) the residual history of
 // the cycle is the same to three digits whether the coarsest level is
@@ -312,8 +313,8 @@ int linear_mixed(pmg_ctx *c, LV<double> *L64, LV<float> *L32,
       fill<float>(L32[0].x, F.d.n(), 0.0f, c->s);
       coarse_solve<float>(c, l0, F.ac, L64[0].cc, r32, L32[0].x);
     } else {
-      vcycle<float>(c, L32, r32, c->L[c->nlev - 1].ac, cc64c, true, 0.0,
-                    nlev);
+      vcycle<float>(c, L32, r32, c->L[c->nlev - 1].ac, cc64c, true,
+                    kMixedCoarseTol, nlev);
     }
     add_correction<float>(F.d, L32[0].x, u, c->s);
     iters++;
