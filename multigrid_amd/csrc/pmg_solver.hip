@@ -1224,12 +1224,9 @@ int newton_level(pmg_ctx *c, int mode, int l0, double *x, int itmax,
     // Vgetjac (newtond.cc:488-531)
     copy_interior(d, w0n, F.rhs, s);
     dc_vec(F.cc, x, F.cpr, nf, s);
-    for (int l = l0 + 1; l < c->nlev; l++) {
-      restrict_<double>(c->L[l - 1].d, c->L[l].d, c->L[l - 1].rhs,
-                        c->L[l].rhs, c->L[l].pc, s);
-      restrict_<double>(c->L[l - 1].d, c->L[l].d, c->L[l - 1].cpr,
-                        c->L[l].cpr, c->L[l].pc, s);
-    }
+    for (int l = l0 + 1; l < c->nlev; l++)
+      restrict2(c->L[l - 1].d, c->L[l].d, c->L[l - 1].rhs, c->L[l - 1].cpr,
+                c->L[l].rhs, c->L[l].cpr, c->L[l].pc, s, nullptr);
     if (mode == PMG_MIXED)
       for (int l = l0; l < c->nlev; l++)
         convert<double, float>(c->L[l].cpr, c->L[l].cpr32, c->L[l].d.n(), s);
