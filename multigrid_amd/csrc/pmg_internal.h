@@ -198,7 +198,7 @@ void fbound_dirichlet(const Dim &d, double *x, const double *gx,
 // ---- ion terms (pmgc/mypdec.cc) ----
 void set_ions(int nion, const double *charge, const double *sconc);
 void dc_vec(const double *coef, const double *u, double *out, size_t n,
-            hipStream_t s);
+            hipStream_t s, float *out32 = nullptr);
 // x_out = x + damp * dx on the interior (boundary of x_out = boundary of x)
 void xpay_interior(const Dim &d, const double *x, double damp,
                    const double *dx, double *out, hipStream_t s);
@@ -217,7 +217,7 @@ void buildA(const Dim &d, double *ac, double *cc, double *fc,
 // mgsubd.cc:370-372) in one pass over P, each sum in restrict_'s term order.
 void restrict2(const Dim &f, const Dim &c, const double *a, const double *b,
                double *ao, double *bo, const double *pc, hipStream_t s,
-               float *ao32);
+               float *ao32, float *bo32 = nullptr);
 void buildP(const Dim &f, const Dim &c, int mgprol, int numdia,
             const double *acf, double *pc, hipStream_t s,
             float *pc32 = nullptr);

@@ -1078,18 +1078,23 @@ __device__ inline double ion_term(double coef, double u, int deriv) {
   return out;
 }
 
+// out32: the FP32 copy for the mixed-precision inner solves, rounded as a
+// separate conversion would.
 __global__ void k_dc_vec(const double *coef, const double *u, double *out,
-                         size_t n) {
+                         size_t n, float *out32) {
   for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n;
-       t += (size_t)gridDim.x * blockDim.x)
-    out[t] = ion_term(coef[t], u[t], 1);
+       t += (size_t)gridDim.x * blockDim.x) {
+    const double v = ion_term(coef[t], u[t], 1);
+    out[t] = v;
+    if (out32) out32[t] = (float)v;
+  }
 }
 
 void dc_vec(const double *coef, const double *u, double *out, size_t n,
-            hipStream_t s) {
+            hipStream_t s, float *out32) {
   int nb = (int)((n + 255) / 256);
   if (nb > 65535) nb = 65535;
-  k_dc_vec<<<nb, 256, 0, s>>>(coef, u, out, n);
+  k_dc_vec<<<nb, 256, 0, s>>>(coef, u, out, n, out32);
   check_launch("dc_vec");
 }
 
@@ -1324,7 +1329,7 @@ __global__ __launch_bounds__(256) void k_restrict2(
     int nxf, int nyf, int nxc, int nyc, int nzc, const double *__restrict__ a,
     const double *__restrict__ b, double *__restrict__ ao,
     double *__restrict__ bo, const double *__restrict__ pc, XG xg,
-    float *__restrict__ ao32) {
+    float *__restrict__ ao32, float *__restrict__ bo32) {
 #pragma clang fp contract(off)
   int bx, by, bz;
   if (!xcd_tile(xg, bx, by, bz)) return;
@@ -1362,12 +1367,14 @@ __global__ __launch_bounds__(256) void k_restrict2(
   const double va = ta[0] + ta[1] + ta[2];
   ao[pC] = va;
   if (ao32) ao32[pC] = (float)va;
-  bo[pC] = tb[0] + tb[1] + tb[2];
+  const double vb = tb[0] + tb[1] + tb[2];
+  bo[pC] = vb;
+  if (bo32) bo32[pC] = (float)vb;
 }
 
 void restrict2(const Dim &f, const Dim &c, const double *a, const double *b,
                double *ao, double *bo, const double *pc, hipStream_t s,
-               float *ao32) {
+               float *ao32, float *bo32) {
   if (c.nx < 3 || c.ny < 3 || c.nz < 3) return;
   if (f.nx != 2 * c.nx - 1 || f.ny != 2 * c.ny - 1 || f.nz != 2 * c.nz - 1) {
     fprintf(stderr, "pmg: restrict shape mismatch\n");
@@ -1375,7 +1382,7 @@ void restrict2(const Dim &f, const Dim &c, const double *a, const double *b,
   }
   dim3 g(cdiv(c.nx - 2, BX), cdiv(c.ny - 2, BY), c.nz - 2);
   k_restrict2<<<xcd_blocks(g), dim3(BX, BY), 0, s>>>(
-      f.nx, f.ny, c.nx, c.ny, c.nz, a, b, ao, bo, pc, xg_of(g), ao32);
+      f.nx, f.ny, c.nx, c.ny, c.nz, a, b, ao, bo, pc, xg_of(g), ao32, bo32);
   check_launch("restrict2");
 }
 

@@ -1223,13 +1223,14 @@ int newton_level(pmg_ctx *c, int mode, int l0, double *x, int itmax,
     iters++;
     // Vgetjac (newtond.cc:488-531)
     copy_interior(d, w0n, F.rhs, s);
-    dc_vec(F.cc, x, F.cpr, nf, s);
+    // (the FP32 copies of the Helmholtz term for mixed inner solves are
+    // written by the same kernels)
+    const bool m32 = mode == PMG_MIXED;
+    dc_vec(F.cc, x, F.cpr, nf, s, m32 ? F.cpr32 : nullptr);
     for (int l = l0 + 1; l < c->nlev; l++)
       restrict2(c->L[l - 1].d, c->L[l].d, c->L[l - 1].rhs, c->L[l - 1].cpr,
-                c->L[l].rhs, c->L[l].cpr, c->L[l].pc, s, nullptr);
-    if (mode == PMG_MIXED)
-      for (int l = l0; l < c->nlev; l++)
-        convert<double, float>(c->L[l].cpr, c->L[l].cpr32, c->L[l].d.n(), s);
+                c->L[l].rhs, c->L[l].cpr, c->L[l].pc, s, nullptr,
+                m32 ? c->L[l].cpr32 : nullptr);
     const double errtol_s = std::min(0.9 * xold, 1.0 * std::pow(xold, 2.0));
     fill<double>(xtmp, nf, 0.0, s);
     int its = 0;
