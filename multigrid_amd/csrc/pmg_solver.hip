@@ -1222,25 +1222,26 @@ int newton_level(pmg_ctx *c, int mode, int l0, double *x, int itmax,
   for (;;) {
     iters++;
     // Vgetjac (newtond.cc:488-531)
-    copy_interior(d, w0n, F.rhs, s);
-    // (the FP32 copies of the Helmholtz term for mixed inner solves are
-    // written by the same kernels)
+    // the right-hand side of the Newton correction is the nonlinear
+    // residual w0n itself (interior; the boundary is zero), read in place;
+    // the FP32 copies of the Helmholtz term for mixed inner solves are
+    // written by the kernels that produce it
     const bool m32 = mode == PMG_MIXED;
     dc_vec(F.cc, x, F.cpr, nf, s, m32 ? F.cpr32 : nullptr);
     for (int l = l0 + 1; l < c->nlev; l++)
-      restrict2(c->L[l - 1].d, c->L[l].d, c->L[l - 1].rhs, c->L[l - 1].cpr,
-                c->L[l].rhs, c->L[l].cpr, c->L[l].pc, s, nullptr,
-                m32 ? c->L[l].cpr32 : nullptr);
+      restrict2(c->L[l - 1].d, c->L[l].d, l - 1 == l0 ? w0n : c->L[l - 1].rhs,
+                c->L[l - 1].cpr, c->L[l].rhs, c->L[l].cpr, c->L[l].pc, s,
+                nullptr, m32 ? c->L[l].cpr32 : nullptr);
     const double errtol_s = std::min(0.9 * xold, 1.0 * std::pow(xold, 2.0));
     fill<double>(xtmp, nf, 0.0, s);
     int its = 0;
     double rel = 0.0;
     const double *cc64c = c->L[c->nlev - 1].cpr;
     if (mode == PMG_MIXED)
-      linear_mixed(c, L64 + l0, L32 + l0, F.rhs, cc64c, xtmp, 0, errtol_s,
+      linear_mixed(c, L64 + l0, L32 + l0, w0n, cc64c, xtmp, 0, errtol_s,
                    1000, &its, &rel, l0);
     else
-      linear_fp64(c, L64 + l0, F.rhs, cc64c, 0, errtol_s, 1000, &its, &rel,
+      linear_fp64(c, L64 + l0, w0n, cc64c, 0, errtol_s, 1000, &its, &rel,
                   l0);
     if (idamp == 1) {  // damping (newtond.cc:313-376)
       xpay_interior(d, x, 1.0, xtmp, w1, s);
