@@ -289,7 +289,9 @@ int linear_fp64(pmg_ctx *c, LV<double> *L, const double *frhs,
 // Mixed precision: FP64 defect correction around FP32 V-cycles.
 //   r = f - A u (FP64, stored as the FP32 right-hand side of the cycle)
 //   e = V-cycle_FP32(0, r);  u += e (FP64)
-// The stopping test is the FP64 L1 residual, as in linear_fp64.
+// The stopping test is the FP64 L1 residual, made after each cycle as in
+// linear_fp64 (at least one cycle, so the starting residual's norm is not
+// read back).
 int linear_mixed(pmg_ctx *c, LV<double> *L64, LV<float> *L32,
                  const double *frhs, const double *cc64c, double *u,
                  int istop, double errtol, int itmax, int *iters_out,
@@ -303,12 +305,12 @@ int linear_mixed(pmg_ctx *c, LV<double> *L64, LV<float> *L32,
   }
   if (rsden == 0.0) rsden = 1.0;
   float *r32 = F.w032;  // fine right-hand side of the FP32 cycle
-  resid<double, float>(F.d, F.numdia, F.ac, L64[0].cc, frhs, u, r32, &rnrm,
-                       c->s, L64[0].dg);
-  double rsnrm = host_scalar(c, S_RSNRM);
+  resid<double, float>(F.d, F.numdia, F.ac, L64[0].cc, frhs, u, r32,
+                       (const Red *)nullptr, c->s, L64[0].dg);
+  double rsnrm = 0.0;
   int iters = 0;
   const int nlev = c->nlev - l0;
-  while (iters < itmax && rsnrm / rsden > errtol) {
+  do {
     if (nlev == 1) {
       fill<float>(L32[0].x, F.d.n(), 0.0f, c->s);
       coarse_solve<float>(c, l0, F.ac, L64[0].cc, r32, L32[0].x);
@@ -321,7 +323,7 @@ int linear_mixed(pmg_ctx *c, LV<double> *L64, LV<float> *L32,
     resid<double, float>(F.d, F.numdia, F.ac, L64[0].cc, frhs, u, r32, &rnrm,
                          c->s, L64[0].dg);
     rsnrm = host_scalar(c, S_RSNRM);
-  }
+  } while (iters < itmax && rsnrm / rsden > errtol);
   *iters_out = iters;
   *rel_out = rsnrm / rsden;
   return (iters < itmax || rsnrm / rsden <= errtol) ? PMG_OK : PMG_ITMAX;
