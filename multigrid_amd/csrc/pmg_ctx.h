@@ -45,6 +45,10 @@ struct Level {
   // Newton iterate of a coarse level (nested iteration)
   double *cpr = nullptr, *rhs = nullptr, *xn = nullptr;
   float *cpr32 = nullptr;
+  // Newton, fine level: the diagonal oC + cpr of the Jacobian (FP64, and the
+  // FP32 sum of the FP32 copies for mixed inner solves)
+  double *dgn = nullptr;
+  float *dgn32 = nullptr;
   // 7-point (fine) level: the diagonal oC + cc, precomputed for the sweeps
   // and residuals (FP64; FP32 as the sum of the FP32 copies)
   double *dg = nullptr;

@@ -197,8 +197,12 @@ void fbound_dirichlet(const Dim &d, double *x, const double *gx,
 
 // ---- ion terms (pmgc/mypdec.cc) ----
 void set_ions(int nion, const double *charge, const double *sconc);
+// out32: FP32 copy of out; oc, dg: dg = oc + out (and dg32 = oc32 + out32
+// in FP32), the diagonal of the Jacobian's 7-point operator.
 void dc_vec(const double *coef, const double *u, double *out, size_t n,
-            hipStream_t s, float *out32 = nullptr);
+            hipStream_t s, float *out32 = nullptr, const double *oc = nullptr,
+            double *dg = nullptr, const float *oc32 = nullptr,
+            float *dg32 = nullptr);
 // x_out = x + damp * dx on the interior (boundary of x_out = boundary of x)
 void xpay_interior(const Dim &d, const double *x, double damp,
                    const double *dx, double *out, hipStream_t s);

@@ -1080,21 +1080,27 @@ __device__ inline double ion_term(double coef, double u, int deriv) {
 
 // out32: the FP32 copy for the mixed-precision inner solves, rounded as a
 // separate conversion would.
+// dg, dg32: the Jacobian's diagonal oC + out, summed as the sweeps and
+// residuals would (FP32: the sum of the FP32 copies).
 __global__ void k_dc_vec(const double *coef, const double *u, double *out,
-                         size_t n, float *out32) {
+                         size_t n, float *out32, const double *oc,
+                         double *dg, const float *oc32, float *dg32) {
   for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n;
        t += (size_t)gridDim.x * blockDim.x) {
     const double v = ion_term(coef[t], u[t], 1);
     out[t] = v;
     if (out32) out32[t] = (float)v;
+    if (dg) dg[t] = oc[t] + v;
+    if (dg32) dg32[t] = oc32[t] + (float)v;
   }
 }
 
 void dc_vec(const double *coef, const double *u, double *out, size_t n,
-            hipStream_t s, float *out32) {
+            hipStream_t s, float *out32, const double *oc, double *dg,
+            const float *oc32, float *dg32) {
   int nb = (int)((n + 255) / 256);
   if (nb > 65535) nb = 65535;
-  k_dc_vec<<<nb, 256, 0, s>>>(coef, u, out, n, out32);
+  k_dc_vec<<<nb, 256, 0, s>>>(coef, u, out, n, out32, oc, dg, oc32, dg32);
   check_launch("dc_vec");
 }
 

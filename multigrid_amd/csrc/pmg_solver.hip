@@ -1168,6 +1168,7 @@ int alloc_newton(pmg_ctx *c) {
     Level &L = c->L[l];
     L.cpr = c->alloc<double>(L.d.n());
     L.rhs = c->alloc<double>(L.d.n());
+    if (l == 0) L.dgn = c->alloc<double>(L.d.n());
     // the Newton iterate of a coarse level (nested iteration, mgkey 1)
     if (l > 0 && c->p.mgkey == 1) L.xn = c->alloc<double>(L.d.n());
   }
@@ -1209,14 +1210,19 @@ int newton_level(pmg_ctx *c, int mode, int l0, double *x, int itmax,
   int idamp = 1, iters = 0;
   LV<double> L64[MAXLEV];
   LV<float> L32[MAXLEV];
+  // The Jacobian term replaces cc; on the 7-point fine level its diagonal
+  // oC + cpr is written beside cpr each iteration (dc_vec), elsewhere the
+  // kernels form it themselves.
+  const bool fine7 = l0 == 0 && F.numdia == 7 && F.dgn &&
+                     (mode != PMG_MIXED || F.dgn32);
   for (int l = l0; l < c->nlev; l++) {
     L64[l] = view<double>(c->L[l]);
-    L64[l].cc = c->L[l].cpr;  // the Jacobian term: no precomputed diagonal
-    L64[l].dg = nullptr;
+    L64[l].cc = c->L[l].cpr;
+    L64[l].dg = (fine7 && l == l0) ? F.dgn : nullptr;
     if (mode == PMG_MIXED) {
       L32[l] = view<float>(c->L[l]);
       L32[l].cc = c->L[l].cpr32;
-      L32[l].dg = nullptr;
+      L32[l].dg = (fine7 && l == l0) ? F.dgn32 : nullptr;
     }
   }
   double *xtmp = F.x;  // iterate of the inner linear solve
@@ -1229,7 +1235,9 @@ int newton_level(pmg_ctx *c, int mode, int l0, double *x, int itmax,
     // the FP32 copies of the Helmholtz term for mixed inner solves are
     // written by the kernels that produce it
     const bool m32 = mode == PMG_MIXED;
-    dc_vec(F.cc, x, F.cpr, nf, s, m32 ? F.cpr32 : nullptr);
+    dc_vec(F.cc, x, F.cpr, nf, s, m32 ? F.cpr32 : nullptr,
+           fine7 ? F.ac : nullptr, fine7 ? F.dgn : nullptr,
+           fine7 && m32 ? F.ac32 : nullptr, fine7 && m32 ? F.dgn32 : nullptr);
     for (int l = l0 + 1; l < c->nlev; l++)
       restrict2(c->L[l - 1].d, c->L[l].d, l - 1 == l0 ? w0n : c->L[l - 1].rhs,
                 c->L[l - 1].cpr, c->L[l].rhs, c->L[l].cpr, c->L[l].pc, s,
@@ -1308,6 +1316,7 @@ int ctx_newton(pmg_ctx *c, int mode) {
     c->ops32_ok = true;
     for (int l = 0; l < c->nlev; l++)
       if (!c->L[l].cpr32) c->L[l].cpr32 = c->alloc<float>(c->L[l].d.n());
+    if (!c->L[0].dgn32) c->L[0].dgn32 = c->alloc<float>(c->L[0].d.n());
     if (!c->ok) return PMG_ENOMEM;
   }
   if (!coarse_fits(c)) return PMG_EBADPARM;
