@@ -64,8 +64,10 @@ def test_gsrb_large(mg, orc, numdia, n):
     runs as two paired-class launches per sweep (strips of 7 rows at 97^3
     and 131^3, of 3 rows at 41^3), an odd third sweep as the four class
     launches.  The fused 7-point sweep marches its k
-    chunk five planes per loop trip: 8-plane chunks leave 3 planes over, and
-    19^3, 22^3, 23^3 end on a chunk of 1, 4 and 5 planes."""
+    chunk five planes per loop trip: 97^3 takes chunks of 6 planes (one trip
+    and one plane over, the last chunk of 5 one trip), 19^3, 22^3, 23^3
+    chunks of 4 planes (no full trip) ending on a chunk of 1, 4 and 1
+    planes."""
     print(f"gsrb large n={n} numdia={numdia}", flush=True)
     rng = np.random.default_rng(n + numdia)
     ac, cc = H.random_op_vec(n, numdia, rng)
