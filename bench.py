@@ -13,7 +13,8 @@ The FP64 capacity, and p50/p99 latency of both precisions under Poisson
 arrivals at 0.8x the FP64 capacity, sit beside it.
 
 Further legs: the 161^3 and 289^3 single solves (p50/p99 over >= 10
-timed solves each), the 129^3 Newton solve, the banded coarsest solve,
+timed solves each), the 129^3 Newton solve (median of 5 timed solves),
+the banded coarsest solve,
 the class-style PCG; the roofline of the fine FP32 sweep at 289^3; the CPU
 oracle on the host cores.
 
@@ -86,9 +87,11 @@ def sweep_bytes(pts, sweeps, nu=(2, 2)):
     (V(nu1, nu2) cycles).  A sweep reads the diagonal oC + cc (precomputed
     once per setup), fc, oE, oN, uC and x and writes x: 7 arrays x 4 B per
     interior point.  The first pre-smoothing sweep of each correction cycle
-    starts from x = 0 and does not read x: 6 arrays."""
-    zero_start = sweeps // (nu[0] + nu[1])
-    return 4 * pts * (7 * sweeps - zero_start)
+    starts from x = 0 and does not read x: 6 arrays.  The last
+    post-smoothing sweep adds its result to the FP64 iterate u instead of
+    writing x: 6 arrays x 4 B + u read and written (2 x 8 B)."""
+    cycles = sweeps // (nu[0] + nu[1])
+    return 4 * pts * (7 * sweeps - cycles) + 12 * pts * cycles
 
 
 def measured_traffic(n):
@@ -333,10 +336,11 @@ def single_leg(mg, name, steps, oracle=None):
     return out
 
 
-def newton_leg(mg, oracle=None):
+def newton_leg(mg, oracle=None, steps=5):
     """The 129^3 nonlinear PBE (sinh ionic term, 0.3 M 1:1 salt, 800 atoms):
-    Newton with FP64 and with mixed-precision inner V-cycles to 1e-8, and
-    (with the CPU leg) the error against the FP64 oracle's Newton."""
+    Newton with FP64 and with mixed-precision inner V-cycles to 1e-8 (median
+    and largest of `steps` timed solves after one untimed one), and (with
+    the CPU leg) the error against the FP64 oracle's Newton."""
     cfg = mg.CONFIGS["newton129"]
     n, nlev = cfg["n"], cfg["nlev"]
     atoms = mg.pbe_molecule(cfg["natoms"], cfg["sphere_r"], cfg["charge_mode"], cfg["seed"])
@@ -349,11 +353,14 @@ def newton_leg(mg, oracle=None):
     us = {}
     for name, mode in (("fp64", mg.PMG_FP64), ("mixed", mg.PMG_MIXED)):
         ctx.solve(mode)  # warm
-        t0 = time.perf_counter()
-        ctx.solve(mode)
-        dt = time.perf_counter() - t0
+        lat = []
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            ctx.solve(mode)
+            lat.append((time.perf_counter() - t0) * 1e3)
         st = ctx.stats()
-        out[name] = {"ms": dt * 1e3, "newton_iters": st["iters"], "relres": st["relres"]}
+        out[name] = {"ms": float(np.median(lat)), "max_ms": max(lat), "timed_solves": steps,
+                     "newton_iters": st["iters"], "relres": st["relres"]}
         us[name] = ctx.u()
     ctx.close()
     out["mixed_rel_err_vs_fp64_gpu"] = rel_err(us["mixed"], us["fp64"])

@@ -46,10 +46,14 @@ void gs7(const Dim &d, const T *ac, const T *cc, const T *fc, T *x,
 // read, its boundary entries must already be zero.
 // dg (may be null): the diagonal oC + cc of the operator, precomputed
 // (diag_sum); the sweep then reads it instead of oC and cc.
+// uadd (FP32 sweeps with dg, not zero_init): the last sweep adds its result
+// to this FP64 iterate (u += (double)x at the interior points) instead of
+// storing it in x; returns true when it did.
 template <class T>
-void gs7_fused(const Dim &d, const T *ac, const T *cc, const T *fc, T *x,
+bool gs7_fused(const Dim &d, const T *ac, const T *cc, const T *fc, T *x,
                T *tmp, int nsweeps, int iadjoint, hipStream_t s,
-               bool zero_init = false, const T *dg = nullptr);
+               bool zero_init = false, const T *dg = nullptr,
+               double *uadd = nullptr);
 // dg[p] = ac[p] + cc[p] for the n points of a 7-point level (ac: its oC
 // plane), the diagonal the 7-point sweeps and residuals use.
 template <class T>

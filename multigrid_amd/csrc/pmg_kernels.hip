@@ -303,12 +303,17 @@ struct SwCo {  // own-point operator values of one plane
 // DG: the diagonal oC + cc comes precomputed in dg (the same sum, so the
 // same bits; one array less to read); ac and cc then serve only for the
 // couplings.
-template <class T, bool ZX, bool DG>
+// ADD: the sweep's result is not stored; it is added to the FP64 iterate
+// instead, uadd[p] += (double)x[p] at each interior point, with the
+// arithmetic of k_addcorr (the last sweep of a mixed-precision correction
+// cycle; every interior point is read and written by its owning thread
+// only, so no other thread sees the update).
+template <class T, bool ZX, bool DG, bool ADD = false>
 __device__ __forceinline__ void gs7_sweep_body(
     int nx, int ny, int nz, const T *__restrict__ ac, const T *__restrict__ cc,
     const T *__restrict__ fc, const T *__restrict__ xin, T *__restrict__ xout,
     int c1, int kc, int gx, int gy, int gz, unsigned bid,
-    const T *__restrict__ dg) {
+    const T *__restrict__ dg, double *__restrict__ uadd = nullptr) {
   __shared__ T SX[2][SW_TY][SW_TX], SE[3][SW_TY][SW_TX], SN[3][SW_TY][SW_TX],
       SM[3][SW_TY][SW_TX];
   // XCD-aware tile order: workgroups are dispatched round-robin over the 8
@@ -364,6 +369,15 @@ __device__ __forceinline__ void gs7_sweep_body(
   SwCo<T> C[5];
   T X[5];
   T mprev, mcur;
+  // ADD: the FP64 iterate of the owned column, two planes ahead (plane q in
+  // slot (q - kb) mod 5, as in the rings above)
+  double U[5];
+  if constexpr (ADD) {
+    if (owned) {
+      U[0] = uadd[at(kb)];
+      U[1] = uadd[at(min(kb + 1, ke))];
+    }
+  }
   {
     // Load order as in the loop's steady state: the last loads issued here
     // (C[2], X[3]) are the ones the first step does not read, so that step
@@ -404,6 +418,9 @@ __device__ __forceinline__ void gs7_sweep_body(
     // unconditional
     load(min(k + 3, ke + 1), C[(r + 3) % 5]);
     X[(r + 4) % 5] = xat(min(k + 4, ke + 2));
+    if constexpr (ADD) {
+      if (owned) U[(r + 2) % 5] = uadd[at(min(k + 2, ke))];
+    }
     const T x0 = X[r], x1 = X[(r + 1) % 5], x2 = X[(r + 2) % 5];
     const int s1 = (k + 1) % 3, s0 = k % 3;
     SX[(k + 1) & 1][ty][tx] = x1;
@@ -422,7 +439,10 @@ __device__ __forceinline__ void gs7_sweep_body(
                      uCm * mprev + P0.uC * mnext;
         v = (P0.fc + sm) / (P0.oC + P0.cc);
       }
-      xout[p0 + (size_t)(k - 1) * sz] = v;
+      if constexpr (ADD)
+        uadd[p0 + (size_t)(k - 1) * sz] = U[r] + (double)v;
+      else
+        xout[p0 + (size_t)(k - 1) * sz] = v;
     }
     mprev = mcur;
     mcur = mnext;
@@ -446,13 +466,14 @@ __device__ __forceinline__ void gs7_sweep_body(
   if (k + 3 <= ke) step(k + 3, I3{});
 }
 
-template <class T, bool ZX, bool DG>
+template <class T, bool ZX, bool DG, bool ADD = false>
 __global__ __launch_bounds__(SW_TX *SW_TY) void k_gs7_sweep(
     int nx, int ny, int nz, const T *__restrict__ ac, const T *__restrict__ cc,
     const T *__restrict__ fc, const T *__restrict__ xin, T *__restrict__ xout,
-    int c1, int kc, int gx, int gy, int gz, const T *__restrict__ dg) {
-  gs7_sweep_body<T, ZX, DG>(nx, ny, nz, ac, cc, fc, xin, xout, c1, kc, gx, gy,
-                            gz, blockIdx.x, dg);
+    int c1, int kc, int gx, int gy, int gz, const T *__restrict__ dg,
+    double *__restrict__ uadd) {
+  gs7_sweep_body<T, ZX, DG, ADD>(nx, ny, nz, ac, cc, fc, xin, xout, c1, kc,
+                                 gx, gy, gz, blockIdx.x, dg, uadd);
 }
 
 // Compute units of the current device (cached per device).
@@ -478,11 +499,14 @@ static int device_cus() {
 // shorter marches: 65^3 mixed solve 1.70 -> 1.62 ms against an 8-plane
 // floor).  zero_init: x starts
 // at zero (Vazeros), boundary included, and its old contents are not read.
+// uadd (FP32 with dg only): the last sweep adds its result to this FP64
+// iterate instead of storing it, and x is left as it was (returns true).
 template <class T>
-void gs7_fused(const Dim &d, const T *ac, const T *cc, const T *fc, T *x,
+bool gs7_fused(const Dim &d, const T *ac, const T *cc, const T *fc, T *x,
                T *tmp, int nsweeps, int iadjoint, hipStream_t s,
-               bool zero_init, const T *dg) {
-  if (d.nx < 3 || d.ny < 3 || d.nz < 3 || nsweeps <= 0) return;
+               bool zero_init, const T *dg, double *uadd) {
+  if (d.nx < 3 || d.ny < 3 || d.nz < 3 || nsweeps <= 0) return false;
+  const bool add = std::is_same<T, float>::value && uadd && dg && !zero_init;
   const int gx = cdiv(d.nx - 2, SW_TX - 4), gy = cdiv(d.ny - 2, SW_TY - 4);
   const int zb = std::max(1, device_cus() / (gx * gy));
   const int kc = std::max(4, cdiv(d.nz - 2, zb));
@@ -492,15 +516,20 @@ void gs7_fused(const Dim &d, const T *ac, const T *cc, const T *fc, T *x,
   const int c1 = iadjoint ? 1 : 0;
   T *src = x, *dst = tmp;
   for (int it = 0; it < nsweeps; it++) {
-#define SWEEP(ZX, DG)                                                         \
-  k_gs7_sweep<T, ZX, DG><<<nblk, b, 0, s>>>(d.nx, d.ny, d.nz, ac, cc, fc, src, \
-                                            dst, c1, kc, gx, gy, gz, dg)
+#define SWEEP(ZX, DG, ADD)                                                    \
+  k_gs7_sweep<T, ZX, DG, ADD><<<nblk, b, 0, s>>>(                             \
+      d.nx, d.ny, d.nz, ac, cc, fc, src, dst, c1, kc, gx, gy, gz, dg, uadd)
+    if (add && it == nsweeps - 1) {
+      if constexpr (std::is_same<T, float>::value) SWEEP(false, true, true);
+      check_launch("gs7_sweep");
+      return true;
+    }
     if (zero_init && it == 0) {
-      if (dg) SWEEP(true, true);
-      else SWEEP(true, false);
+      if (dg) SWEEP(true, true, false);
+      else SWEEP(true, false, false);
     } else {
-      if (dg) SWEEP(false, true);
-      else SWEEP(false, false);
+      if (dg) SWEEP(false, true, false);
+      else SWEEP(false, false, false);
     }
 #undef SWEEP
     T *t = src;
@@ -510,6 +539,7 @@ void gs7_fused(const Dim &d, const T *ac, const T *cc, const T *fc, T *x,
   if (src != x)
     hipMemcpyAsync(x, src, d.n() * sizeof(T), hipMemcpyDeviceToDevice, s);
   check_launch("gs7_sweep");
+  return false;
 }
 
 template <class T>
@@ -3308,9 +3338,9 @@ template void dot_energy<double>(const Dim &, int, const double *,
 #define INST(T)                                                               \
   template void gs7<T>(const Dim &, const T *, const T *, const T *, T *,     \
                        int, int, hipStream_t);                                \
-  template void gs7_fused<T>(const Dim &, const T *, const T *, const T *,    \
+  template bool gs7_fused<T>(const Dim &, const T *, const T *, const T *,    \
                              T *, T *, int, int, hipStream_t, bool,           \
-                             const T *);                                      \
+                             const T *, double *);                            \
   template void diag_sum<T>(const T *, const T *, T *, size_t, hipStream_t);  \
   template void gs27<T>(const Dim &, const T *, const T *, const T *, T *,    \
                         int, int, hipStream_t, bool, T *);                    \

@@ -142,22 +142,25 @@ LV<float> view<float>(Level &l) {
 
 // zero_init: x starts at zero (Vazeros).  bnd_zero: x's boundary entries
 // are zero already, so a 7-point level skips clearing x and the first sweep
-// does not read it.
+// does not read it.  uadd: where the level's fused FP32 sweep can, its last
+// sweep adds x to this FP64 iterate instead of storing x (returns true).
 template <class T>
-void smooth(pmg_ctx *c, const LV<T> &l, const T *f, T *x, int nu, int iadj,
-            bool fine, bool zero_init = false, bool bnd_zero = false) {
+bool smooth(pmg_ctx *c, const LV<T> &l, const T *f, T *x, int nu, int iadj,
+            bool fine, bool zero_init = false, bool bnd_zero = false,
+            double *uadd = nullptr) {
   if (nu <= 0) {
     if (zero_init) fill<T>(x, l.d.n(), (T)0, c->s);
-    return;
+    return false;
   }
   bool timed = c->ktime && !c->capturing && fine &&
                c->nev + 2 <= (int)c->ev.size();
   if (timed) hipEventRecord(c->ev[c->nev], c->s);
+  bool added = false;
   if (l.numdia == 7) {
     if (l.t && l.t != x) {
       if (zero_init && !bnd_zero) fill<T>(x, l.d.n(), (T)0, c->s);
-      gs7_fused<T>(l.d, l.ac, l.cc, f, x, l.t, nu, iadj, c->s,
-                   zero_init && bnd_zero, l.dg);
+      added = gs7_fused<T>(l.d, l.ac, l.cc, f, x, l.t, nu, iadj, c->s,
+                           zero_init && bnd_zero, l.dg, uadd);
     } else {
       if (zero_init) fill<T>(x, l.d.n(), (T)0, c->s);
       gs7<T>(l.d, l.ac, l.cc, f, x, nu, iadj, c->s);
@@ -172,6 +175,7 @@ void smooth(pmg_ctx *c, const LV<T> &l, const T *f, T *x, int nu, int iadj,
     c->nev += 2;
     c->ktime_count += nu;
   }
+  return added;
 }
 
 // Coarsest-level solve: Vcghs with 100 iterations and errtol = epsiln = 0
@@ -213,10 +217,14 @@ bool coarse_fits(pmg_ctx *c) {
 // mixed solve); its boundary entries are zero already.
 // nlevd: the cycle runs on the last nlevd levels of the hierarchy (L[0] is
 // the first of them; 0: all levels).
+// uadd: the FP64 iterate the cycle's fine result corrects.  Where the last
+// fine post-smoothing sweep can, it makes the update uadd += (double)x
+// itself and the cycle returns true (L[0].x then does not hold the
+// result); otherwise the caller adds L[0].x.
 template <class T>
-void vcycle(pmg_ctx *c, LV<T> *L, const T *frhs, const double *ac64c,
+bool vcycle(pmg_ctx *c, LV<T> *L, const T *frhs, const double *ac64c,
             const double *cc64c, bool fine_zero = false,
-            double coarse_tol = 0.0, int nlevd = 0) {
+            double coarse_tol = 0.0, int nlevd = 0, double *uadd = nullptr) {
   const int nlev = nlevd > 0 ? nlevd : c->nlev;
   const Params &p = c->p;
   hipStream_t s = c->s;
@@ -243,8 +251,12 @@ void vcycle(pmg_ctx *c, LV<T> *L, const T *frhs, const double *ac64c,
                             L[cl].x, L[cl].w0, L[cl].t, c->part, s);
     interp_damped<T>(L[cl].d, L[l].d, L[cl].x, L[cl].pc, L[l].x, c->part, nb,
                      s);
-    smooth<T>(c, L[l], l == 0 ? frhs : L[l].w0, L[l].x, p.nu2, 1, l == 0);
+    if (l == 0)
+      return smooth<T>(c, L[0], frhs, L[0].x, p.nu2, 1, true, false, false,
+                       uadd);
+    smooth<T>(c, L[l], L[l].w0, L[l].x, p.nu2, 1, false);
   }
+  return false;
 }
 
 // Linear solve with FP64 V-cycles.  L[0].x holds the iterate (zeroed by the
@@ -311,14 +323,15 @@ int linear_mixed(pmg_ctx *c, LV<double> *L64, LV<float> *L32,
   int iters = 0;
   const int nlev = c->nlev - l0;
   do {
+    bool added = false;
     if (nlev == 1) {
       fill<float>(L32[0].x, F.d.n(), 0.0f, c->s);
       coarse_solve<float>(c, l0, F.ac, L64[0].cc, r32, L32[0].x);
     } else {
-      vcycle<float>(c, L32, r32, c->L[c->nlev - 1].ac, cc64c, true,
-                    kMixedCoarseTol, nlev);
+      added = vcycle<float>(c, L32, r32, c->L[c->nlev - 1].ac, cc64c, true,
+                            kMixedCoarseTol, nlev, u);
     }
-    add_correction<float>(F.d, L32[0].x, u, c->s);
+    if (!added) add_correction<float>(F.d, L32[0].x, u, c->s);
     iters++;
     resid<double, float>(F.d, F.numdia, F.ac, L64[0].cc, frhs, u, r32, &rnrm,
                          c->s, L64[0].dg);
@@ -631,14 +644,15 @@ void step_body(pmg_ctx *c, int mode) {
   const bool resid_norm = c->p.istop <= 1;
   if (mode == PMG_MIXED) {
     float *r32 = F.w032;
+    bool added = false;
     if (nlev == 1) {
       fill<float>(L32[0].x, F.d.n(), 0.0f, c->s);
       coarse_solve<float>(c, 0, F.ac, F.cc, r32, L32[0].x);
     } else {
-      vcycle<float>(c, L32, r32, c->L[nlev - 1].ac, cc64c, true,
-                    kMixedCoarseTol);
+      added = vcycle<float>(c, L32, r32, c->L[nlev - 1].ac, cc64c, true,
+                            kMixedCoarseTol, 0, c->u);
     }
-    add_correction<float>(F.d, L32[0].x, c->u, c->s);
+    if (!added) add_correction<float>(F.d, L32[0].x, c->u, c->s);
     resid<double, float>(F.d, F.numdia, F.ac, F.cc, F.fc, c->u, r32,
                          resid_norm ? &rnrm : nullptr, c->s, F.dg);
   } else {
