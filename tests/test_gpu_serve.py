@@ -52,7 +52,10 @@ def test_same_alone_and_in_mix(mg):
 def test_grouped_server_same_results(mg, groups):
     """Grouped iterations (pmg_server_create_grouped): every request of the
     uneven mix, both precisions in flight at once, gives the bits, the
-    iteration count and the residual of the ungrouped server."""
+    iteration count and the residual of the ungrouped server.  The grouped
+    mixed iterations add the FP32 correction to u in a pass of their own,
+    the ungrouped ones in the last fine post-smoothing sweep,
+    so this also checks that the two updates give the same bits."""
     reqs = _requests(mg) * 2
     res = {}
     for g in (0, groups):
