@@ -973,16 +973,16 @@ void gs27(const Dim &d, const T *ac, const T *cc, const T *fc, T *x,
 }
 
 // ---------------------------------------------------------------------------
-// Residual (Vmresid7_1s / Vmresid27_1s) with optional L1 partial sums
+// Residual (Vmresid7_1s / Vmresid27_1s) with optional L1 partial sums: one
+// thread per (i, j) column of a chunk of kc planes (7-point levels with the
+// precomputed diagonal take the march k_resid7m below instead)
 // ---------------------------------------------------------------------------
-// DG (7-point only): the diagonal oC + cc comes precomputed in dg (same
-// bits, one array less to read).
-template <class T, class TO, int ND, bool DG = false>
+template <class T, class TO, int ND>
 __device__ __forceinline__ void resid_body(
     int nx, int ny, int nz, const T *__restrict__ ac,
     const T *__restrict__ cc, const T *__restrict__ fc,
     const T *__restrict__ x, TO *__restrict__ r, Red rd, int do_red, int kc,
-    XG xg, unsigned bid, const T *__restrict__ dg = nullptr) {
+    XG xg, unsigned bid) {
   __shared__ double red[4];
   const size_t n = (size_t)nx * ny * nz, sy = nx, sz = (size_t)nx * ny;
   int bx, by, bz;
@@ -1000,7 +1000,7 @@ __device__ __forceinline__ void resid_body(
       const int k = min(k0 + kk, nz - 1);
       const size_t p = ix(nx, ny, i, j, k);
       const T *oE = ac + n, *oN = ac + 2 * n, *uC = ac + 3 * n;
-      const T dgp = DG ? dg[p] : ac[p] + cc[p];
+      const T dgp = ac[p] + cc[p];
       vv[kk] = fc[p] + oN[p] * x[p + sy] + oN[p - sy] * x[p - sy] +
                oE[p] * x[p + 1] + oE[p - 1] * x[p - 1] +
                uC[p - sz] * x[p - sz] + uC[p] * x[p + sz] - dgp * x[p];
@@ -1019,7 +1019,7 @@ __device__ __forceinline__ void resid_body(
       T v;
       if (ND == 7) {
         const T *oE = ac + n, *oN = ac + 2 * n, *uC = ac + 3 * n;
-        const T dgp = DG ? dg[p] : ac[p] + cc[p];
+        const T dgp = ac[p] + cc[p];
         v = fc[p] + oN[p] * x[p + sy] + oN[p - sy] * x[p - sy] +
             oE[p] * x[p + 1] + oE[p - 1] * x[p - 1] +
             uC[p - sz] * x[p - sz] + uC[p] * x[p + sz] - dgp * x[p];
@@ -1039,14 +1039,154 @@ __device__ __forceinline__ void resid_body(
   }
 }
 
-template <class T, class TO, int ND, bool DG>
+template <class T, class TO, int ND>
 __global__ __launch_bounds__(256) void k_resid(
     int nx, int ny, int nz, const T *__restrict__ ac,
     const T *__restrict__ cc, const T *__restrict__ fc,
     const T *__restrict__ x, TO *__restrict__ r, Red rd, int do_red, int kc,
-    XG xg, const T *__restrict__ dg) {
-  resid_body<T, TO, ND, DG>(nx, ny, nz, ac, cc, fc, x, r, rd, do_red, kc, xg,
-                            blockIdx.x, dg);
+    XG xg) {
+  resid_body<T, TO, ND>(nx, ny, nz, ac, cc, fc, x, r, rd, do_red, kc, xg,
+                        blockIdx.x);
+}
+
+// The 7-point residual with the precomputed diagonal as a march in k (the
+// fine levels' residual, the FP64 outer residual of the mixed solve among
+// them).  A 1024-thread workgroup owns a (RM_TX-2) x (RM_TY-2) tile of
+// (i, j) columns plus one halo ring and marches a chunk of planes.  Each
+// thread loads only its own point of dg, fc, oE, oN, uC and x, into
+// five-slot register rings (the operator three planes ahead, x four, as in
+// the fused sweep), and takes its in-plane neighbours' x and the west and
+// south couplings oE(i-1), oN(j-1) from LDS: two slots, one barrier per
+// plane (a slot is rewritten two planes later, after a barrier every reader
+// has passed).  Against one thread per point with seven x loads through
+// the caches, every element is fetched once per tile.  The sum is
+// resid_body's, term for term, so r has the same bits; the L1 norm is
+// summed per thread over its planes, then over the workgroup.
+constexpr int RM_TX = 64, RM_TY = 16;
+template <class T, class TO>
+__device__ __forceinline__ void resid7m_body(
+    int nx, int ny, int nz, const T *__restrict__ ac,
+    const T *__restrict__ fc, const T *__restrict__ x, TO *__restrict__ r,
+    const T *__restrict__ dg, const Red &rd, int do_red, int kc, int gx,
+    int gy, int gz, unsigned bid) {
+  __shared__ T SX[2][RM_TY][RM_TX], SE[2][RM_TY][RM_TX], SN[2][RM_TY][RM_TX];
+  __shared__ double red[RM_TX * RM_TY / 64];
+  // XCD-aware tile order, as in the fused sweep
+  const int ntile = gx * gy * gz, per = (ntile + 7) / 8;
+  const int t = (int)(bid % 8) * per + (int)(bid / 8);
+  if (t >= ntile) {  // whole workgroup, before any barrier: a zero partial
+    if (do_red) red_store_at<1>(rd, bid, 0.0, 0.0);
+    return;
+  }
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int i = (t % gx) * (RM_TX - 2) + tx + 1;
+  const int j = ((t / gx) % gy) * (RM_TY - 2) + ty + 1;
+  const int kb = 2 + (t / (gx * gy)) * kc;
+  const int ke = min(kb + kc - 1, nz - 1);
+  const size_t n = (size_t)nx * ny * nz, sz = (size_t)nx * ny;
+  const T *oE = ac + n, *oN = ac + 2 * n, *uC = ac + 3 * n;
+  const bool owned = tx >= 1 && tx <= RM_TX - 2 && ty >= 1 &&
+                     ty <= RM_TY - 2 && i <= nx - 1 && j <= ny - 1;
+  // column base (i, j >= 1 by construction), clamped into the grid: loads
+  // stay unconditional
+  const size_t p0 = ix(nx, ny, min(i, nx), min(j, ny), 1);
+  auto at = [&](int q) { return p0 + (size_t)(min(max(q, 1), nz) - 1) * sz; };
+  struct Co {
+    T dg, fc, oE, oN, uC;
+  };
+  auto load = [&](int q, Co &P) {
+    const size_t p = at(q);
+    P.dg = dg[p], P.fc = fc[p], P.oE = oE[p], P.oN = oN[p], P.uC = uC[p];
+  };
+  // plane q in slot (q - kb) mod 5 of both rings; step k reads the operator
+  // of planes k-1 (uC) and k and x of planes k-1, k, k+1; the operator is
+  // read up to plane ke and x up to ke+1 (later steps re-read those)
+  Co C[5];
+  T X[5];
+  load(kb - 1, C[4]);
+  X[4] = x[at(kb - 1)];
+  load(min(kb, ke), C[0]);
+  X[0] = x[at(kb)];
+  load(min(kb + 1, ke), C[1]);
+  X[1] = x[at(min(kb + 1, ke + 1))];
+  load(min(kb + 2, ke), C[2]);
+  X[2] = x[at(min(kb + 2, ke + 1))];
+  X[3] = x[at(min(kb + 3, ke + 1))];
+  double acc = 0.0;
+  auto step = [&](int k, auto slot) __attribute__((always_inline)) {
+    constexpr int s5 = decltype(slot)::value;
+    const T uCm = C[(s5 + 4) % 5].uC, xm = X[(s5 + 4) % 5];
+    load(min(k + 3, ke), C[(s5 + 3) % 5]);
+    X[(s5 + 4) % 5] = x[at(min(k + 4, ke + 1))];
+    const Co &P = C[s5];
+    const T x0 = X[s5], xp = X[(s5 + 1) % 5];
+    const int s = k & 1;
+    SX[s][ty][tx] = x0;
+    SE[s][ty][tx] = P.oE;
+    SN[s][ty][tx] = P.oN;
+    __syncthreads();
+    if (owned) {
+      const T v = P.fc + P.oN * SX[s][ty + 1][tx] +
+                  SN[s][ty - 1][tx] * SX[s][ty - 1][tx] +
+                  P.oE * SX[s][ty][tx + 1] +
+                  SE[s][ty][tx - 1] * SX[s][ty][tx - 1] + uCm * xm +
+                  P.uC * xp - P.dg * x0;
+      if (r) r[p0 + (size_t)(k - 1) * sz] = (TO)v;
+      acc += fabs((double)v);
+    }
+  };
+  using I0 = std::integral_constant<int, 0>;
+  using I1 = std::integral_constant<int, 1>;
+  using I2 = std::integral_constant<int, 2>;
+  using I3 = std::integral_constant<int, 3>;
+  using I4 = std::integral_constant<int, 4>;
+  int k = kb;
+  for (; k + 4 <= ke; k += 5) {
+    step(k, I0{});
+    step(k + 1, I1{});
+    step(k + 2, I2{});
+    step(k + 3, I3{});
+    step(k + 4, I4{});
+  }
+  if (k <= ke) step(k, I0{});
+  if (k + 1 <= ke) step(k + 1, I1{});
+  if (k + 2 <= ke) step(k + 2, I2{});
+  if (k + 3 <= ke) step(k + 3, I3{});
+  if (do_red) {
+    const double tsum = block_sum<RM_TX * RM_TY>(acc, red);
+    red_store_at<1>(rd, bid, tsum, 0.0);
+  }
+}
+template <class T, class TO>
+__global__ __launch_bounds__(RM_TX *RM_TY) void k_resid7m(
+    int nx, int ny, int nz, const T *__restrict__ ac,
+    const T *__restrict__ fc, const T *__restrict__ x, TO *__restrict__ r,
+    const T *__restrict__ dg, Red rd, int do_red, int kc, int gx, int gy,
+    int gz) {
+  resid7m_body<T, TO>(nx, ny, nz, ac, fc, x, r, dg, rd, do_red, kc, gx, gy,
+                      gz, blockIdx.x);
+}
+
+// Launch shape of k_resid7m: one round of workgroups (one per compute
+// unit) covers the grid, with chunks of at least 4 planes.  Returns the
+// number of workgroups (and norm partials).
+static unsigned resid7m_shape(const Dim &d, int &kc, int &gx, int &gy,
+                              int &gz) {
+  gx = cdiv(d.nx - 2, RM_TX - 2), gy = cdiv(d.ny - 2, RM_TY - 2);
+  const int zb = std::max(1, device_cus() / (gx * gy));
+  kc = std::max(4, cdiv(d.nz - 2, zb));
+  gz = cdiv(d.nz - 2, kc);
+  return 8u * (unsigned)cdiv(gx * gy * gz, 8);
+}
+
+// The march serves a 7-point level with the precomputed diagonal when its
+// tiles of RM_TX - 2 owned columns need no more tiles across i than
+// resid_body's blocks of BX columns: with 63 or 127 interior columns (65^3,
+// 129^3) a third of its workgroups' columns would be idle (129^3 Newton
+// 7.61 -> 7.79 ms), and resid_body, computing the same sum in-kernel, is
+// the faster form there.
+static bool resid7m_fits(const Dim &d) {
+  return cdiv(d.nx - 2, RM_TX - 2) == cdiv(d.nx - 2, BX);
 }
 
 size_t resid_partials(const Dim &d) {
@@ -1058,21 +1198,27 @@ size_t resid_partials(const Dim &d) {
 template <class T, class TO>
 void resid(const Dim &d, int numdia, const T *ac, const T *cc, const T *fc,
            const T *x, TO *r, const Red *red, hipStream_t s, const T *dg) {
+  if (numdia == 7 && dg && resid7m_fits(d)) {
+    int kcm, gx, gy, gz;
+    const unsigned nb = resid7m_shape(d, kcm, gx, gy, gz);
+    const Red rdm = red ? *red : Red{};
+    k_resid7m<T, TO><<<nb, dim3(RM_TX, RM_TY), 0, s>>>(
+        d.nx, d.ny, d.nz, ac, fc, x, r, dg, rdm, red ? 1 : 0, kcm, gx, gy,
+        gz);
+    if (red) red_final<1, 0>(rdm, dim3(nb), s);
+    check_launch("resid");
+    return;
+  }
   const int kc = kchunk(d, red ? 40 : 80);
   dim3 g(cdiv(d.nx - 2, BX), cdiv(d.ny - 2, BY), cdiv(d.nz - 2, kc));
   dim3 b(BX, BY);
   const Red rd = red ? *red : Red{};
-  if (numdia == 7 && dg)
-    k_resid<T, TO, 7, true><<<xcd_blocks(g), b, 0, s>>>(
-        d.nx, d.ny, d.nz, ac, cc, fc, x, r, rd, red ? 1 : 0, kc, xg_of(g), dg);
-  else if (numdia == 7)
-    k_resid<T, TO, 7, false><<<xcd_blocks(g), b, 0, s>>>(
-        d.nx, d.ny, d.nz, ac, cc, fc, x, r, rd, red ? 1 : 0, kc, xg_of(g),
-        nullptr);
+  if (numdia == 7)
+    k_resid<T, TO, 7><<<xcd_blocks(g), b, 0, s>>>(
+        d.nx, d.ny, d.nz, ac, cc, fc, x, r, rd, red ? 1 : 0, kc, xg_of(g));
   else
-    k_resid<T, TO, 27, false><<<xcd_blocks(g), b, 0, s>>>(
-        d.nx, d.ny, d.nz, ac, cc, fc, x, r, rd, red ? 1 : 0, kc, xg_of(g),
-        nullptr);
+    k_resid<T, TO, 27><<<xcd_blocks(g), b, 0, s>>>(
+        d.nx, d.ny, d.nz, ac, cc, fc, x, r, rd, red ? 1 : 0, kc, xg_of(g));
   if (red) red_final<1, 0>(rd, dim3(xcd_blocks(g)), s);
   check_launch("resid");
 }
@@ -2869,7 +3015,7 @@ void g_gs7_fused(const GSmooth<T> *j, int n, int nsweeps, int iadjoint,
     G &e = g[q];
     e.gx = cdiv(d.nx - 2, SW_TX - 4), e.gy = cdiv(d.ny - 2, SW_TY - 4);
     const int zb = std::max(1, cus / (e.gx * e.gy));
-    e.kc = std::max(8, cdiv(d.nz - 2, zb));
+    e.kc = std::max(4, cdiv(d.nz - 2, zb));
     e.gz = cdiv(d.nz - 2, e.kc);
     e.nblk = 8 * cdiv(e.gx * e.gy * e.gz, 8);
     e.src = j[q].x, e.dst = j[q].tmp;
@@ -3026,12 +3172,22 @@ struct A_res {
   unsigned nblk;
   const T *dg;
 };
-template <class T, class TO, int ND, bool DG>
+template <class T, class TO, int ND>
 __global__ __launch_bounds__(256) void kb_resid(const GTab<A_res<T, TO>> t) {
   const A_res<T, TO> &a = t.a[blockIdx.y];
   if (blockIdx.x >= a.nblk) return;
-  resid_body<T, TO, ND, DG>(a.nx, a.ny, a.nz, a.ac, a.cc, a.fc, a.x, a.r,
-                            a.rd, a.do_red, a.kc, a.xg, blockIdx.x, a.dg);
+  resid_body<T, TO, ND>(a.nx, a.ny, a.nz, a.ac, a.cc, a.fc, a.x, a.r, a.rd,
+                        a.do_red, a.kc, a.xg, blockIdx.x);
+}
+// 7-point levels with the precomputed diagonal: k_resid7m's march (xg holds
+// its tile counts)
+template <class T, class TO>
+__global__ __launch_bounds__(RM_TX *RM_TY) void kb_resid7m(
+    const GTab<A_res<T, TO>> t) {
+  const A_res<T, TO> &a = t.a[blockIdx.y];
+  if (blockIdx.x >= a.nblk) return;
+  resid7m_body<T, TO>(a.nx, a.ny, a.nz, a.ac, a.fc, a.x, a.r, a.dg, a.rd,
+                      a.do_red, a.kc, a.xg.gx, a.xg.gy, a.xg.gz, blockIdx.x);
 }
 
 struct A_rf {
@@ -3053,13 +3209,22 @@ void g_resid(const GResid<T, TO> *j, int n, hipStream_t s) {
   for (int q = 0; q < n; q++) {  // the launch shape of resid
     const Dim &d = j[q].d;
     const bool red = j[q].red != nullptr;
-    const int kc = kchunk(d, red ? 40 : 80);
-    const dim3 g(cdiv(d.nx - 2, BX), cdiv(d.ny - 2, BY), cdiv(d.nz - 2, kc));
-    const T *dg = j[q].numdia == 7 ? j[q].dg : nullptr;
-    a[q] = A_res<T, TO>{d.nx,    d.ny,   d.nz,    j[q].ac,
+    const T *dg = j[q].numdia == 7 && resid7m_fits(d) ? j[q].dg : nullptr;
+    int kc;
+    XG xg;
+    unsigned nblk;
+    if (dg) {
+      nblk = resid7m_shape(d, kc, xg.gx, xg.gy, xg.gz);
+    } else {
+      kc = kchunk(d, red ? 40 : 80);
+      const dim3 g(cdiv(d.nx - 2, BX), cdiv(d.ny - 2, BY),
+                   cdiv(d.nz - 2, kc));
+      xg = xg_of(g), nblk = xcd_blocks(g);
+    }
+    a[q] = A_res<T, TO>{d.nx,    d.ny,    d.nz,   j[q].ac,
                         j[q].cc, j[q].fc, j[q].x, j[q].r,
                         red ? *j[q].red : Red{}, red ? 1 : 0, kc,
-                        xg_of(g), xcd_blocks(g), dg};
+                        xg,      nblk,    dg};
   }
   // variant key: 27 (27-point), 7 (7-point), 8 (7-point with dg)
   g_split<A_res<T, TO>>(
@@ -3070,11 +3235,11 @@ void g_resid(const GResid<T, TO> *j, int n, hipStream_t s) {
         for (int q = 0; q < m; q++) mx = std::max(mx, t.a[q].nblk);
         const dim3 grid(mx, m), b(BX, BY);
         if (key == 8)
-          kb_resid<T, TO, 7, true><<<grid, b, 0, s>>>(t);
+          kb_resid7m<T, TO><<<grid, dim3(RM_TX, RM_TY), 0, s>>>(t);
         else if (key == 7)
-          kb_resid<T, TO, 7, false><<<grid, b, 0, s>>>(t);
+          kb_resid<T, TO, 7><<<grid, b, 0, s>>>(t);
         else
-          kb_resid<T, TO, 27, false><<<grid, b, 0, s>>>(t);
+          kb_resid<T, TO, 27><<<grid, b, 0, s>>>(t);
       });
   std::vector<int> rq;
   for (int q = 0; q < n; q++)

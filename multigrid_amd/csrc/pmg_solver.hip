@@ -1780,8 +1780,11 @@ int mresid_t(int nx, int ny, int nz, int numdia, const T *ac, const T *cc,
   Scratch S;
   T *dac = S.put(ac, (numdia == 7 ? 4 : 14) * n), *dcc = S.put(cc, n),
     *dfc = S.put(fc, n), *dx = S.put(x, n), *dr = S.put(r, n);
+  // 7-point levels: the precomputed diagonal, as the cycle's residual
+  T *ddg = numdia == 7 ? S.put((const T *)nullptr, n) : nullptr;
   if (!S.ok) return PMG_ENOMEM;
-  resid<T, T>(d, numdia, dac, dcc, dfc, dx, dr, nullptr, 0);
+  if (ddg) diag_sum<T>(dac, dcc, ddg, n, 0);
+  resid<T, T>(d, numdia, dac, dcc, dfc, dx, dr, nullptr, 0, ddg);
   return finish(S, r, dr, n);
 }
 
