@@ -41,6 +41,30 @@ def test_matvec_resid(mg, orc, n, numdia):
     assert _rel(r, orc.mresid(n, numdia, ac, cc, f, x)) < 1e-13
 
 
+@pytest.mark.parametrize("dims", [(126, 40, 300), (131, 70, 45), (129, 41, 9)])
+def test_mresid7_shapes(mg, orc, dims):
+    """The 7-point residual on non-cubic grids: the k-march over several
+    tiles in i and j with ragged last tiles and a two-plane last k chunk
+    (126 x 40 x 300, 131 x 70 x 45), and a grid of 127 interior columns,
+    which the column kernel serves (129 x 41 x 9)."""
+    print(f"mresid7 dims={dims}", flush=True)
+    nx, ny, nz = dims
+    N = nx * ny * nz
+    rng = np.random.default_rng(sum(dims))
+    ac = rng.uniform(0.1, 1.0, 4 * N)
+    ac[:N] += 6.0
+    cc = rng.uniform(0.0, 1.0, N)
+    mask = np.zeros((nz, ny, nx))
+    mask[1:-1, 1:-1, 1:-1] = 1
+    f = (rng.standard_normal((nz, ny, nx)) * mask).ravel()
+    x = (rng.standard_normal((nz, ny, nx)) * mask).ravel()
+    r = np.zeros(N)
+    _call(mg, "pmg_mresid", nx, ny, nz, 7, dp(ac), dp(cc), dp(f), dp(x), dp(r))
+    ref = np.zeros(N)
+    orc.olib().pmgo_mresid(nx, ny, nz, 7, dp(ac), dp(cc), dp(f), dp(x), dp(ref))
+    assert _rel(r, ref) < 1e-13
+
+
 @pytest.mark.parametrize("numdia", [7, 27])
 @pytest.mark.parametrize("iadj", [0, 1])
 def test_gsrb(mg, orc, numdia, iadj):
