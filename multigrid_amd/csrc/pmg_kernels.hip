@@ -1181,12 +1181,17 @@ static unsigned resid7m_shape(const Dim &d, int &kc, int &gx, int &gy,
 
 // The march serves a 7-point level with the precomputed diagonal when its
 // tiles of RM_TX - 2 owned columns need no more tiles across i than
-// resid_body's blocks of BX columns: with 63 or 127 interior columns (65^3,
-// 129^3) a third of its workgroups' columns would be idle (129^3 Newton
-// 7.61 -> 7.79 ms), and resid_body, computing the same sum in-kernel, is
-// the faster form there.
+// resid_body's blocks of BX columns, and its k chunks are long enough to
+// hide the four planes its rings load before the first result.  With 63
+// or 127 interior columns (65^3, 129^3) a third of its workgroups' columns
+// would be idle (129^3 Newton 7.61 -> 7.79 ms); at 97^3 its chunks are 6
+// planes.  resid_body, computing the same sum in-kernel, serves there.
+constexpr int RM_MIN_PLANES = 16;
 static bool resid7m_fits(const Dim &d) {
-  return cdiv(d.nx - 2, RM_TX - 2) == cdiv(d.nx - 2, BX);
+  int kc, gx, gy, gz;
+  resid7m_shape(d, kc, gx, gy, gz);
+  return cdiv(d.nx - 2, RM_TX - 2) == cdiv(d.nx - 2, BX) &&
+         kc >= RM_MIN_PLANES;
 }
 
 size_t resid_partials(const Dim &d) {

@@ -87,12 +87,13 @@ def _abs_matvec(orc, dims, numdia, ac, cc, x):
 
 
 # (145, 37, 21), 27 points: rows longer than 129 points (two point pairs per
-# lane, all four points' operands loaded before any update).  (126, 40, 300)
-# and (131, 70, 45), 7 points: the residual's march over several tiles in i
-# and j, ragged last tiles, and a last k chunk of two planes at 300.
+# lane, all four points' operands loaded before any update).  (126, 40, 1000)
+# and (131, 70, 300), 7 points: the residual's march (k chunks of 24 and 18
+# planes on 256 compute units) over several tiles in i and j, with ragged
+# last tiles and ragged last chunks.
 CASES = [((17, 17, 17), 7), ((17, 17, 17), 27), ((97, 97, 97), 7), ((97, 97, 97), 27),
          ((33, 25, 19), 7), ((33, 25, 19), 27), ((145, 37, 21), 27),
-         ((126, 40, 300), 7), ((131, 70, 45), 7)]
+         ((126, 40, 1000), 7), ((131, 70, 300), 7)]
 
 
 @pytest.mark.parametrize("dims,numdia", CASES)

@@ -41,12 +41,13 @@ def test_matvec_resid(mg, orc, n, numdia):
     assert _rel(r, orc.mresid(n, numdia, ac, cc, f, x)) < 1e-13
 
 
-@pytest.mark.parametrize("dims", [(126, 40, 300), (131, 70, 45), (129, 41, 9)])
+@pytest.mark.parametrize("dims", [(126, 40, 1000), (131, 70, 300), (129, 41, 9), (126, 40, 60)])
 def test_mresid7_shapes(mg, orc, dims):
-    """The 7-point residual on non-cubic grids: the k-march over several
-    tiles in i and j with ragged last tiles and a two-plane last k chunk
-    (126 x 40 x 300, 131 x 70 x 45), and a grid of 127 interior columns,
-    which the column kernel serves (129 x 41 x 9)."""
+    """The 7-point residual on non-cubic grids: the k-march (k chunks of 24
+    and 18 planes on 256 compute units) over several tiles in i and j with
+    ragged last tiles and ragged last chunks (126 x 40 x 1000,
+    131 x 70 x 300), and the column kernel on a grid of 127 interior columns
+    (129 x 41 x 9) and on one whose chunks would be short (126 x 40 x 60)."""
     print(f"mresid7 dims={dims}", flush=True)
     nx, ny, nz = dims
     N = nx * ny * nz
